@@ -1,0 +1,202 @@
+/*
+ * salsa_amd.h -- C ABI of the MI355X SALSA coagulation + condensation step.
+ *
+ * This is the drop-in boundary for UCLALES-SALSA's per-grid-cell microphysics
+ * call (run_SALSA, src/src_salsa/mo_salsa_driver.f90:112-435, restricted to the
+ * coagulation and condensation processes of salsa(), mo_salsa.f90:92-107).
+ * The Fortran LES binds these symbols through ISO_C_BINDING
+ * (uclales-salsa_amd/fortran/salsa_amd_cabi.f90); plain pointers and sizes
+ * only, no torch or HIP types in any signature.
+ *
+ * Array layout (identical to the LES FieldArray storage):
+ *   3-D fields      f(k,i,j)          -> f[k + nzp*(i + nxp*j)]   (Fortran order)
+ *   number fields   n(k,i,j,b)        -> n[cell + ncell3d*b]
+ *   mass fields     m(k,i,j,(s-1)*nb+b) -> m[cell + ncell3d*(s*nb + b)]
+ *                   (util.f90:955-961 getMassIndex)
+ * with ncell3d = nzp*nxp*nyp.  Interior cells are k = hz..nzp-1-hz,
+ * i,j = hxy..n-1-hxy (0-based); the LES uses hz = 1, hxy = 2
+ * (mo_salsa_driver.f90:180-182).  Categories are ordered
+ * [aerosol, cloud, precipitation, ice] (classSection phase 1..4).
+ * Units follow the LES: #/kg and kg/kg; conversion to #/m3 and m3/m3 with
+ * the air density field dn happens inside the kernels.
+ */
+#ifndef SALSA_AMD_H
+#define SALSA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SALSA_MAXBIN 64
+#define SALSA_MAXSPEC 9
+
+/* ---- status codes (the reference STOPs or prints instead) ---- */
+enum {
+  SALSA_OK = 0,
+  SALSA_ERR_ARG = -1,          /* invalid argument / unsupported configuration     */
+  SALSA_ERR_HIP = -2,          /* a HIP runtime call failed                         */
+  SALSA_ERR_NOCONV = -3,       /* equilibration exceeded 100 iterations in some cell
+                                  (mo_salsa_properties.f90:238-243, :352-357)       */
+  SALSA_ERR_NONFINITE = -4,    /* NaN/Inf in the result (check_arrays analogue,
+                                  mo_salsa.f90:157-192)                              */
+  SALSA_ERR_NOMEM = -5
+};
+
+/* per-cell fault bits returned by salsa_get_cell_flags() */
+enum {
+  SALSA_CELL_NEG_NUMC = 1,     /* numc < -1e-6 clamped after coagulation (mo_salsa_dynamics.f90:191) */
+  SALSA_CELL_NOCONV = 2,       /* equilibration did not converge in 100 iterations                 */
+  SALSA_CELL_SUBSTEPS = 4,     /* > 100 condensation substeps (mo_salsa_dynamics.f90:1051)         */
+  SALSA_CELL_NONFINITE = 8     /* NaN/Inf produced                                                  */
+};
+
+/* precision modes */
+enum {
+  SALSA_FP64 = 0,              /* plain FP64 everywhere                                             */
+  SALSA_FP32_MIXED = 1         /* FP32 kernel coefficients and particle properties; FP64 state,
+                                  gain/loss sums, equilibration and vapour/mass closure             */
+};
+
+/* step flags */
+enum {
+  SALSA_STEP_TENDENCY = 0,     /* run_SALSA semantics: p-fields read-only, t-fields accumulated     */
+  SALSA_STEP_INPLACE = 1,      /* box-model / device-resident: write the new state into the p-fields */
+  SALSA_STEP_NO_COMPACT = 2,   /* do not reorder cells by cost class (debug / comparison)          */
+  SALSA_STEP_NO_GRAPH = 4      /* launch directly instead of through the captured hipGraph          */
+};
+
+/*
+ * Bin layout and species table: what set_aerobins / set_cloudbins /
+ * set_icebins / Species() derive from NAMELIST &salsa
+ * (mo_salsa_init.f90:148-450, classSpecies.f90:205-299).  0-based indices.
+ */
+typedef struct {
+  int32_t cur, par;            /* t_parallelbin, mo_submctl.f90:13-17 */
+} salsa_parbin;
+
+typedef struct {
+  /* configuration */
+  int32_t nbin[2];             /* nbin(1:2)                          */
+  double reglim[4];            /* reglim(1:4) [m]                    */
+  int32_t nprc; double prc_dlo, prc_vratio;   /* bloPrc            */
+  int32_t nice; double ice_dlo, ice_vratio;   /* bloIce            */
+  int32_t nspec_dry;
+  int32_t dry_species[7];      /* 0 SO4, 1 OC, 2 BC, 3 DU, 4 SS, 5 NO, 6 NH (namelist order) */
+  /* derived indices (0-based, category-local) */
+  int32_t in1a, fn1a, in2a, fn2a, in2b, fn2b, nbins;
+  salsa_parbin ica, fca, icb, fcb;
+  int32_t ncld;
+  int32_t ntotal;              /* nbins + ncld + nprc + nice         */
+  /* species in mass-array order: dry species, H2O, rime */
+  int32_t nspec, iwa, irim;
+  double rholiq[SALSA_MAXSPEC], rhoice[SALSA_MAXSPEC], mm[SALSA_MAXSPEC], diss[SALSA_MAXSPEC];
+  /* per bin, master order [aero | cloud | precip | ice] */
+  int32_t phase[SALSA_MAXBIN];
+  double vlolim[SALSA_MAXBIN], vhilim[SALSA_MAXBIN], dmid[SALSA_MAXBIN];
+  double nlim[SALSA_MAXBIN], dlim[SALSA_MAXBIN];
+  double precpbins[SALSA_MAXBIN];
+} salsa_bin_desc;
+
+/* The &salsa switches and constants that the coagulation + condensation path
+ * reads (mo_submctl.f90:21-142, mo_ice_shape.f90:25-30), plus driver options. */
+typedef struct {
+  int32_t level;               /* thermodynamical level 4 (warm) or 5 (ice)      */
+  int32_t lscoag, lscnd;       /* master switch states                           */
+  int32_t lsauto, lsauto_mode; /* lsauto%state and %mode (1: coagulation-based precip formation) */
+  int32_t lscgaa, lscgcc, lscgpp, lscgii, lscgca, lscgpa, lscgia, lscgpc, lscgic, lscgip;
+  int32_t lscndh2oae, lscndh2ocl, lscndh2oic;
+  int32_t lsfreeRH;            /* lsfreeRH%state; else rv limited to rhlim*rs    */
+  double rhlim;
+  double eddy_dis, Eiagg_max, Eiagg_min;
+  double ice_alpha, ice_beta, ice_gamma, ice_sigma;
+  int32_t lcharge;             /* charge enhancement with chargeTime = chargeTimeMax */
+  double charge_coll_enh;      /* chargeCollEnh: kernel scaled by (1 + this)       */
+  int32_t precision;           /* SALSA_FP64 or SALSA_FP32_MIXED                  */
+  int32_t hz, hxy;             /* halo widths of the 3-D arrays (LES: 1, 2)       */
+} salsa_params;
+
+typedef struct salsa_ctx salsa_ctx;
+
+/* Fill the bin/species descriptor (replaces set_aerobins/set_cloudbins/
+ * set_icebins/Species(), mo_salsa_init.f90:148-450, 792-858). */
+int salsa_bin_desc_init(salsa_bin_desc *desc, const int32_t nbin[2], const double reglim[4],
+                        int32_t nprc, double prc_dlo, double prc_vratio,
+                        int32_t nice, double ice_dlo, double ice_vratio,
+                        int32_t nspec_dry, const int32_t dry_species[7]);
+
+/* Reference defaults for the hot-path switches (mo_submctl.f90) with the
+ * level applied as define_salsa does (mo_salsa_init.f90:615-635). */
+int salsa_params_default(salsa_params *p, int32_t level);
+
+/* Create a context for one (nzp,nxp,nyp) grid on the current HIP device:
+ * device tables, work buffers, a stream and the tuned launch shapes.
+ * Replaces initialize_salsa_driver + the module state of run_SALSA
+ * (mo_salsa_driver.f90:481-561). */
+int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *params,
+                     int32_t nzp, int32_t nxp, int32_t nyp, salsa_ctx **out);
+
+void salsa_ctx_destroy(salsa_ctx *ctx);
+
+/*
+ * One SALSA coagulation + condensation step over all interior cells
+ * (run_SALSA with tstep = dt, mo_salsa_driver.f90:112-435).
+ *   press, temp, rsl, rsi, dn : Diag fields, (nzp,nxp,nyp)
+ *   rv_p : water vapour mixing ratio (Prog rp, tlev 1); rv_t : its tendency (+=)
+ *   n_p[c], m_p[c] : category c number / mass mixing ratios (read-only unless
+ *                    SALSA_STEP_INPLACE); n_t[c], m_t[c] : tendencies (+=).
+ *   Category 3 (ice) pointers are ignored at level 4 and may be NULL.
+ *   rates : NULL (process-rate diagnostics are not produced by this build)
+ *   time  : model time, used for the lsfreeRH switch state
+ * salsa_step_host takes host pointers (copies in and out on the context
+ * stream); salsa_step_device takes device pointers and only enqueues work
+ * (call salsa_ctx_sync before reading results).
+ */
+int salsa_step_host(salsa_ctx *ctx, const double *press, const double *temp,
+                    const double *rsl, const double *rsi, const double *dn,
+                    const double *rv_p, double *rv_t,
+                    const double *const n_p[4], double *const n_t[4],
+                    const double *const m_p[4], double *const m_t[4],
+                    double *rates, double tstep, double time, int32_t istp, int32_t flags);
+
+int salsa_step_device(salsa_ctx *ctx, const double *press, const double *temp,
+                      const double *rsl, const double *rsi, const double *dn,
+                      const double *rv_p, double *rv_t,
+                      const double *const n_p[4], double *const n_t[4],
+                      const double *const m_p[4], double *const m_t[4],
+                      double *rates, double tstep, double time, int32_t istp, int32_t flags);
+
+/* Wait for the context stream; returns SALSA_ERR_NOCONV / SALSA_ERR_NONFINITE
+ * if any cell of the last step raised the corresponding fault. */
+int salsa_ctx_sync(salsa_ctx *ctx);
+
+/* Copy the per-cell fault bits of the last step (one byte per interior cell,
+ * interior order k fastest) to host memory. */
+int salsa_get_cell_flags(salsa_ctx *ctx, uint8_t *flags_host, int64_t n);
+
+/* Launch-shape selection.  lanes_per_cell in {8,16,32,64} for each cost class
+ * (0 = clear and dry, 1 = clear and humid, 2 = cloudy); 0 keeps the current
+ * value.  salsa_autotune times the candidates on the given device-resident
+ * inputs and keeps the fastest per class. */
+int salsa_set_launch_shape(salsa_ctx *ctx, int32_t cls, int32_t lanes_per_cell);
+int salsa_get_launch_shape(salsa_ctx *ctx, int32_t cls);
+int salsa_autotune(salsa_ctx *ctx, const double *press, const double *temp,
+                   const double *rsl, const double *rsi, const double *dn,
+                   const double *rv_p, const double *const n_p[4],
+                   const double *const m_p[4], double tstep);
+
+/* Number of interior cells and of active bins in the context's layout. */
+int64_t salsa_ctx_ncells(const salsa_ctx *ctx);
+int32_t salsa_ctx_ntotal(const salsa_ctx *ctx);
+
+/* The HIP stream the context enqueues on (as an opaque pointer). */
+void *salsa_ctx_stream(salsa_ctx *ctx);
+
+/* Last error message of the calling thread (static storage). */
+const char *salsa_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

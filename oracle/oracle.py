@@ -1,0 +1,162 @@
+"""ctypes binding of the FP64 CPU oracle (libsalsa_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the thing that
+is measured or shipped.  See salsa_oracle.h for the reference citations.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "_build", "libsalsa_oracle.so")
+
+MAXBIN = 64
+MAXSPEC = 9
+NRATE = 20
+
+
+class ParBin(C.Structure):
+    _fields_ = [("cur", C.c_int), ("par", C.c_int)]
+
+
+class Layout(C.Structure):
+    _fields_ = [
+        ("nbin", C.c_int * 2), ("reglim", C.c_double * 4),
+        ("nprc", C.c_int), ("prc_dlo", C.c_double), ("prc_vratio", C.c_double),
+        ("nice", C.c_int), ("ice_dlo", C.c_double), ("ice_vratio", C.c_double),
+        ("nspec_dry", C.c_int), ("dry_species", C.c_int * 7),
+        ("in1a", C.c_int), ("fn1a", C.c_int), ("in2a", C.c_int), ("fn2a", C.c_int),
+        ("in2b", C.c_int), ("fn2b", C.c_int), ("nbins", C.c_int),
+        ("ica", ParBin), ("fca", ParBin), ("icb", ParBin), ("fcb", ParBin),
+        ("ncld", C.c_int),
+        ("iaero", C.c_int), ("icloud", C.c_int), ("iprecp", C.c_int), ("iice", C.c_int),
+        ("ntotal", C.c_int),
+        ("nspec", C.c_int), ("nwet", C.c_int), ("ndry", C.c_int), ("iwa", C.c_int), ("irim", C.c_int),
+        ("spec_id", C.c_int * MAXSPEC),
+        ("rholiq", C.c_double * MAXSPEC), ("rhoice", C.c_double * MAXSPEC),
+        ("mm", C.c_double * MAXSPEC), ("diss", C.c_double * MAXSPEC),
+        ("phase", C.c_int * MAXBIN),
+        ("vlolim", C.c_double * MAXBIN), ("vhilim", C.c_double * MAXBIN), ("dmid", C.c_double * MAXBIN),
+        ("nlim", C.c_double * MAXBIN), ("dlim", C.c_double * MAXBIN),
+        ("precpbins", C.c_double * MAXBIN),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("lscoag", C.c_int), ("lscnd", C.c_int),
+        ("lscgaa", C.c_int), ("lscgcc", C.c_int), ("lscgpp", C.c_int), ("lscgii", C.c_int),
+        ("lscgca", C.c_int), ("lscgpa", C.c_int), ("lscgia", C.c_int), ("lscgpc", C.c_int),
+        ("lscgic", C.c_int), ("lscgip", C.c_int),
+        ("lscndh2oae", C.c_int), ("lscndh2ocl", C.c_int), ("lscndh2oic", C.c_int),
+        ("lsauto_state", C.c_int), ("lsauto_mode", C.c_int),
+        ("eddy_dis", C.c_double), ("Eiagg_max", C.c_double), ("Eiagg_min", C.c_double),
+        ("ice_alpha", C.c_double), ("ice_beta", C.c_double), ("ice_gamma", C.c_double),
+        ("ice_sigma", C.c_double),
+        ("lcharge", C.c_int), ("charge_coll_enh", C.c_double),
+    ]
+
+
+class Rate(C.Structure):
+    _fields_ = [("volc", C.c_double * MAXSPEC), ("numc", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("oracle library missing: run `make -C oracle`")
+        L = C.CDLL(LIBPATH)
+        dp = C.POINTER(C.c_double)
+        L.so_layout_init.argtypes = [C.POINTER(Layout), C.POINTER(C.c_int), dp, C.c_int, C.c_double,
+                                     C.c_double, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.POINTER(C.c_int)]
+        L.so_params_default.argtypes = [C.POINTER(Params), C.c_int]
+        L.so_step_cell.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_double, C.c_double,
+                                   C.c_double, C.c_double, dp, C.c_double, dp, dp, C.POINTER(Rate)]
+        L.so_step_cell.restype = C.c_int
+        L.so_coagc.argtypes = [C.POINTER(Layout), C.POINTER(Params), dp, dp, C.c_int, C.c_int,
+                               C.c_double, C.c_double]
+        L.so_coagc.restype = C.c_double
+        L.so_terminal_vel_liquid.argtypes = [C.c_double] * 5
+        L.so_terminal_vel_liquid.restype = C.c_double
+        L.so_acth2o.argtypes = [C.POINTER(Layout), dp, dp]
+        L.so_acth2o.restype = C.c_double
+        L.so_rslf.argtypes = [C.c_double, C.c_double]
+        L.so_rslf.restype = C.c_double
+        L.so_rsif.argtypes = [C.c_double, C.c_double]
+        L.so_rsif.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def make_layout(nbin=(3, 4), reglim=(3.e-9, 9.703e-8, 1.e-5, 1.e-5), nprc=7, prc_dlo=20.e-6,
+                prc_vratio=4., nice=7, ice_dlo=2.e-6, ice_vratio=20., species=(0,)):
+    L = Layout()
+    nb = (C.c_int * 2)(*nbin)
+    rg = (C.c_double * 4)(*reglim)
+    sp = (C.c_int * 7)(*(list(species) + [0] * (7 - len(species))))
+    rc = lib().so_layout_init(C.byref(L), nb, rg, nprc, prc_dlo, prc_vratio, nice, ice_dlo,
+                              ice_vratio, len(species), sp)
+    if rc != 0:
+        raise ValueError("so_layout_init failed: %d" % rc)
+    return L
+
+
+def make_params(level=4, **kw):
+    P = Params()
+    lib().so_params_default(C.byref(P), level)
+    for k, v in kw.items():
+        setattr(P, k, v)
+    return P
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def step_cells(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, rates=False):
+    """Run `nsteps` box-model steps for each cell (SALSA units).
+
+    numc: [ncell, ntotal]; volc: [ncell, ntotal, 9].  Returns
+    (rv, numc, volc, status[, rates[ncell, 20, 10]]) as new arrays.
+    """
+    lb = lib()
+    n = len(T)
+    rv = np.array(rv, dtype=np.float64, copy=True)
+    numc = np.ascontiguousarray(numc, dtype=np.float64).copy()
+    volc = np.ascontiguousarray(volc, dtype=np.float64).copy()
+    status = np.zeros(n, dtype=np.int32)
+    rr = np.zeros((n, NRATE, MAXSPEC + 1)) if rates else None
+    rbuf = (Rate * NRATE)()
+    for i in range(n):
+        x = C.c_double(rv[i])
+        nc = numc[i]
+        vc = volc[i]
+        st = 0
+        for _ in range(nsteps):
+            st |= lb.so_step_cell(C.byref(L), C.byref(P), float(T[i]), float(p[i]), float(rs[i]),
+                                  float(rsi[i]), C.byref(x), dt, _dp(nc), _dp(vc),
+                                  rbuf if rates else None)
+        rv[i] = x.value
+        status[i] = st
+        if rates:
+            for k in range(NRATE):
+                rr[i, k, :MAXSPEC] = rbuf[k].volc[:]
+                rr[i, k, MAXSPEC] = rbuf[k].numc
+    if rates:
+        return rv, numc, volc, status, rr
+    return rv, numc, volc, status
+
+
+def rslf(p, t):
+    return lib().so_rslf(p, t)
+
+
+def rsif(p, t):
+    return lib().so_rsif(p, t)

@@ -1,0 +1,140 @@
+/*
+ * salsa_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of UCLALES-SALSA's per-grid-cell coagulation +
+ * water condensation step, used as the checker for the HIP implementation.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load or call it.  The product path never routes through this code.
+ *
+ * Every function in salsa_oracle.c cites the reference file:line it follows
+ * (paths relative to the UCLALES-SALSA source tree).
+ *
+ * Indexing is 0-based throughout; the reference's 1-based ranges are
+ * translated one-to-one (in1a = 0, ...).
+ */
+#ifndef SALSA_ORACLE_H
+#define SALSA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SO_MAXBIN 64   /* max bins over all categories */
+#define SO_MAXSPEC 9   /* Section%volc(9), src/src_salsa/classSection.f90:22 */
+#define SO_NRATE 20    /* classProcessRates.f90:41 nproc = 20 */
+
+/* t_parallelbin, src/src_shared/mo_submctl.f90:13-17 */
+typedef struct { int cur, par; } so_parbin;
+
+/* Bin layout + species table (mo_submctl.f90:204-249, mo_salsa_init.f90:148-450,
+ * classSpecies.f90:119-299).  Built by so_layout_init(). */
+typedef struct {
+  /* ---- configuration (NAMELIST &salsa) ---- */
+  int nbin[2];          /* nbin(1:2)                        mo_submctl.f90:213 */
+  double reglim[4];     /* reglim(1:4) [m]                  mo_submctl.f90:209 */
+  int nprc;  double prc_dlo, prc_vratio;   /* bloPrc        mo_submctl.f90:232 */
+  int nice;  double ice_dlo, ice_vratio;   /* bloIce        mo_submctl.f90:233 */
+  int nspec_dry;        /* number of dry species            mo_submctl.f90:187 */
+  int dry_species[7];   /* listspec as ids: 0 SO4,1 OC,2 BC,3 DU,4 SS,5 NO,6 NH */
+
+  /* ---- derived indices (mo_salsa_init.f90:800-814, 255-284) ---- */
+  int in1a, fn1a, in2a, fn2a, in2b, fn2b, nbins;
+  so_parbin ica, fca, icb, fcb;
+  int ncld;
+  int iaero, icloud, iprecp, iice, ntotal;   /* offsets into the master array */
+
+  /* ---- species (classSpecies.f90:205-299) ---- */
+  int nspec;            /* total incl. H2O and rime          */
+  int nwet, ndry;       /* nspec-1, nspec-2                  */
+  int iwa, irim;        /* 0-based H2O and rime slots        */
+  int spec_id[SO_MAXSPEC];      /* allNames index per slot (7 = H2O, 8 = rime) */
+  double rholiq[SO_MAXSPEC], rhoice[SO_MAXSPEC], mm[SO_MAXSPEC], diss[SO_MAXSPEC];
+
+  /* ---- per-bin constants (over the master array of ntotal bins) ---- */
+  int phase[SO_MAXBIN];
+  double vlolim[SO_MAXBIN], vhilim[SO_MAXBIN], dmid[SO_MAXBIN];
+  double nlim[SO_MAXBIN], dlim[SO_MAXBIN];
+  double precpbins[SO_MAXBIN];  /* precip low-limit diameters (mo_salsa_init.f90:347-350) */
+} so_layout;
+
+/* Switches and parameters read by the hot path (mo_submctl.f90:21-142,
+ * mo_ice_shape.f90:25-30).  All logicals are 0/1. */
+typedef struct {
+  int lscoag, lscnd;                          /* master %state                 */
+  int lscgaa, lscgcc, lscgpp, lscgii, lscgca, lscgpa, lscgia, lscgpc, lscgic, lscgip;
+  int lscndh2oae, lscndh2ocl, lscndh2oic;
+  int lsauto_state, lsauto_mode;
+  double eddy_dis, Eiagg_max, Eiagg_min;
+  double ice_alpha, ice_beta, ice_gamma, ice_sigma;
+  int lcharge;                 /* charge enhancement on                         */
+  double charge_coll_enh;      /* chargeCollEnh; with chargeTime = chargeTimeMax
+                                  the kernel is scaled by (1 + charge_coll_enh) */
+} so_params;
+
+/* One Rate (classProcessRates.f90:40-45): volc(9) and numc */
+typedef struct { double volc[SO_MAXSPEC]; double numc; } so_rate;
+
+/* Rate slots, same order as classProcessRates.f90:60-98 */
+enum {
+  SO_R_AUTOCONV = 0, SO_R_AUTOCONV50, SO_R_AUTOCONV80, SO_R_ACCR, SO_R_ACCR50,
+  SO_R_ACCR80, SO_R_ACCOLL, SO_R_APCOLL, SO_R_AICOLL, SO_R_ACTIV, SO_R_ICEHOM,
+  SO_R_ICEIMM, SO_R_ICEDEP, SO_R_CONDA, SO_R_CONDC, SO_R_CONDP, SO_R_CONDI,
+  SO_R_DRFR, SO_R_RMSPL, SO_R_IIBR
+};
+
+/* Per-cell status bits (the reference prints or STOPs instead) */
+enum {
+  SO_OK = 0,
+  SO_WARN_NEG_NUMC = 1,       /* numc < -1e-6 after coagulation (mo_salsa_dynamics.f90:191-193) */
+  SO_ERR_EQUIL_NOCONV = 2,    /* equilibration > 100 iterations (mo_salsa_properties.f90:238-243) */
+  SO_WARN_SUBSTEPS = 4        /* > 100 condensation substeps (mo_salsa_dynamics.f90:1051) */
+};
+
+/* Fill the layout from nbin/reglim/bloPrc/bloIce/species; returns 0 on success */
+int so_layout_init(so_layout *L, const int nbin[2], const double reglim[4],
+                   int nprc, double prc_dlo, double prc_vratio,
+                   int nice, double ice_dlo, double ice_vratio,
+                   int nspec_dry, const int dry_species[]);
+
+/* Reference defaults for the hot-path switches (mo_submctl.f90) with
+ * `level` applied as define_salsa does (mo_salsa_init.f90:615-635). */
+void so_params_default(so_params *P, int level);
+
+/*
+ * One SALSA coagulation + condensation step for one cell, SALSA units
+ * (#/m3, m3/m3).  numc[ntotal], volc[ntotal*SO_MAXSPEC] (row = bin) are
+ * updated in place; *rv (kg/kg) is updated in place.  rates (nullable)
+ * receives SO_NRATE accumulators (reset here, like mo_salsa_driver.f90:293).
+ * Returns a bitmask of SO_* status bits.
+ */
+int so_step_cell(const so_layout *L, const so_params *P,
+                 double ptemp, double ppres, double prs, double prsi,
+                 double *prv, double ptstep,
+                 double *numc, double *volc, so_rate *rates);
+
+/* Individual pieces, exported so that tests can pin them separately. */
+double so_coagc(const so_layout *L, const so_params *P,
+                const double *numc, const double *volc,
+                int b1, int b2, double temp, double pres);
+double so_terminal_vel_liquid(double diam, double rhop, double rhoa,
+                              double visc, double beta);
+double so_acth2o(const so_layout *L, const double *volc_bin, const double *pcw);
+double so_rslf(double p, double t);
+double so_rsif(double p, double t);
+
+/* Driver-level pack (mo_salsa_driver.f90:230-279) and tendency unpack
+ * (:310-388) for one cell.  n_p[ntotal] (#/kg), m_p[ntotal*... ] in the LES
+ * mass layout of each category: m(cat)[(nc)*nb_cat + b].  These operate on
+ * one cell's vectors gathered by the caller. */
+void so_pack_cell(const so_layout *L, int level, double dn,
+                  const double *n_cat[4], const double *m_cat[4],
+                  double *numc, double *volc);
+void so_unpack_tend(const so_layout *L, int level, double dn, double tstep,
+                    const double *numc, const double *volc,
+                    const double *n_cat[4], const double *m_cat[4],
+                    double *nt_cat[4], double *mt_cat[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
