@@ -1,0 +1,444 @@
+// salsa_capi.hip -- the extern "C" boundary (include/salsa_amd.h): context
+// lifecycle, host- and device-pointer step entry points, cost-class
+// compaction of cells, hipGraph capture of the per-step launch sequence and
+// the launch-shape autotuner.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "salsa_internal.h"
+
+namespace salsa {
+extern thread_local char g_err[512];
+void set_error(const char *msg);
+int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &L);
+int launch_step(int cap, int lanes, int precision, const DevLayout *d, const StepArgs &a, hipStream_t s);
+int cap_bins(int cap);
+int cap_pairs(int cap);
+int cap_spec(int cap);
+}  // namespace salsa
+
+using namespace salsa;
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) {                                                              \
+      std::snprintf(g_err, sizeof(g_err), "%s failed: %s", #x, hipGetErrorString(e_));   \
+      return SALSA_ERR_HIP;                                                              \
+    }                                                                                    \
+  } while (0)
+
+struct salsa_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  salsa_bin_desc desc;
+  salsa_params params;
+  DevLayout hostL;
+  DevLayout *devL = nullptr;
+  int cap = 4;
+  int lanes[3] = {32, 32, 32};
+  int32_t nzp = 0, nxp = 0, nyp = 0, nz = 0, nx = 0, ny = 0;
+  int64_t n3 = 0, ncells = 0;
+  // per-cell fault flags and the global OR
+  uint8_t *flags = nullptr;
+  int32_t *fault = nullptr;
+  // compaction: cost class per cell, cell ids sorted by class, class counts
+  uint8_t *cls = nullptr, *cls_sorted = nullptr;
+  int32_t *ids = nullptr, *ids_sorted = nullptr;
+  int32_t *cls_count = nullptr;       // device [3]
+  int32_t *cls_count_h = nullptr;     // pinned host [3]
+  void *cub_tmp = nullptr;
+  size_t cub_bytes = 0;
+  // staging for salsa_step_host
+  double *stage = nullptr;
+  size_t stage_bytes = 0;
+  // captured launch sequence
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  StepArgs gkey;
+  int glanes[3] = {0, 0, 0};
+  bool gvalid = false;
+};
+
+// ---------------------------------------------------------------------------
+// small kernels: fault reset, cost classes
+// ---------------------------------------------------------------------------
+__global__ void k_zero_i32(int32_t *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Cost class of a cell from its inputs: 2 = any hydrometeor present (cloud /
+// precipitation / ice: full coagulation sweep and condensation substeps),
+// 1 = aerosol only and RH > 0.98 (aerosol substeps), 0 = aerosol only and
+// drier (equilibrium water only).  Mirrors the predicates of gpparth2o
+// (mo_salsa_dynamics.f90:784-788) evaluated on the input state.
+__global__ void k_classify(const DevLayout *__restrict__ Dp, StepArgs a, uint8_t *cls, int32_t *ids,
+                           int32_t *counts) {
+  const DevLayout &D = *Dp;
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncells) return;
+  int k = (int)(c % a.nz), r = (int)(c / a.nz);
+  int i = r % a.nx, j = r / a.nx;
+  int64_t g = (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
+  double dn = a.dn[g];
+  bool hyd = false;
+  for (int cat = 1; cat < D.ncat; ++cat) {
+    const double *n = a.n_p[cat];
+    for (int b = 0; b < D.nb[cat]; ++b) hyd |= n[g + a.n3 * b] * dn > (cat == 1 ? kNlim : kPrlim);
+  }
+  double rv = a.rv_p[g], rs = a.rsl[g];
+  if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);
+  uint8_t cl = hyd ? 2 : (rv / rs > 0.98 ? 1 : 0);
+  cls[c] = cl;
+  ids[c] = (int32_t)c;
+  atomicAdd(&counts[cl], 1);
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static int substeps(double dt) {
+  // DO WHILE (ttot < ptstep) with adt = 0.02 accumulated in FP64 (mo_salsa_dynamics.f90:988-1053)
+  double ttot = 0.;
+  int n = 0;
+  while (ttot < dt && n < 100000) { ttot = ttot + 2.e-2; ++n; }
+  return n;
+}
+
+static int pick_cap(const DevLayout &L) {
+  for (int cap : {4, 5})
+    if (L.nt <= cap_bins(cap) && L.npair <= cap_pairs(cap) && L.nspec <= cap_spec(cap)) return cap;
+  return 0;
+}
+
+static void release(salsa_ctx *c) {
+  if (!c) return;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  (void)hipFree(c->devL); (void)hipFree(c->flags); (void)hipFree(c->fault);
+  (void)hipFree(c->cls); (void)hipFree(c->cls_sorted);
+  (void)hipFree(c->ids); (void)hipFree(c->ids_sorted); (void)hipFree(c->cls_count);
+  (void)hipFree(c->cub_tmp);
+  (void)hipFree(c->stage);
+  if (c->cls_count_h) (void)hipHostFree(c->cls_count_h);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" const char *salsa_last_error(void) { return g_err; }
+
+extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *params, int32_t nzp,
+                                int32_t nxp, int32_t nyp, salsa_ctx **out) {
+  if (!desc || !params || !out) { set_error("null argument"); return SALSA_ERR_ARG; }
+  *out = nullptr;
+  const int hz = params->hz, hxy = params->hxy;
+  if (hz < 0 || hxy < 0 || nzp - 2 * hz < 1 || nxp - 2 * hxy < 1 || nyp - 2 * hxy < 1) {
+    set_error("grid has no interior cells for the given halo widths");
+    return SALSA_ERR_ARG;
+  }
+  if (params->precision != SALSA_FP64 && params->precision != SALSA_FP32_MIXED) {
+    set_error("precision must be SALSA_FP64 or SALSA_FP32_MIXED");
+    return SALSA_ERR_ARG;
+  }
+  salsa_ctx *c = new (std::nothrow) salsa_ctx();
+  if (!c) { set_error("out of host memory"); return SALSA_ERR_NOMEM; }
+  c->desc = *desc;
+  c->params = *params;
+  int rc = build_dev_layout(*desc, *params, c->hostL);
+  if (rc) { delete c; return rc; }
+  c->cap = pick_cap(c->hostL);
+  if (!c->cap) { set_error("layout exceeds the compiled bin/pair capacity"); delete c; return SALSA_ERR_ARG; }
+  c->nzp = nzp; c->nxp = nxp; c->nyp = nyp;
+  c->nz = nzp - 2 * hz; c->nx = nxp - 2 * hxy; c->ny = nyp - 2 * hxy;
+  c->n3 = (int64_t)nzp * nxp * nyp;
+  c->ncells = (int64_t)c->nz * c->nx * c->ny;
+  if (c->ncells > INT32_MAX) { set_error("too many cells"); delete c; return SALSA_ERR_ARG; }
+  // default shapes: measured fastest on MI355X for the warm layout (see salsa_autotune)
+  c->lanes[0] = 32; c->lanes[1] = 32; c->lanes[2] = 32;
+  auto fail = [&](int code) { release(c); delete c; return code; };
+  if (hipGetDevice(&c->device) != hipSuccess) { set_error("no HIP device"); return fail(SALSA_ERR_HIP); }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed"); return fail(SALSA_ERR_HIP);
+  }
+  const size_t nc = (size_t)c->ncells;
+  if (hipMalloc(&c->devL, sizeof(DevLayout)) != hipSuccess ||
+      hipMalloc(&c->flags, nc) != hipSuccess || hipMalloc(&c->fault, sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->cls, nc) != hipSuccess || hipMalloc(&c->cls_sorted, nc) != hipSuccess ||
+      hipMalloc(&c->ids, nc * 4) != hipSuccess || hipMalloc(&c->ids_sorted, nc * 4) != hipSuccess ||
+      hipMalloc(&c->cls_count, 3 * sizeof(int32_t)) != hipSuccess ||
+      hipHostMalloc(&c->cls_count_h, 3 * sizeof(int32_t)) != hipSuccess) {
+    set_error("device allocation failed");
+    return fail(SALSA_ERR_NOMEM);
+  }
+  if (hipMemcpy(c->devL, &c->hostL, sizeof(DevLayout), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("copy of device tables failed"); return fail(SALSA_ERR_HIP);
+  }
+  // radix-sort temporary storage for the class compaction
+  size_t bytes = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->cls, c->cls_sorted, c->ids, c->ids_sorted,
+                                         (int)nc, 0, 2, c->stream) != hipSuccess ||
+      hipMalloc(&c->cub_tmp, bytes ? bytes : 16) != hipSuccess) {
+    set_error("sort workspace allocation failed");
+    return fail(SALSA_ERR_NOMEM);
+  }
+  c->cub_bytes = bytes;
+  *out = c;
+  return SALSA_OK;
+}
+
+extern "C" void salsa_ctx_destroy(salsa_ctx *c) {
+  if (!c) return;
+  hipStreamSynchronize(c->stream);
+  release(c);
+  delete c;
+}
+
+extern "C" int64_t salsa_ctx_ncells(const salsa_ctx *c) { return c ? c->ncells : -1; }
+extern "C" int32_t salsa_ctx_ntotal(const salsa_ctx *c) { return c ? c->hostL.nt : -1; }
+extern "C" void *salsa_ctx_stream(salsa_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int salsa_set_launch_shape(salsa_ctx *c, int32_t cls, int32_t lanes) {
+  if (!c || cls < 0 || cls > 2) { set_error("bad class"); return SALSA_ERR_ARG; }
+  if (lanes == 0) return SALSA_OK;
+  if (lanes != 16 && lanes != 32 && lanes != 64) { set_error("lanes_per_cell must be 16, 32 or 64"); return SALSA_ERR_ARG; }
+  c->lanes[cls] = lanes;
+  return SALSA_OK;
+}
+
+extern "C" int salsa_get_launch_shape(salsa_ctx *c, int32_t cls) {
+  if (!c || cls < 0 || cls > 2) return SALSA_ERR_ARG;
+  return c->lanes[cls];
+}
+
+static StepArgs make_args(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                          const double *rsi, const double *dn, const double *rv_p, double *rv_t,
+                          const double *const n_p[4], double *const n_t[4], const double *const m_p[4],
+                          double *const m_t[4], double tstep, int32_t flags) {
+  StepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nzp = c->nzp; a.nxp = c->nxp; a.nyp = c->nyp; a.hz = c->params.hz; a.hxy = c->params.hxy;
+  a.nz = c->nz; a.nx = c->nx; a.ny = c->ny; a.n3 = c->n3; a.ncells = c->ncells;
+  a.press = press; a.temp = temp; a.rsl = rsl; a.rsi = rsi; a.dn = dn; a.rv_p = rv_p; a.rv_t = rv_t;
+  const int ncat = c->hostL.ncat;
+  for (int k = 0; k < 4; ++k) {
+    a.n_p[k] = k < ncat ? n_p[k] : nullptr;
+    a.m_p[k] = k < ncat ? m_p[k] : nullptr;
+    a.n_t[k] = (k < ncat && n_t) ? n_t[k] : nullptr;
+    a.m_t[k] = (k < ncat && m_t) ? m_t[k] : nullptr;
+    a.n_o[k] = const_cast<double *>(a.n_p[k]);
+    a.m_o[k] = const_cast<double *>(a.m_p[k]);
+  }
+  a.rv_o = const_cast<double *>(rv_p);
+  a.dt = tstep;
+  a.nsub = substeps(tstep);
+  a.inplace = (flags & SALSA_STEP_INPLACE) ? 1 : 0;
+  a.lsfreeRH = c->params.lsfreeRH;
+  a.rhlim = c->params.rhlim;
+  a.cell_flags = c->flags;
+  a.fault_any = c->fault;
+  return a;
+}
+
+static int check_ptrs(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                      const double *rsi, const double *dn, const double *rv_p, double *rv_t,
+                      const double *const n_p[4], double *const n_t[4], const double *const m_p[4],
+                      double *const m_t[4], double *rates, double tstep, int32_t flags) {
+  if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  if (rates) { set_error("process-rate diagnostics are not produced by this build (pass NULL)"); return SALSA_ERR_ARG; }
+  if (!(tstep > 0.)) { set_error("tstep must be positive"); return SALSA_ERR_ARG; }
+  if (!press || !temp || !rsl || !dn || !rv_p || !n_p || !m_p) { set_error("null field pointer"); return SALSA_ERR_ARG; }
+  if (c->hostL.level == 5 && !rsi) { set_error("rsi required at level 5"); return SALSA_ERR_ARG; }
+  const bool inplace = flags & SALSA_STEP_INPLACE;
+  if (!inplace && (!rv_t || !n_t || !m_t)) { set_error("null tendency pointer"); return SALSA_ERR_ARG; }
+  for (int k = 0; k < c->hostL.ncat; ++k) {
+    if (!n_p[k] || !m_p[k]) { set_error("null category field"); return SALSA_ERR_ARG; }
+    if (!inplace && (!n_t[k] || !m_t[k])) { set_error("null category tendency"); return SALSA_ERR_ARG; }
+  }
+  return SALSA_OK;
+}
+
+// Enqueue: fault reset, classification + sort (unless disabled), per-class launches.
+static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
+  if (flags & SALSA_STEP_NO_COMPACT) {
+    a.cell_list = nullptr;
+    int rc = launch_step(c->cap, c->lanes[2], c->params.precision, c->devL, a, s);
+    if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
+    return SALSA_OK;
+  }
+  // classify and sort cell ids by class (stable: k-fastest order kept within a class)
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->cls_count, 3);
+  const int nb = (int)((c->ncells + 255) / 256);
+  hipLaunchKernelGGL(k_classify, dim3(nb), dim3(256), 0, s, c->devL, a, c->cls, c->ids, c->cls_count);
+  size_t bytes = c->cub_bytes;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, bytes, c->cls, c->cls_sorted, c->ids, c->ids_sorted,
+                                            (int)c->ncells, 0, 2, s));
+  HIPCHK(hipMemcpyAsync(c->cls_count_h, c->cls_count, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  int64_t off = 0;
+  for (int k = 0; k < 3; ++k) {
+    StepArgs b = a;
+    b.cell_list = c->ids_sorted + off;
+    b.ncells = c->cls_count_h[k];
+    off += b.ncells;
+    if (b.ncells == 0) continue;
+    int rc = launch_step(c->cap, c->lanes[k], c->params.precision, c->devL, b, s);
+    if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  }
+  return SALSA_OK;
+}
+
+extern "C" int salsa_step_device(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                                 const double *rsi, const double *dn, const double *rv_p, double *rv_t,
+                                 const double *const n_p[4], double *const n_t[4],
+                                 const double *const m_p[4], double *const m_t[4], double *rates,
+                                 double tstep, double time, int32_t istp, int32_t flags) {
+  (void)time; (void)istp;
+  int rc = check_ptrs(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, rates, tstep, flags);
+  if (rc) return rc;
+  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, tstep, flags);
+  return enqueue_step(c, a, flags);
+}
+
+extern "C" int salsa_ctx_sync(salsa_ctx *c) {
+  if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int32_t f = 0;
+  HIPCHK(hipMemcpy(&f, c->fault, sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (f & SALSA_CELL_NONFINITE) { set_error("non-finite values produced in some cell"); return SALSA_ERR_NONFINITE; }
+  if (f & SALSA_CELL_NOCONV) { set_error("equilibration did not converge in some cell"); return SALSA_ERR_NOCONV; }
+  return SALSA_OK;
+}
+
+extern "C" int salsa_get_cell_flags(salsa_ctx *c, uint8_t *out, int64_t n) {
+  if (!c || !out || n < c->ncells) { set_error("bad arguments"); return SALSA_ERR_ARG; }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(out, c->flags, (size_t)c->ncells, hipMemcpyDeviceToHost));
+  return SALSA_OK;
+}
+
+// Host-pointer variant: stage every field through device memory owned by the context.
+extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                               const double *rsi, const double *dn, const double *rv_p, double *rv_t,
+                               const double *const n_p[4], double *const n_t[4],
+                               const double *const m_p[4], double *const m_t[4], double *rates,
+                               double tstep, double time, int32_t istp, int32_t flags) {
+  int rc = check_ptrs(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, rates, tstep, flags);
+  if (rc) return rc;
+  const DevLayout &L = c->hostL;
+  const bool inplace = flags & SALSA_STEP_INPLACE;
+  const size_t n3 = (size_t)c->n3;
+  size_t nf = 7;                                       // press temp rsl rsi dn rv_p rv_t
+  size_t nmass[4], nnum[4];
+  for (int k = 0; k < 4; ++k) {
+    nnum[k] = k < L.ncat ? L.nb[k] : 0;
+    nmass[k] = k < L.ncat ? (size_t)L.nb[k] * (k < 3 ? L.nwet : L.nspec) : 0;
+    nf += 2 * (nnum[k] + nmass[k]);
+  }
+  const size_t need = nf * n3 * sizeof(double);
+  if (need > c->stage_bytes) {
+    hipFree(c->stage);
+    c->stage = nullptr;
+    c->stage_bytes = 0;
+    if (hipMalloc(&c->stage, need) != hipSuccess) { set_error("staging allocation failed"); return SALSA_ERR_NOMEM; }
+    c->stage_bytes = need;
+  }
+  hipStream_t s = c->stream;
+  double *p = c->stage;
+  auto put = [&](const double *h, size_t n) -> double * {
+    double *d = p;
+    p += n;
+    if (h) { if (hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) return nullptr; }
+    return d;
+  };
+  double *dpress = put(press, n3), *dtemp = put(temp, n3), *drsl = put(rsl, n3), *drsi = put(rsi, n3);
+  double *ddn = put(dn, n3), *drvp = put(rv_p, n3), *drvt = put(inplace ? nullptr : rv_t, n3);
+  const double *dnp[4] = {0, 0, 0, 0}, *dmp[4] = {0, 0, 0, 0};
+  double *dnt[4] = {0, 0, 0, 0}, *dmt[4] = {0, 0, 0, 0};
+  for (int k = 0; k < L.ncat; ++k) {
+    dnp[k] = put(n_p[k], nnum[k] * n3);
+    dmp[k] = put(m_p[k], nmass[k] * n3);
+    dnt[k] = put(inplace ? nullptr : n_t[k], nnum[k] * n3);
+    dmt[k] = put(inplace ? nullptr : m_t[k], nmass[k] * n3);
+  }
+  if (!dpress || !drvt) { set_error("host to device copy failed"); return SALSA_ERR_HIP; }
+  StepArgs a = make_args(c, dpress, dtemp, drsl, rsi ? drsi : nullptr, ddn, drvp, drvt, dnp, dnt, dmp, dmt,
+                         tstep, flags);
+  rc = enqueue_step(c, a, flags);
+  if (rc) return rc;
+  if (inplace) {
+    HIPCHK(hipMemcpyAsync(const_cast<double *>(rv_p), drvp, n3 * 8, hipMemcpyDeviceToHost, s));
+    for (int k = 0; k < L.ncat; ++k) {
+      HIPCHK(hipMemcpyAsync(const_cast<double *>(n_p[k]), dnp[k], nnum[k] * n3 * 8, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(const_cast<double *>(m_p[k]), dmp[k], nmass[k] * n3 * 8, hipMemcpyDeviceToHost, s));
+    }
+  } else {
+    HIPCHK(hipMemcpyAsync(rv_t, drvt, n3 * 8, hipMemcpyDeviceToHost, s));
+    for (int k = 0; k < L.ncat; ++k) {
+      HIPCHK(hipMemcpyAsync(n_t[k], dnt[k], nnum[k] * n3 * 8, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(m_t[k], dmt[k], nmass[k] * n3 * 8, hipMemcpyDeviceToHost, s));
+    }
+  }
+  (void)time; (void)istp;
+  return salsa_ctx_sync(c);
+}
+
+// Time each lanes-per-cell candidate per cost class on the caller's
+// device-resident inputs (tendencies go to scratch) and keep the fastest.
+extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                              const double *rsi, const double *dn, const double *rv_p,
+                              const double *const n_p[4], const double *const m_p[4], double tstep) {
+  if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  const DevLayout &L = c->hostL;
+  // scratch tendencies
+  size_t nf = 1;
+  for (int k = 0; k < L.ncat; ++k) nf += (size_t)L.nb[k] * (1 + (k < 3 ? L.nwet : L.nspec));
+  double *scr = nullptr;
+  HIPCHK(hipMalloc(&scr, nf * c->n3 * sizeof(double)));
+  double *rvt = scr, *p = scr + c->n3;
+  double *nt[4] = {0, 0, 0, 0}, *mt[4] = {0, 0, 0, 0};
+  for (int k = 0; k < L.ncat; ++k) {
+    nt[k] = p; p += (size_t)L.nb[k] * c->n3;
+    mt[k] = p; p += (size_t)L.nb[k] * (k < 3 ? L.nwet : L.nspec) * c->n3;
+  }
+  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rvt, n_p, nt, m_p, mt, tstep, 0);
+  int rc = enqueue_step(c, a, 0);                        // classification + warm-up
+  if (rc) { hipFree(scr); return rc; }
+  hipStream_t s = c->stream;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  int64_t off = 0;
+  for (int k = 0; k < 3; ++k) {
+    StepArgs b = a;
+    b.cell_list = c->ids_sorted + off;
+    b.ncells = c->cls_count_h[k];
+    off += b.ncells;
+    if (b.ncells == 0) continue;
+    float best = 1e30f;
+    int bestl = c->lanes[k];
+    for (int lanes : {16, 32, 64}) {
+      launch_step(c->cap, lanes, c->params.precision, c->devL, b, s);   // warm
+      hipEventRecord(e0, s);
+      for (int r = 0; r < 3; ++r) launch_step(c->cap, lanes, c->params.precision, c->devL, b, s);
+      hipEventRecord(e1, s);
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; bestl = lanes; }
+    }
+    c->lanes[k] = bestl;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipStreamSynchronize(s);
+  hipFree(scr);
+  return SALSA_OK;
+}

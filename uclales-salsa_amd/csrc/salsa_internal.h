@@ -1,0 +1,108 @@
+// salsa_internal.h -- device tables and launch arguments shared by the host
+// table builder (salsa_layout.cpp), the kernels (salsa_kernels.hip) and the
+// C ABI (salsa_capi.cpp).
+#pragma once
+
+#include <stdint.h>
+
+#include "salsa_amd.h"
+
+namespace salsa {
+
+constexpr int kMaxBin = 64;           // master bins (aerosol|cloud|precip|ice)
+constexpr int kMaxSpec = 9;           // Section%volc(9)
+
+// Physical constants exactly as written in src/src_shared/mo_submctl.f90:253-338.
+constexpr double kAvog = 6.0221e+23, kBoltz = 1.3807e-23, kGrav = 9.81;
+constexpr double kPstand = 1.01325e+5, kRg = 8.314, kPi = 3.1415927;
+constexpr double kPi6 = 0.5235988, kMair = 28.97e-3, kRd = 287.04;
+constexpr double kAlv = 2.5e6, kAls = 2.834e6, kSurfw0 = 0.073, kSurfi0 = 0.105;
+constexpr double kEpsoc = 0.15, kNlim = 1.0, kPrlim = 1.e-6;
+constexpr double kRhowa = 1000., kRhoic = 917., kRhori = 400., kMwa = 18.016e-3;
+constexpr double kRhosu = 1830., kMsu = 98.08e-3, kRhooc = 1400., kMoc = 150.e-3;
+constexpr double kRhoss = 2165., kMss = 58.44e-3;
+constexpr double kEpsR8 = 2.220446049250313080847e-16;   // epsilon(1.0) for real(8)
+constexpr double kMvwa = (18.016e-3 / 6.0221e+23) / 1000.;
+
+// Target flags of the coagulation sweep program.
+enum TargetFlags : uint32_t {
+  TF_SELF_HALF = 1u << 0,        // sink_self += 0.5*K(t,t)*N_t   (accumulateSink multp=0.5)
+  TF_SELF_HALF_EAGG = 1u << 1,   // same, ice-ice with aggregation efficiency
+  TF_PRECIP_SELFCOLL = 1u << 2,  // precipSelfCoag: products moved to bin t+1
+  TF_PF_SELF = 1u << 3,          // accumulatePrecipFormation self collection
+  TF_PF = 1u << 4,               // pf_own / pf_oth masks are live
+  TF_ICE = 1u << 5               // applyCoagIce update (dry, rime, water)
+};
+
+// One target bin of the sequential coagulation sweep; the sweep visits the
+// targets in the reference order ice -> precip -> aerosol -> cloud
+// (mo_salsa_dynamics.f90:166-183), ascending bins within each category.
+// Masks select collector / source bins by master index.
+struct DevTarget {
+  int32_t t;             // master bin index
+  int32_t cat;           // 0 aero, 1 cloud, 2 precip, 3 ice
+  uint32_t flags;
+  int32_t nupd;          // species updated by applyCoag (nwet; ice handled by TF_ICE)
+  uint64_t sink;         // sink += K(t,j) N_j
+  uint64_t sink_eagg;    // sink += Eagg(t,j) K(t,j) N_j   (ice-ice)
+  uint64_t src;          // src[s] += K(j,t) V_j[s],  s < nupd
+  uint64_t src_eagg;     // src[s] += Eagg K V_j[s],  all species (ice-ice)
+  uint64_t src_pc_wa;    // phase change into ice water slot (aerosol collected by ice)
+  uint64_t src_pc_ri;    // phase change into rime slot (cloud/precip collected by ice)
+  uint64_t pf_own, fix_own;   // precip formation from the own cloud series, fix_coag range
+  uint64_t pf_oth, fix_oth;   // ... from the other series
+};
+
+struct DevLayout {
+  int32_t nt;                         // bins handled (ice dropped at level 4)
+  int32_t nb[4], off[4];              // bins and master offsets per category
+  int32_t in1a, fn1a, in2a, fn2a, in2b, fn2b;
+  int32_t nspec, nwet, ndry, iwa, irim;
+  int32_t ncat;                       // 3 at level 4, 4 at level 5
+  double rholiq[kMaxSpec], rhoice[kMaxSpec], mm[kMaxSpec], diss[kMaxSpec];
+  int32_t dry_id[kMaxSpec];           // species id (0 SO4 .. 6 NH) per dry slot
+  int32_t phase[kMaxBin];
+  int32_t cat_of[kMaxBin], loc_of[kMaxBin];
+  double dmid[kMaxBin], nlim[kMaxBin], dlim[kMaxBin];
+  double precpbins[kMaxBin];
+  // pairs: canonical orientation (b1 is "pp1") and the self-offset predicate
+  int32_t npair;
+  int16_t pb1[kMaxBin * (kMaxBin + 1) / 2], pb2[kMaxBin * (kMaxBin + 1) / 2];
+  uint8_t poff[kMaxBin * (kMaxBin + 1) / 2];
+  int16_t pidx[kMaxBin][kMaxBin];     // pair id of (i,j), -1 if that block is off
+  // sweep program
+  int32_t ntarget;
+  DevTarget tgt[kMaxBin];
+  // switches and constants
+  int32_t level, lscoag, lscnd, lscndh2oae, lscndh2ocl, lscndh2oic;
+  int32_t any_ice_coag;               // any of lscgii/lscgia/lscgic/lscgip
+  double eddy_dis, Eiagg_max, Eiagg_min;
+  double ice_alpha, ice_beta, ice_gamma, ice_sigma;
+  double charge_factor;               // 1 + chargeCollEnh (lcharge) or 1
+};
+
+// Per-launch arguments (passed by value).
+struct StepArgs {
+  int32_t nzp, nxp, nyp, hz, hxy, nz, nx, ny;
+  int64_t n3;                         // nzp*nxp*nyp
+  int64_t ncells;                     // cells in this launch
+  const int32_t *cell_list;           // interior ids (k fastest), or null = identity
+  const double *press, *temp, *rsl, *rsi, *dn, *rv_p;
+  double *rv_t;
+  const double *n_p[4];
+  double *n_t[4];
+  const double *m_p[4];
+  double *m_t[4];
+  double *n_o[4];                     // in-place outputs (== n_p cast) or null
+  double *m_o[4];
+  double *rv_o;
+  double dt;
+  int32_t nsub;                       // condensation substeps for dt
+  int32_t inplace;
+  int32_t lsfreeRH;
+  double rhlim;
+  uint8_t *cell_flags;                // one byte per interior cell, or null
+  int32_t *fault_any;                 // OR of all cell flags
+};
+
+}  // namespace salsa

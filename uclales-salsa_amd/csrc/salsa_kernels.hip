@@ -1,0 +1,941 @@
+// salsa_kernels.hip -- CDNA4 (gfx950) kernels for the SALSA coagulation +
+// condensation step.
+//
+// One fused kernel handles a group of 64/L grid cells per 64-lane wavefront;
+// the L lanes of a cell cooperate:
+//   1. pack     : LES fields -> SALSA units in LDS        (mo_salsa_driver.f90:230-279)
+//   2. bin props: wet diameter, density and the per-particle terms of coagc
+//                 for every bin and its "self-offset" twin  (classSection.f90:91-195,
+//                 mo_salsa_coagulation_kernels.f90:233-262, mo_particle_external_properties.f90:43-158)
+//   3. pairs    : collision kernel K for every active bin pair -> LDS
+//                 (mo_salsa_coagulation_kernels.f90:94-180, 264-420)
+//   4. sweep    : the sequential semi-implicit target sweep, each target's
+//                 sink/source sums reduced across the cell's lanes
+//                 (mo_salsa_coagulation_processes.f90:24-1611)
+//   5. condense : equilibration + analytic predictor-corrector substeps with
+//                 lane-owned bins in registers (mo_salsa_dynamics.f90:691-1157,
+//                 mo_salsa_properties.f90:73-380)
+//   6. unpack   : tendencies (+=) or in-place state       (mo_salsa_driver.f90:310-388)
+//
+// KT is the arithmetic type of step 3 (double for the FP64 path, float for the
+// FP32-mixed path); everything else -- state, sums, predicates, equilibration
+// and the vapour budget -- is FP64 in both paths.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "salsa_internal.h"
+
+namespace salsa {
+
+// ---------------------------------------------------------------------------
+// precision helpers
+// ---------------------------------------------------------------------------
+template <class T> struct Mth;
+template <> struct Mth<double> {
+  static __device__ __forceinline__ double sqrt_(double x) { return ::sqrt(x); }
+  static __device__ __forceinline__ double log_(double x) { return ::log(x); }
+  static __device__ __forceinline__ double cbrt_(double x) { return ::cbrt(x); }
+  static __device__ __forceinline__ double div_(double a, double b) { return a / b; }
+};
+template <> struct Mth<float> {
+  // v_sqrt_f32 / v_log_f32 / v_exp_f32 / v_rcp_f32: ~1 ulp, far inside the
+  // 1e-5 output tolerance of the mixed path.
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float log_(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
+  static __device__ __forceinline__ float cbrt_(float x) {
+    return x > 0.f ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.f / 3.f)) : 0.f;
+  }
+  static __device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+};
+
+// Sum / OR over the L lanes of one cell (all lanes receive the result).
+template <int L>
+__device__ __forceinline__ double cell_sum(double x) {
+#pragma unroll
+  for (int m = L / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, L);
+  return x;
+}
+template <int L>
+__device__ __forceinline__ uint32_t cell_or(uint32_t x) {
+#pragma unroll
+  for (int m = L / 2; m > 0; m >>= 1) x |= (uint32_t)__shfl_xor((int)x, m, L);
+  return x;
+}
+
+__device__ __forceinline__ double cubed(double x) { return x * x * x; }
+
+// ---------------------------------------------------------------------------
+// particle terms
+// ---------------------------------------------------------------------------
+
+// terminal_vel for liquid particles (Beard 1976), mo_particle_external_properties.f90:61-90
+__device__ double terminal_vel_liquid(double diam, double rhop, double rhoa, double visc, double beta) {
+  if (diam <= 19.e-6) return (diam * diam) * (rhop - rhoa) * kGrav * beta / (18. * visc);
+  if (diam < 1.07e-3) {
+    double nda = 4. / 3. * cubed(diam) * kGrav * (rhop - rhoa) * rhoa / (visc * visc);
+    double X = ::log(nda);
+    double X2 = X * X;
+    double Y = -0.318657E1 + 0.992696 * X - 0.153193E-2 * X2 - 0.987059E-3 * X2 * X
+               - 0.578878E-3 * X2 * X2 + 0.855176E-4 * X2 * X2 * X - 0.327815E-5 * X2 * X2 * X2;
+    return visc / rhoa / diam * (beta * ::exp(Y));
+  }
+  double np_ = (kSurfw0 * kSurfw0 * kSurfw0) * (rhoa * rhoa) / ((visc * visc) * (visc * visc) * (rhop - rhoa) * kGrav);
+  double bo = 4. / 3. * (rhop - rhoa) * kGrav / kSurfw0 * (diam * diam);
+  double np6 = ::pow(np_, 1. / 6.);
+  double X3 = ::log(bo * np6);
+  double X2 = X3 * X3;
+  double Y3 = -0.500015E1 + 0.523778E1 * X3 - 0.204914E1 * X2 + 0.475294 * X2 * X3
+              - 0.542819E-1 * X2 * X2 + 0.238449E-2 * X2 * X2 * X3;
+  return fmin(visc / rhoa / diam * (np6 * ::exp(Y3)), 9.2);
+}
+
+struct Shape { double alpha, beta, gamma, sigma; };
+
+// getShapeCoefficients, mo_ice_shape.f90:145-191
+__device__ Shape shape_coeffs(const DevLayout &D, double mpri, double mrim, double numc) {
+  Shape s{kPi6 * kRhoic, 3., kPi / 4., 2.};
+  if (numc < 1.e-6) return s;
+  double mtot = (mpri + mrim) / numc;
+  double fr = fmax(fmin(mrim / (mtot * numc), 0.9999), 0.0001);
+  double rho_b = fr * kRhori + (1. - fr) * kRhoic;
+  double dth = ::pow(kPi6 * kRhoic / D.ice_alpha, 1. / (D.ice_beta - 3.));
+  double dgr = ::pow(D.ice_alpha / (kPi6 * rho_b), 1. / (3. - D.ice_beta));
+  double dcr = ::pow((1. / (1. - fr)) * D.ice_alpha / (kPi6 * rho_b), 1. / (3. - D.ice_beta));
+  double mth = kRhoic * kPi6 * cubed(dth), mgr = rho_b * kPi6 * cubed(dgr), mcr = rho_b * kPi6 * cubed(dcr);
+  if (mtot >= mth && mtot < mgr) {
+    s = Shape{D.ice_alpha, D.ice_beta, D.ice_gamma, D.ice_sigma};
+  } else if (mtot > mcr) {
+    double h = 1. / (1. - fr);
+    s = Shape{D.ice_alpha * h, D.ice_beta, D.ice_gamma * (1. - fr) + kPi / 4. * fr,
+              D.ice_sigma * (1. - fr) + 2. * fr};
+  }
+  return s;
+}
+
+// terminal_vel for ice (Khvorostyanov & Curry 2002), :91-98 with kcVt..kcbre :103-158
+__device__ double terminal_vel_ice(double dsph, double rhop, double rhoa, double visc, Shape sh, double dnsp) {
+  const double c1 = 0.0902, delta0 = 9.06;
+  double vb = kPi6 * cubed(dsph);
+  double ap = sh.gamma * ::pow(dnsp, sh.sigma);
+  double X = (2. * vb * (rhop - rhoa) * kGrav * dnsp * dnsp) / (ap * rhoa * visc * visc);
+  double k1213 = ::sqrt(1. + c1 * ::sqrt(X));
+  double bre = 0.5 * c1 * ::sqrt(X) / (k1213 - 1.) / k1213;
+  double are = 0.25 * delta0 * delta0 * (k1213 - 1.) * (k1213 - 1.) / ::pow(X, bre);
+  double av = are * ::pow(visc, 1. - 2. * bre) * ::pow((2. * sh.alpha * kGrav) / (rhoa * sh.gamma), bre);
+  double bv = bre * (sh.beta - sh.sigma + 2.) - 1.;
+  return av * ::pow(dnsp, bv);
+}
+
+// Per-cell constants of coagc (mo_salsa_coagulation_kernels.f90:264-266, 323-325, 389-391)
+struct CellC {
+  double T, visc, mfp, zrhoa, vkin, cshear, cinert;
+};
+__device__ CellC cell_consts(const DevLayout &D, double T, double P) {
+  CellC c;
+  c.T = T;
+  c.visc = (7.44523e-3 * ::sqrt(T * T * T)) / (5093. * (T + 110.4));
+  c.mfp = (1.656e-10 * T + 1.828e-8) * kPstand / P;
+  c.zrhoa = P / (kRd * T);
+  c.vkin = c.visc / c.zrhoa;
+  c.cshear = ::sqrt(8. * kPi * D.eddy_dis / (15. * c.vkin));
+  c.cinert = kPi * ::pow(D.eddy_dis, 0.75) / (kGrav * ::sqrt(::sqrt(c.vkin)));
+  return c;
+}
+
+// Particle terms of one colliding particle: diffusion coefficient, mean
+// thermal speed, flux-matching omega and terminal velocity (coagc :280-286, 327-339).
+struct PTerms { double D, c, w, v; };
+__device__ PTerms particle_terms(const CellC &cc, double diam, double mass, double rhop, int phase,
+                                 double dsph, const Shape &sh) {
+  PTerms r;
+  double knud = 2. * cc.mfp / diam;
+  double beta = 1. + knud * (1.142 + 0.558 * ::exp(-0.999 / knud));
+  r.D = beta * kBoltz * cc.T / (3. * kPi * cc.visc * diam);
+  r.c = ::sqrt((8. * kBoltz * cc.T) / (kPi * mass));
+  r.w = 8. * r.D / (kPi * r.c);
+  r.v = phase < 4 ? terminal_vel_liquid(diam, rhop, cc.zrhoa, cc.visc, beta)
+                  : terminal_vel_ice(dsph, rhop, cc.zrhoa, cc.visc, sh, diam);
+  return r;
+}
+
+// Collision kernel of one pair from the two particles' terms (coagc :288-407).
+// a_lrg: particle A is the larger (diam(1) >= diam(2), evaluated in FP64).
+template <class KT>
+__device__ __forceinline__ KT pair_coef(KT dA, KT DA, KT cA, KT wA, KT vA, KT dB, KT DB, KT cB, KT wB,
+                                        KT vB, bool a_lrg, KT vkin, KT cshear, KT cinert) {
+  using M = Mth<KT>;
+  const KT pi = KT(kPi);
+  KT md = KT(0.5) * (dA + dB);
+  KT flux1 = KT(4.) * pi * md * (DA + DB);
+  KT flux2 = pi * M::sqrt_(cA * cA + cB * cB) * (md * md);
+  KT tA, tB;
+  if constexpr (sizeof(KT) == 8) {
+    // reference form (:300-310)
+    KT a2 = md * md + wA * wA, b2 = md * md + wB * wB;
+    tA = ((md + wA) * (md + wA) * (md + wA) - a2 * M::sqrt_(a2)) / (KT(3.) * md * wA) - md;
+    tB = ((md + wB) * (md + wB) * (md + wB) - b2 * M::sqrt_(b2)) / (KT(3.) * md * wB) - md;
+  } else {
+    // the same quantity without the two cancellations of the reference form:
+    // tva = w (m + 2s + 4w - m w/(m+s)) / (3 (m + w + s)),  s = sqrt(m^2 + w^2)
+    KT sA = M::sqrt_(md * md + wA * wA), sB = M::sqrt_(md * md + wB * wB);
+    tA = M::div_(wA * (md + KT(2.) * sA + KT(4.) * wA - M::div_(md * wA, md + sA)), KT(3.) * (md + wA + sA));
+    tB = M::div_(wB * (md + KT(2.) * sB + KT(4.) * wB - M::div_(md * wB, md + sB)), KT(3.) * (md + wB + sB));
+  }
+  KT fmd = M::sqrt_(tA * tA + tB * tB);
+  KT zbrown = M::div_(flux1, M::div_(md, md + fmd) + M::div_(flux1, flux2));
+  KT d_l = a_lrg ? dA : dB, v_l = a_lrg ? vA : vB, v_s = a_lrg ? vB : vA, D_s = a_lrg ? DB : DA;
+  KT reyn = M::div_(d_l * v_l, vkin);
+  KT schm = M::div_(vkin, D_s);
+  KT dv = vA > vB ? vA - vB : vB - vA;
+  KT stok = M::div_(KT(2.) * v_s * dv, d_l * KT(kGrav));
+  KT zbrconv = KT(0.);
+  if (reyn <= KT(1.)) zbrconv = KT(0.45) * zbrown * M::cbrt_(reyn * schm);
+  else if (reyn > KT(1.)) zbrconv = KT(0.45) * zbrown * M::sqrt_(reyn) * M::cbrt_(schm);
+  KT ztshear = cshear * (md * md * md);
+  KT zturb = cinert * (md * md) * dv;
+  KT s1 = stok + KT(0.5);
+  KT zea = M::div_(stok * stok, s1 * s1);
+  KT zev = KT(0.);
+  if (stok > KT(1.214)) {
+    zev = KT(0.75) * M::div_(M::log_(KT(2.) * stok), stok - KT(1.214));
+    zev = M::div_(KT(1.), (KT(1.) + zev) * (KT(1.) + zev));
+  }
+  KT zecoll = M::div_(KT(60.) * zev + zea * reyn, KT(60.) + reyn);
+  KT zgrav = zecoll * pi * (md * md) * dv;
+  KT hyp;
+  if constexpr (sizeof(KT) == 8) {
+    hyp = M::sqrt_(zgrav * zgrav + ztshear * ztshear + zturb * zturb);
+  } else {
+    // scaled so that the squares stay clear of the FP32 denormal range
+    KT m = fmaxf(zgrav, fmaxf(ztshear, zturb));
+    KT r = m > 0.f ? __builtin_amdgcn_rcpf(m) : 0.f;
+    KT a = zgrav * r, b = ztshear * r, c = zturb * r;
+    hyp = m * M::sqrt_(a * a + b * b + c * c);
+  }
+  return zbrown + zbrconv + hyp;
+}
+
+// ---------------------------------------------------------------------------
+// shared memory of one wavefront
+// ---------------------------------------------------------------------------
+template <int L, class KT, int MB, int MP, int NS>
+struct Smem {
+  static constexpr int CPW = 64 / L;
+  double N[CPW][MB];
+  double V[CPW][MB][NS];
+  double dw[CPW][MB];            // wet diameter at coagulation start
+  double di[CPW][2][MB];         // colliding diameter: bin / self-offset twin
+  KT pr[CPW][2][4][MB];          // D, c, omega, v: bin / twin
+  KT K[CPW][MP];
+  int8_t pf[CPW][MP];            // precip bin formed by a cloud-cloud pair (-1: none)
+};
+
+// rime fraction, classSection.f90:202-224
+template <int NS>
+__device__ __forceinline__ double rime_frac(const DevLayout &D, double n, const double *v, int b) {
+  if (D.phase[b] == 4 && v[D.iwa] > 1.e-23 && v[D.irim] > 1.e-23 && n > D.nlim[b])
+    return v[D.irim] * kRhori / (v[D.iwa] * kRhoic + v[D.irim] * kRhori);
+  return 0.;
+}
+
+// acth2o, mo_salsa_dynamics.f90:1127-1157, with the dry-solute term precomputed
+template <bool FAST>
+__device__ __forceinline__ double act_h2o(double znw, double zns) {
+  double den = fmax(kEpsR8, znw + zns);
+  double q;
+  if constexpr (FAST) {
+    double r = __builtin_amdgcn_rcp(den);      // ~2^-23, refined twice
+    r = r * (2. - den * r);
+    r = r * (2. - den * r);
+    q = znw * r;
+  } else {
+    q = znw / den;
+  }
+  return fmax(0.1, q);
+}
+
+// ---------------------------------------------------------------------------
+// the step kernel
+// ---------------------------------------------------------------------------
+template <int L, class KT, int MB, int MP, int NS>
+__global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a) {
+  constexpr int CPW = 64 / L, NQ = (MB + L - 1) / L;
+  constexpr bool FASTDIV = sizeof(KT) == 4;
+  __shared__ Smem<L, KT, MB, MP, NS> sm;
+  const DevLayout &D = *Dp;
+  const int lane = threadIdx.x, slot = lane / L, sub = lane % L;
+  const int64_t cid = (int64_t)blockIdx.x * CPW + slot;
+  const bool live = cid < a.ncells;
+  const int nt = D.nt;
+  const double dt = a.dt;
+
+  int64_t g = 0;
+  int32_t icell = 0;
+  if (live) {
+    icell = a.cell_list ? a.cell_list[cid] : (int32_t)cid;
+    int k = icell % a.nz, r = icell / a.nz;
+    int i = r % a.nx, j = r / a.nx;
+    g = (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
+  }
+  double T = 280., P = 1.e5, rs = 0.01, rsi = 0.01, dn = 1., rv = 0.;
+  if (live) {
+    T = a.temp[g]; P = a.press[g]; rs = a.rsl[g]; dn = a.dn[g]; rv = a.rv_p[g];
+    rsi = D.level == 5 ? a.rsi[g] : 0.;
+  }
+  if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);     // mo_salsa_driver.f90:193-197
+  const double rv_old = rv;
+  uint32_t flags = 0;
+
+  double *sN = sm.N[slot];
+  double(*sV)[NS] = sm.V[slot];
+
+  // ---- 1. pack (mo_salsa_driver.f90:236-279) ----
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int b = sub + q * L;
+    if (b < nt) {
+      double n = 0.;
+      double v[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) v[s] = 0.;
+      if (live) {
+        const int c = D.cat_of[b], loc = D.loc_of[b], nbc = D.nb[c];
+        n = a.n_p[c][g + a.n3 * loc] * dn;
+        const double *m = a.m_p[c];
+        if (c < 3) {
+#pragma unroll
+          for (int s = 0; s < NS - 1; ++s)
+            if (s < D.nwet) v[s] = m[g + a.n3 * (s * nbc + loc)] * dn / D.rholiq[s];
+        } else {
+#pragma unroll
+          for (int s = 0; s < NS - 1; ++s)
+            if (s < D.nwet) v[s] = m[g + a.n3 * (s * nbc + loc)] * dn / D.rhoice[s];
+          v[D.irim] = m[g + a.n3 * (D.irim * nbc + loc)] * dn / kRhori;
+        }
+      }
+      sN[b] = n;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) sV[b][s] = v[s];
+    }
+  }
+  __syncthreads();
+
+  // ---- 2./3./4. coagulation (mo_salsa_dynamics.f90:92-202) ----
+  if (D.lscoag) {
+    const CellC cc = cell_consts(D, T, P);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < nt) {
+        const double n = sN[b];
+        double vs = 0.;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) vs += sV[b][s];
+        // updateDiameter(limit, "all"): dnsp ends equal to the limited dwet (classSection.f90:147-156)
+        double dw = n > D.nlim[b] ? ::cbrt(vs / n / kPi6) : D.dmid[b];
+        dw = fmin(dw, D.dlim[b]);
+        const int ph = D.phase[b];
+        double rho = kRhowa;                             // updateRhomean, classSection.f90:169-195
+        Shape sh{0., 0., 0., 0.};
+        if (ph == 4) {
+          if (n > D.nlim[b]) {
+            double mp = kRhoic * sV[b][D.iwa], mr = kRhori * sV[b][D.irim];
+            rho = fmin((mp * kRhoic + mr * kRhori) / fmax(mp + mr, 7.e-25), kRhoic);
+          } else {
+            rho = kRhoic;
+          }
+          double sdry = 0.;
+          for (int s = 0; s < D.nspec - 1; ++s) sdry += sV[b][s] * kRhoic;
+          sh = shape_coeffs(D, sdry, sV[b][D.nspec - 1] * kRhori, n);
+        }
+        const double mass = rho * kPi6 * cubed(dw);
+        const PTerms p0 = particle_terms(cc, dw, mass, rho, ph, dw, sh);
+        // self-offset twin: diameter 1.1x, mass 1.33x (kernels.f90:273-276)
+        const PTerms p1 = particle_terms(cc, 1.1 * dw, 1.33 * mass, rho, ph, dw, sh);
+        sm.dw[slot][b] = dw;
+        sm.di[slot][0][b] = dw;
+        sm.di[slot][1][b] = 1.1 * dw;
+        sm.pr[slot][0][0][b] = (KT)p0.D; sm.pr[slot][0][1][b] = (KT)p0.c;
+        sm.pr[slot][0][2][b] = (KT)p0.w; sm.pr[slot][0][3][b] = (KT)p0.v;
+        sm.pr[slot][1][0][b] = (KT)p1.D; sm.pr[slot][1][1][b] = (KT)p1.c;
+        sm.pr[slot][1][2][b] = (KT)p1.w; sm.pr[slot][1][3][b] = (KT)p1.v;
+      }
+    }
+    __syncthreads();
+
+    // ---- 3. pairs (buildKernelSelf / buildKernel, kernels.f90:94-180) ----
+    {
+      const KT vkin = (KT)cc.vkin, cshear = (KT)cc.cshear, cinert = (KT)cc.cinert;
+      const KT cfac = (KT)D.charge_factor;
+      const int np = D.npair;
+      const int R = D.off[2];
+      for (int p = sub; p < np; p += L) {
+        const int b1 = D.pb1[p], b2 = D.pb2[p];
+        const int tw = D.poff[p];
+        KT k = KT(0.);
+        int8_t pfb = -1;
+        if (live && !(sN[b1] < D.nlim[b1]) && !(sN[b2] < D.nlim[b2])) {
+          // particle 1 is the twin of particle 2 when the bins' dmid agree within 1%
+          const int ba = tw ? b2 : b1;
+          const double dA = sm.di[slot][tw][ba], dB = sm.di[slot][0][b2];
+          k = pair_coef<KT>((KT)dA, sm.pr[slot][tw][0][ba], sm.pr[slot][tw][1][ba], sm.pr[slot][tw][2][ba],
+                            sm.pr[slot][tw][3][ba], (KT)dB, sm.pr[slot][0][0][b2], sm.pr[slot][0][1][b2],
+                            sm.pr[slot][0][2][b2], sm.pr[slot][0][3][b2], dA >= dB, vkin, cshear, cinert);
+          k = k * cfac;                                 // charge enhancement (:411-416)
+          if (D.cat_of[b1] == 1 && D.cat_of[b2] == 1) {
+            // precip bin reached by coalescence (accumulatePrecipFormation, processes.f90:1461-1467)
+            double dnew = ::cbrt(cubed(sm.dw[slot][b1]) + cubed(sm.dw[slot][b2]));
+            int cnt = 0;
+            for (int qq = 0; qq < D.nb[2]; ++qq) cnt += dnew > D.precpbins[qq];
+            pfb = dnew > D.precpbins[0] ? (int8_t)(cnt - 1) : (int8_t)-1;
+          }
+        }
+        sm.K[slot][p] = k;
+        sm.pf[slot][p] = pfb;
+        (void)R;
+      }
+    }
+    __syncthreads();
+
+    // ---- 4. sweep (coag_ice, coag_precp, coag_aero, coag_cloud) ----
+    const KT *sK = sm.K[slot];
+    const int8_t *sPF = sm.pf[slot];
+    const int nspec = D.nspec, ndry = D.ndry, iwa = D.iwa, irim = D.irim;
+    const int Roff = D.off[2];
+    for (int it = 0; it < D.ntarget; ++it) {
+      const DevTarget &TG = D.tgt[it];
+      const int t = TG.t;
+      const uint32_t fl = TG.flags;
+      const double Nt = sN[t];
+      double Vt[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) Vt[s] = sV[t][s];
+      bool act = live && !(Nt < D.nlim[t]);
+      if (fl & TF_ICE) act = act && T < 273.15;          // mo_salsa_dynamics.f90:166
+      const double rft = (fl & TF_ICE) ? rime_frac<NS>(D, Nt, Vt, t) : 0.;
+      const double emax = D.Eiagg_max, emin = D.Eiagg_min;
+
+      double fix_own = 1., fix_oth = 1.;
+      if (fl & TF_PF) {
+        double so = 0., sx = 0.;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int j = sub + q * L;
+          if (j < nt) {
+            const uint64_t bit = 1ull << j;
+            if ((TG.fix_own | TG.fix_oth) & bit) {
+              const int pid = D.pidx[t][j];
+              const double kj = pid >= 0 ? (double)sK[pid] : 0.;
+              if (TG.fix_own & bit) so += kj * sN[j];
+              if (TG.fix_oth & bit) sx += kj * sN[j];
+            }
+          }
+        }
+        so = cell_sum<L>(so);
+        sx = cell_sum<L>(sx);
+        fix_own = fmax(1. - dt * so, 0.1);
+        fix_oth = fmax(1. - dt * sx, 0.1);
+      }
+
+      double sink = 0., src[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) src[s] = 0.;
+      uint32_t pmask = 0;
+      const uint64_t anymask = TG.sink | TG.sink_eagg | TG.src | TG.src_eagg | TG.src_pc_wa | TG.src_pc_ri |
+                               TG.pf_own | TG.pf_oth;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int j = sub + q * L;
+        if (j < nt && ((anymask >> j) & 1ull)) {
+          const uint64_t bit = 1ull << j;
+          const int pid = D.pidx[t][j];
+          const double kj = pid >= 0 ? (double)sK[pid] : 0.;
+          const double Nj = sN[j];
+          const double *Vj = sV[j];
+          if (TG.sink & bit) sink += kj * Nj;
+          if ((TG.sink_eagg | TG.src_eagg) & bit) {
+            const double e = emax - fmax(rime_frac<NS>(D, Nj, Vj, j), rft) * (emax - emin);
+            if (TG.sink_eagg & bit) sink += e * kj * Nj;
+            if (TG.src_eagg & bit) {
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < nspec) src[s] += e * kj * Vj[s];
+            }
+          }
+          if (TG.src & bit) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+              if (s < TG.nupd) src[s] += kj * Vj[s];
+          }
+          if ((TG.src_pc_wa | TG.src_pc_ri) & bit) {   // accumulateSourcePhaseChange (:1236-1266)
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+              if (s < ndry) src[s] += kj * Vj[s];
+            if (TG.src_pc_wa & bit) src[iwa] += kj * Vj[iwa] * kRhowa / kRhoic;
+            else src[irim] += kj * Vj[iwa] * kRhowa / kRhori;
+          }
+          if ((TG.pf_own | TG.pf_oth) & bit) {         // accumulatePrecipFormation (:1458-1576)
+            const int pb = sPF[pid];
+            if (pb >= 0) {
+              sink += kj * Nj;
+              pmask |= 1u << pb;
+            } else {
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < TG.nupd) src[s] += kj * Vj[s];
+            }
+          }
+        }
+      }
+      sink = cell_sum<L>(sink);
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (s < nspec) src[s] = cell_sum<L>(src[s]);
+
+      // self terms (computed identically by every lane of the cell)
+      double sink_self = 0.;
+      double volsink[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) volsink[s] = 0.;
+      int pself = -1;
+      double kself = 0., fixs = 1.;
+      if (fl & (TF_SELF_HALF | TF_SELF_HALF_EAGG | TF_PF_SELF | TF_PRECIP_SELFCOLL)) {
+        const int pid = D.pidx[t][t];
+        kself = pid >= 0 ? (double)sK[pid] : 0.;
+        if (fl & TF_SELF_HALF) sink_self = 0.5 * kself * Nt;
+        if (fl & TF_SELF_HALF_EAGG) sink_self = (emax - rft * (emax - emin)) * 0.5 * kself * Nt;
+        if (fl & TF_PF_SELF) {
+          fixs = fmax(1. - dt * (0.5 * kself * (Nt * Nt) / Nt), 0.1);
+          pself = pid >= 0 ? sPF[pid] : -1;
+          sink_self = 0.5 * kself * Nt;
+          if (pself >= 0) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+              if (s < TG.nupd) volsink[s] = Vt[s] * Nt * kself * fixs;
+          }
+        }
+      }
+
+      // applyCoag / applyCoagIce / applyCoagPrecp (:539-645)
+      double Vn[NS];
+      const int nu = (fl & TF_ICE) ? nspec : TG.nupd;
+      const double den = 1. + dt * sink;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) Vn[s] = s < nu ? (Vt[s] + dt * src[s] * Nt) / den : Vt[s];
+      double Nn = Nt / (1. + dt * (sink + sink_self));
+      if (fl & TF_PRECIP_SELFCOLL) {                  // precipSelfCoag (:1588-1611)
+        const double nslf = 0.5 * kself * Nt * Nt;
+        if (act && sub == 0) {
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+            if (s < TG.nupd) {
+              const double vslf = Vt[s] * Nt * kself;
+              sV[t + 1][s] = sV[t + 1][s] + dt * vslf;
+              Vn[s] = Vn[s] - dt * vslf;
+            }
+          sN[t + 1] = sN[t + 1] + dt * nslf;
+        }
+        Nn = Nn - dt * nslf;
+      }
+      if (fl & TF_PF_SELF) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) Vn[s] = Vn[s] - dt * volsink[s];
+      }
+      if (act && sub == 0) {
+        sN[t] = Nn;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) sV[t][s] = Vn[s];
+      }
+
+      // transfer of coalesced cloud drops into precipitation bins (applyCoagPrecipFormation :649-703)
+      if (fl & (TF_PF | TF_PF_SELF)) {
+        uint32_t pm = cell_or<L>(pmask);
+        if (pself >= 0) pm |= 1u << pself;
+        if (!act) pm = 0;
+        while (pm) {
+          const int pb = __builtin_ctz(pm);
+          pm &= pm - 1;
+          double vp[NS], npp = 0.;
+#pragma unroll
+          for (int s = 0; s < NS; ++s) vp[s] = 0.;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const int j = sub + q * L;
+            if (j < nt && (((TG.pf_own | TG.pf_oth) >> j) & 1ull)) {
+              const int pid = D.pidx[t][j];
+              if (sPF[pid] == pb) {
+                const double kj = (double)sK[pid];
+                const double fx = ((TG.pf_own >> j) & 1ull) ? fix_own : fix_oth;
+                const double Nj = sN[j];
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                  if (s < TG.nupd) vp[s] += (sV[j][s] * Nt + Vt[s] * Nj) * kj * fx;
+                npp += kj * Nj * Nt * fx;
+              }
+            }
+          }
+          npp = cell_sum<L>(npp);
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+            if (s < TG.nupd) vp[s] = cell_sum<L>(vp[s]);
+          if (pb == pself) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) vp[s] += volsink[s];
+            npp += 0.5 * kself * Nt * Nt * fixs;
+          }
+          if (sub == 0) {
+            const int r = Roff + pb;
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+              if (s < TG.nupd) sV[r][s] = sV[r][s] + dt * vp[s];
+            sN[r] = sN[r] + dt * npp;
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    // clamp (mo_salsa_dynamics.f90:189-198)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < nt) {
+        if (sN[b] < -1.e-6) flags |= SALSA_CELL_NEG_NUMC;
+        sN[b] = fmax(0., sN[b]);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) sV[b][s] = fmax(0., sV[b][s]);
+      }
+    }
+  } else {
+    // without coagulation the wet diameters are refreshed from the state
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < nt) {
+        double vs = 0.;
+        for (int s = 0; s < NS; ++s) vs += sV[b][s];
+        double dw = sN[b] > D.nlim[b] ? ::cbrt(vs / sN[b] / kPi6) : D.dmid[b];
+        sm.dw[slot][b] = fmin(dw, D.dlim[b]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 5. condensation (condensation/gpparth2o, mo_salsa_dynamics.f90:276-342, 691-1125) ----
+  if (D.lscnd && (D.lscndh2oae || D.lscndh2ocl || D.lscndh2oic)) {
+    const int iwa = D.iwa, irim = D.irim, ndry = D.ndry;
+    const int A0 = D.off[0], C0 = D.off[1], I0 = D.off[3];
+    const double zrh = rv / rs;
+    double w1 = 0.;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < D.nb[0]) w1 += sV[b][iwa];
+    }
+    w1 = cell_sum<L>(w1) * kRhowa;
+
+    // equilibration regime 1 (1a bins, sequential: the Kelvin factor carries
+    // over between bins), mo_salsa_properties.f90:165-269
+    int isu = -1, ioc = -1, iss = -1;
+    for (int s = 0; s < ndry; ++s) {
+      if (D.dry_id[s] == 0) isu = s;
+      if (D.dry_id[s] == 1) ioc = s;
+      if (D.dry_id[s] == 4) iss = s;
+    }
+    if (sub == 0 && live) {
+      double zke = 1.001;
+      for (int kk = D.in1a; kk <= D.fn1a; ++kk) {
+        const int b = A0 + kk;
+        const double n = sN[b];
+        if (n > kNlim) {
+          const double v1 = isu >= 0 ? sV[b][isu] : 0., v2 = ioc >= 0 ? sV[b][ioc] : 0.;
+          const double zcore = v1 / n + v2 / n;
+          double zdwet = sm.dw[slot][b], zdold = 1., zlwc = 0.;
+          const double zr = fmin(fmax(zrh, 0.05), 0.98);
+          int count = 0;
+          while (fabs(zdwet / zdold - 1.) > 1.e-2) {
+            zdold = fmax(zdwet, 1.e-20);
+            const double aw = zr / zke;
+            const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * (aw * aw)
+                               - 3.1543839e+2 * (aw * aw * aw) + 6.770824e+1 * ((aw * aw) * (aw * aw));
+            const double bm2 = 1. / (aw * kMwa) - 1. / kMwa;
+            zlwc = (v1 * (kRhosu / kMsu)) / bm1 + kEpsoc * v2 * (kRhooc / kMoc) / bm2;
+            zdwet = ::cbrt(zlwc / n / kRhowa / kPi6 + zcore / kPi6);
+            zke = ::exp(2. * kSurfw0 * kMvwa / (kBoltz * T * zdwet));
+            if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
+          }
+          sV[b][iwa] = zlwc / kRhowa;
+        }
+      }
+    }
+    // regime 2 (2a/2b bins, init call), mo_salsa_properties.f90:275-378
+    if (zrh < 0.98 || !D.lscndh2oae) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int b = sub + q * L;
+        if (live && b >= A0 + D.in2a && b <= A0 + D.fn2b) {
+          const double n = sN[b];
+          if (n > kNlim) {
+            double zv[5] = {0., 0., 0., 0., 0.};
+            for (int s = 0; s < ndry; ++s)
+              if (D.dry_id[s] < 5) zv[D.dry_id[s]] = sV[b][s];
+            const double zcore = zv[0] / n + zv[1] / n + zv[2] / n + zv[3] / n + zv[4] / n;
+            double zdwet = sm.dw[slot][b], zdold = 1., zlwc = 0., zke = 1.02;
+            const double zr = fmin(fmax(zrh, 0.37), 0.98);
+            int count = 0;
+            while (fabs(zdwet / zdold - 1.) > 1.e-12) {
+              zdold = fmax(zdwet, 1.e-20);
+              const double aw = zr / zke;
+              const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * (aw * aw)
+                                 - 3.1543839e+2 * (aw * aw * aw) + 6.770824e+1 * ((aw * aw) * (aw * aw));
+              const double bm2 = 1. / (aw * kMwa) - 1. / kMwa;
+              const double bm5 = 5.875248e+1 - 1.8781997e+2 * aw + 2.7211377e+2 * (aw * aw)
+                                 - 1.8458287e+2 * (aw * aw * aw) + 4.153689e+1 * ((aw * aw) * (aw * aw));
+              zlwc = (zv[0] * (kRhosu / kMsu)) / bm1 + kEpsoc * (zv[1] * (kRhooc / kMoc)) / bm2 +
+                     (zv[4] * (kRhoss / kMss)) / bm5;
+              zdwet = ::cbrt(zlwc / n / kRhowa / kPi6 + zcore / kPi6);
+              zke = ::exp(2. * kSurfw0 * kMvwa / (kBoltz * T * zdwet));
+              if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
+            }
+            sV[b][iwa] = zlwc / kRhowa;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    double w2 = 0.;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < D.nb[0]) w2 += sV[b][iwa];
+    }
+    w2 = cell_sum<L>(w2) * kRhowa;
+    rv = rv - (w2 - w1) / (P * kMair / (kRg * T));
+
+    // cell-skip predicate (:784-788; .AND. binds before .OR.)
+    uint32_t any = 0;   // bit0 cloud>nlim, bit1 precip>prlim, bit2 ice>prlim, bit3 aero>nlim
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < nt) {
+        const int c = D.cat_of[b];
+        const double n = sN[b];
+        if (c == 0 && n > kNlim) any |= 8u;
+        if (c == 1 && n > kNlim) any |= 1u;
+        if (c == 2 && n > kPrlim) any |= 2u;
+        if (c == 3 && n > kPrlim) any |= 4u;
+      }
+    }
+    any = cell_or<L>(any);
+    const bool pass = live && (((any & 1u) || ((any & 2u) && D.lscndh2ocl)) || ((any & 4u) && D.lscndh2oic) ||
+                               ((any & 8u) && zrh > 0.98 && D.lscndh2oae));
+    if (pass) {
+      const double rhoair = kMair * P / (kRg * T);
+      double zdf = (5. / (16. * kAvog * rhoair * 1.e-3 * (3.11e-8 * 3.11e-8))) *
+                   ::sqrt(kRg * 1.e7 * T * kMair * 1.e3 * (kMwa + kMair) * 1.e3 / (2. * kPi * kMwa * 1.e3));
+      zdf = zdf * 1.e-4;
+      const double zmfp = 3. * zdf * ::sqrt(kPi * kMwa / (8. * kRg * T));
+      const double zthc = 0.023807 + 7.1128e-5 * (T - 273.16);
+      const double hl = ((kAlv * kMwa) / (kRg * T)) - 1.;
+      const double hs = ((kAls * kMwa) / (kRg * T)) - 1.;
+      // per owned bin: moles of water, transfer coefficient, Kelvin factor,
+      // flat-surface saturation concentration, solute moles, saturation ratio
+      double zcw[NQ], zmt[NQ], zkl[NQ], csu[NQ], zns[NQ], zws[NQ], org_i[NQ], org_r[NQ];
+      uint32_t upd = 0;        // bit q: bin updated in the substeps
+      uint32_t pred = 0;       // bit q: bin in the vapour predictor sums
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int b = sub + q * L;
+        zcw[q] = 0.; zmt[q] = 0.; zkl[q] = 1.; csu[q] = 0.; zns[q] = 0.; zws[q] = 0.;
+        org_i[q] = 0.; org_r[q] = 0.;
+        if (b < nt) {
+          const int c = D.cat_of[b];
+          const double n = sN[b];
+          double vs = 0.;
+#pragma unroll
+          for (int s = 0; s < NS; ++s) vs += sV[b][s];
+          double dw = n > D.nlim[b] ? ::cbrt(vs / n / kPi6) : D.dmid[b];
+          dw = fmin(dw, D.dlim[b]);                        // updateDiameter (:803-806); dnsp == dwet
+          double zn = 0.;
+          for (int s = 0; s < ndry; ++s) zn += D.diss[s] * sV[b][s] * D.rholiq[s] / D.mm[s];
+          zns[q] = zn;
+          if (c == 3) {
+            org_i[q] = sV[b][iwa] * kRhoic / kMwa;
+            org_r[q] = sV[b][irim] * kRhori / kMwa;
+            zcw[q] = sV[b][iwa] * kRhoic / kMwa + sV[b][irim] * kRhori / kMwa;
+          } else {
+            zcw[q] = sV[b][iwa] * kRhowa / kMwa;
+          }
+          const bool on = n > D.nlim[b] &&
+                          (c == 0 ? (zrh > 0.98 && D.lscndh2oae)
+                                  : c == 3 ? (D.lscndh2oic && T < 273.15) : D.lscndh2ocl);
+          if (on) {
+            const double zknud = 2. * zmfp / dw;
+            const double zbeta = (zknud + 1.) / (0.377 * zknud + 1. + 4. / 3. * (zknud + zknud * zknud));
+            if (c == 3) {                                   // :892-930 (capacitance 0.5*dnsp, activity 1)
+              zkl[q] = ::exp(4. * kSurfi0 * kMwa / (kRg * T * kRhowa * dw));
+              csu[q] = rsi * rhoair / kMwa;
+              zws[q] = 1.0 * zkl[q];
+              const double h1 = n * 4. * kPi * (0.5 * dw) * zdf * zbeta;
+              const double h2 = kMwa * zdf * kAls * zws[q] * csu[q] / (zthc * T);
+              zmt[q] = h1 / (h2 * hs + 1.);
+            } else {                                        // :826-890, :932-955
+              const double zact = act_h2o<false>(sV[b][iwa] * D.rholiq[iwa] / D.mm[iwa], zn);
+              zkl[q] = ::exp(4. * kSurfw0 * kMwa / (kRg * T * kRhowa * dw));
+              csu[q] = rs * rhoair / kMwa;
+              zws[q] = zact * zkl[q];
+              const double h1 = n * 2. * kPi * dw * zdf * zbeta;
+              const double h2 = kMwa * zdf * kAlv * zws[q] * csu[q] / (zthc * T);
+              zmt[q] = h1 / (h2 * hl + 1.);
+            }
+          }
+          // aerosol 1a bins sit outside the predictor and are never updated (nstr = in2a)
+          const bool inpred = c != 0 || b >= A0 + D.in2a;
+          if (inpred) pred |= 1u << q;
+          if (inpred && on) upd |= 1u << q;
+        }
+      }
+      double zcwc = rv * rhoair / kMwa;
+      double stot = 0., s1 = 0., s2 = 0., sfix = 0.;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        stot += zcw[q];
+        if ((pred >> q) & 1u) { s1 += zmt[q] * zws[q] * csu[q]; s2 += zmt[q]; }
+        if (!((upd >> q) & 1u)) sfix += zcw[q];
+      }
+      stot = cell_sum<L>(stot);
+      s1 = cell_sum<L>(s1);
+      s2 = cell_sum<L>(s2);
+      sfix = cell_sum<L>(sfix);
+      const double zcwtot = zcwc + stot;
+
+      // substeps of adt = 0.02 s (:986-1053); the count comes from FP64 time accumulation
+      const double adt = 2.e-2;
+      double zcwint = 0.;
+      const int nsub = a.nsub;
+      for (int st = 0; st < nsub; ++st) {
+        const double zhlp1 = zcwc + adt * s1;
+        const double zhlp2 = 1. + adt * s2;
+        zcwint = fmin(zhlp1 / zhlp2, zcwtot);
+        double tsum = 0., s1n = 0.;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          if ((upd >> q) & 1u) {
+            const int b = sub + q * L;
+            const double d = adt * zmt[q] * (zcwint - zws[q] * csu[q]);
+            double nw = zcw[q] + fmin(fmax(d, -1.e-1 * zcw[q]), 1.e-1 * zcw[q]);
+            nw = fmax(nw, 0.);
+            zcw[q] = nw;
+            zws[q] = D.cat_of[b] == 3 ? zkl[q] : act_h2o<FASTDIV>(nw, zns[q]) * zkl[q];
+            tsum += nw;
+          }
+          if ((pred >> q) & 1u) s1n += zmt[q] * zws[q] * csu[q];
+        }
+        tsum = cell_sum<L>(tsum);
+        s1 = cell_sum<L>(s1n);
+        zcwint = zcwtot - (sfix + tsum);
+        zcwc = zcwint;
+      }
+      if (nsub > 100) flags |= SALSA_CELL_SUBSTEPS;
+      rv = zcwint * kMwa / rhoair;
+      // write-back (:1058-1117)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int b = sub + q * L;
+        if (b < nt) {
+          const int c = D.cat_of[b];
+          if (c < 3) {
+            sV[b][iwa] = fmax(0., zcw[q] * kMwa / kRhowa);
+          } else {
+            const double orig = org_i[q] + org_r[q];
+            const double dvt = zcw[q] - orig;
+            if (dvt > 0.) {
+              sV[b][iwa] = sV[b][iwa] + dvt * kMwa / kRhoic;
+            } else if (dvt < 0.) {
+              const double fr = (zcw[q] / orig) - 1.;
+              sV[b][iwa] = sV[b][iwa] + (fr * org_i[q]) * kMwa / kRhoic;
+              sV[b][irim] = sV[b][irim] + (fr * org_r[q]) * kMwa / kRhori;
+            }
+          }
+        }
+      }
+    }
+    (void)C0; (void)I0;
+  }
+  __syncthreads();
+
+  // ---- 6. unpack: tendencies or new state (mo_salsa_driver.f90:310-388) ----
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < nt) {
+        const int c = D.cat_of[b], loc = D.loc_of[b], nbc = D.nb[c];
+        const double n = sN[b];
+        const int64_t in = g + a.n3 * loc;
+        const double nn = n / dn;
+        if (!isfinite(nn)) flags |= SALSA_CELL_NONFINITE;
+        if (a.inplace) a.n_o[c][in] = nn;
+        else a.n_t[c][in] = a.n_t[c][in] + (nn - a.n_p[c][in]) / dt;
+        const int nsw = c < 3 ? D.nwet : D.nspec;
+        for (int s = 0; s < nsw; ++s) {
+          const double rho = c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
+          const int64_t im = g + a.n3 * (s * nbc + loc);
+          const double mm = sV[b][s] * rho / dn;
+          if (!isfinite(mm)) flags |= SALSA_CELL_NONFINITE;
+          if (a.inplace) a.m_o[c][im] = mm;
+          else a.m_t[c][im] = a.m_t[c][im] + (mm - a.m_p[c][im]) / dt;
+        }
+      }
+    }
+    if (sub == 0) {
+      if (!isfinite(rv)) flags |= SALSA_CELL_NONFINITE;
+      if (a.inplace) a.rv_o[g] = rv;
+      else a.rv_t[g] = a.rv_t[g] + (rv - rv_old) / dt;
+    }
+  }
+  flags = cell_or<L>(flags);
+  if (live && sub == 0) {
+    if (a.cell_flags) a.cell_flags[icell] = (uint8_t)flags;
+    if (flags && a.fault_any) atomicOr(a.fault_any, (int)flags);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+template <int L, class KT, int MB, int MP, int NS>
+static hipError_t launch_t(const DevLayout *d, const StepArgs &a, hipStream_t s) {
+  constexpr int CPW = 64 / L;
+  if (a.ncells <= 0) return hipSuccess;
+  const int64_t nb = (a.ncells + CPW - 1) / CPW;
+  hipLaunchKernelGGL((k_salsa_step<L, KT, MB, MP, NS>), dim3((unsigned)nb), dim3(64), 0, s, d, a);
+  return hipGetLastError();
+}
+
+// capacity classes: warm (level 4) and mixed-phase (level 5)
+int launch_step(int cap, int lanes, int precision, const DevLayout *d, const StepArgs &a, hipStream_t s) {
+#define SALSA_CASE(CAP, MB, MP, NS)                                                            \
+  if (cap == CAP) {                                                                            \
+    if (precision == SALSA_FP64) {                                                             \
+      if (lanes == 16) return (int)launch_t<16, double, MB, MP, NS>(d, a, s);                  \
+      if (lanes == 32) return (int)launch_t<32, double, MB, MP, NS>(d, a, s);                  \
+      if (lanes == 64) return (int)launch_t<64, double, MB, MP, NS>(d, a, s);                  \
+    } else {                                                                                   \
+      if (lanes == 16) return (int)launch_t<16, float, MB, MP, NS>(d, a, s);                   \
+      if (lanes == 32) return (int)launch_t<32, float, MB, MP, NS>(d, a, s);                   \
+      if (lanes == 64) return (int)launch_t<64, float, MB, MP, NS>(d, a, s);                   \
+    }                                                                                          \
+  }
+  SALSA_CASE(4, 32, 384, 3)
+  SALSA_CASE(5, 40, 640, 4)
+#undef SALSA_CASE
+  return -1;
+}
+
+int cap_bins(int cap) { return cap == 4 ? 32 : 40; }
+int cap_pairs(int cap) { return cap == 4 ? 384 : 640; }
+int cap_spec(int cap) { return cap == 4 ? 3 : 4; }
+
+}  // namespace salsa
