@@ -1,0 +1,106 @@
+"""Write the golden fixtures of tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/ref_harness (the UCLALES-SALSA Fortran compiled from its own
+sources by `make -C oracle ref`) on seeded cells and stores inputs and outputs
+as small compressed .npz files (data only).  Needs the reference tree, so it
+runs only in the build container:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "uclales-salsa_amd"))
+
+import oracle as O  # noqa: E402
+import refio  # noqa: E402
+from salsa_amd import workload as W  # noqa: E402
+
+
+def pad(L, numc, volc):
+    n, nt0 = numc.shape
+    nc = np.zeros((n, L.ntotal))
+    vc = np.zeros((n, L.ntotal, 9))
+    nc[:, :nt0] = numc
+    vc[:, :nt0] = volc
+    return nc, vc
+
+
+def pick(f, n, seed):
+    """Sample cells so that cloudy, humid and dry cells are all present."""
+    rng = np.random.default_rng(seed)
+    T, p, rs, rsi, rv, numc, volc = f
+    rh = rv / rs
+    cloudy = numc[:, 11:19].max(axis=1) > 1.0
+    groups = [np.where(cloudy)[0], np.where(~cloudy & (rh > 0.98))[0], np.where(~cloudy & (rh <= 0.98))[0]]
+    out = []
+    for g in groups:
+        if len(g):
+            out.append(rng.choice(g, size=min(len(g), n // 3 + 1), replace=False))
+    return np.sort(np.concatenate(out))[:n]
+
+
+def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0):
+    T, p, rs, rsi, rv, nc, vc = cells
+    R = refio.run(L, P, level, T, p, rs, rsi, rv, nc, vc, nsteps=nsteps, lcharge=lcharge, charge_enh=enh)
+    ns = L.nspec
+    np.savez_compressed(
+        os.path.join(HERE, name),
+        level=level, nsteps=nsteps, lcharge=lcharge, charge_enh=enh,
+        lsauto=P.lsauto_state, nbin=np.array(list(L.nbin)), nprc=L.nprc, nice=L.nice,
+        species=np.array(list(L.dry_species)[:L.nspec_dry]),
+        T=T, p=p, rs=rs, rsi=rsi, rv=rv, numc=nc, volc=vc[:, :, :ns],
+        out_rv=R["rv"], out_numc=R["numc"], out_volc=R["volc"][:, :, :ns], out_rates=R["rates"])
+    print("wrote", name, len(T), "cells")
+
+
+def main():
+    if not refio.available():
+        sys.exit("oracle/_ref/ref_harness missing: run `make -C oracle ref` first")
+    # warm layout nbin=(3,4): aerosol 11, cloud 8, precip 7 (+ 7 inert ice bins)
+    L = O.make_layout()
+    P4 = O.make_params(level=4)
+    R = refio.run(L, P4, 4, *[np.zeros(1)] * 5, np.zeros((1, L.ntotal)), np.zeros((1, L.ntotal, 9)))
+    np.savez_compressed(os.path.join(HERE, "bins_legal.npz"), vlolim=R["vlolim"], vhilim=R["vhilim"],
+                        dmid=R["dmid"], precpbins=R["precpbins"])
+    f1 = W.to_salsa_units(L, W.config_fields(L, 1), level=4)
+    T, p, rs, rsi, rv, numc, volc = f1
+    nc, vc = pad(L, numc, volc)
+    cells = (T, p, rs, rsi, rv, nc, vc)
+    sel = pick(cells, 240, 1)
+    save_step("step_cfg1.npz", L, 4, P4, tuple(x[sel] for x in cells))
+    save_step("step_cfg1_charge.npz", L, 4, O.make_params(level=4, lcharge=1, charge_coll_enh=0.3),
+              tuple(x[sel[:60]] for x in cells), lcharge=1, enh=0.3)
+    # kernel matrices of cloudy cells
+    cl = sel[nc[sel, 11:19].max(axis=1) > 1.0][:12]
+    Rk = refio.run(L, P4, 4, T[cl], p[cl], rs[cl], rsi[cl], rv[cl], nc[cl], vc[cl], mode=2)
+    np.savez_compressed(os.path.join(HERE, "kmat_cfg1.npz"), T=T[cl], p=p[cl], numc=nc[cl],
+                        volc=vc[cl][:, :, :L.nspec], kmat=Rk["kmat"])
+    print("wrote kmat_cfg1.npz", len(cl), "cells")
+    # cloudy cells with precipitation present, with and without precip formation
+    f4 = W.to_salsa_units(L, W.config_fields(L, 4, grid_override=(20, 8, 8)), level=4)
+    T, p, rs, rsi, rv, numc, volc = f4
+    nc, vc = pad(L, numc, volc)
+    cells = (T, p, rs, rsi, rv, nc, vc)
+    sel = pick(cells, 120, 4)
+    save_step("step_cfg4.npz", L, 4, P4, tuple(x[sel] for x in cells))
+    save_step("step_cfg4_noauto.npz", L, 4, O.make_params(level=4, lsauto_state=0),
+              tuple(x[sel[:60]] for x in cells))
+    # mixed phase: SO4 + dust, ice bins, T 240-275 K
+    L5 = O.make_layout(species=(0, 3))
+    P5 = O.make_params(level=5)
+    R5 = refio.run(L5, P5, 5, *[np.zeros(1)] * 5, np.zeros((1, L5.ntotal)), np.zeros((1, L5.ntotal, 9)))
+    np.savez_compressed(os.path.join(HERE, "bins_ice.npz"), vlolim=R5["vlolim"], vhilim=R5["vhilim"],
+                        dmid=R5["dmid"], precpbins=R5["precpbins"])
+    f5 = W.to_salsa_units(L5, W.config_fields(L5, 5, grid_override=(20, 8, 8)), level=5)
+    sel = np.random.default_rng(5).choice(len(f5[0]), 120, replace=False)
+    save_step("step_cfg5.npz", L5, 5, P5, tuple(x[np.sort(sel)] for x in f5))
+
+
+if __name__ == "__main__":
+    main()

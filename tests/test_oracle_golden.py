@@ -1,0 +1,68 @@
+"""The C oracle against the compiled reference (golden fixtures written by
+tests/golden/make_golden.py from oracle/_ref)."""
+import numpy as np
+import pytest
+
+from helpers import FLOOR_N, FLOOR_RV, FLOOR_V, full_volc, golden, oracle_layout_for, oracle_params_for, rel
+
+CASES = ["step_cfg1.npz", "step_cfg1_charge.npz", "step_cfg4.npz", "step_cfg4_noauto.npz", "step_cfg5.npz"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_step_matches_reference(oracle_mod, name):
+    g = golden(name)
+    L = oracle_layout_for(g)
+    P = oracle_params_for(g)
+    rv, nc, vc, st = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"],
+                                           full_volc(g["volc"]), nsteps=int(g["nsteps"]))
+    assert not np.any(st & 2)
+    assert rel(rv, g["out_rv"], FLOOR_RV).max() < 1e-12
+    assert rel(nc, g["out_numc"], FLOOR_N).max() < 1e-12
+    ns = g["volc"].shape[2]
+    assert rel(vc[:, :, :ns], g["out_volc"], FLOOR_V).max() < 1e-11
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_rates_match_reference(oracle_mod, name):
+    g = golden(name)
+    L = oracle_layout_for(g)
+    P = oracle_params_for(g)
+    _, _, _, _, rr = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"],
+                                           full_volc(g["volc"]), rates=True)
+    ref = g["out_rates"]
+    # the condensation rates are differences of round-tripped volumes: compare
+    # each rate slot against its own scale
+    scale = np.maximum(np.abs(ref).max(axis=0, keepdims=True), 1e-300)
+    err = np.abs(rr - ref) / scale
+    assert err.max() < 1e-9
+
+
+def test_oracle_kernel_matrices_match_reference(oracle_mod):
+    g = golden("kmat_cfg1.npz")
+    L = oracle_mod.make_layout()
+    P = oracle_mod.make_params(level=4)
+    for c in range(len(g["T"])):
+        nc = np.ascontiguousarray(g["numc"][c])
+        vc = np.ascontiguousarray(full_volc(g["volc"][c:c + 1])[0])
+        K = g["kmat"][c]
+        for i in range(L.ntotal):
+            for j in range(L.ntotal):
+                if K[i, j] == 0.0:
+                    continue
+                a, b = (i, j) if i <= j else (j, i)
+                # canonical orientation of the reference blocks: lower master index is pp1
+                k = oracle_mod.lib().so_coagc(L, P, nc.ctypes.data_as(__import__("ctypes").POINTER(__import__("ctypes").c_double)),
+                                              vc.ctypes.data_as(__import__("ctypes").POINTER(__import__("ctypes").c_double)),
+                                              a, b, float(g["T"][c]), float(g["p"][c]))
+                assert abs(k - K[i, j]) <= 1e-12 * abs(K[i, j]), (c, i, j, k, K[i, j])
+
+
+@pytest.mark.parametrize("name,species", [("bins_legal.npz", (0,)), ("bins_ice.npz", (0, 3))])
+def test_oracle_bins_match_reference(oracle_mod, name, species):
+    g = golden(name)
+    L = oracle_mod.make_layout(species=species)
+    nt = L.ntotal
+    assert np.array_equal(np.array(L.vlolim[:nt]), g["vlolim"])
+    assert np.array_equal(np.array(L.vhilim[:nt]), g["vhilim"])
+    assert np.array_equal(np.array(L.dmid[:nt]), g["dmid"])
+    assert np.array_equal(np.array(L.precpbins[:L.nprc]), g["precpbins"])

@@ -79,26 +79,32 @@ __global__ void k_zero_i32(int32_t *p, int n) {
 // 1 = aerosol only and RH > 0.98 (aerosol substeps), 0 = aerosol only and
 // drier (equilibrium water only).  Mirrors the predicates of gpparth2o
 // (mo_salsa_dynamics.f90:784-788) evaluated on the input state.
-__global__ void k_classify(const DevLayout *__restrict__ Dp, StepArgs a, uint8_t *cls, int32_t *ids,
-                           int32_t *counts) {
+__global__ void __launch_bounds__(256) k_classify(const DevLayout *__restrict__ Dp, StepArgs a, uint8_t *cls,
+                                                  int32_t *ids, int32_t *counts) {
+  __shared__ int32_t bc[3];
   const DevLayout &D = *Dp;
+  if (threadIdx.x < 3) bc[threadIdx.x] = 0;
+  __syncthreads();
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.ncells) return;
-  int k = (int)(c % a.nz), r = (int)(c / a.nz);
-  int i = r % a.nx, j = r / a.nx;
-  int64_t g = (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
-  double dn = a.dn[g];
-  bool hyd = false;
-  for (int cat = 1; cat < D.ncat; ++cat) {
-    const double *n = a.n_p[cat];
-    for (int b = 0; b < D.nb[cat]; ++b) hyd |= n[g + a.n3 * b] * dn > (cat == 1 ? kNlim : kPrlim);
+  if (c < a.ncells) {
+    int k = (int)(c % a.nz), r = (int)(c / a.nz);
+    int i = r % a.nx, j = r / a.nx;
+    int64_t g = (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
+    double dn = a.dn[g];
+    bool hyd = false;
+    for (int cat = 1; cat < D.ncat; ++cat) {
+      const double *n = a.n_p[cat];
+      for (int b = 0; b < D.nb[cat]; ++b) hyd |= n[g + a.n3 * b] * dn > (cat == 1 ? kNlim : kPrlim);
+    }
+    double rv = a.rv_p[g], rs = a.rsl[g];
+    if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);
+    uint8_t cl = hyd ? 2 : (rv / rs > 0.98 ? 1 : 0);
+    cls[c] = cl;
+    ids[c] = (int32_t)c;
+    atomicAdd(&bc[cl], 1);
   }
-  double rv = a.rv_p[g], rs = a.rsl[g];
-  if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);
-  uint8_t cl = hyd ? 2 : (rv / rs > 0.98 ? 1 : 0);
-  cls[c] = cl;
-  ids[c] = (int32_t)c;
-  atomicAdd(&counts[cl], 1);
+  __syncthreads();
+  if (threadIdx.x < 3 && bc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], bc[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------

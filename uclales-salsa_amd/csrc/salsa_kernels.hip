@@ -49,18 +49,46 @@ template <> struct Mth<float> {
   static __device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 };
 
-// Sum / OR over the L lanes of one cell (all lanes receive the result).
+// Sum / OR over the L lanes of one cell; every lane receives the same bits.
+// Within a 16-lane row: DPP quad permutes and row mirrors; across rows the
+// gfx950 v_permlane16_swap / v_permlane32_swap exchanges.  A cell never spans
+// more lanes than its own group, so rows of other cells are never read.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) { return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  return __hiloint2double(dpp_i<CTRL>(__double2hiint(x)), dpp_i<CTRL>(__double2loint(x)));
+}
+__device__ __forceinline__ double swap16_sum(double x) {
+  auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(x), false, false);
+  auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(x), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+__device__ __forceinline__ double swap32_sum(double x) {
+  auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(x), false, false);
+  auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(x), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
 template <int L>
 __device__ __forceinline__ double cell_sum(double x) {
-#pragma unroll
-  for (int m = L / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, L);
+  if constexpr (L >= 2) x += dpp_d<0xB1>(x);     // quad_perm [1,0,3,2]
+  if constexpr (L >= 4) x += dpp_d<0x4E>(x);     // quad_perm [2,3,0,1]
+  if constexpr (L >= 8) x += dpp_d<0x141>(x);    // row_half_mirror
+  if constexpr (L >= 16) x += dpp_d<0x140>(x);   // row_mirror
+  if constexpr (L >= 32) x = swap16_sum(x);
+  if constexpr (L >= 64) x = swap32_sum(x);
   return x;
 }
 template <int L>
 __device__ __forceinline__ uint32_t cell_or(uint32_t x) {
-#pragma unroll
-  for (int m = L / 2; m > 0; m >>= 1) x |= (uint32_t)__shfl_xor((int)x, m, L);
-  return x;
+  int v = (int)x;
+  if constexpr (L >= 2) v |= dpp_i<0xB1>(v);
+  if constexpr (L >= 4) v |= dpp_i<0x4E>(v);
+  if constexpr (L >= 8) v |= dpp_i<0x141>(v);
+  if constexpr (L >= 16) v |= dpp_i<0x140>(v);
+  if constexpr (L >= 32) { auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false); v = p[0] | p[1]; }
+  if constexpr (L >= 64) { auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false); v = p[0] | p[1]; }
+  return (uint32_t)v;
 }
 
 __device__ __forceinline__ double cubed(double x) { return x * x * x; }
@@ -229,6 +257,9 @@ struct Smem {
   KT pr[CPW][2][4][MB];          // D, c, omega, v: bin / twin
   KT K[CPW][MP];
   int8_t pf[CPW][MP];            // precip bin formed by a cloud-cloud pair (-1: none)
+  // layout tables shared by the block's cells
+  int16_t pidx[MB][MB];          // pair id of (i,j)
+  uint16_t pair[MP];             // b1 | b2 << 7 | twin << 14
 };
 
 // rime fraction, classSection.f90:202-224
@@ -319,6 +350,9 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       for (int s = 0; s < NS; ++s) sV[b][s] = v[s];
     }
   }
+  for (int e = lane; e < nt * nt; e += 64) sm.pidx[e / nt][e % nt] = D.pidx[e / nt][e % nt];
+  for (int e = lane; e < D.npair; e += 64)
+    sm.pair[e] = (uint16_t)(D.pb1[e] | (D.pb2[e] << 7) | (D.poff[e] << 14));
   __syncthreads();
 
   // ---- 2./3./4. coagulation (mo_salsa_dynamics.f90:92-202) ----
@@ -371,8 +405,9 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       const int np = D.npair;
       const int R = D.off[2];
       for (int p = sub; p < np; p += L) {
-        const int b1 = D.pb1[p], b2 = D.pb2[p];
-        const int tw = D.poff[p];
+        const int pw = sm.pair[p];
+        const int b1 = pw & 127, b2 = (pw >> 7) & 127;
+        const int tw = pw >> 14;
         KT k = KT(0.);
         int8_t pfb = -1;
         if (live && !(sN[b1] < D.nlim[b1]) && !(sN[b2] < D.nlim[b2])) {
@@ -413,8 +448,10 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       for (int s = 0; s < NS; ++s) Vt[s] = sV[t][s];
       bool act = live && !(Nt < D.nlim[t]);
       if (fl & TF_ICE) act = act && T < 273.15;          // mo_salsa_dynamics.f90:166
+      if (!__any(act)) continue;            // no cell of this wavefront updates the target
       const double rft = (fl & TF_ICE) ? rime_frac<NS>(D, Nt, Vt, t) : 0.;
       const double emax = D.Eiagg_max, emin = D.Eiagg_min;
+      const int nu = (fl & TF_ICE) ? nspec : TG.nupd;
 
       double fix_own = 1., fix_oth = 1.;
       if (fl & TF_PF) {
@@ -425,7 +462,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           if (j < nt) {
             const uint64_t bit = 1ull << j;
             if ((TG.fix_own | TG.fix_oth) & bit) {
-              const int pid = D.pidx[t][j];
+              const int pid = sm.pidx[t][j];
               const double kj = pid >= 0 ? (double)sK[pid] : 0.;
               if (TG.fix_own & bit) so += kj * sN[j];
               if (TG.fix_oth & bit) sx += kj * sN[j];
@@ -449,7 +486,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         const int j = sub + q * L;
         if (j < nt && ((anymask >> j) & 1ull)) {
           const uint64_t bit = 1ull << j;
-          const int pid = D.pidx[t][j];
+          const int pid = sm.pidx[t][j];
           const double kj = pid >= 0 ? (double)sK[pid] : 0.;
           const double Nj = sN[j];
           const double *Vj = sV[j];
@@ -491,7 +528,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       sink = cell_sum<L>(sink);
 #pragma unroll
       for (int s = 0; s < NS; ++s)
-        if (s < nspec) src[s] = cell_sum<L>(src[s]);
+        if (s < nu) src[s] = cell_sum<L>(src[s]);
 
       // self terms (computed identically by every lane of the cell)
       double sink_self = 0.;
@@ -501,7 +538,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       int pself = -1;
       double kself = 0., fixs = 1.;
       if (fl & (TF_SELF_HALF | TF_SELF_HALF_EAGG | TF_PF_SELF | TF_PRECIP_SELFCOLL)) {
-        const int pid = D.pidx[t][t];
+        const int pid = sm.pidx[t][t];
         kself = pid >= 0 ? (double)sK[pid] : 0.;
         if (fl & TF_SELF_HALF) sink_self = 0.5 * kself * Nt;
         if (fl & TF_SELF_HALF_EAGG) sink_self = (emax - rft * (emax - emin)) * 0.5 * kself * Nt;
@@ -519,7 +556,6 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
 
       // applyCoag / applyCoagIce / applyCoagPrecp (:539-645)
       double Vn[NS];
-      const int nu = (fl & TF_ICE) ? nspec : TG.nupd;
       const double den = 1. + dt * sink;
 #pragma unroll
       for (int s = 0; s < NS; ++s) Vn[s] = s < nu ? (Vt[s] + dt * src[s] * Nt) / den : Vt[s];
@@ -563,7 +599,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           for (int q = 0; q < NQ; ++q) {
             const int j = sub + q * L;
             if (j < nt && (((TG.pf_own | TG.pf_oth) >> j) & 1ull)) {
-              const int pid = D.pidx[t][j];
+              const int pid = sm.pidx[t][j];
               if (sPF[pid] == pb) {
                 const double kj = (double)sK[pid];
                 const double fx = ((TG.pf_own >> j) & 1ull) ? fix_own : fix_oth;
