@@ -1,0 +1,268 @@
+"""Benchmark of the SALSA coagulation + condensation step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py --counter-run {mixed,fp64,copy} [--lanes 16,16,16] [--steps 3]
+
+The second form is the program profiled by the hardware-counter runs: it
+runs K steps at a fixed launch shape (no autotune, no timing, no CPU leg), or
+a 512 MiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings scale
+the counters to bytes.
+
+Workload: 64x64x120 cells (491,520) at the reference-legal bin layout
+nbin = (3,4) -> aerosol 11 (1a 3, 2a 4, 2b 4), cloud 8, precipitation 7; SO4 + H2O;
+dt = 1 s; seeded synthetic fields with a cloudy boundary-layer slab z in [60, 90).
+A step is one run_SALSA-style pass (tendency mode) over every interior cell
+with all inputs resident in HBM.  The FP32-mixed path is the headline value;
+the plain FP64 path is timed on the same inputs for the speedup, and the two
+are compared on every bin.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "uclales-salsa_amd"))
+
+METRIC = "cell-steps/s SALSA coag+cond: FP32-mixed vs FP64 GPU at ≤1e-5 rel err"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def _cpu_worker(args):
+    """Time the FP64 C oracle on a slice of the packed cells (spawned process)."""
+    shm_name, shapes, lo, hi = args
+    import numpy as np
+    from multiprocessing import shared_memory
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    shm = shared_memory.SharedMemory(name=shm_name)
+    off = 0
+    arrs = []
+    for shp in shapes:
+        n = int(np.prod(shp))
+        arrs.append(np.ndarray(shp, dtype=np.float64, buffer=shm.buf, offset=off))
+        off += 8 * n
+    T, p, rs, rsi, rv, numc, volc = arrs
+    L = O.make_layout()
+    P = O.make_params(level=4)
+    sl = slice(lo, hi)
+    a = [np.ascontiguousarray(x[sl]) for x in (T, p, rs, rsi, rv, numc, volc)]
+    t0 = time.perf_counter()
+    O.step_batch(L, P, *a)
+    dt = time.perf_counter() - t0
+    del arrs, T, p, rs, rsi, rv, numc, volc
+    shm.close()
+    return hi - lo, dt
+
+
+def cpu_baseline(desc, f, max_cores=16):
+    """FP64 C restatement (oracle/) of the step on the host cores, one process
+    per core over contiguous slices of the same 491,520 cells."""
+    import multiprocessing as mp
+    from multiprocessing import shared_memory
+
+    import numpy as np
+    from salsa_amd import workload as W
+    T, p, rs, rsi, rv, numc, volc = W.to_salsa_units(desc, f, level=4)
+    n, nt0 = numc.shape
+    ntot = nt0 + desc.nice                      # oracle layout keeps the 7 inert ice bins
+    nc = np.zeros((n, ntot))
+    vc = np.zeros((n, ntot, 9))
+    nc[:, :nt0] = numc
+    vc[:, :nt0] = volc
+    del numc, volc
+    arrays = [T, p, rs, rsi, rv, nc, vc]
+    size = sum(a.nbytes for a in arrays)
+    shm = shared_memory.SharedMemory(create=True, size=size)
+    off = 0
+    shapes = []
+    for a in arrays:
+        np.ndarray(a.shape, dtype=np.float64, buffer=shm.buf, offset=off)[...] = a
+        shapes.append(a.shape)
+        off += a.nbytes
+    del arrays, nc, vc
+    try:
+        cores = min(max_cores, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        cores = min(max_cores, os.cpu_count() or 1)
+    bounds = np.linspace(0, n, cores + 1).astype(int)
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores) as pool:
+        t0 = time.perf_counter()
+        res = pool.map(_cpu_worker, [(shm.name, shapes, bounds[i], bounds[i + 1]) for i in range(cores)])
+        wall = time.perf_counter() - t0
+    shm.close()
+    shm.unlink()
+    per_core = sum(r[0] for r in res) / sum(r[1] for r in res)
+    return {"value": n / wall, "unit": "cell-steps/s", "cores": cores, "kind": "port",
+            "per_core": per_core,
+            "sample": "all %d cells of the 64x64x120 workload, 1 step, FP64 C restatement of the "
+                      "reference (oracle/salsa_oracle.c, gcc -O2), one process per core" % n}
+
+
+def counter_run(kind, lanes, steps):
+    """Fixed-shape steps (or the copy calibration) for rocprofv3 --pmc runs."""
+    import numpy as np
+    import torch
+    from salsa_amd import capi, workload as W
+    dev = torch.device("cuda")
+    if kind == "copy":
+        a = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
+        a.fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("counter run: copy of 512 MiB x %d done" % steps, flush=True)
+        return
+    desc = capi.bin_desc()
+    f = W.config_fields(desc, 2)
+    g = f.grid
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fd = dict(press=T(f.press), temp=T(f.temp), rsl=T(f.rsl), rsi=T(f.rsi), dn=T(f.dn), rv=T(f.rv))
+    n_p = [T(x) for x in f.n[:3]]
+    m_p = [T(x) for x in f.m[:3]]
+    n_t = [torch.zeros_like(x) for x in n_p]
+    m_t = [torch.zeros_like(x) for x in m_p]
+    rv_t = torch.zeros_like(fd["rv"])
+    prec = capi.SALSA_FP32_MIXED if kind == "mixed" else capi.SALSA_FP64
+    ctx = capi.Context(desc, capi.params(level=4, precision=prec), g.nzp, g.nxp, g.nyp)
+    for k in range(3):
+        ctx.set_shape(k, lanes[k])
+    for _ in range(steps):
+        ctx.step_device(fd, n_p, m_p, n_t, m_t, rv_t)
+    ctx.sync()
+    ctx.close()
+    print("counter run: %s, lanes %s, %d steps done" % (kind, lanes, steps), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--counter-run", choices=("mixed", "fp64", "copy"))
+    ap.add_argument("--lanes", default="16,16,16")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the SALSA step has no cross-GPU work in this build "
+              "(cells are independent; a multi-GPU run would shard the LES column blocks). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.counter_run:
+        counter_run(args.counter_run, [int(x) for x in args.lanes.split(",")], args.steps)
+        return
+
+    import numpy as np
+    from salsa_amd import capi, opcount, workload as W
+
+    desc = capi.bin_desc()
+    t0 = time.perf_counter()
+    f = W.config_fields(desc, 2)
+    g = f.grid
+    gen_s = time.perf_counter() - t0
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(desc, f)
+
+    import torch
+    dev = torch.device("cuda")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fd = dict(press=T(f.press), temp=T(f.temp), rsl=T(f.rsl), rsi=T(f.rsi), dn=T(f.dn), rv=T(f.rv))
+    n_p = [T(x) for x in f.n[:3]]
+    m_p = [T(x) for x in f.m[:3]]
+    n_t = [torch.zeros_like(x) for x in n_p]
+    m_t = [torch.zeros_like(x) for x in m_p]
+    rv_t = torch.zeros_like(fd["rv"])
+    torch.cuda.synchronize()
+
+    def run(prec):
+        ctx = capi.Context(desc, capi.params(level=4, precision=prec), g.nzp, g.nxp, g.nyp)
+        ctx.autotune(fd, n_p, m_p)
+        for _ in range(args.warmup):
+            ctx.step_device(fd, n_p, m_p, n_t, m_t, rv_t)
+        ctx.sync()
+        torch.cuda.synchronize()
+        ctx.timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.step_device(fd, n_p, m_p, n_t, m_t, rv_t)
+        ctx.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms, ksteps = ctx.kernel_time()
+        ctx.timing(False)
+        shapes = [ctx.shape_of(k) for k in range(3)]
+        # one in-place step for the accuracy comparison
+        rv = fd["rv"].clone()
+        fdd = dict(fd)
+        fdd["rv"] = rv
+        np_ = [x.clone() for x in n_p]
+        mp_ = [x.clone() for x in m_p]
+        ctx.step_device(fdd, np_, mp_, flags=capi.STEP_INPLACE)
+        ctx.sync()
+        ctx.close()
+        st = (rv.cpu().numpy(), [x.cpu().numpy() for x in np_], [x.cpu().numpy() for x in mp_])
+        return wall / args.steps, kms / max(ksteps, 1), shapes, st
+
+    step_mixed, kern_mixed, shapes_mixed, st_mixed = run(capi.SALSA_FP32_MIXED)
+    res64 = None if args.no_fp64 else run(capi.SALSA_FP64)
+
+    ncell = g.ncells
+    value = ncell / step_mixed
+    flops = opcount.flops_per_cell_step(desc, 4) * ncell
+    bytes_ = opcount.bytes_per_cell_step(desc, 4, tendency=True) * ncell
+    achieved_tf = flops / (kern_mixed * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": step_mixed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "SALSA coagulation+condensation step, 64x64x120 = %d cells, bins 11/8/7 "
+                               "(nbin=(3,4)), SO4+H2O, dt=1 s, tendency mode, cloudy slab z in [60,90)" % ncell,
+                   "cells": ncell, "layout": "11/8/7", "precision": "FP32-mixed",
+                   "launch_lanes_per_cell": shapes_mixed},
+        "roofline": {"bound": "mfma", "engine": "VALU (no MFMA: not a dense contraction)",
+                     "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tf / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel_ms_per_step": kern_mixed,
+                     "flop_per_cell_step": opcount.flops_per_cell_step(desc, 4),
+                     "hbm_GBs_algorithmic": bytes_ / (kern_mixed * 1e-3) / 1e9,
+                     "hbm_frac": bytes_ / (kern_mixed * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    prof = os.path.join(ROOT, "profiles", "traffic_k_salsa_step_64x64x120.json")
+    if os.path.exists(prof):
+        with open(prof) as fh:
+            tr = json.load(fh)
+        out["roofline"]["traffic"] = tr.get("bytes_per_step")
+    if cpu is not None:
+        out["cpu_baseline"]["per_core"] = cpu["per_core"]
+        out["speedup_vs_cpu"] = value / cpu["value"]
+    if res64 is not None:
+        step64, kern64, shapes64, st64 = res64
+        out["fp64"] = {"value": ncell / step64, "ms_per_step": step64 * 1e3, "kernel_ms_per_step": kern64,
+                       "launch_lanes_per_cell": shapes64,
+                       "roofline_frac": flops / (kern64 * 1e-3) / 1e12 / 78.6}
+        out["speedup_vs_fp64"] = step64 / step_mixed
+        # accuracy of the mixed path against the FP64 path, every bin of every cell
+        fo = lambda st: W.to_salsa_units(desc, W.Fields(grid=g, level=4, press=f.press, temp=f.temp, rsl=f.rsl,
+                                                         rsi=f.rsi, dn=f.dn, rv=st[0], n=st[1] + [None],
+                                                         m=st[2] + [None]))
+        a, b = fo(st_mixed), fo(st64)
+        rel = lambda x, y, fl: np.abs(x - y) / np.maximum(np.abs(y), fl)
+        e = {"rv": rel(a[4], b[4], 1e-12), "N": rel(a[5], b[5], 1e-3),
+             "V_SO4": rel(a[6][:, :, 0], b[6][:, :, 0], 1e-24), "V_H2O": rel(a[6][:, :, 1], b[6][:, :, 1], 1e-24)}
+        out["accuracy_vs_fp64"] = {k: {"max": float(v.max()), "frac_gt_1e-5": float(np.mean(v > 1e-5))}
+                                   for k, v in e.items()}
+    out["setup_s"] = gen_s
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

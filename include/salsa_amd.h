@@ -148,7 +148,10 @@ void salsa_ctx_destroy(salsa_ctx *ctx);
  *                    SALSA_STEP_INPLACE); n_t[c], m_t[c] : tendencies (+=).
  *   Category 3 (ice) pointers are ignored at level 4 and may be NULL.
  *   rates : NULL (process-rate diagnostics are not produced by this build)
- *   time  : model time, used for the lsfreeRH switch state
+ *   time, istp : model time [s] and step number of run_SALSA; the process
+ *           switch states (set_SALSA_runtime, mo_salsa_driver.f90:445-467) are
+ *           taken from salsa_params as the caller resolved them, so these two
+ *           do not change the result of this build's step
  * salsa_step_host takes host pointers (copies in and out on the context
  * stream); salsa_step_device takes device pointers and only enqueues work
  * (call salsa_ctx_sync before reading results).
@@ -185,6 +188,15 @@ int salsa_autotune(salsa_ctx *ctx, const double *press, const double *temp,
                    const double *rsl, const double *rsi, const double *dn,
                    const double *rv_p, const double *const n_p[4],
                    const double *const m_p[4], double tstep);
+
+/* Kernel timing with HIP events on the context stream.  salsa_ctx_timing(ctx, 1)
+ * resets and enables it: each step then records one event before its first and
+ * one after its last step-kernel launch.  salsa_ctx_kernel_time waits for the
+ * stream and returns the summed milliseconds and the number of steps timed.
+ * (Instrumentation of this build; no counterpart in the reference, which
+ * only prints cpu_time per LES step, step.f90:134-141.) */
+int salsa_ctx_timing(salsa_ctx *ctx, int32_t enable);
+int salsa_ctx_kernel_time(salsa_ctx *ctx, double *ms, int64_t *nsteps);
 
 /* Number of interior cells and of active bins in the context's layout. */
 int64_t salsa_ctx_ncells(const salsa_ctx *ctx);

@@ -80,6 +80,9 @@ def lib():
         L.so_step_cell.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_double, C.c_double,
                                    C.c_double, C.c_double, dp, C.c_double, dp, dp, C.POINTER(Rate)]
         L.so_step_cell.restype = C.c_int
+        L.so_step_batch.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_long, dp, dp, dp, dp, dp,
+                                    C.c_double, C.c_int, dp, dp]
+        L.so_step_batch.restype = C.c_int
         L.so_coagc.argtypes = [C.POINTER(Layout), C.POINTER(Params), dp, dp, C.c_int, C.c_int,
                                C.c_double, C.c_double]
         L.so_coagc.restype = C.c_double
@@ -160,3 +163,11 @@ def rslf(p, t):
 
 def rsif(p, t):
     return lib().so_rsif(p, t)
+
+
+def step_batch(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1):
+    """C-loop batch step (in place on rv, numc [n, ntotal], volc [n, ntotal, 9])."""
+    for a in (T, p, rs, rsi, rv, numc, volc):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return lib().so_step_batch(C.byref(L), C.byref(P), len(T), _dp(T), _dp(p), _dp(rs), _dp(rsi), _dp(rv),
+                               dt, nsteps, _dp(numc), _dp(volc))

@@ -1390,3 +1390,15 @@ void so_unpack_tend(const so_layout *L, int level, double dn, double tstep,
     nt_cat[cat][loc] = nt_cat[cat][loc] + ((numc[b] / dn) - n_cat[cat][loc]) / tstep;
   }
 }
+
+/* Batch driver for timing: n cells, each stepped nsteps times (SALSA units). */
+int so_step_batch(const so_layout *L, const so_params *P, long n, const double *T, const double *p,
+                  const double *rs, const double *rsi, double *rv, double ptstep, int nsteps,
+                  double *numc, double *volc) {
+  int st = 0;
+  for (long i = 0; i < n; ++i)
+    for (int k = 0; k < nsteps; ++k)
+      st |= so_step_cell(L, P, T[i], p[i], rs[i], rsi[i], &rv[i], ptstep,
+                         numc + i * L->ntotal, volc + i * L->ntotal * SO_MAXSPEC, 0);
+  return st;
+}

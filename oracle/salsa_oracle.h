@@ -112,6 +112,11 @@ int so_step_cell(const so_layout *L, const so_params *P,
                  double *prv, double ptstep,
                  double *numc, double *volc, so_rate *rates);
 
+/* n cells (numc [n*ntotal], volc [n*ntotal*SO_MAXSPEC]), nsteps each. */
+int so_step_batch(const so_layout *L, const so_params *P, long n, const double *T, const double *p,
+                  const double *rs, const double *rsi, double *rv, double ptstep, int nsteps,
+                  double *numc, double *volc);
+
 /* Individual pieces, exported so that tests can pin them separately. */
 double so_coagc(const so_layout *L, const so_params *P,
                 const double *numc, const double *volc,

@@ -64,6 +64,9 @@ struct salsa_ctx {
   StepArgs gkey;
   int glanes[3] = {0, 0, 0};
   bool gvalid = false;
+  // kernel timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;         // start/stop pairs of the timed steps
 };
 
 // ---------------------------------------------------------------------------
@@ -126,6 +129,8 @@ static int pick_cap(const DevLayout &L) {
 
 static void release(salsa_ctx *c) {
   if (!c) return;
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  c->ev.clear();
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   (void)hipFree(c->devL); (void)hipFree(c->flags); (void)hipFree(c->fault);
@@ -270,14 +275,24 @@ static int check_ptrs(salsa_ctx *c, const double *press, const double *temp, con
 }
 
 // Enqueue: fault reset, classification + sort (unless disabled), per-class launches.
+static int timing_mark(salsa_ctx *c) {
+  if (!c->timing) return SALSA_OK;
+  hipEvent_t e;
+  HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(e, c->stream));
+  c->ev.push_back(e);
+  return SALSA_OK;
+}
+
 static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
   hipStream_t s = c->stream;
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
   if (flags & SALSA_STEP_NO_COMPACT) {
     a.cell_list = nullptr;
+    if (timing_mark(c)) return SALSA_ERR_HIP;
     int rc = launch_step(c->cap, c->lanes[2], c->params.precision, c->devL, a, s);
     if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
-    return SALSA_OK;
+    return timing_mark(c);
   }
   // classify and sort cell ids by class (stable: k-fastest order kept within a class)
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->cls_count, 3);
@@ -289,6 +304,7 @@ static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
   HIPCHK(hipMemcpyAsync(c->cls_count_h, c->cls_count, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   int64_t off = 0;
+  if (timing_mark(c)) return SALSA_ERR_HIP;
   for (int k = 0; k < 3; ++k) {
     StepArgs b = a;
     b.cell_list = c->ids_sorted + off;
@@ -298,6 +314,29 @@ static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
     int rc = launch_step(c->cap, c->lanes[k], c->params.precision, c->devL, b, s);
     if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
   }
+  return timing_mark(c);
+}
+
+extern "C" int salsa_ctx_timing(salsa_ctx *c, int32_t enable) {
+  if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  c->ev.clear();
+  c->timing = enable != 0;
+  return SALSA_OK;
+}
+
+extern "C" int salsa_ctx_kernel_time(salsa_ctx *c, double *ms, int64_t *nsteps) {
+  if (!c || !ms || !nsteps) { set_error("null argument"); return SALSA_ERR_ARG; }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double tot = 0.;
+  for (size_t k = 0; k + 1 < c->ev.size(); k += 2) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
+    tot += t;
+  }
+  *ms = tot;
+  *nsteps = (int64_t)(c->ev.size() / 2);
   return SALSA_OK;
 }
 

@@ -41,7 +41,8 @@ EXPORTS = [
     "salsa_bin_desc_init", "salsa_params_default", "salsa_ctx_create", "salsa_ctx_destroy",
     "salsa_step_host", "salsa_step_device", "salsa_ctx_sync", "salsa_get_cell_flags",
     "salsa_set_launch_shape", "salsa_get_launch_shape", "salsa_autotune", "salsa_ctx_ncells",
-    "salsa_ctx_ntotal", "salsa_ctx_stream", "salsa_last_error",
+    "salsa_ctx_ntotal", "salsa_ctx_stream", "salsa_last_error", "salsa_ctx_timing",
+    "salsa_ctx_kernel_time",
 ]
 
 
@@ -105,6 +106,14 @@ def lib():
     if not os.path.exists(LIBPATH):
         raise RuntimeError("libsalsa_amd.so is not built (run __graft_entry__.build()); "
                            "there is no CPU fallback for the SALSA step")
+    # PyTorch-ROCm wheels carry their own HIP runtime (libamdhip64.so) next to
+    # ROCm's libamdhip64.so.7 that this library links.  When both live in one
+    # process, torch's runtime has to be loaded first or the second runtime
+    # finds no device; importing torch here fixes the order.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIBPATH)
     L.salsa_bin_desc_init.argtypes = [C.POINTER(BinDesc), C.POINTER(C.c_int32), _dp, C.c_int32,
                                       C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double,
@@ -139,6 +148,10 @@ def lib():
     L.salsa_ctx_ntotal.restype = C.c_int32
     L.salsa_ctx_stream.argtypes = [C.c_void_p]
     L.salsa_ctx_stream.restype = C.c_void_p
+    L.salsa_ctx_timing.argtypes = [C.c_void_p, C.c_int32]
+    L.salsa_ctx_timing.restype = C.c_int
+    L.salsa_ctx_kernel_time.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.salsa_ctx_kernel_time.restype = C.c_int
     L.salsa_last_error.argtypes = []
     L.salsa_last_error.restype = C.c_char_p
     _lib = L
@@ -251,6 +264,17 @@ class Context:
         out = np.zeros(self.ncells, dtype=np.uint8)
         _check(lib().salsa_get_cell_flags(self._h, out.ctypes.data, out.size))
         return out
+
+    def timing(self, enable=True):
+        """Reset and enable (or disable) HIP-event timing of the step kernels."""
+        _check(lib().salsa_ctx_timing(self._h, 1 if enable else 0))
+
+    def kernel_time(self):
+        """(milliseconds, steps) of step-kernel time since timing(True)."""
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(lib().salsa_ctx_kernel_time(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def set_shape(self, cls, lanes):
         _check(lib().salsa_set_launch_shape(self._h, cls, lanes))
