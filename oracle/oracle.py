@@ -56,6 +56,7 @@ class Params(C.Structure):
         ("ice_alpha", C.c_double), ("ice_beta", C.c_double), ("ice_gamma", C.c_double),
         ("ice_sigma", C.c_double),
         ("lcharge", C.c_int), ("charge_coll_enh", C.c_double),
+        ("kpert", C.c_double), ("kseed", C.c_ulonglong),
     ]
 
 

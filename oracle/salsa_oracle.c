@@ -187,6 +187,7 @@ void so_params_default(so_params *P, int level) {
   P->ice_alpha = 15.56999e-3; P->ice_beta = 2.02;   /* mo_ice_shape.f90:25-30 */
   P->ice_gamma = 0.55; P->ice_sigma = 1.97;
   P->lcharge = 0; P->charge_coll_enh = 0.;
+  P->kpert = 0.; P->kseed = 0ull;
   if (level == 4) {
     P->lscgia = P->lscgic = P->lscgii = P->lscgip = 0;
     P->lscndh2oic = 0;
@@ -331,6 +332,19 @@ static double terminal_vel_ice(double diam, double rhop, double rhoa, double vis
   return Av * pow(dnsp, Bv);
 }
 
+/* uniform number in [-1,1) from (seed, pair, T, p): splitmix64 finaliser */
+static double kpert_unit(unsigned long long seed, int b1, int b2, double temp, double pres) {
+  unsigned long long x, bt, bp;
+  memcpy(&bt, &temp, 8);
+  memcpy(&bp, &pres, 8);
+  x = seed ^ ((unsigned long long)b1 << 32) ^ (unsigned long long)(b2 * 2654435761u) ^ bt ^ (bp << 1);
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return (double)(x >> 11) * (2. / 9007199254740992.) - 1.;
+}
+
 /* coagc(pp1,pp2,temp,pres,lcharge,kernel=2),
  * src/src_salsa/mo_salsa_coagulation_kernels.f90:185-420 */
 static double coagc_cell(const so_layout *L, const so_params *P, const so_cell *c,
@@ -408,6 +422,7 @@ static double coagc_cell(const so_layout *L, const so_params *P, const so_cell *
   double k = zbrown + zbrconv + sqrt(zgrav * zgrav + ztshear * ztshear + zturbinert * zturbinert);
   /* charge enhancement :411-416 with chargeTime = chargeTimeMax for both bins */
   if (P->lcharge) k = k + k * P->charge_coll_enh * 0.5 * (1000. + 1000.) / 1000.;
+  if (P->kpert != 0.) k *= 1. + P->kpert * kpert_unit(P->kseed, b1, b2, temp, pres);
   return k;
 }
 

@@ -69,6 +69,12 @@ typedef struct {
   int lcharge;                 /* charge enhancement on                         */
   double charge_coll_enh;      /* chargeCollEnh; with chargeTime = chargeTimeMax
                                   the kernel is scaled by (1 + charge_coll_enh) */
+  /* Test hook with no counterpart in the reference: when kpert != 0 every
+   * kernel value is multiplied by (1 + kpert*u), u in [-1,1) a hash of
+   * (kseed, bin pair, T, p).  Used to measure how strongly the reference's
+   * answer depends on rounding-sized kernel errors.  Default 0. */
+  double kpert;
+  unsigned long long kseed;
 } so_params;
 
 /* One Rate (classProcessRates.f90:40-45): volc(9) and numc */

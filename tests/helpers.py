@@ -113,16 +113,33 @@ class FlatCells:
         return self.n, 1, 1
 
 
-def oracle_sensitivity(oracle_mod, L, P, T, p, rs, rsi, rv, numc, volc, eps=1e-13, seed=0):
+def oracle_sensitivity(oracle_mod, L, P, T, p, rs, rsi, rv, numc, volc, eps=1e-13, seed=0, kernel_eps=0.0):
     """Spread of the oracle's own outputs under a relative input perturbation
     of size eps: the part of the reference's answer that its own rounding
-    does not determine.  Returns (d_rv, d_numc, d_volc) relative spreads."""
+    does not determine.  With kernel_eps > 0 the spread under a random
+    per-pair relative kernel perturbation of that size (two seeds) is folded
+    in as well: what FP32-rounded collision kernels cannot determine.
+    Returns (d_rv, d_numc, d_volc) relative spreads."""
     rng = np.random.default_rng(seed)
     base = oracle_mod.step_cells(L, P, T, p, rs, rsi, rv, numc, volc)
-    pert = oracle_mod.step_cells(L, P, T, p, rs, rsi, rv * (1 + eps * rng.standard_normal(len(rv))),
-                                 numc * (1 + eps * rng.standard_normal(numc.shape)),
-                                 volc * (1 + eps * rng.standard_normal(volc.shape)))
-    return (rel(pert[0], base[0], FLOOR_RV), rel(pert[1], base[1], FLOOR_N), rel(pert[2], base[2], FLOOR_V))
+    runs = [oracle_mod.step_cells(L, P, T, p, rs, rsi, rv * (1 + eps * rng.standard_normal(len(rv))),
+                                  numc * (1 + eps * rng.standard_normal(numc.shape)),
+                                  volc * (1 + eps * rng.standard_normal(volc.shape)))]
+    if kernel_eps > 0:
+        for ks in (1, 2):
+            Pk = type(P).from_buffer_copy(P)
+            Pk.kpert = kernel_eps
+            Pk.kseed = ks
+            runs.append(oracle_mod.step_cells(L, Pk, T, p, rs, rsi, rv, numc, volc))
+    out = []
+    for i, fl in ((0, FLOOR_RV), (1, FLOOR_N), (2, FLOOR_V)):
+        out.append(np.max([rel(r[i], base[i], fl) for r in runs], axis=0))
+    return tuple(out)
+
+
+# Kernel perturbation that stands for FP32 rounding of the collision kernels
+# (about two units in the last place of a float).
+KERNEL_EPS_FP32 = 2.0 ** -22
 
 
 def check_condition_aware(got, ref, sens, tol, ns, label=""):

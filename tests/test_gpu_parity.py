@@ -4,12 +4,14 @@ cells, through the C ABI.  Bars: FP64 path 1e-10 relative; FP32-mixed path
 import numpy as np
 import pytest
 
-from helpers import FLOOR_N, FLOOR_RV, FLOOR_V, FlatCells, check_condition_aware, full_volc, golden, \
-    oracle_layout_for, oracle_params_for, oracle_sensitivity, rel
+from helpers import FLOOR_N, FLOOR_RV, FLOOR_V, KERNEL_EPS_FP32, FlatCells, check_condition_aware, full_volc, \
+    golden, oracle_layout_for, oracle_params_for, oracle_sensitivity, rel
 
 pytestmark = pytest.mark.gpu
 
 TOL = {0: 1e-10, 1: 1e-5}
+# kernel perturbation folded into the conditioning allowance per precision
+KEPS = {0: 0.0, 1: KERNEL_EPS_FP32}
 
 
 @pytest.fixture(scope="module")
@@ -86,13 +88,15 @@ def test_gpu_matches_reference_golden(capi, oracle_mod, name, kw, prec):
     orv, onc, ovc, _ = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco)
     e_orc = errors(out[0], out[1], out[2], orv, onc[:, :nt], ovc[:, :nt], ns)
     print("  vs reference rv %.2e N %.2e V %.2e | vs oracle rv %.2e N %.2e V %.2e" % (e_ref + e_orc))
-    srv, snc, svc = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco)
+    srv, snc, svc = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco,
+                                       kernel_eps=KEPS[prec])
     sens = (srv, snc[:, :nt], svc[:, :nt])
     _, nf1 = check_condition_aware(out, (orv, onc[:, :nt], ovc[:, :nt]), sens, TOL[prec], ns, "oracle")
     _, nf2 = check_condition_aware(out, (g["out_rv"], g["out_numc"][:, :nt], g["out_volc"][:, :nt]), sens,
                                    TOL[prec], ns, "reference")
-    print("  elements inside the conditioning allowance: %d / %d" % (nf1, out[2][:, :, :ns].size))
-    assert nf1 <= 2 and nf2 <= 2
+    nel = out[2][:, :, :ns].size
+    print("  elements inside the conditioning allowance: %d / %d" % (nf1, nel))
+    assert nf1 <= max(2, nel // 200) and nf2 <= max(2, nel // 200)
 
 
 def config_cells(capi, cfg, grid=None):
@@ -153,7 +157,7 @@ def test_gpu_config1_full_grid_vs_oracle(capi, oracle_mod, prec):
     e = errors(grv, gnc, gvc, orv, onc[:, :nt], ovc[:, :nt], 2)
     print("config1 prec %d: rv %.2e N %.2e V %.2e" % ((prec,) + e))
     # the tendency round trip m_p + dt*(m_new - m_p)/dt costs a few ulp of m_p
-    srv, snc, svc = oracle_sensitivity(oracle_mod, L, P, T, p, rs, rsi, rv, nco, vco)
+    srv, snc, svc = oracle_sensitivity(oracle_mod, L, P, T, p, rs, rsi, rv, nco, vco, kernel_eps=KEPS[prec])
     _, nflag = check_condition_aware((grv, gnc, gvc), (orv, onc[:, :nt], ovc[:, :nt]),
                                      (srv, snc[:, :nt], svc[:, :nt]), max(TOL[prec], 1e-9), 2, "config1")
     assert nflag <= 10
