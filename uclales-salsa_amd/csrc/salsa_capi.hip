@@ -17,9 +17,10 @@ namespace salsa {
 extern thread_local char g_err[512];
 void set_error(const char *msg);
 int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &L);
-int launch_step(int cap, int lanes, int precision, const DevLayout *d, const StepArgs &a, hipStream_t s);
+int launch_step(int cap, int lanes, int precision, bool hyd, const DevLayout *d, const StepArgs &a,
+                const SmemGeom &G, hipStream_t s);
+int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom &G);
 int cap_bins(int cap);
-int cap_pairs(int cap);
 int cap_spec(int cap);
 }  // namespace salsa
 
@@ -77,11 +78,13 @@ __global__ void k_zero_i32(int32_t *p, int n) {
   if (i < n) p[i] = 0;
 }
 
-// Cost class of a cell from its inputs: 2 = any hydrometeor present (cloud /
-// precipitation / ice: full coagulation sweep and condensation substeps),
-// 1 = aerosol only and RH > 0.98 (aerosol substeps), 0 = aerosol only and
-// drier (equilibrium water only).  Mirrors the predicates of gpparth2o
-// (mo_salsa_dynamics.f90:784-788) evaluated on the input state.
+// Cost class of a cell from its inputs: 2 = some cloud / precipitation / ice
+// bin at or above its activity limit (numc >= nlim: it takes part in the
+// collision kernels, mo_salsa_coagulation_kernels.f90:116-168, and in the
+// condensation predicate, mo_salsa_dynamics.f90:784-788); 1 = no such bin and
+// RH > 0.98 (aerosol substeps); 0 = no such bin and drier (equilibrium water
+// only).  Classes 0 and 1 run the kernel variant without hydrometeor pairs,
+// which is exact for them: coagulation only moves aerosol into aerosol bins.
 __global__ void __launch_bounds__(256) k_classify(const DevLayout *__restrict__ Dp, StepArgs a, uint8_t *cls,
                                                   int32_t *ids, int32_t *counts) {
   __shared__ int32_t bc[3];
@@ -97,7 +100,7 @@ __global__ void __launch_bounds__(256) k_classify(const DevLayout *__restrict__ 
     bool hyd = false;
     for (int cat = 1; cat < D.ncat; ++cat) {
       const double *n = a.n_p[cat];
-      for (int b = 0; b < D.nb[cat]; ++b) hyd |= n[g + a.n3 * b] * dn > (cat == 1 ? kNlim : kPrlim);
+      for (int b = 0; b < D.nb[cat]; ++b) hyd |= !(n[g + a.n3 * b] * dn < D.nlim[D.off[cat] + b]);
     }
     double rv = a.rv_p[g], rs = a.rsl[g];
     if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);
@@ -123,7 +126,7 @@ static int substeps(double dt) {
 
 static int pick_cap(const DevLayout &L) {
   for (int cap : {4, 5})
-    if (L.nt <= cap_bins(cap) && L.npair <= cap_pairs(cap) && L.nspec <= cap_spec(cap)) return cap;
+    if (L.nt <= cap_bins(cap) && L.nspec <= cap_spec(cap)) return cap;
   return 0;
 }
 
@@ -217,7 +220,15 @@ extern "C" void *salsa_ctx_stream(salsa_ctx *c) { return c ? (void *)c->stream :
 extern "C" int salsa_set_launch_shape(salsa_ctx *c, int32_t cls, int32_t lanes) {
   if (!c || cls < 0 || cls > 2) { set_error("bad class"); return SALSA_ERR_ARG; }
   if (lanes == 0) return SALSA_OK;
-  if (lanes != 16 && lanes != 32 && lanes != 64) { set_error("lanes_per_cell must be 16, 32 or 64"); return SALSA_ERR_ARG; }
+  if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) {
+    set_error("lanes_per_cell must be 8, 16, 32 or 64");
+    return SALSA_ERR_ARG;
+  }
+  SmemGeom G;
+  if (make_geom(c->hostL, cls == 2, c->params.precision == SALSA_FP64 ? 8 : 4, lanes, G)) {
+    set_error("this lanes_per_cell needs more than 64 KiB of LDS per block for the layout");
+    return SALSA_ERR_ARG;
+  }
   c->lanes[cls] = lanes;
   return SALSA_OK;
 }
@@ -284,13 +295,23 @@ static int timing_mark(salsa_ctx *c) {
   return SALSA_OK;
 }
 
+static int launch_class(salsa_ctx *c, int k, const StepArgs &a) {
+  const bool hyd = k == 2;
+  SmemGeom G;
+  if (make_geom(c->hostL, hyd, c->params.precision == SALSA_FP64 ? 8 : 4, c->lanes[k], G)) {
+    set_error("launch shape exceeds 64 KiB of LDS per block");
+    return SALSA_ERR_ARG;
+  }
+  return launch_step(c->cap, c->lanes[k], c->params.precision, hyd, c->devL, a, G, c->stream);
+}
+
 static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
   hipStream_t s = c->stream;
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
   if (flags & SALSA_STEP_NO_COMPACT) {
     a.cell_list = nullptr;
     if (timing_mark(c)) return SALSA_ERR_HIP;
-    int rc = launch_step(c->cap, c->lanes[2], c->params.precision, c->devL, a, s);
+    int rc = launch_class(c, 2, a);
     if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
     return timing_mark(c);
   }
@@ -311,7 +332,7 @@ static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
     b.ncells = c->cls_count_h[k];
     off += b.ncells;
     if (b.ncells == 0) continue;
-    int rc = launch_step(c->cap, c->lanes[k], c->params.precision, c->devL, b, s);
+    int rc = launch_class(c, k, b);
     if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
   }
   return timing_mark(c);
@@ -468,11 +489,15 @@ extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *t
     off += b.ncells;
     if (b.ncells == 0) continue;
     float best = 1e30f;
-    int bestl = c->lanes[k];
-    for (int lanes : {16, 32, 64}) {
-      launch_step(c->cap, lanes, c->params.precision, c->devL, b, s);   // warm
+    const int keep = c->lanes[k];
+    int bestl = keep;
+    for (int lanes : {8, 16, 32, 64}) {
+      SmemGeom G;
+      if (make_geom(c->hostL, k == 2, c->params.precision == SALSA_FP64 ? 8 : 4, lanes, G)) continue;
+      c->lanes[k] = lanes;
+      if (launch_class(c, k, b)) { c->lanes[k] = keep; continue; }   // warm
       hipEventRecord(e0, s);
-      for (int r = 0; r < 3; ++r) launch_step(c->cap, lanes, c->params.precision, c->devL, b, s);
+      for (int r = 0; r < 3; ++r) launch_class(c, k, b);
       hipEventRecord(e1, s);
       hipEventSynchronize(e1);
       float ms = 0.f;

@@ -70,8 +70,12 @@ struct DevLayout {
   int16_t pb1[kMaxBin * (kMaxBin + 1) / 2], pb2[kMaxBin * (kMaxBin + 1) / 2];
   uint8_t poff[kMaxBin * (kMaxBin + 1) / 2];
   int16_t pidx[kMaxBin][kMaxBin];     // pair id of (i,j), -1 if that block is off
+  uint32_t pw[kMaxBin * (kMaxBin + 1) / 2];   // packed pair word: b1 | b2 << 8 | offset << 16
+  int32_t npair_aa;                   // aerosol-aerosol pairs (the first block)
+  int32_t pf0, ncc;                   // first cloud-cloud pair id and their count
   // sweep program
   int32_t ntarget;
+  int32_t it_aero0, it_aero1;         // sweep entries of the aerosol targets [it_aero0, it_aero1)
   DevTarget tgt[kMaxBin];
   // switches and constants
   int32_t level, lscoag, lscnd, lscndh2oae, lscndh2ocl, lscndh2oic;
@@ -79,6 +83,21 @@ struct DevLayout {
   double eddy_dis, Eiagg_max, Eiagg_min;
   double ice_alpha, ice_beta, ice_gamma, ice_sigma;
   double charge_factor;               // 1 + chargeCollEnh (lcharge) or 1
+};
+
+// LDS geometry of one launch (built on the host for the layout, the kernel
+// variant and the cells per wavefront).  The block region starts with the
+// pair-id table [ntab][ntab] (int16), followed by one region per cell:
+//   N[nt], V[nt][nsV], dw[nt] (double), particle terms [2][ntab] (4 x KT),
+//   K[npairs] (KT), precipitation-formation bin per cloud-cloud pair (int8).
+struct SmemGeom {
+  int32_t tab_bytes, cell_bytes, total_bytes;
+  int32_t oN, oV, odw, opr, oK, opf;
+  int32_t ntab;        // bins with particle terms and kernels
+  int32_t npairs;      // pairs evaluated (the aerosol block only without hydrometeors)
+  int32_t nsV;         // volume slots per bin
+  int32_t it0, it1;    // sweep entries visited
+  uint64_t jmask;      // collector bins that can be active
 };
 
 // Per-launch arguments (passed by value).

@@ -245,25 +245,12 @@ __device__ __forceinline__ KT pair_coef(KT dA, KT DA, KT cA, KT wA, KT vA, KT dB
 }
 
 // ---------------------------------------------------------------------------
-// shared memory of one wavefront
+// per-cell LDS views
 // ---------------------------------------------------------------------------
-template <int L, class KT, int MB, int MP, int NS>
-struct Smem {
-  static constexpr int CPW = 64 / L;
-  double N[CPW][MB];
-  double V[CPW][MB][NS];
-  double dw[CPW][MB];            // wet diameter at coagulation start
-  double di[CPW][2][MB];         // colliding diameter: bin / self-offset twin
-  KT pr[CPW][2][4][MB];          // D, c, omega, v: bin / twin
-  KT K[CPW][MP];
-  int8_t pf[CPW][MP];            // precip bin formed by a cloud-cloud pair (-1: none)
-  // layout tables shared by the block's cells
-  int16_t pidx[MB][MB];          // pair id of (i,j)
-  uint16_t pair[MP];             // b1 | b2 << 7 | twin << 14
-};
+template <class KT>
+struct PT4 { KT D, c, w, v; };      // particle terms of one bin: diffusion, speed, omega, fall speed
 
 // rime fraction, classSection.f90:202-224
-template <int NS>
 __device__ __forceinline__ double rime_frac(const DevLayout &D, double n, const double *v, int b) {
   if (D.phase[b] == 4 && v[D.iwa] > 1.e-23 && v[D.irim] > 1.e-23 && n > D.nlim[b])
     return v[D.irim] * kRhori / (v[D.iwa] * kRhoic + v[D.irim] * kRhori);
@@ -286,20 +273,47 @@ __device__ __forceinline__ double act_h2o(double znw, double zns) {
   return fmax(0.1, q);
 }
 
+template <int L>
+__device__ __forceinline__ uint64_t cell_or64(uint64_t x) {
+  return (uint64_t)cell_or<L>((uint32_t)x) | ((uint64_t)cell_or<L>((uint32_t)(x >> 32)) << 32);
+}
+
 // ---------------------------------------------------------------------------
 // the step kernel
+//
+// HYD = false is the variant for cells whose cloud, precipitation and ice bins
+// are all below their activity limit (numc < nlim): their kernel values are
+// zero and their sweep targets are skipped in the reference as well, so only
+// the aerosol-aerosol pairs and the aerosol targets are evaluated.  The
+// condensation stage still visits every bin.
 // ---------------------------------------------------------------------------
-template <int L, class KT, int MB, int MP, int NS>
-__global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a) {
+template <int L, class KT, int MB, int NS, bool HYD>
+__global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   constexpr int CPW = 64 / L, NQ = (MB + L - 1) / L;
   constexpr bool FASTDIV = sizeof(KT) == 4;
-  __shared__ Smem<L, KT, MB, MP, NS> sm;
+  extern __shared__ __align__(16) char smem[];
   const DevLayout &D = *Dp;
   const int lane = threadIdx.x, slot = lane / L, sub = lane % L;
-  const int64_t cid = (int64_t)blockIdx.x * CPW + slot;
+  // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+  // range of cells so that neighbouring cells (which share cache lines of
+  // every field) meet in one L2.
+  const uint32_t nblk = gridDim.x, bid = blockIdx.x;
+  const uint32_t xcd = bid & 7u, rnk = bid >> 3, per = nblk >> 3, rem = nblk & 7u;
+  const uint32_t lb = xcd * per + (xcd < rem ? xcd : rem) + rnk;
+  const int64_t cid = (int64_t)lb * CPW + slot;
   const bool live = cid < a.ncells;
-  const int nt = D.nt;
+  const int nt = D.nt, nsV = G.nsV, ntab = G.ntab;
   const double dt = a.dt;
+
+  int16_t *tpidx = reinterpret_cast<int16_t *>(smem);
+  char *cb = smem + G.tab_bytes + slot * G.cell_bytes;
+  double *sN = reinterpret_cast<double *>(cb + G.oN);
+  double *sV = reinterpret_cast<double *>(cb + G.oV);       // [nt][nsV]
+  double *sDW = reinterpret_cast<double *>(cb + G.odw);
+  PT4<KT> *sPR = reinterpret_cast<PT4<KT> *>(cb + G.opr);   // [2][ntab]: bin, self-offset twin
+  KT *sK = reinterpret_cast<KT *>(cb + G.oK);
+  int8_t *sPF = reinterpret_cast<int8_t *>(cb + G.opf);
+#define VV(b, s) sV[(b) * nsV + (s)]
 
   int64_t g = 0;
   int32_t icell = 0;
@@ -318,13 +332,16 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
   const double rv_old = rv;
   uint32_t flags = 0;
 
-  double *sN = sm.N[slot];
-  double(*sV)[NS] = sm.V[slot];
-
-  // ---- 1. pack (mo_salsa_driver.f90:236-279) ----
+  // ---- 1. pack (mo_salsa_driver.f90:236-279); the LES values stay in
+  //      registers for the tendency, and the tendency inputs are fetched now
+  //      so that their latency hides behind the computation ----
+  double pn[NQ], pm[NQ][NS], tn[NQ], tm[NQ][NS];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int b = sub + q * L;
+    pn[q] = 0.; tn[q] = 0.;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) { pm[q][s] = 0.; tm[q][s] = 0.; }
     if (b < nt) {
       double n = 0.;
       double v[NS];
@@ -332,32 +349,39 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       for (int s = 0; s < NS; ++s) v[s] = 0.;
       if (live) {
         const int c = D.cat_of[b], loc = D.loc_of[b], nbc = D.nb[c];
-        n = a.n_p[c][g + a.n3 * loc] * dn;
+        const int64_t in = g + a.n3 * loc;
+        const int nsw = c < 3 ? D.nwet : D.nspec;
+        pn[q] = a.n_p[c][in];
+        if (!a.inplace) tn[q] = a.n_t[c][in];
         const double *m = a.m_p[c];
-        if (c < 3) {
 #pragma unroll
-          for (int s = 0; s < NS - 1; ++s)
-            if (s < D.nwet) v[s] = m[g + a.n3 * (s * nbc + loc)] * dn / D.rholiq[s];
-        } else {
+        for (int s = 0; s < NS; ++s)
+          if (s < nsw) {
+            const int64_t im = g + a.n3 * (s * nbc + loc);
+            pm[q][s] = m[im];
+            if (!a.inplace) tm[q][s] = a.m_t[c][im];
+          }
+        n = pn[q] * dn;
 #pragma unroll
-          for (int s = 0; s < NS - 1; ++s)
-            if (s < D.nwet) v[s] = m[g + a.n3 * (s * nbc + loc)] * dn / D.rhoice[s];
-          v[D.irim] = m[g + a.n3 * (D.irim * nbc + loc)] * dn / kRhori;
-        }
+        for (int s = 0; s < NS; ++s)
+          if (s < nsw) {
+            const double rho = c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
+            v[s] = pm[q][s] * dn / rho;
+          }
       }
       sN[b] = n;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) sV[b][s] = v[s];
+      for (int s = 0; s < NS; ++s)
+        if (s < nsV) VV(b, s) = v[s];
     }
   }
-  for (int e = lane; e < nt * nt; e += 64) sm.pidx[e / nt][e % nt] = D.pidx[e / nt][e % nt];
-  for (int e = lane; e < D.npair; e += 64)
-    sm.pair[e] = (uint16_t)(D.pb1[e] | (D.pb2[e] << 7) | (D.poff[e] << 14));
+  for (int e = lane; e < ntab * ntab; e += 64) tpidx[e] = D.pidx[e / ntab][e % ntab];
   __syncthreads();
 
   // ---- 2./3./4. coagulation (mo_salsa_dynamics.f90:92-202) ----
   if (D.lscoag) {
     const CellC cc = cell_consts(D, T, P);
+    uint64_t abits = 0;      // bins taking part in kernels: numc >= nlim (kernels.f90:116-168)
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int b = sub + q * L;
@@ -365,93 +389,91 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         const double n = sN[b];
         double vs = 0.;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) vs += sV[b][s];
+        for (int s = 0; s < NS; ++s)
+          if (s < nsV) vs += VV(b, s);
         // updateDiameter(limit, "all"): dnsp ends equal to the limited dwet (classSection.f90:147-156)
         double dw = n > D.nlim[b] ? ::cbrt(vs / n / kPi6) : D.dmid[b];
         dw = fmin(dw, D.dlim[b]);
-        const int ph = D.phase[b];
-        double rho = kRhowa;                             // updateRhomean, classSection.f90:169-195
-        Shape sh{0., 0., 0., 0.};
-        if (ph == 4) {
-          if (n > D.nlim[b]) {
-            double mp = kRhoic * sV[b][D.iwa], mr = kRhori * sV[b][D.irim];
-            rho = fmin((mp * kRhoic + mr * kRhori) / fmax(mp + mr, 7.e-25), kRhoic);
-          } else {
-            rho = kRhoic;
+        sDW[b] = dw;
+        if (b < ntab) {
+          if (!(n < D.nlim[b])) abits |= 1ull << b;
+          const int ph = D.phase[b];
+          double rho = kRhowa;                             // updateRhomean, classSection.f90:169-195
+          Shape sh{0., 0., 0., 0.};
+          if (ph == 4) {
+            if (n > D.nlim[b]) {
+              double mp = kRhoic * VV(b, D.iwa), mr = kRhori * VV(b, D.irim);
+              rho = fmin((mp * kRhoic + mr * kRhori) / fmax(mp + mr, 7.e-25), kRhoic);
+            } else {
+              rho = kRhoic;
+            }
+            double sdry = 0.;
+            for (int s = 0; s < D.nspec - 1; ++s) sdry += VV(b, s) * kRhoic;
+            sh = shape_coeffs(D, sdry, VV(b, D.nspec - 1) * kRhori, n);
           }
-          double sdry = 0.;
-          for (int s = 0; s < D.nspec - 1; ++s) sdry += sV[b][s] * kRhoic;
-          sh = shape_coeffs(D, sdry, sV[b][D.nspec - 1] * kRhori, n);
+          const double mass = rho * kPi6 * cubed(dw);
+          const PTerms p0 = particle_terms(cc, dw, mass, rho, ph, dw, sh);
+          // self-offset twin: diameter 1.1x, mass 1.33x (kernels.f90:273-276)
+          const PTerms p1 = particle_terms(cc, 1.1 * dw, 1.33 * mass, rho, ph, dw, sh);
+          sPR[b] = PT4<KT>{(KT)p0.D, (KT)p0.c, (KT)p0.w, (KT)p0.v};
+          sPR[ntab + b] = PT4<KT>{(KT)p1.D, (KT)p1.c, (KT)p1.w, (KT)p1.v};
         }
-        const double mass = rho * kPi6 * cubed(dw);
-        const PTerms p0 = particle_terms(cc, dw, mass, rho, ph, dw, sh);
-        // self-offset twin: diameter 1.1x, mass 1.33x (kernels.f90:273-276)
-        const PTerms p1 = particle_terms(cc, 1.1 * dw, 1.33 * mass, rho, ph, dw, sh);
-        sm.dw[slot][b] = dw;
-        sm.di[slot][0][b] = dw;
-        sm.di[slot][1][b] = 1.1 * dw;
-        sm.pr[slot][0][0][b] = (KT)p0.D; sm.pr[slot][0][1][b] = (KT)p0.c;
-        sm.pr[slot][0][2][b] = (KT)p0.w; sm.pr[slot][0][3][b] = (KT)p0.v;
-        sm.pr[slot][1][0][b] = (KT)p1.D; sm.pr[slot][1][1][b] = (KT)p1.c;
-        sm.pr[slot][1][2][b] = (KT)p1.w; sm.pr[slot][1][3][b] = (KT)p1.v;
       }
     }
+    abits = cell_or64<L>(abits);
     __syncthreads();
 
     // ---- 3. pairs (buildKernelSelf / buildKernel, kernels.f90:94-180) ----
     {
       const KT vkin = (KT)cc.vkin, cshear = (KT)cc.cshear, cinert = (KT)cc.cinert;
       const KT cfac = (KT)D.charge_factor;
-      const int np = D.npair;
-      const int R = D.off[2];
+      const int np = G.npairs, pf0 = D.pf0, pf1 = D.pf0 + D.ncc;
       for (int p = sub; p < np; p += L) {
-        const int pw = sm.pair[p];
-        const int b1 = pw & 127, b2 = (pw >> 7) & 127;
-        const int tw = pw >> 14;
+        const uint32_t pw = D.pw[p];
+        const int b1 = pw & 255u, b2 = (pw >> 8) & 255u, tw = (pw >> 16) & 1u;
         KT k = KT(0.);
         int8_t pfb = -1;
-        if (live && !(sN[b1] < D.nlim[b1]) && !(sN[b2] < D.nlim[b2])) {
+        if (live && ((abits >> b1) & (abits >> b2) & 1ull)) {
           // particle 1 is the twin of particle 2 when the bins' dmid agree within 1%
-          const int ba = tw ? b2 : b1;
-          const double dA = sm.di[slot][tw][ba], dB = sm.di[slot][0][b2];
-          k = pair_coef<KT>((KT)dA, sm.pr[slot][tw][0][ba], sm.pr[slot][tw][1][ba], sm.pr[slot][tw][2][ba],
-                            sm.pr[slot][tw][3][ba], (KT)dB, sm.pr[slot][0][0][b2], sm.pr[slot][0][1][b2],
-                            sm.pr[slot][0][2][b2], sm.pr[slot][0][3][b2], dA >= dB, vkin, cshear, cinert);
+          const double dB = sDW[b2];
+          const double dA = tw ? 1.1 * dB : sDW[b1];
+          const PT4<KT> A = sPR[tw ? ntab + b2 : b1], B = sPR[b2];
+          k = pair_coef<KT>((KT)dA, A.D, A.c, A.w, A.v, (KT)dB, B.D, B.c, B.w, B.v, dA >= dB, vkin, cshear,
+                            cinert);
           k = k * cfac;                                 // charge enhancement (:411-416)
-          if (D.cat_of[b1] == 1 && D.cat_of[b2] == 1) {
+          if (HYD && p >= pf0 && p < pf1) {
             // precip bin reached by coalescence (accumulatePrecipFormation, processes.f90:1461-1467)
-            double dnew = ::cbrt(cubed(sm.dw[slot][b1]) + cubed(sm.dw[slot][b2]));
+            double dnew = ::cbrt(cubed(sDW[b1]) + cubed(dB));
             int cnt = 0;
             for (int qq = 0; qq < D.nb[2]; ++qq) cnt += dnew > D.precpbins[qq];
             pfb = dnew > D.precpbins[0] ? (int8_t)(cnt - 1) : (int8_t)-1;
           }
         }
-        sm.K[slot][p] = k;
-        sm.pf[slot][p] = pfb;
-        (void)R;
+        sK[p] = k;
+        if (HYD && p >= pf0 && p < pf1) sPF[p - pf0] = pfb;
       }
     }
     __syncthreads();
 
     // ---- 4. sweep (coag_ice, coag_precp, coag_aero, coag_cloud) ----
-    const KT *sK = sm.K[slot];
-    const int8_t *sPF = sm.pf[slot];
     const int nspec = D.nspec, ndry = D.ndry, iwa = D.iwa, irim = D.irim;
-    const int Roff = D.off[2];
-    for (int it = 0; it < D.ntarget; ++it) {
+    const int Roff = D.off[2], pf0 = D.pf0;
+    for (int it = G.it0; it < G.it1; ++it) {
       const DevTarget &TG = D.tgt[it];
       const int t = TG.t;
       const uint32_t fl = TG.flags;
       const double Nt = sN[t];
       double Vt[NS];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) Vt[s] = sV[t][s];
+      for (int s = 0; s < NS; ++s) Vt[s] = s < nsV ? VV(t, s) : 0.;
       bool act = live && !(Nt < D.nlim[t]);
       if (fl & TF_ICE) act = act && T < 273.15;          // mo_salsa_dynamics.f90:166
       if (!__any(act)) continue;            // no cell of this wavefront updates the target
-      const double rft = (fl & TF_ICE) ? rime_frac<NS>(D, Nt, Vt, t) : 0.;
+      const double rft = (fl & TF_ICE) ? rime_frac(D, Nt, Vt, t) : 0.;
       const double emax = D.Eiagg_max, emin = D.Eiagg_min;
       const int nu = (fl & TF_ICE) ? nspec : TG.nupd;
+      const int16_t *prow = tpidx + t * ntab;
+      const uint64_t jm = G.jmask;
 
       double fix_own = 1., fix_oth = 1.;
       if (fl & TF_PF) {
@@ -462,7 +484,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           if (j < nt) {
             const uint64_t bit = 1ull << j;
             if ((TG.fix_own | TG.fix_oth) & bit) {
-              const int pid = sm.pidx[t][j];
+              const int pid = prow[j];
               const double kj = pid >= 0 ? (double)sK[pid] : 0.;
               if (TG.fix_own & bit) so += kj * sN[j];
               if (TG.fix_oth & bit) sx += kj * sN[j];
@@ -479,20 +501,20 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
 #pragma unroll
       for (int s = 0; s < NS; ++s) src[s] = 0.;
       uint32_t pmask = 0;
-      const uint64_t anymask = TG.sink | TG.sink_eagg | TG.src | TG.src_eagg | TG.src_pc_wa | TG.src_pc_ri |
-                               TG.pf_own | TG.pf_oth;
+      const uint64_t anymask = (TG.sink | TG.sink_eagg | TG.src | TG.src_eagg | TG.src_pc_wa | TG.src_pc_ri |
+                                TG.pf_own | TG.pf_oth) & jm;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int j = sub + q * L;
         if (j < nt && ((anymask >> j) & 1ull)) {
           const uint64_t bit = 1ull << j;
-          const int pid = sm.pidx[t][j];
+          const int pid = prow[j];
           const double kj = pid >= 0 ? (double)sK[pid] : 0.;
           const double Nj = sN[j];
-          const double *Vj = sV[j];
+          const double *Vj = sV + j * nsV;
           if (TG.sink & bit) sink += kj * Nj;
           if ((TG.sink_eagg | TG.src_eagg) & bit) {
-            const double e = emax - fmax(rime_frac<NS>(D, Nj, Vj, j), rft) * (emax - emin);
+            const double e = emax - fmax(rime_frac(D, Nj, Vj, j), rft) * (emax - emin);
             if (TG.sink_eagg & bit) sink += e * kj * Nj;
             if (TG.src_eagg & bit) {
 #pragma unroll
@@ -512,8 +534,8 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
             if (TG.src_pc_wa & bit) src[iwa] += kj * Vj[iwa] * kRhowa / kRhoic;
             else src[irim] += kj * Vj[iwa] * kRhowa / kRhori;
           }
-          if ((TG.pf_own | TG.pf_oth) & bit) {         // accumulatePrecipFormation (:1458-1576)
-            const int pb = sPF[pid];
+          if (HYD && ((TG.pf_own | TG.pf_oth) & bit)) {   // accumulatePrecipFormation (:1458-1576)
+            const int pb = sPF[pid - pf0];
             if (pb >= 0) {
               sink += kj * Nj;
               pmask |= 1u << pb;
@@ -538,13 +560,13 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       int pself = -1;
       double kself = 0., fixs = 1.;
       if (fl & (TF_SELF_HALF | TF_SELF_HALF_EAGG | TF_PF_SELF | TF_PRECIP_SELFCOLL)) {
-        const int pid = sm.pidx[t][t];
+        const int pid = prow[t];
         kself = pid >= 0 ? (double)sK[pid] : 0.;
         if (fl & TF_SELF_HALF) sink_self = 0.5 * kself * Nt;
         if (fl & TF_SELF_HALF_EAGG) sink_self = (emax - rft * (emax - emin)) * 0.5 * kself * Nt;
-        if (fl & TF_PF_SELF) {
+        if (HYD && (fl & TF_PF_SELF)) {
           fixs = fmax(1. - dt * (0.5 * kself * (Nt * Nt) / Nt), 0.1);
-          pself = pid >= 0 ? sPF[pid] : -1;
+          pself = pid >= 0 ? sPF[pid - pf0] : -1;
           sink_self = 0.5 * kself * Nt;
           if (pself >= 0) {
 #pragma unroll
@@ -560,32 +582,33 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
 #pragma unroll
       for (int s = 0; s < NS; ++s) Vn[s] = s < nu ? (Vt[s] + dt * src[s] * Nt) / den : Vt[s];
       double Nn = Nt / (1. + dt * (sink + sink_self));
-      if (fl & TF_PRECIP_SELFCOLL) {                  // precipSelfCoag (:1588-1611)
+      if (HYD && (fl & TF_PRECIP_SELFCOLL)) {         // precipSelfCoag (:1588-1611)
         const double nslf = 0.5 * kself * Nt * Nt;
         if (act && sub == 0) {
 #pragma unroll
           for (int s = 0; s < NS; ++s)
             if (s < TG.nupd) {
               const double vslf = Vt[s] * Nt * kself;
-              sV[t + 1][s] = sV[t + 1][s] + dt * vslf;
+              VV(t + 1, s) = VV(t + 1, s) + dt * vslf;
               Vn[s] = Vn[s] - dt * vslf;
             }
           sN[t + 1] = sN[t + 1] + dt * nslf;
         }
         Nn = Nn - dt * nslf;
       }
-      if (fl & TF_PF_SELF) {
+      if (HYD && (fl & TF_PF_SELF)) {
 #pragma unroll
         for (int s = 0; s < NS; ++s) Vn[s] = Vn[s] - dt * volsink[s];
       }
       if (act && sub == 0) {
         sN[t] = Nn;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) sV[t][s] = Vn[s];
+        for (int s = 0; s < NS; ++s)
+          if (s < nsV) VV(t, s) = Vn[s];
       }
 
       // transfer of coalesced cloud drops into precipitation bins (applyCoagPrecipFormation :649-703)
-      if (fl & (TF_PF | TF_PF_SELF)) {
+      if (HYD && (fl & (TF_PF | TF_PF_SELF))) {
         uint32_t pm = cell_or<L>(pmask);
         if (pself >= 0) pm |= 1u << pself;
         if (!act) pm = 0;
@@ -599,14 +622,14 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           for (int q = 0; q < NQ; ++q) {
             const int j = sub + q * L;
             if (j < nt && (((TG.pf_own | TG.pf_oth) >> j) & 1ull)) {
-              const int pid = sm.pidx[t][j];
-              if (sPF[pid] == pb) {
+              const int pid = prow[j];
+              if (sPF[pid - pf0] == pb) {
                 const double kj = (double)sK[pid];
                 const double fx = ((TG.pf_own >> j) & 1ull) ? fix_own : fix_oth;
                 const double Nj = sN[j];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                  if (s < TG.nupd) vp[s] += (sV[j][s] * Nt + Vt[s] * Nj) * kj * fx;
+                  if (s < TG.nupd) vp[s] += (VV(j, s) * Nt + Vt[s] * Nj) * kj * fx;
                 npp += kj * Nj * Nt * fx;
               }
             }
@@ -624,7 +647,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
             const int r = Roff + pb;
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-              if (s < TG.nupd) sV[r][s] = sV[r][s] + dt * vp[s];
+              if (s < TG.nupd) VV(r, s) = VV(r, s) + dt * vp[s];
             sN[r] = sN[r] + dt * npp;
           }
         }
@@ -640,7 +663,8 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         if (sN[b] < -1.e-6) flags |= SALSA_CELL_NEG_NUMC;
         sN[b] = fmax(0., sN[b]);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) sV[b][s] = fmax(0., sV[b][s]);
+        for (int s = 0; s < NS; ++s)
+          if (s < nsV) VV(b, s) = fmax(0., VV(b, s));
       }
     }
   } else {
@@ -650,9 +674,9 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       const int b = sub + q * L;
       if (b < nt) {
         double vs = 0.;
-        for (int s = 0; s < NS; ++s) vs += sV[b][s];
+        for (int s = 0; s < nsV; ++s) vs += VV(b, s);
         double dw = sN[b] > D.nlim[b] ? ::cbrt(vs / sN[b] / kPi6) : D.dmid[b];
-        sm.dw[slot][b] = fmin(dw, D.dlim[b]);
+        sDW[b] = fmin(dw, D.dlim[b]);
       }
     }
   }
@@ -661,23 +685,22 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
   // ---- 5. condensation (condensation/gpparth2o, mo_salsa_dynamics.f90:276-342, 691-1125) ----
   if (D.lscnd && (D.lscndh2oae || D.lscndh2ocl || D.lscndh2oic)) {
     const int iwa = D.iwa, irim = D.irim, ndry = D.ndry;
-    const int A0 = D.off[0], C0 = D.off[1], I0 = D.off[3];
+    const int A0 = D.off[0];
     const double zrh = rv / rs;
     double w1 = 0.;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int b = sub + q * L;
-      if (b < D.nb[0]) w1 += sV[b][iwa];
+      if (b < D.nb[0]) w1 += VV(b, iwa);
     }
     w1 = cell_sum<L>(w1) * kRhowa;
 
     // equilibration regime 1 (1a bins, sequential: the Kelvin factor carries
     // over between bins), mo_salsa_properties.f90:165-269
-    int isu = -1, ioc = -1, iss = -1;
+    int isu = -1, ioc = -1;
     for (int s = 0; s < ndry; ++s) {
       if (D.dry_id[s] == 0) isu = s;
       if (D.dry_id[s] == 1) ioc = s;
-      if (D.dry_id[s] == 4) iss = s;
     }
     if (sub == 0 && live) {
       double zke = 1.001;
@@ -685,9 +708,9 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         const int b = A0 + kk;
         const double n = sN[b];
         if (n > kNlim) {
-          const double v1 = isu >= 0 ? sV[b][isu] : 0., v2 = ioc >= 0 ? sV[b][ioc] : 0.;
+          const double v1 = isu >= 0 ? VV(b, isu) : 0., v2 = ioc >= 0 ? VV(b, ioc) : 0.;
           const double zcore = v1 / n + v2 / n;
-          double zdwet = sm.dw[slot][b], zdold = 1., zlwc = 0.;
+          double zdwet = sDW[b], zdold = 1., zlwc = 0.;
           const double zr = fmin(fmax(zrh, 0.05), 0.98);
           int count = 0;
           while (fabs(zdwet / zdold - 1.) > 1.e-2) {
@@ -701,7 +724,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
             zke = ::exp(2. * kSurfw0 * kMvwa / (kBoltz * T * zdwet));
             if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
           }
-          sV[b][iwa] = zlwc / kRhowa;
+          VV(b, iwa) = zlwc / kRhowa;
         }
       }
     }
@@ -715,9 +738,9 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           if (n > kNlim) {
             double zv[5] = {0., 0., 0., 0., 0.};
             for (int s = 0; s < ndry; ++s)
-              if (D.dry_id[s] < 5) zv[D.dry_id[s]] = sV[b][s];
+              if (D.dry_id[s] < 5) zv[D.dry_id[s]] = VV(b, s);
             const double zcore = zv[0] / n + zv[1] / n + zv[2] / n + zv[3] / n + zv[4] / n;
-            double zdwet = sm.dw[slot][b], zdold = 1., zlwc = 0., zke = 1.02;
+            double zdwet = sDW[b], zdold = 1., zlwc = 0., zke = 1.02;
             const double zr = fmin(fmax(zrh, 0.37), 0.98);
             int count = 0;
             while (fabs(zdwet / zdold - 1.) > 1.e-12) {
@@ -734,7 +757,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
               zke = ::exp(2. * kSurfw0 * kMvwa / (kBoltz * T * zdwet));
               if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
             }
-            sV[b][iwa] = zlwc / kRhowa;
+            VV(b, iwa) = zlwc / kRhowa;
           }
         }
       }
@@ -744,7 +767,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int b = sub + q * L;
-      if (b < D.nb[0]) w2 += sV[b][iwa];
+      if (b < D.nb[0]) w2 += VV(b, iwa);
     }
     w2 = cell_sum<L>(w2) * kRhowa;
     rv = rv - (w2 - w1) / (P * kMair / (kRg * T));
@@ -780,6 +803,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       double zcw[NQ], zmt[NQ], zkl[NQ], csu[NQ], zns[NQ], zws[NQ], org_i[NQ], org_r[NQ];
       uint32_t upd = 0;        // bit q: bin updated in the substeps
       uint32_t pred = 0;       // bit q: bin in the vapour predictor sums
+      uint32_t iceq = 0;       // bit q: ice bin
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int b = sub + q * L;
@@ -790,18 +814,20 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           const double n = sN[b];
           double vs = 0.;
 #pragma unroll
-          for (int s = 0; s < NS; ++s) vs += sV[b][s];
+          for (int s = 0; s < NS; ++s)
+            if (s < nsV) vs += VV(b, s);
           double dw = n > D.nlim[b] ? ::cbrt(vs / n / kPi6) : D.dmid[b];
           dw = fmin(dw, D.dlim[b]);                        // updateDiameter (:803-806); dnsp == dwet
           double zn = 0.;
-          for (int s = 0; s < ndry; ++s) zn += D.diss[s] * sV[b][s] * D.rholiq[s] / D.mm[s];
+          for (int s = 0; s < ndry; ++s) zn += D.diss[s] * VV(b, s) * D.rholiq[s] / D.mm[s];
           zns[q] = zn;
           if (c == 3) {
-            org_i[q] = sV[b][iwa] * kRhoic / kMwa;
-            org_r[q] = sV[b][irim] * kRhori / kMwa;
-            zcw[q] = sV[b][iwa] * kRhoic / kMwa + sV[b][irim] * kRhori / kMwa;
+            iceq |= 1u << q;
+            org_i[q] = VV(b, iwa) * kRhoic / kMwa;
+            org_r[q] = VV(b, irim) * kRhori / kMwa;
+            zcw[q] = VV(b, iwa) * kRhoic / kMwa + VV(b, irim) * kRhori / kMwa;
           } else {
-            zcw[q] = sV[b][iwa] * kRhowa / kMwa;
+            zcw[q] = VV(b, iwa) * kRhowa / kMwa;
           }
           const bool on = n > D.nlim[b] &&
                           (c == 0 ? (zrh > 0.98 && D.lscndh2oae)
@@ -817,7 +843,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
               const double h2 = kMwa * zdf * kAls * zws[q] * csu[q] / (zthc * T);
               zmt[q] = h1 / (h2 * hs + 1.);
             } else {                                        // :826-890, :932-955
-              const double zact = act_h2o<false>(sV[b][iwa] * D.rholiq[iwa] / D.mm[iwa], zn);
+              const double zact = act_h2o<false>(VV(b, iwa) * D.rholiq[iwa] / D.mm[iwa], zn);
               zkl[q] = ::exp(4. * kSurfw0 * kMwa / (kRg * T * kRhowa * dw));
               csu[q] = rs * rhoair / kMwa;
               zws[q] = zact * zkl[q];
@@ -858,12 +884,11 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           if ((upd >> q) & 1u) {
-            const int b = sub + q * L;
             const double d = adt * zmt[q] * (zcwint - zws[q] * csu[q]);
             double nw = zcw[q] + fmin(fmax(d, -1.e-1 * zcw[q]), 1.e-1 * zcw[q]);
             nw = fmax(nw, 0.);
             zcw[q] = nw;
-            zws[q] = D.cat_of[b] == 3 ? zkl[q] : act_h2o<FASTDIV>(nw, zns[q]) * zkl[q];
+            zws[q] = ((iceq >> q) & 1u) ? zkl[q] : act_h2o<FASTDIV>(nw, zns[q]) * zkl[q];
             tsum += nw;
           }
           if ((pred >> q) & 1u) s1n += zmt[q] * zws[q] * csu[q];
@@ -880,24 +905,22 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       for (int q = 0; q < NQ; ++q) {
         const int b = sub + q * L;
         if (b < nt) {
-          const int c = D.cat_of[b];
-          if (c < 3) {
-            sV[b][iwa] = fmax(0., zcw[q] * kMwa / kRhowa);
+          if (!((iceq >> q) & 1u)) {
+            VV(b, iwa) = fmax(0., zcw[q] * kMwa / kRhowa);
           } else {
             const double orig = org_i[q] + org_r[q];
             const double dvt = zcw[q] - orig;
             if (dvt > 0.) {
-              sV[b][iwa] = sV[b][iwa] + dvt * kMwa / kRhoic;
+              VV(b, iwa) = VV(b, iwa) + dvt * kMwa / kRhoic;
             } else if (dvt < 0.) {
               const double fr = (zcw[q] / orig) - 1.;
-              sV[b][iwa] = sV[b][iwa] + (fr * org_i[q]) * kMwa / kRhoic;
-              sV[b][irim] = sV[b][irim] + (fr * org_r[q]) * kMwa / kRhori;
+              VV(b, iwa) = VV(b, iwa) + (fr * org_i[q]) * kMwa / kRhoic;
+              VV(b, irim) = VV(b, irim) + (fr * org_r[q]) * kMwa / kRhori;
             }
           }
         }
       }
     }
-    (void)C0; (void)I0;
   }
   __syncthreads();
 
@@ -908,21 +931,22 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       const int b = sub + q * L;
       if (b < nt) {
         const int c = D.cat_of[b], loc = D.loc_of[b], nbc = D.nb[c];
-        const double n = sN[b];
         const int64_t in = g + a.n3 * loc;
-        const double nn = n / dn;
+        const double nn = sN[b] / dn;
         if (!isfinite(nn)) flags |= SALSA_CELL_NONFINITE;
         if (a.inplace) a.n_o[c][in] = nn;
-        else a.n_t[c][in] = a.n_t[c][in] + (nn - a.n_p[c][in]) / dt;
+        else a.n_t[c][in] = tn[q] + (nn - pn[q]) / dt;
         const int nsw = c < 3 ? D.nwet : D.nspec;
-        for (int s = 0; s < nsw; ++s) {
-          const double rho = c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
-          const int64_t im = g + a.n3 * (s * nbc + loc);
-          const double mm = sV[b][s] * rho / dn;
-          if (!isfinite(mm)) flags |= SALSA_CELL_NONFINITE;
-          if (a.inplace) a.m_o[c][im] = mm;
-          else a.m_t[c][im] = a.m_t[c][im] + (mm - a.m_p[c][im]) / dt;
-        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          if (s < nsw) {
+            const double rho = c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
+            const int64_t im = g + a.n3 * (s * nbc + loc);
+            const double mm = VV(b, s) * rho / dn;
+            if (!isfinite(mm)) flags |= SALSA_CELL_NONFINITE;
+            if (a.inplace) a.m_o[c][im] = mm;
+            else a.m_t[c][im] = tm[q][s] + (mm - pm[q][s]) / dt;
+          }
       }
     }
     if (sub == 0) {
@@ -936,42 +960,78 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
     if (a.cell_flags) a.cell_flags[icell] = (uint8_t)flags;
     if (flags && a.fault_any) atomicOr(a.fault_any, (int)flags);
   }
+#undef VV
 }
 
 // ---------------------------------------------------------------------------
-// launch wrappers
+// LDS geometry and launch wrappers
 // ---------------------------------------------------------------------------
-template <int L, class KT, int MB, int MP, int NS>
-static hipError_t launch_t(const DevLayout *d, const StepArgs &a, hipStream_t s) {
+static inline int32_t align_up(int32_t x, int32_t a) { return (x + a - 1) / a * a; }
+
+int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom &G) {
+  const int cpw = 64 / lanes;
+  G.ntab = hyd ? Lay.nt : Lay.nb[0];
+  G.npairs = hyd ? Lay.npair : Lay.npair_aa;
+  G.nsV = Lay.level == 5 ? Lay.nspec : Lay.nwet;
+  G.it0 = hyd ? 0 : Lay.it_aero0;
+  G.it1 = hyd ? Lay.ntarget : Lay.it_aero1;
+  G.jmask = hyd ? ~0ull : (Lay.nb[0] >= 64 ? ~0ull : ((1ull << Lay.nb[0]) - 1ull));
+  G.tab_bytes = align_up(2 * G.ntab * G.ntab, 16);
+  int32_t o = 0;
+  G.oN = o; o += 8 * Lay.nt;
+  G.oV = o; o += 8 * Lay.nt * G.nsV;
+  G.odw = o; o += 8 * Lay.nt;
+  o = align_up(o, 16);
+  G.opr = o; o += 2 * G.ntab * 4 * kt_bytes;
+  G.oK = o; o += G.npairs * kt_bytes;
+  G.opf = o; o += hyd ? Lay.ncc : 0;
+  o = align_up(o, 16);
+  // keep neighbouring cells of a block off the same LDS banks
+  if (o % 256 == 0) o += 16;
+  G.cell_bytes = o;
+  G.total_bytes = G.tab_bytes + cpw * G.cell_bytes;
+  return G.total_bytes <= 65536 ? 0 : -1;
+}
+
+template <int L, class KT, int MB, int NS, bool HYD>
+static hipError_t launch_t(const DevLayout *d, const StepArgs &a, const SmemGeom &G, hipStream_t s) {
   constexpr int CPW = 64 / L;
   if (a.ncells <= 0) return hipSuccess;
   const int64_t nb = (a.ncells + CPW - 1) / CPW;
-  hipLaunchKernelGGL((k_salsa_step<L, KT, MB, MP, NS>), dim3((unsigned)nb), dim3(64), 0, s, d, a);
+  hipLaunchKernelGGL((k_salsa_step<L, KT, MB, NS, HYD>), dim3((unsigned)nb), dim3(64), (size_t)G.total_bytes, s, d,
+                     a, G);
   return hipGetLastError();
 }
 
-// capacity classes: warm (level 4) and mixed-phase (level 5)
-int launch_step(int cap, int lanes, int precision, const DevLayout *d, const StepArgs &a, hipStream_t s) {
-#define SALSA_CASE(CAP, MB, MP, NS)                                                            \
-  if (cap == CAP) {                                                                            \
-    if (precision == SALSA_FP64) {                                                             \
-      if (lanes == 16) return (int)launch_t<16, double, MB, MP, NS>(d, a, s);                  \
-      if (lanes == 32) return (int)launch_t<32, double, MB, MP, NS>(d, a, s);                  \
-      if (lanes == 64) return (int)launch_t<64, double, MB, MP, NS>(d, a, s);                  \
-    } else {                                                                                   \
-      if (lanes == 16) return (int)launch_t<16, float, MB, MP, NS>(d, a, s);                   \
-      if (lanes == 32) return (int)launch_t<32, float, MB, MP, NS>(d, a, s);                   \
-      if (lanes == 64) return (int)launch_t<64, float, MB, MP, NS>(d, a, s);                   \
-    }                                                                                          \
+template <class KT, int MB, int NS, bool HYD>
+static int launch_l(int lanes, const DevLayout *d, const StepArgs &a, const SmemGeom &G, hipStream_t s) {
+  switch (lanes) {
+    case 8: return (int)launch_t<8, KT, MB, NS, HYD>(d, a, G, s);
+    case 16: return (int)launch_t<16, KT, MB, NS, HYD>(d, a, G, s);
+    case 32: return (int)launch_t<32, KT, MB, NS, HYD>(d, a, G, s);
+    case 64: return (int)launch_t<64, KT, MB, NS, HYD>(d, a, G, s);
   }
-  SALSA_CASE(4, 32, 384, 3)
-  SALSA_CASE(5, 40, 640, 4)
+  return -1;
+}
+
+// capacity classes: warm (level 4) and mixed-phase (level 5)
+int launch_step(int cap, int lanes, int precision, bool hyd, const DevLayout *d, const StepArgs &a,
+                const SmemGeom &G, hipStream_t s) {
+#define SALSA_CASE(CAP, MB, NS)                                                                   \
+  if (cap == CAP) {                                                                              \
+    if (precision == SALSA_FP64)                                                                 \
+      return hyd ? launch_l<double, MB, NS, true>(lanes, d, a, G, s)                             \
+                 : launch_l<double, MB, NS, false>(lanes, d, a, G, s);                           \
+    return hyd ? launch_l<float, MB, NS, true>(lanes, d, a, G, s)                                \
+               : launch_l<float, MB, NS, false>(lanes, d, a, G, s);                              \
+  }
+  SALSA_CASE(4, 32, 3)
+  SALSA_CASE(5, 40, 4)
 #undef SALSA_CASE
   return -1;
 }
 
 int cap_bins(int cap) { return cap == 4 ? 32 : 40; }
-int cap_pairs(int cap) { return cap == 4 ? 384 : 640; }
 int cap_spec(int cap) { return cap == 4 ? 3 : 4; }
 
 }  // namespace salsa

@@ -221,6 +221,7 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
     L.pb1[np] = (int16_t)a; L.pb2[np] = (int16_t)b;
     L.poff[np] = std::fabs(d.dmid[a] - d.dmid[b]) / d.dmid[a] < 1.e-2;   // kernels.f90:273
     L.pidx[a][b] = L.pidx[b][a] = (int16_t)np;
+    L.pw[np] = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)L.poff[np] << 16);
     ++np;
   };
   auto self_block = [&](int off, int n) { for (int m = 0; m < n; ++m) for (int k = m; k < n; ++k) add(off + m, off + k); };
@@ -230,7 +231,10 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
   const int A = L.off[0], C = L.off[1], R = L.off[2], I = L.off[3];
   const int nb = d.nbins, nc = d.ncld, nr = d.nprc, ni = L.nb[3];
   if (cgaa) self_block(A, nb);
+  L.npair_aa = np;
+  L.pf0 = np;
   if (cgcc) self_block(C, nc);
+  L.ncc = np - L.pf0;
   if (cgpp) self_block(R, nr);
   if (cgii) self_block(I, ni);
   if (cgca) cross_block(A, nb, C, nc);
@@ -290,6 +294,7 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
         if (cgpp && kk > 0) T.src |= bits(R, 0, kk - 1);
       }
     }
+    L.it_aero0 = nt;
     if (any_aero) {                // coag_aero, processes.f90:24-215
       for (int kk = d.in1a; kk <= d.fn1a; ++kk) {
         DevTarget &T = L.tgt[nt++];
@@ -354,6 +359,7 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
         }
       }
     }
+    L.it_aero1 = nt;
     if (any_cloud) {               // coag_cloud, processes.f90:221-412
       for (int series = 0; series < 2; ++series) {
         const int cur0 = series ? icb.cur : ica.cur, cur1 = series ? fcb.cur : fca.cur;
