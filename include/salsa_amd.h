@@ -153,8 +153,10 @@ void salsa_ctx_destroy(salsa_ctx *ctx);
  *           taken from salsa_params as the caller resolved them, so these two
  *           do not change the result of this build's step
  * salsa_step_host takes host pointers (copies in and out on the context
- * stream); salsa_step_device takes device pointers and only enqueues work
- * (call salsa_ctx_sync before reading results).
+ * stream); salsa_step_device takes device pointers and only enqueues work on
+ * the context stream (salsa_ctx_stream): work that produces its inputs must
+ * be complete or ordered before that stream, and results are ready after
+ * salsa_ctx_sync (or an event recorded on that stream).
  */
 int salsa_step_host(salsa_ctx *ctx, const double *press, const double *temp,
                     const double *rsl, const double *rsi, const double *dn,

@@ -170,7 +170,7 @@ def test_gpu_modes_and_shapes_agree(capi, prec):
     from salsa_amd import workload as W
     desc, f = config_cells(capi, 1, grid=(20, 6, 6))
     ref = None
-    for lanes in (16, 32, 64):
+    for lanes in (8, 16, 32, 64):
         per = []
         for flags in (0, capi.STEP_NO_COMPACT):
             n_t, m_t, rv_t, _ = run_grid(capi, desc, f, prec, flags=flags, lanes=lanes)

@@ -20,6 +20,9 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
 int launch_step(int cap, int lanes, int precision, bool hyd, const DevLayout *d, const StepArgs &a,
                 const SmemGeom &G, hipStream_t s);
 int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom &G);
+int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, double *rec, uint8_t *cls,
+                  int32_t *ids, int32_t *counts, hipStream_t s);
+int launch_scatter(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, hipStream_t s);
 int cap_bins(int cap);
 int cap_spec(int cap);
 }  // namespace salsa
@@ -46,6 +49,8 @@ struct salsa_ctx {
   int lanes[3] = {32, 32, 32};
   int32_t nzp = 0, nxp = 0, nyp = 0, nz = 0, nx = 0, ny = 0;
   int64_t n3 = 0, ncells = 0;
+  // cell records: LES values gathered per cell, step results per cell
+  double *rec_in = nullptr, *rec_out = nullptr;
   // per-cell fault flags and the global OR
   uint8_t *flags = nullptr;
   int32_t *fault = nullptr;
@@ -78,41 +83,6 @@ __global__ void k_zero_i32(int32_t *p, int n) {
   if (i < n) p[i] = 0;
 }
 
-// Cost class of a cell from its inputs: 2 = some cloud / precipitation / ice
-// bin at or above its activity limit (numc >= nlim: it takes part in the
-// collision kernels, mo_salsa_coagulation_kernels.f90:116-168, and in the
-// condensation predicate, mo_salsa_dynamics.f90:784-788); 1 = no such bin and
-// RH > 0.98 (aerosol substeps); 0 = no such bin and drier (equilibrium water
-// only).  Classes 0 and 1 run the kernel variant without hydrometeor pairs,
-// which is exact for them: coagulation only moves aerosol into aerosol bins.
-__global__ void __launch_bounds__(256) k_classify(const DevLayout *__restrict__ Dp, StepArgs a, uint8_t *cls,
-                                                  int32_t *ids, int32_t *counts) {
-  __shared__ int32_t bc[3];
-  const DevLayout &D = *Dp;
-  if (threadIdx.x < 3) bc[threadIdx.x] = 0;
-  __syncthreads();
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < a.ncells) {
-    int k = (int)(c % a.nz), r = (int)(c / a.nz);
-    int i = r % a.nx, j = r / a.nx;
-    int64_t g = (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
-    double dn = a.dn[g];
-    bool hyd = false;
-    for (int cat = 1; cat < D.ncat; ++cat) {
-      const double *n = a.n_p[cat];
-      for (int b = 0; b < D.nb[cat]; ++b) hyd |= !(n[g + a.n3 * b] * dn < D.nlim[D.off[cat] + b]);
-    }
-    double rv = a.rv_p[g], rs = a.rsl[g];
-    if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);
-    uint8_t cl = hyd ? 2 : (rv / rs > 0.98 ? 1 : 0);
-    cls[c] = cl;
-    ids[c] = (int32_t)c;
-    atomicAdd(&bc[cl], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x < 3 && bc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], bc[threadIdx.x]);
-}
-
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -137,6 +107,7 @@ static void release(salsa_ctx *c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   (void)hipFree(c->devL); (void)hipFree(c->flags); (void)hipFree(c->fault);
+  (void)hipFree(c->rec_in); (void)hipFree(c->rec_out);
   (void)hipFree(c->cls); (void)hipFree(c->cls_sorted);
   (void)hipFree(c->ids); (void)hipFree(c->ids_sorted); (void)hipFree(c->cls_count);
   (void)hipFree(c->cub_tmp);
@@ -186,6 +157,8 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
       hipMalloc(&c->cls, nc) != hipSuccess || hipMalloc(&c->cls_sorted, nc) != hipSuccess ||
       hipMalloc(&c->ids, nc * 4) != hipSuccess || hipMalloc(&c->ids_sorted, nc * 4) != hipSuccess ||
       hipMalloc(&c->cls_count, 3 * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->rec_in, nc * c->hostL.nslot * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->rec_out, nc * c->hostL.nout * sizeof(double)) != hipSuccess ||
       hipHostMalloc(&c->cls_count_h, 3 * sizeof(int32_t)) != hipSuccess) {
     set_error("device allocation failed");
     return fail(SALSA_ERR_NOMEM);
@@ -262,6 +235,8 @@ static StepArgs make_args(salsa_ctx *c, const double *press, const double *temp,
   a.inplace = (flags & SALSA_STEP_INPLACE) ? 1 : 0;
   a.lsfreeRH = c->params.lsfreeRH;
   a.rhlim = c->params.rhlim;
+  a.rec_in = c->rec_in;
+  a.rec_out = c->rec_out;
   a.cell_flags = c->flags;
   a.fault_any = c->fault;
   return a;
@@ -307,34 +282,39 @@ static int launch_class(salsa_ctx *c, int k, const StepArgs &a) {
 
 static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
-  if (flags & SALSA_STEP_NO_COMPACT) {
-    a.cell_list = nullptr;
-    if (timing_mark(c)) return SALSA_ERR_HIP;
-    int rc = launch_class(c, 2, a);
-    if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
-    return timing_mark(c);
-  }
-  // classify and sort cell ids by class (stable: k-fastest order kept within a class)
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->cls_count, 3);
-  const int nb = (int)((c->ncells + 255) / 256);
-  hipLaunchKernelGGL(k_classify, dim3(nb), dim3(256), 0, s, c->devL, a, c->cls, c->ids, c->cls_count);
-  size_t bytes = c->cub_bytes;
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, bytes, c->cls, c->cls_sorted, c->ids, c->ids_sorted,
-                                            (int)c->ncells, 0, 2, s));
-  HIPCHK(hipMemcpyAsync(c->cls_count_h, c->cls_count, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  int64_t off = 0;
+  const bool compact = !(flags & SALSA_STEP_NO_COMPACT);
   if (timing_mark(c)) return SALSA_ERR_HIP;
-  for (int k = 0; k < 3; ++k) {
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
+  if (compact) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->cls_count, 3);
+  // LES rows -> cell records (+ cost classes)
+  int rc = launch_gather(c->hostL, c->devL, a, c->rec_in, compact ? c->cls : nullptr, c->ids, c->cls_count, s);
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "gather launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  if (!compact) {
     StepArgs b = a;
-    b.cell_list = c->ids_sorted + off;
-    b.ncells = c->cls_count_h[k];
-    off += b.ncells;
-    if (b.ncells == 0) continue;
-    int rc = launch_class(c, k, b);
+    b.cell_list = nullptr;
+    rc = launch_class(c, 2, b);
     if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  } else {
+    // sort cell ids by class (stable: k-fastest order kept within a class)
+    size_t bytes = c->cub_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, bytes, c->cls, c->cls_sorted, c->ids, c->ids_sorted,
+                                              (int)c->ncells, 0, 2, s));
+    HIPCHK(hipMemcpyAsync(c->cls_count_h, c->cls_count, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int64_t off = 0;
+    for (int k = 0; k < 3; ++k) {
+      StepArgs b = a;
+      b.cell_list = c->ids_sorted + off;
+      b.ncells = c->cls_count_h[k];
+      off += b.ncells;
+      if (b.ncells == 0) continue;
+      rc = launch_class(c, k, b);
+      if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
+    }
   }
+  // cell records -> tendencies or new state
+  rc = launch_scatter(c->hostL, c->devL, a, s);
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "scatter launch failed (%d)", rc); return SALSA_ERR_HIP; }
   return timing_mark(c);
 }
 

@@ -11,6 +11,8 @@ namespace salsa {
 
 constexpr int kMaxBin = 64;           // master bins (aerosol|cloud|precip|ice)
 constexpr int kMaxSpec = 9;           // Section%volc(9)
+constexpr int kThermo = 6;            // T, p, rs, rsi, dn, rv at the head of a cell record
+constexpr int kMaxSlot = kThermo + kMaxBin * (1 + kMaxSpec);
 
 // Physical constants exactly as written in src/src_shared/mo_submctl.f90:253-338.
 constexpr double kAvog = 6.0221e+23, kBoltz = 1.3807e-23, kGrav = 9.81;
@@ -63,6 +65,16 @@ struct DevLayout {
   int32_t dry_id[kMaxSpec];           // species id (0 SO4 .. 6 NH) per dry slot
   int32_t phase[kMaxBin];
   int32_t cat_of[kMaxBin], loc_of[kMaxBin];
+  // Cell records.  The gather kernel copies each cell's LES values into one
+  // contiguous record of nslot doubles: T, p, rs, rsi, dn, rv, then per bin
+  // (master order) its number and its nsw(c) masses.  The step kernel writes
+  // nout doubles per cell (rv, then the same per-bin layout) that the scatter
+  // kernel adds to the tendencies (or stores as the new state).
+  int32_t nslot, nout;
+  int32_t sbase[kMaxBin];             // slot of bin b's number in the input record
+  int16_t slot_cat[kMaxSlot];         // category of a bin slot (-1: thermodynamic slot)
+  int32_t slot_row[kMaxSlot];         // row of the slot inside its category array (units of n3)
+  uint8_t slot_mass[kMaxSlot];        // 1: mass slot, 0: number slot
   double dmid[kMaxBin], nlim[kMaxBin], dlim[kMaxBin];
   double precpbins[kMaxBin];
   // pairs: canonical orientation (b1 is "pp1") and the self-offset predicate
@@ -115,6 +127,8 @@ struct StepArgs {
   double *n_o[4];                     // in-place outputs (== n_p cast) or null
   double *m_o[4];
   double *rv_o;
+  const double *rec_in;               // cell records [ncells_all][nslot]
+  double *rec_out;                    // step results [ncells_all][nout]
   double dt;
   int32_t nsub;                       // condensation substeps for dt
   int32_t inplace;

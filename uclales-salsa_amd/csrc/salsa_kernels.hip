@@ -315,52 +315,40 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
   int8_t *sPF = reinterpret_cast<int8_t *>(cb + G.opf);
 #define VV(b, s) sV[(b) * nsV + (s)]
 
-  int64_t g = 0;
   int32_t icell = 0;
-  if (live) {
-    icell = a.cell_list ? a.cell_list[cid] : (int32_t)cid;
-    int k = icell % a.nz, r = icell / a.nz;
-    int i = r % a.nx, j = r / a.nx;
-    g = (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
-  }
+  const double *rec = nullptr;
   double T = 280., P = 1.e5, rs = 0.01, rsi = 0.01, dn = 1., rv = 0.;
   if (live) {
-    T = a.temp[g]; P = a.press[g]; rs = a.rsl[g]; dn = a.dn[g]; rv = a.rv_p[g];
-    rsi = D.level == 5 ? a.rsi[g] : 0.;
+    icell = a.cell_list ? a.cell_list[cid] : (int32_t)cid;
+    rec = a.rec_in + (int64_t)icell * D.nslot;
+    T = rec[0]; P = rec[1]; rs = rec[2]; rsi = rec[3]; dn = rec[4]; rv = rec[5];
   }
   if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);     // mo_salsa_driver.f90:193-197
   const double rv_old = rv;
   uint32_t flags = 0;
 
-  // ---- 1. pack (mo_salsa_driver.f90:236-279); the LES values stay in
-  //      registers for the tendency, and the tendency inputs are fetched now
-  //      so that their latency hides behind the computation ----
-  double pn[NQ], pm[NQ][NS], tn[NQ], tm[NQ][NS];
+  // ---- 1. pack (mo_salsa_driver.f90:236-279) from the cell record; the LES
+  //      values stay in registers for the tendency ----
+  double pn[NQ], pm[NQ][NS];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int b = sub + q * L;
-    pn[q] = 0.; tn[q] = 0.;
+    pn[q] = 0.;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) { pm[q][s] = 0.; tm[q][s] = 0.; }
+    for (int s = 0; s < NS; ++s) pm[q][s] = 0.;
     if (b < nt) {
       double n = 0.;
       double v[NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) v[s] = 0.;
       if (live) {
-        const int c = D.cat_of[b], loc = D.loc_of[b], nbc = D.nb[c];
-        const int64_t in = g + a.n3 * loc;
+        const int c = D.cat_of[b];
         const int nsw = c < 3 ? D.nwet : D.nspec;
-        pn[q] = a.n_p[c][in];
-        if (!a.inplace) tn[q] = a.n_t[c][in];
-        const double *m = a.m_p[c];
+        const double *r = rec + D.sbase[b];
+        pn[q] = r[0];
 #pragma unroll
         for (int s = 0; s < NS; ++s)
-          if (s < nsw) {
-            const int64_t im = g + a.n3 * (s * nbc + loc);
-            pm[q][s] = m[im];
-            if (!a.inplace) tm[q][s] = a.m_t[c][im];
-          }
+          if (s < nsw) pm[q][s] = r[1 + s];
         n = pn[q] * dn;
 #pragma unroll
         for (int s = 0; s < NS; ++s)
@@ -924,35 +912,33 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
   }
   __syncthreads();
 
-  // ---- 6. unpack: tendencies or new state (mo_salsa_driver.f90:310-388) ----
+  // ---- 6. unpack (mo_salsa_driver.f90:310-388): the tendency contribution
+  //      (new - old)/dt, or the new state, into the cell's output record ----
   if (live) {
+    double *o = a.rec_out + (int64_t)icell * D.nout;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int b = sub + q * L;
       if (b < nt) {
-        const int c = D.cat_of[b], loc = D.loc_of[b], nbc = D.nb[c];
-        const int64_t in = g + a.n3 * loc;
+        const int c = D.cat_of[b];
+        double *ob = o + (D.sbase[b] - kThermo + 1);
         const double nn = sN[b] / dn;
         if (!isfinite(nn)) flags |= SALSA_CELL_NONFINITE;
-        if (a.inplace) a.n_o[c][in] = nn;
-        else a.n_t[c][in] = tn[q] + (nn - pn[q]) / dt;
+        ob[0] = a.inplace ? nn : (nn - pn[q]) / dt;
         const int nsw = c < 3 ? D.nwet : D.nspec;
 #pragma unroll
         for (int s = 0; s < NS; ++s)
           if (s < nsw) {
             const double rho = c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
-            const int64_t im = g + a.n3 * (s * nbc + loc);
             const double mm = VV(b, s) * rho / dn;
             if (!isfinite(mm)) flags |= SALSA_CELL_NONFINITE;
-            if (a.inplace) a.m_o[c][im] = mm;
-            else a.m_t[c][im] = tm[q][s] + (mm - pm[q][s]) / dt;
+            ob[1 + s] = a.inplace ? mm : (mm - pm[q][s]) / dt;
           }
       }
     }
     if (sub == 0) {
       if (!isfinite(rv)) flags |= SALSA_CELL_NONFINITE;
-      if (a.inplace) a.rv_o[g] = rv;
-      else a.rv_t[g] = a.rv_t[g] + (rv - rv_old) / dt;
+      o[0] = a.inplace ? rv : (rv - rv_old) / dt;
     }
   }
   flags = cell_or<L>(flags);
@@ -961,6 +947,142 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
     if (flags && a.fault_any) atomicOr(a.fault_any, (int)flags);
   }
 #undef VV
+}
+
+// ---------------------------------------------------------------------------
+// gather / scatter between the LES field rows and the per-cell records.
+// A 256-thread block handles a tile of kTileCells consecutive interior cells:
+// every LES row is read (written) as one contiguous run of the tile's cells,
+// and the records pass through LDS so that both sides are coalesced.  Each
+// row visit thus covers 32 cells, which keeps the address-translation misses
+// per cell low (every bin of every field is its own row, 8*nzp*nxp*nyp bytes
+// from the next).
+// ---------------------------------------------------------------------------
+constexpr int kTileCells = 32;
+
+__device__ __forceinline__ int64_t les_index(const StepArgs &a, int64_t cell) {
+  const int k = (int)(cell % a.nz);
+  const int64_t r = cell / a.nz;
+  const int i = (int)(r % a.nx), j = (int)(r / a.nx);
+  return (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
+}
+
+// LES rows -> cell records, plus the cost class of every cell: 2 = some cloud /
+// precipitation / ice bin at or above its activity limit (numc >= nlim: it
+// takes part in the collision kernels, mo_salsa_coagulation_kernels.f90:116-168,
+// and in the condensation predicate, mo_salsa_dynamics.f90:784-788); 1 = no
+// such bin and RH > 0.98 (aerosol substeps); 0 = no such bin and drier
+// (equilibrium water only).  Classes 0 and 1 run the step-kernel variant
+// without hydrometeor pairs, which is exact for them.
+__global__ void __launch_bounds__(256) k_gather(const DevLayout *__restrict__ Dp, StepArgs a, double *rec,
+                                                uint8_t *cls, int32_t *ids, int32_t *counts) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ int32_t bc[3];
+  const DevLayout &D = *Dp;
+  const int ns = D.nslot, ld = ns + 1;
+  double *tile = reinterpret_cast<double *>(smem);                 // [kTileCells][ld]
+  int64_t *gi = reinterpret_cast<int64_t *>(tile + kTileCells * ld);
+  const int64_t c0 = (int64_t)blockIdx.x * kTileCells;
+  const int nc = (int)(a.ncells - c0 < kTileCells ? a.ncells - c0 : kTileCells);
+  const int tid = threadIdx.x;
+  if (tid < kTileCells) gi[tid] = tid < nc ? les_index(a, c0 + tid) : 0;
+  if (tid < 3) bc[tid] = 0;
+  __syncthreads();
+  for (int e = tid; e < ns * kTileCells; e += 256) {
+    const int r = e / kTileCells, c = e % kTileCells;
+    if (c < nc) {
+      const int64_t g = gi[c];
+      const int cat = D.slot_cat[r];
+      double v;
+      if (cat >= 0) {
+        const double *base = D.slot_mass[r] ? a.m_p[cat] : a.n_p[cat];
+        v = base[g + a.n3 * D.slot_row[r]];
+      } else {
+        const double *f = r == 0 ? a.temp : r == 1 ? a.press : r == 2 ? a.rsl
+                        : r == 3 ? (D.level == 5 ? a.rsi : nullptr) : r == 4 ? a.dn : a.rv_p;
+        v = f ? f[g] : 0.;
+      }
+      tile[c * ld + r] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nc * ns; e += 256) {
+    const int c = e / ns, r = e % ns;
+    rec[(c0 + c) * ns + r] = tile[c * ld + r];
+  }
+  if (cls && tid < nc) {
+    const double *t = tile + tid * ld;
+    const double dn = t[4];
+    bool hyd = false;
+    for (int b = D.off[1]; b < D.nt; ++b) hyd |= !(t[D.sbase[b]] * dn < D.nlim[b]);
+    double rv = t[5];
+    const double rs = t[2];
+    if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);
+    const int cl = hyd ? 2 : (rv / rs > 0.98 ? 1 : 0);
+    cls[c0 + tid] = (uint8_t)cl;
+    ids[c0 + tid] = (int32_t)(c0 + tid);
+    atomicAdd(&bc[cl], 1);
+  }
+  __syncthreads();
+  if (cls && tid < 3 && bc[tid]) atomicAdd(&counts[tid], bc[tid]);
+}
+
+// output records -> tendencies (+=) or, in place, the new state
+__global__ void __launch_bounds__(256) k_scatter(const DevLayout *__restrict__ Dp, StepArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  const DevLayout &D = *Dp;
+  const int no = D.nout, ld = no + 1;
+  double *tile = reinterpret_cast<double *>(smem);
+  int64_t *gi = reinterpret_cast<int64_t *>(tile + kTileCells * ld);
+  const int64_t c0 = (int64_t)blockIdx.x * kTileCells;
+  const int nc = (int)(a.ncells - c0 < kTileCells ? a.ncells - c0 : kTileCells);
+  const int tid = threadIdx.x;
+  if (tid < kTileCells) gi[tid] = tid < nc ? les_index(a, c0 + tid) : 0;
+  for (int e = tid; e < nc * no; e += 256) {
+    const int c = e / no, r = e % no;
+    tile[c * ld + r] = a.rec_out[(c0 + c) * no + r];
+  }
+  __syncthreads();
+  for (int e = tid; e < no * kTileCells; e += 256) {
+    const int r = e / kTileCells, c = e % kTileCells;
+    if (c < nc) {
+      const int64_t g = gi[c];
+      const double x = tile[c * ld + r];
+      if (r == 0) {
+        if (a.inplace) a.rv_o[g] = x;
+        else a.rv_t[g] = a.rv_t[g] + x;
+      } else {
+        const int sr = r - 1 + kThermo;
+        const int cat = D.slot_cat[sr];
+        const int64_t i = g + a.n3 * D.slot_row[sr];
+        if (a.inplace) {
+          double *dst = D.slot_mass[sr] ? a.m_o[cat] : a.n_o[cat];
+          dst[i] = x;
+        } else {
+          double *dst = D.slot_mass[sr] ? a.m_t[cat] : a.n_t[cat];
+          dst[i] = dst[i] + x;
+        }
+      }
+    }
+  }
+}
+
+size_t record_tile_lds(const DevLayout &Lay) {
+  const int n = Lay.nslot > Lay.nout ? Lay.nslot : Lay.nout;
+  return (size_t)kTileCells * (n + 1) * sizeof(double) + kTileCells * sizeof(int64_t);
+}
+
+int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, double *rec, uint8_t *cls,
+                  int32_t *ids, int32_t *counts, hipStream_t s) {
+  const unsigned nb = (unsigned)((a.ncells + kTileCells - 1) / kTileCells);
+  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), record_tile_lds(Lay), s, d, a, rec, cls, ids, counts);
+  return (int)hipGetLastError();
+}
+
+int launch_scatter(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, hipStream_t s) {
+  const unsigned nb = (unsigned)((a.ncells + kTileCells - 1) / kTileCells);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), record_tile_lds(Lay), s, d, a);
+  return (int)hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

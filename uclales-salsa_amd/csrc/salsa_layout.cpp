@@ -198,6 +198,22 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
     L.cat_of[b] = c; L.loc_of[b] = b - L.off[c];
   }
   for (int k = 0; k < d.nprc; ++k) L.precpbins[k] = d.precpbins[k];
+  // cell-record slots
+  {
+    int sl = kThermo;
+    for (int r = 0; r < kThermo; ++r) { L.slot_cat[r] = -1; L.slot_row[r] = 0; L.slot_mass[r] = 0; }
+    for (int b = 0; b < L.nt; ++b) {
+      const int c = L.cat_of[b], loc = L.loc_of[b], nbc = L.nb[c];
+      const int nsw = c < 3 ? L.nwet : L.nspec;
+      L.sbase[b] = sl;
+      L.slot_cat[sl] = (int16_t)c; L.slot_row[sl] = loc; L.slot_mass[sl] = 0; ++sl;
+      for (int q = 0; q < nsw; ++q) {
+        L.slot_cat[sl] = (int16_t)c; L.slot_row[sl] = q * nbc + loc; L.slot_mass[sl] = 1; ++sl;
+      }
+    }
+    L.nslot = sl;
+    L.nout = sl - kThermo + 1;
+  }
 
   L.lscoag = p.lscoag; L.lscnd = p.lscnd;
   L.lscndh2oae = p.lscndh2oae; L.lscndh2ocl = p.lscndh2ocl;

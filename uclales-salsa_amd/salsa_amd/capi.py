@@ -251,11 +251,24 @@ class Context:
         """run_SALSA on host numpy arrays (staged through device memory)."""
         return _check(self._call(lib().salsa_step_host, f, n_p, m_p, n_t, m_t, rv_t, tstep, time, istp, flags))
 
+    def _ordered(self, enqueue):
+        """Run `enqueue` with the context stream ordered after the work already
+        queued on torch's current stream, and torch's current stream ordered
+        after the enqueued step (torch and the library share one HIP runtime,
+        so the two streams can wait on each other's events)."""
+        import torch
+        cur = torch.cuda.current_stream()
+        ext = torch.cuda.ExternalStream(self.stream)
+        ext.wait_stream(cur)
+        rc = enqueue()
+        cur.wait_stream(ext)
+        return rc
+
     def step_device(self, f, n_p, m_p, n_t=None, m_t=None, rv_t=None, tstep=1.0, time=10.0, istp=2,
                     flags=STEP_TENDENCY):
         """run_SALSA on device-resident torch tensors; enqueues only (see sync())."""
-        return _check(self._call(lib().salsa_step_device, f, n_p, m_p, n_t, m_t, rv_t, tstep, time, istp,
-                                 flags))
+        return _check(self._ordered(lambda: self._call(lib().salsa_step_device, f, n_p, m_p, n_t, m_t, rv_t,
+                                                       tstep, time, istp, flags)))
 
     def sync(self):
         return _check(lib().salsa_ctx_sync(self._h))
@@ -284,6 +297,6 @@ class Context:
 
     def autotune(self, f, n_p, m_p, tstep=1.0):
         P = lambda xs: _P4(*[_ptr(x) for x in (list(xs) + [None] * (4 - len(xs)))])
-        return _check(lib().salsa_autotune(self._h, _ptr(f["press"]), _ptr(f["temp"]), _ptr(f["rsl"]),
-                                           _ptr(f.get("rsi")), _ptr(f["dn"]), _ptr(f["rv"]), P(n_p), P(m_p),
-                                           tstep))
+        return _check(self._ordered(lambda: lib().salsa_autotune(
+            self._h, _ptr(f["press"]), _ptr(f["temp"]), _ptr(f["rsl"]), _ptr(f.get("rsi")), _ptr(f["dn"]),
+            _ptr(f["rv"]), P(n_p), P(m_p), tstep)))
