@@ -36,6 +36,20 @@ enum TargetFlags : uint32_t {
   TF_ICE = 1u << 5               // applyCoagIce update (dry, rime, water)
 };
 
+// Roles of a collector/source bin j in the sweep entry of a target t.
+enum EntryRole : uint32_t {
+  R_SINK = 1u << 0,       // sink += K(t,j) N_j
+  R_SINK_EAGG = 1u << 1,  // sink += Eagg K N_j
+  R_SRC = 1u << 2,        // src[s] += K V_j[s], s < nupd
+  R_SRC_EAGG = 1u << 3,   // src[s] += Eagg K V_j[s], all species
+  R_PC_WA = 1u << 4,      // phase change into the ice water slot
+  R_PC_RI = 1u << 5,      // phase change into the rime slot
+  R_PF_OWN = 1u << 6,     // precipitation formation, own cloud series
+  R_PF_OTH = 1u << 7,     // ... other series
+  R_FIX_OWN = 1u << 8,    // fix_coag sums
+  R_FIX_OTH = 1u << 9
+};
+
 // One target bin of the sequential coagulation sweep; the sweep visits the
 // targets in the reference order ice -> precip -> aerosol -> cloud
 // (mo_salsa_dynamics.f90:166-183), ascending bins within each category.
@@ -53,6 +67,9 @@ struct DevTarget {
   uint64_t src_pc_ri;    // phase change into rime slot (cloud/precip collected by ice)
   uint64_t pf_own, fix_own;   // precip formation from the own cloud series, fix_coag range
   uint64_t pf_oth, fix_oth;   // ... from the other series
+  // the bins named by the masks above as a list of entries
+  // j | role << 6 | pair id << 16, aerosol bins first
+  int32_t ent0, nent, nent_aero;
 };
 
 struct DevLayout {
@@ -89,12 +106,15 @@ struct DevLayout {
   int32_t ntarget;
   int32_t it_aero0, it_aero1;         // sweep entries of the aerosol targets [it_aero0, it_aero1)
   DevTarget tgt[kMaxBin];
+  uint32_t ent[kMaxBin * kMaxBin];    // sweep entries of all targets
+  int32_t nent_total;
   // switches and constants
   int32_t level, lscoag, lscnd, lscndh2oae, lscndh2ocl, lscndh2oic;
   int32_t any_ice_coag;               // any of lscgii/lscgia/lscgic/lscgip
   double eddy_dis, Eiagg_max, Eiagg_min;
   double ice_alpha, ice_beta, ice_gamma, ice_sigma;
   double charge_factor;               // 1 + chargeCollEnh (lcharge) or 1
+  double eddy_075;                    // eddy_dis**0.75 (coagc :391)
 };
 
 // LDS geometry of one launch (built on the host for the layout, the kernel

@@ -47,6 +47,10 @@ template <> struct Mth<float> {
     return x > 0.f ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.f / 3.f)) : 0.f;
   }
   static __device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float exp_(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+  static __device__ __forceinline__ float pow_(float x, float y) {
+    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * y);
+  }
 };
 
 // Sum / OR over the L lanes of one cell; every lane receives the same bits.
@@ -92,6 +96,24 @@ __device__ __forceinline__ uint32_t cell_or(uint32_t x) {
 }
 
 __device__ __forceinline__ double cubed(double x) { return x * x * x; }
+
+// FP64 quotient from the hardware reciprocal refined by two Newton steps
+// (within about one unit in the last place of the correctly rounded quotient).
+__device__ __forceinline__ double rcp_nr(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = r * (2. - b * r);
+  return r * (2. - b * r);
+}
+__device__ __forceinline__ double div_nr(double a, double b) { return a * rcp_nr(b); }
+
+// x**(1/3) for positive x: an FP32 seed and one Halley step in FP64, which
+// converges cubically from ~1e-7 to the FP64 rounding level.
+__device__ __forceinline__ double cbrt_fast(double x) {
+  if (!(x > 1.e-36 && x < 1.e36)) return ::cbrt(x);
+  const double y0 = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)x) * (1.f / 3.f));
+  const double y3 = y0 * y0 * y0;
+  return y0 * div_nr(y3 + 2. * x, 2. * y3 + x);
+}
 
 // ---------------------------------------------------------------------------
 // particle terms
@@ -167,7 +189,7 @@ __device__ CellC cell_consts(const DevLayout &D, double T, double P) {
   c.zrhoa = P / (kRd * T);
   c.vkin = c.visc / c.zrhoa;
   c.cshear = ::sqrt(8. * kPi * D.eddy_dis / (15. * c.vkin));
-  c.cinert = kPi * ::pow(D.eddy_dis, 0.75) / (kGrav * ::sqrt(::sqrt(c.vkin)));
+  c.cinert = kPi * D.eddy_075 / (kGrav * ::sqrt(::sqrt(c.vkin)));
   return c;
 }
 
@@ -184,6 +206,32 @@ __device__ PTerms particle_terms(const CellC &cc, double diam, double mass, doub
   r.w = 8. * r.D / (kPi * r.c);
   r.v = phase < 4 ? terminal_vel_liquid(diam, rhop, cc.zrhoa, cc.visc, beta)
                   : terminal_vel_ice(dsph, rhop, cc.zrhoa, cc.visc, sh, diam);
+  return r;
+}
+
+// The same terms in FP32 for a liquid particle (mixed path: they only feed the
+// FP32 kernel coefficients).  The terminal-velocity regime is chosen on the
+// FP64 diameter so that both precision paths take the same branch.
+struct PTermsF { float D, c, w, v; };
+__device__ PTermsF particle_terms_liquid_f(const CellC &cc, double diam_d, double mass_d, double rhop_d) {
+  using M = Mth<float>;
+  const float diam = (float)diam_d, mass = (float)mass_d, rhop = (float)rhop_d;
+  const float T = (float)cc.T, mfp = (float)cc.mfp, visc = (float)cc.visc, rhoa = (float)cc.zrhoa;
+  const float pi = (float)kPi, g = (float)kGrav, kb = (float)kBoltz;
+  PTermsF r;
+  const float knud = M::div_(2.f * mfp, diam);
+  const float beta = 1.f + knud * (1.142f + 0.558f * M::exp_(M::div_(-0.999f, knud)));
+  r.D = M::div_(beta * kb * T, 3.f * pi * visc * diam);
+  r.c = M::sqrt_(M::div_(8.f * kb * T, pi * mass));
+  r.w = M::div_(8.f * r.D, pi * r.c);
+  // terminal_vel, Beard (1976), mo_particle_external_properties.f90:61-90: the
+  // Stokes regime in FP32; the two polynomial regimes (drops above 19 um) in
+  // FP64, since their degree-6 polynomials in log(Davies/Bond number) cancel
+  // too much for FP32 at the 1e-5 output tolerance
+  if (diam_d <= 19.e-6)
+    r.v = M::div_((diam * diam) * (rhop - rhoa) * g * beta, 18.f * visc);
+  else
+    r.v = (float)terminal_vel_liquid(diam_d, rhop_d, cc.zrhoa, cc.visc, (double)beta);
   return r;
 }
 
@@ -273,6 +321,29 @@ __device__ __forceinline__ double act_h2o(double znw, double zns) {
   return fmax(0.1, q);
 }
 
+// Particle terms of an ice bin and of its self-offset twin (FP64), written to
+// LDS; updateRhomean (classSection.f90:169-195) and getShapeCoefficients
+// (mo_ice_shape.f90:145-191).
+template <class KT>
+__device__ __forceinline__ void ice_bin_terms(const DevLayout &D, const CellC &cc, double n, double nlim, const double *v,
+                                           double dw, PT4<KT> *dst, PT4<KT> *dst_twin) {
+  double rho;
+  if (n > nlim) {
+    const double mp = kRhoic * v[D.iwa], mr = kRhori * v[D.irim];
+    rho = fmin((mp * kRhoic + mr * kRhori) / fmax(mp + mr, 7.e-25), kRhoic);
+  } else {
+    rho = kRhoic;
+  }
+  double sdry = 0.;
+  for (int s = 0; s < D.nspec - 1; ++s) sdry += v[s] * kRhoic;
+  const Shape sh = shape_coeffs(D, sdry, v[D.nspec - 1] * kRhori, n);
+  const double mass = rho * kPi6 * cubed(dw);
+  const PTerms p0 = particle_terms(cc, dw, mass, rho, 4, dw, sh);
+  const PTerms p1 = particle_terms(cc, 1.1 * dw, 1.33 * mass, rho, 4, dw, sh);
+  *dst = PT4<KT>{(KT)p0.D, (KT)p0.c, (KT)p0.w, (KT)p0.v};
+  *dst_twin = PT4<KT>{(KT)p1.D, (KT)p1.c, (KT)p1.w, (KT)p1.v};
+}
+
 template <int L>
 __device__ __forceinline__ uint64_t cell_or64(uint64_t x) {
   return (uint64_t)cell_or<L>((uint32_t)x) | ((uint64_t)cell_or<L>((uint32_t)(x >> 32)) << 32);
@@ -288,9 +359,9 @@ __device__ __forceinline__ uint64_t cell_or64(uint64_t x) {
 // condensation stage still visits every bin.
 // ---------------------------------------------------------------------------
 template <int L, class KT, int MB, int NS, bool HYD>
-__global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   constexpr int CPW = 64 / L, NQ = (MB + L - 1) / L;
-  constexpr bool FASTDIV = sizeof(KT) == 4;
   extern __shared__ __align__(16) char smem[];
   const DevLayout &D = *Dp;
   const int lane = threadIdx.x, slot = lane / L, sub = lane % L;
@@ -305,7 +376,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
   const int nt = D.nt, nsV = G.nsV, ntab = G.ntab;
   const double dt = a.dt;
 
-  int16_t *tpidx = reinterpret_cast<int16_t *>(smem);
+  uint32_t *tent = reinterpret_cast<uint32_t *>(smem);       // sweep entries of the layout
   char *cb = smem + G.tab_bytes + slot * G.cell_bytes;
   double *sN = reinterpret_cast<double *>(cb + G.oN);
   double *sV = reinterpret_cast<double *>(cb + G.oV);       // [nt][nsV]
@@ -324,18 +395,12 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
     T = rec[0]; P = rec[1]; rs = rec[2]; rsi = rec[3]; dn = rec[4]; rv = rec[5];
   }
   if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);     // mo_salsa_driver.f90:193-197
-  const double rv_old = rv;
   uint32_t flags = 0;
 
-  // ---- 1. pack (mo_salsa_driver.f90:236-279) from the cell record; the LES
-  //      values stay in registers for the tendency ----
-  double pn[NQ], pm[NQ][NS];
+  // ---- 1. pack (mo_salsa_driver.f90:236-279) from the cell record ----
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int b = sub + q * L;
-    pn[q] = 0.;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) pm[q][s] = 0.;
     if (b < nt) {
       double n = 0.;
       double v[NS];
@@ -345,16 +410,12 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         const int c = D.cat_of[b];
         const int nsw = c < 3 ? D.nwet : D.nspec;
         const double *r = rec + D.sbase[b];
-        pn[q] = r[0];
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-          if (s < nsw) pm[q][s] = r[1 + s];
-        n = pn[q] * dn;
+        n = r[0] * dn;
 #pragma unroll
         for (int s = 0; s < NS; ++s)
           if (s < nsw) {
             const double rho = c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
-            v[s] = pm[q][s] * dn / rho;
+            v[s] = r[1 + s] * dn / rho;
           }
       }
       sN[b] = n;
@@ -363,14 +424,14 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         if (s < nsV) VV(b, s) = v[s];
     }
   }
-  for (int e = lane; e < ntab * ntab; e += 64) tpidx[e] = D.pidx[e / ntab][e % ntab];
+  for (int e = lane; e < D.nent_total; e += 64) tent[e] = D.ent[e];
   __syncthreads();
 
   // ---- 2./3./4. coagulation (mo_salsa_dynamics.f90:92-202) ----
   if (D.lscoag) {
     const CellC cc = cell_consts(D, T, P);
     uint64_t abits = 0;      // bins taking part in kernels: numc >= nlim (kernels.f90:116-168)
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < NQ; ++q) {
       const int b = sub + q * L;
       if (b < nt) {
@@ -380,31 +441,31 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         for (int s = 0; s < NS; ++s)
           if (s < nsV) vs += VV(b, s);
         // updateDiameter(limit, "all"): dnsp ends equal to the limited dwet (classSection.f90:147-156)
-        double dw = n > D.nlim[b] ? ::cbrt(vs / n / kPi6) : D.dmid[b];
+        double dw = n > D.nlim[b] ? cbrt_fast(vs / n / kPi6) : D.dmid[b];
         dw = fmin(dw, D.dlim[b]);
         sDW[b] = dw;
         if (b < ntab) {
           if (!(n < D.nlim[b])) abits |= 1ull << b;
           const int ph = D.phase[b];
-          double rho = kRhowa;                             // updateRhomean, classSection.f90:169-195
-          Shape sh{0., 0., 0., 0.};
           if (ph == 4) {
-            if (n > D.nlim[b]) {
-              double mp = kRhoic * VV(b, D.iwa), mr = kRhori * VV(b, D.irim);
-              rho = fmin((mp * kRhoic + mr * kRhori) / fmax(mp + mr, 7.e-25), kRhoic);
+            ice_bin_terms<KT>(D, cc, n, D.nlim[b], sV + b * nsV, dw, sPR + b, sPR + ntab + b);
+          } else {
+            const double rho = kRhowa;                       // updateRhomean, classSection.f90:169-195
+            const double mass = rho * kPi6 * cubed(dw);
+            // self-offset twin: diameter 1.1x, mass 1.33x (kernels.f90:273-276)
+            if (sizeof(KT) == 4) {
+              const PTermsF p0 = particle_terms_liquid_f(cc, dw, mass, rho);
+              const PTermsF p1 = particle_terms_liquid_f(cc, 1.1 * dw, 1.33 * mass, rho);
+              sPR[b] = PT4<KT>{(KT)p0.D, (KT)p0.c, (KT)p0.w, (KT)p0.v};
+              sPR[ntab + b] = PT4<KT>{(KT)p1.D, (KT)p1.c, (KT)p1.w, (KT)p1.v};
             } else {
-              rho = kRhoic;
+              const Shape sh{0., 0., 0., 0.};
+              const PTerms p0 = particle_terms(cc, dw, mass, rho, ph, dw, sh);
+              const PTerms p1 = particle_terms(cc, 1.1 * dw, 1.33 * mass, rho, ph, dw, sh);
+              sPR[b] = PT4<KT>{(KT)p0.D, (KT)p0.c, (KT)p0.w, (KT)p0.v};
+              sPR[ntab + b] = PT4<KT>{(KT)p1.D, (KT)p1.c, (KT)p1.w, (KT)p1.v};
             }
-            double sdry = 0.;
-            for (int s = 0; s < D.nspec - 1; ++s) sdry += VV(b, s) * kRhoic;
-            sh = shape_coeffs(D, sdry, VV(b, D.nspec - 1) * kRhori, n);
           }
-          const double mass = rho * kPi6 * cubed(dw);
-          const PTerms p0 = particle_terms(cc, dw, mass, rho, ph, dw, sh);
-          // self-offset twin: diameter 1.1x, mass 1.33x (kernels.f90:273-276)
-          const PTerms p1 = particle_terms(cc, 1.1 * dw, 1.33 * mass, rho, ph, dw, sh);
-          sPR[b] = PT4<KT>{(KT)p0.D, (KT)p0.c, (KT)p0.w, (KT)p0.v};
-          sPR[ntab + b] = PT4<KT>{(KT)p1.D, (KT)p1.c, (KT)p1.w, (KT)p1.v};
         }
       }
     }
@@ -431,7 +492,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           k = k * cfac;                                 // charge enhancement (:411-416)
           if (HYD && p >= pf0 && p < pf1) {
             // precip bin reached by coalescence (accumulatePrecipFormation, processes.f90:1461-1467)
-            double dnew = ::cbrt(cubed(sDW[b1]) + cubed(dB));
+            double dnew = cbrt_fast(cubed(sDW[b1]) + cubed(dB));
             int cnt = 0;
             for (int qq = 0; qq < D.nb[2]; ++qq) cnt += dnew > D.precpbins[qq];
             pfb = dnew > D.precpbins[0] ? (int8_t)(cnt - 1) : (int8_t)-1;
@@ -444,6 +505,9 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
     __syncthreads();
 
     // ---- 4. sweep (coag_ice, coag_precp, coag_aero, coag_cloud) ----
+    // Each target visits only its entry list (the bins its masks name, with
+    // their roles); the lanes of a cell split the entries and the partial
+    // sums are reduced across them.
     const int nspec = D.nspec, ndry = D.ndry, iwa = D.iwa, irim = D.irim;
     const int Roff = D.off[2], pf0 = D.pf0;
     for (int it = G.it0; it < G.it1; ++it) {
@@ -460,69 +524,59 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       const double rft = (fl & TF_ICE) ? rime_frac(D, Nt, Vt, t) : 0.;
       const double emax = D.Eiagg_max, emin = D.Eiagg_min;
       const int nu = (fl & TF_ICE) ? nspec : TG.nupd;
-      const int16_t *prow = tpidx + t * ntab;
-      const uint64_t jm = G.jmask;
+      const int nupd = TG.nupd;
+      const uint32_t *ent = tent + TG.ent0;
+      const int ne = HYD ? TG.nent : TG.nent_aero;
 
-      double fix_own = 1., fix_oth = 1.;
-      if (fl & TF_PF) {
-        double so = 0., sx = 0.;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int j = sub + q * L;
-          if (j < nt) {
-            const uint64_t bit = 1ull << j;
-            if ((TG.fix_own | TG.fix_oth) & bit) {
-              const int pid = prow[j];
-              const double kj = pid >= 0 ? (double)sK[pid] : 0.;
-              if (TG.fix_own & bit) so += kj * sN[j];
-              if (TG.fix_oth & bit) sx += kj * sN[j];
-            }
-          }
-        }
-        so = cell_sum<L>(so);
-        sx = cell_sum<L>(sx);
-        fix_own = fmax(1. - dt * so, 0.1);
-        fix_oth = fmax(1. - dt * sx, 0.1);
-      }
-
-      double sink = 0., src[NS];
+      double sink = 0., so = 0., sx = 0., src[NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) src[s] = 0.;
       uint32_t pmask = 0;
-      const uint64_t anymask = (TG.sink | TG.sink_eagg | TG.src | TG.src_eagg | TG.src_pc_wa | TG.src_pc_ri |
-                                TG.pf_own | TG.pf_oth) & jm;
+      if (!HYD || !(fl & (TF_ICE | TF_PF))) {
+        // plain collection (accumulateSink / accumulateSource, :707-1210)
+        for (int e = sub; e < ne; e += L) {
+          const uint32_t w = ent[e];
+          const int j = w & 63u, pid = (int)(w >> 16);
+          const uint32_t role = (w >> 6) & 1023u;
+          const double kj = (double)sK[pid];
+          const double ks = (role & R_SINK) ? kj : 0., kv = (role & R_SRC) ? kj : 0.;
+          const double *Vj = sV + j * nsV;
+          sink += ks * sN[j];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int j = sub + q * L;
-        if (j < nt && ((anymask >> j) & 1ull)) {
-          const uint64_t bit = 1ull << j;
-          const int pid = prow[j];
-          const double kj = pid >= 0 ? (double)sK[pid] : 0.;
+          for (int s = 0; s < NS; ++s)
+            if (s < nupd) src[s] += kv * Vj[s];
+        }
+      } else {
+        for (int e = sub; e < ne; e += L) {
+          const uint32_t w = ent[e];
+          const int j = w & 63u, pid = (int)(w >> 16);
+          const uint32_t role = (w >> 6) & 1023u;
+          const double kj = (double)sK[pid];
           const double Nj = sN[j];
           const double *Vj = sV + j * nsV;
-          if (TG.sink & bit) sink += kj * Nj;
-          if ((TG.sink_eagg | TG.src_eagg) & bit) {
-            const double e = emax - fmax(rime_frac(D, Nj, Vj, j), rft) * (emax - emin);
-            if (TG.sink_eagg & bit) sink += e * kj * Nj;
-            if (TG.src_eagg & bit) {
+          if (role & R_SINK) sink += kj * Nj;
+          if (role & (R_SINK_EAGG | R_SRC_EAGG)) {      // ice aggregation efficiency (:758-769)
+            const double eg = emax - fmax(rime_frac(D, Nj, Vj, j), rft) * (emax - emin);
+            if (role & R_SINK_EAGG) sink += eg * kj * Nj;
+            if (role & R_SRC_EAGG) {
 #pragma unroll
               for (int s = 0; s < NS; ++s)
-                if (s < nspec) src[s] += e * kj * Vj[s];
+                if (s < nspec) src[s] += eg * kj * Vj[s];
             }
           }
-          if (TG.src & bit) {
+          if (role & R_SRC) {
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-              if (s < TG.nupd) src[s] += kj * Vj[s];
+              if (s < nupd) src[s] += kj * Vj[s];
           }
-          if ((TG.src_pc_wa | TG.src_pc_ri) & bit) {   // accumulateSourcePhaseChange (:1236-1266)
+          if (role & (R_PC_WA | R_PC_RI)) {              // accumulateSourcePhaseChange (:1236-1266)
 #pragma unroll
             for (int s = 0; s < NS; ++s)
               if (s < ndry) src[s] += kj * Vj[s];
-            if (TG.src_pc_wa & bit) src[iwa] += kj * Vj[iwa] * kRhowa / kRhoic;
+            if (role & R_PC_WA) src[iwa] += kj * Vj[iwa] * kRhowa / kRhoic;
             else src[irim] += kj * Vj[iwa] * kRhowa / kRhori;
           }
-          if (HYD && ((TG.pf_own | TG.pf_oth) & bit)) {   // accumulatePrecipFormation (:1458-1576)
+          if (role & (R_PF_OWN | R_PF_OTH)) {            // accumulatePrecipFormation (:1458-1576)
             const int pb = sPF[pid - pf0];
             if (pb >= 0) {
               sink += kj * Nj;
@@ -530,15 +584,24 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
             } else {
 #pragma unroll
               for (int s = 0; s < NS; ++s)
-                if (s < TG.nupd) src[s] += kj * Vj[s];
+                if (s < nupd) src[s] += kj * Vj[s];
             }
           }
+          if (role & R_FIX_OWN) so += kj * Nj;
+          if (role & R_FIX_OTH) sx += kj * Nj;
         }
       }
       sink = cell_sum<L>(sink);
 #pragma unroll
       for (int s = 0; s < NS; ++s)
         if (s < nu) src[s] = cell_sum<L>(src[s]);
+      double fix_own = 1., fix_oth = 1.;
+      if (HYD && (fl & TF_PF)) {                          // fix_coag (:1440-1456)
+        so = cell_sum<L>(so);
+        sx = cell_sum<L>(sx);
+        fix_own = fmax(1. - dt * so, 0.1);
+        fix_oth = fmax(1. - dt * sx, 0.1);
+      }
 
       // self terms (computed identically by every lane of the cell)
       double sink_self = 0.;
@@ -548,7 +611,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       int pself = -1;
       double kself = 0., fixs = 1.;
       if (fl & (TF_SELF_HALF | TF_SELF_HALF_EAGG | TF_PF_SELF | TF_PRECIP_SELFCOLL)) {
-        const int pid = prow[t];
+        const int pid = D.pidx[t][t];
         kself = pid >= 0 ? (double)sK[pid] : 0.;
         if (fl & TF_SELF_HALF) sink_self = 0.5 * kself * Nt;
         if (fl & TF_SELF_HALF_EAGG) sink_self = (emax - rft * (emax - emin)) * 0.5 * kself * Nt;
@@ -559,7 +622,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           if (pself >= 0) {
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-              if (s < TG.nupd) volsink[s] = Vt[s] * Nt * kself * fixs;
+              if (s < nupd) volsink[s] = Vt[s] * Nt * kself * fixs;
           }
         }
       }
@@ -575,7 +638,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         if (act && sub == 0) {
 #pragma unroll
           for (int s = 0; s < NS; ++s)
-            if (s < TG.nupd) {
+            if (s < nupd) {
               const double vslf = Vt[s] * Nt * kself;
               VV(t + 1, s) = VV(t + 1, s) + dt * vslf;
               Vn[s] = Vn[s] - dt * vslf;
@@ -606,26 +669,25 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           double vp[NS], npp = 0.;
 #pragma unroll
           for (int s = 0; s < NS; ++s) vp[s] = 0.;
+          for (int e = sub; e < ne; e += L) {
+            const uint32_t w = ent[e];
+            const uint32_t role = (w >> 6) & 1023u;
+            if (!(role & (R_PF_OWN | R_PF_OTH))) continue;
+            const int j = w & 63u, pid = (int)(w >> 16);
+            if (sPF[pid - pf0] == pb) {
+              const double kj = (double)sK[pid];
+              const double fx = (role & R_PF_OWN) ? fix_own : fix_oth;
+              const double Nj = sN[j];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const int j = sub + q * L;
-            if (j < nt && (((TG.pf_own | TG.pf_oth) >> j) & 1ull)) {
-              const int pid = prow[j];
-              if (sPF[pid - pf0] == pb) {
-                const double kj = (double)sK[pid];
-                const double fx = ((TG.pf_own >> j) & 1ull) ? fix_own : fix_oth;
-                const double Nj = sN[j];
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                  if (s < TG.nupd) vp[s] += (VV(j, s) * Nt + Vt[s] * Nj) * kj * fx;
-                npp += kj * Nj * Nt * fx;
-              }
+              for (int s = 0; s < NS; ++s)
+                if (s < nupd) vp[s] += (VV(j, s) * Nt + Vt[s] * Nj) * kj * fx;
+              npp += kj * Nj * Nt * fx;
             }
           }
           npp = cell_sum<L>(npp);
 #pragma unroll
           for (int s = 0; s < NS; ++s)
-            if (s < TG.nupd) vp[s] = cell_sum<L>(vp[s]);
+            if (s < nupd) vp[s] = cell_sum<L>(vp[s]);
           if (pb == pself) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) vp[s] += volsink[s];
@@ -635,7 +697,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
             const int r = Roff + pb;
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-              if (s < TG.nupd) VV(r, s) = VV(r, s) + dt * vp[s];
+              if (s < nupd) VV(r, s) = VV(r, s) + dt * vp[s];
             sN[r] = sN[r] + dt * npp;
           }
         }
@@ -663,7 +725,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       if (b < nt) {
         double vs = 0.;
         for (int s = 0; s < nsV; ++s) vs += VV(b, s);
-        double dw = sN[b] > D.nlim[b] ? ::cbrt(vs / sN[b] / kPi6) : D.dmid[b];
+        double dw = sN[b] > D.nlim[b] ? cbrt_fast(vs / sN[b] / kPi6) : D.dmid[b];
         sDW[b] = fmin(dw, D.dlim[b]);
       }
     }
@@ -703,13 +765,13 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           int count = 0;
           while (fabs(zdwet / zdold - 1.) > 1.e-2) {
             zdold = fmax(zdwet, 1.e-20);
-            const double aw = zr / zke;
+            const double aw = div_nr(zr, zke);
             const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * (aw * aw)
                                - 3.1543839e+2 * (aw * aw * aw) + 6.770824e+1 * ((aw * aw) * (aw * aw));
-            const double bm2 = 1. / (aw * kMwa) - 1. / kMwa;
-            zlwc = (v1 * (kRhosu / kMsu)) / bm1 + kEpsoc * v2 * (kRhooc / kMoc) / bm2;
-            zdwet = ::cbrt(zlwc / n / kRhowa / kPi6 + zcore / kPi6);
-            zke = ::exp(2. * kSurfw0 * kMvwa / (kBoltz * T * zdwet));
+            const double bm2 = rcp_nr(aw * kMwa) - 1. / kMwa;
+            zlwc = div_nr(v1 * (kRhosu / kMsu), bm1) + div_nr(kEpsoc * v2 * (kRhooc / kMoc), bm2);
+            zdwet = cbrt_fast(div_nr(div_nr(div_nr(zlwc, n), kRhowa), kPi6) + div_nr(zcore, kPi6));
+            zke = ::exp(div_nr(2. * kSurfw0 * kMvwa, kBoltz * T * zdwet));
             if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
           }
           VV(b, iwa) = zlwc / kRhowa;
@@ -718,7 +780,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
     }
     // regime 2 (2a/2b bins, init call), mo_salsa_properties.f90:275-378
     if (zrh < 0.98 || !D.lscndh2oae) {
-#pragma unroll
+#pragma unroll 1
       for (int q = 0; q < NQ; ++q) {
         const int b = sub + q * L;
         if (live && b >= A0 + D.in2a && b <= A0 + D.fn2b) {
@@ -733,16 +795,16 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
             int count = 0;
             while (fabs(zdwet / zdold - 1.) > 1.e-12) {
               zdold = fmax(zdwet, 1.e-20);
-              const double aw = zr / zke;
+              const double aw = div_nr(zr, zke);
               const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * (aw * aw)
                                  - 3.1543839e+2 * (aw * aw * aw) + 6.770824e+1 * ((aw * aw) * (aw * aw));
-              const double bm2 = 1. / (aw * kMwa) - 1. / kMwa;
+              const double bm2 = rcp_nr(aw * kMwa) - 1. / kMwa;
               const double bm5 = 5.875248e+1 - 1.8781997e+2 * aw + 2.7211377e+2 * (aw * aw)
                                  - 1.8458287e+2 * (aw * aw * aw) + 4.153689e+1 * ((aw * aw) * (aw * aw));
-              zlwc = (zv[0] * (kRhosu / kMsu)) / bm1 + kEpsoc * (zv[1] * (kRhooc / kMoc)) / bm2 +
-                     (zv[4] * (kRhoss / kMss)) / bm5;
-              zdwet = ::cbrt(zlwc / n / kRhowa / kPi6 + zcore / kPi6);
-              zke = ::exp(2. * kSurfw0 * kMvwa / (kBoltz * T * zdwet));
+              zlwc = div_nr(zv[0] * (kRhosu / kMsu), bm1) + div_nr(kEpsoc * (zv[1] * (kRhooc / kMoc)), bm2) +
+                     div_nr(zv[4] * (kRhoss / kMss), bm5);
+              zdwet = cbrt_fast(div_nr(div_nr(div_nr(zlwc, n), kRhowa), kPi6) + div_nr(zcore, kPi6));
+              zke = ::exp(div_nr(2. * kSurfw0 * kMvwa, kBoltz * T * zdwet));
               if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
             }
             VV(b, iwa) = zlwc / kRhowa;
@@ -786,36 +848,37 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       const double zthc = 0.023807 + 7.1128e-5 * (T - 273.16);
       const double hl = ((kAlv * kMwa) / (kRg * T)) - 1.;
       const double hs = ((kAls * kMwa) / (kRg * T)) - 1.;
-      // per owned bin: moles of water, transfer coefficient, Kelvin factor,
-      // flat-surface saturation concentration, solute moles, saturation ratio
-      double zcw[NQ], zmt[NQ], zkl[NQ], csu[NQ], zns[NQ], zws[NQ], org_i[NQ], org_r[NQ];
+      // flat-surface saturation concentrations of liquid and ice
+      const double csul = rs * rhoair / kMwa, csui = rsi * rhoair / kMwa;
+      // Per bin: moles of water, transfer coefficient, Kelvin factor, solute
+      // moles and saturation ratio.  The set-up runs as a plain loop into the
+      // cell's (now free) kernel region of LDS, so that its long transcendental
+      // chains are not interleaved across a lane's bins; the substeps then keep
+      // the lane's bins in registers.
+      double *cs = reinterpret_cast<double *>(cb + G.opr);   // [5][nt]
       uint32_t upd = 0;        // bit q: bin updated in the substeps
       uint32_t pred = 0;       // bit q: bin in the vapour predictor sums
       uint32_t iceq = 0;       // bit q: ice bin
-#pragma unroll
+#pragma unroll 1
       for (int q = 0; q < NQ; ++q) {
         const int b = sub + q * L;
-        zcw[q] = 0.; zmt[q] = 0.; zkl[q] = 1.; csu[q] = 0.; zns[q] = 0.; zws[q] = 0.;
-        org_i[q] = 0.; org_r[q] = 0.;
         if (b < nt) {
+          double zcw0, zmt0 = 0., zkl0 = 1., zws0 = 0.;
           const int c = D.cat_of[b];
           const double n = sN[b];
           double vs = 0.;
 #pragma unroll
           for (int s = 0; s < NS; ++s)
             if (s < nsV) vs += VV(b, s);
-          double dw = n > D.nlim[b] ? ::cbrt(vs / n / kPi6) : D.dmid[b];
+          double dw = n > D.nlim[b] ? cbrt_fast(vs / n / kPi6) : D.dmid[b];
           dw = fmin(dw, D.dlim[b]);                        // updateDiameter (:803-806); dnsp == dwet
           double zn = 0.;
           for (int s = 0; s < ndry; ++s) zn += D.diss[s] * VV(b, s) * D.rholiq[s] / D.mm[s];
-          zns[q] = zn;
           if (c == 3) {
             iceq |= 1u << q;
-            org_i[q] = VV(b, iwa) * kRhoic / kMwa;
-            org_r[q] = VV(b, irim) * kRhori / kMwa;
-            zcw[q] = VV(b, iwa) * kRhoic / kMwa + VV(b, irim) * kRhori / kMwa;
+            zcw0 = VV(b, iwa) * kRhoic / kMwa + VV(b, irim) * kRhori / kMwa;
           } else {
-            zcw[q] = VV(b, iwa) * kRhowa / kMwa;
+            zcw0 = VV(b, iwa) * kRhowa / kMwa;
           }
           const bool on = n > D.nlim[b] &&
                           (c == 0 ? (zrh > 0.98 && D.lscndh2oae)
@@ -824,26 +887,35 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
             const double zknud = 2. * zmfp / dw;
             const double zbeta = (zknud + 1.) / (0.377 * zknud + 1. + 4. / 3. * (zknud + zknud * zknud));
             if (c == 3) {                                   // :892-930 (capacitance 0.5*dnsp, activity 1)
-              zkl[q] = ::exp(4. * kSurfi0 * kMwa / (kRg * T * kRhowa * dw));
-              csu[q] = rsi * rhoair / kMwa;
-              zws[q] = 1.0 * zkl[q];
+              zkl0 = ::exp(4. * kSurfi0 * kMwa / (kRg * T * kRhowa * dw));
+              zws0 = 1.0 * zkl0;
               const double h1 = n * 4. * kPi * (0.5 * dw) * zdf * zbeta;
-              const double h2 = kMwa * zdf * kAls * zws[q] * csu[q] / (zthc * T);
-              zmt[q] = h1 / (h2 * hs + 1.);
+              const double h2 = kMwa * zdf * kAls * zws0 * csui / (zthc * T);
+              zmt0 = h1 / (h2 * hs + 1.);
             } else {                                        // :826-890, :932-955
               const double zact = act_h2o<false>(VV(b, iwa) * D.rholiq[iwa] / D.mm[iwa], zn);
-              zkl[q] = ::exp(4. * kSurfw0 * kMwa / (kRg * T * kRhowa * dw));
-              csu[q] = rs * rhoair / kMwa;
-              zws[q] = zact * zkl[q];
+              zkl0 = ::exp(4. * kSurfw0 * kMwa / (kRg * T * kRhowa * dw));
+              zws0 = zact * zkl0;
               const double h1 = n * 2. * kPi * dw * zdf * zbeta;
-              const double h2 = kMwa * zdf * kAlv * zws[q] * csu[q] / (zthc * T);
-              zmt[q] = h1 / (h2 * hl + 1.);
+              const double h2 = kMwa * zdf * kAlv * zws0 * csul / (zthc * T);
+              zmt0 = h1 / (h2 * hl + 1.);
             }
           }
+          cs[b] = zcw0; cs[nt + b] = zmt0; cs[2 * nt + b] = zkl0; cs[3 * nt + b] = zn; cs[4 * nt + b] = zws0;
           // aerosol 1a bins sit outside the predictor and are never updated (nstr = in2a)
           const bool inpred = c != 0 || b >= A0 + D.in2a;
           if (inpred) pred |= 1u << q;
           if (inpred && on) upd |= 1u << q;
+        }
+      }
+      double zcw[NQ], zmt[NQ], zkl[NQ], zns[NQ], zws[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int b = sub + q * L;
+        zcw[q] = 0.; zmt[q] = 0.; zkl[q] = 1.; zns[q] = 0.; zws[q] = 0.;
+        if (b < nt) {
+          zcw[q] = cs[b]; zmt[q] = cs[nt + b]; zkl[q] = cs[2 * nt + b]; zns[q] = cs[3 * nt + b];
+          zws[q] = cs[4 * nt + b];
         }
       }
       double zcwc = rv * rhoair / kMwa;
@@ -851,7 +923,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         stot += zcw[q];
-        if ((pred >> q) & 1u) { s1 += zmt[q] * zws[q] * csu[q]; s2 += zmt[q]; }
+        if ((pred >> q) & 1u) { s1 += zmt[q] * zws[q] * (((iceq >> q) & 1u) ? csui : csul); s2 += zmt[q]; }
         if (!((upd >> q) & 1u)) sfix += zcw[q];
       }
       stot = cell_sum<L>(stot);
@@ -867,19 +939,20 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
       for (int st = 0; st < nsub; ++st) {
         const double zhlp1 = zcwc + adt * s1;
         const double zhlp2 = 1. + adt * s2;
-        zcwint = fmin(zhlp1 / zhlp2, zcwtot);
+        zcwint = fmin(div_nr(zhlp1, zhlp2), zcwtot);
         double tsum = 0., s1n = 0.;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           if ((upd >> q) & 1u) {
-            const double d = adt * zmt[q] * (zcwint - zws[q] * csu[q]);
+            const double csu = ((iceq >> q) & 1u) ? csui : csul;
+            const double d = adt * zmt[q] * (zcwint - zws[q] * csu);
             double nw = zcw[q] + fmin(fmax(d, -1.e-1 * zcw[q]), 1.e-1 * zcw[q]);
             nw = fmax(nw, 0.);
             zcw[q] = nw;
-            zws[q] = ((iceq >> q) & 1u) ? zkl[q] : act_h2o<FASTDIV>(nw, zns[q]) * zkl[q];
+            zws[q] = ((iceq >> q) & 1u) ? zkl[q] : act_h2o<true>(nw, zns[q]) * zkl[q];
             tsum += nw;
           }
-          if ((pred >> q) & 1u) s1n += zmt[q] * zws[q] * csu[q];
+          if ((pred >> q) & 1u) s1n += zmt[q] * zws[q] * (((iceq >> q) & 1u) ? csui : csul);
         }
         tsum = cell_sum<L>(tsum);
         s1 = cell_sum<L>(s1n);
@@ -896,14 +969,15 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
           if (!((iceq >> q) & 1u)) {
             VV(b, iwa) = fmax(0., zcw[q] * kMwa / kRhowa);
           } else {
-            const double orig = org_i[q] + org_r[q];
+            const double org_i = VV(b, iwa) * kRhoic / kMwa, org_r = VV(b, irim) * kRhori / kMwa;
+            const double orig = org_i + org_r;
             const double dvt = zcw[q] - orig;
             if (dvt > 0.) {
               VV(b, iwa) = VV(b, iwa) + dvt * kMwa / kRhoic;
             } else if (dvt < 0.) {
               const double fr = (zcw[q] / orig) - 1.;
-              VV(b, iwa) = VV(b, iwa) + (fr * org_i[q]) * kMwa / kRhoic;
-              VV(b, irim) = VV(b, irim) + (fr * org_r[q]) * kMwa / kRhori;
+              VV(b, iwa) = VV(b, iwa) + (fr * org_i) * kMwa / kRhoic;
+              VV(b, irim) = VV(b, irim) + (fr * org_r) * kMwa / kRhori;
             }
           }
         }
@@ -912,8 +986,8 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
   }
   __syncthreads();
 
-  // ---- 6. unpack (mo_salsa_driver.f90:310-388): the tendency contribution
-  //      (new - old)/dt, or the new state, into the cell's output record ----
+  // ---- 6. unpack (mo_salsa_driver.f90:310-388): the new state in LES units
+  //      into the cell's output record (the scatter forms the tendencies) ----
   if (live) {
     double *o = a.rec_out + (int64_t)icell * D.nout;
 #pragma unroll
@@ -924,7 +998,7 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
         double *ob = o + (D.sbase[b] - kThermo + 1);
         const double nn = sN[b] / dn;
         if (!isfinite(nn)) flags |= SALSA_CELL_NONFINITE;
-        ob[0] = a.inplace ? nn : (nn - pn[q]) / dt;
+        ob[0] = nn;
         const int nsw = c < 3 ? D.nwet : D.nspec;
 #pragma unroll
         for (int s = 0; s < NS; ++s)
@@ -932,13 +1006,13 @@ __global__ void __launch_bounds__(64) k_salsa_step(const DevLayout *__restrict__
             const double rho = c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
             const double mm = VV(b, s) * rho / dn;
             if (!isfinite(mm)) flags |= SALSA_CELL_NONFINITE;
-            ob[1 + s] = a.inplace ? mm : (mm - pm[q][s]) / dt;
+            ob[1 + s] = mm;
           }
       }
     }
     if (sub == 0) {
       if (!isfinite(rv)) flags |= SALSA_CELL_NONFINITE;
-      o[0] = a.inplace ? rv : (rv - rv_old) / dt;
+      o[0] = rv;
     }
   }
   flags = cell_or<L>(flags);
@@ -1049,8 +1123,13 @@ __global__ void __launch_bounds__(256) k_scatter(const DevLayout *__restrict__ D
       const int64_t g = gi[c];
       const double x = tile[c * ld + r];
       if (r == 0) {
-        if (a.inplace) a.rv_o[g] = x;
-        else a.rv_t[g] = a.rv_t[g] + x;
+        if (a.inplace) {
+          a.rv_o[g] = x;
+        } else {
+          double rv_old = a.rv_p[g];                 // mo_salsa_driver.f90:193-197, 387-388
+          if (!a.lsfreeRH) rv_old = fmin(rv_old, a.rsl[g] * a.rhlim);
+          a.rv_t[g] = a.rv_t[g] + (x - rv_old) / a.dt;
+        }
       } else {
         const int sr = r - 1 + kThermo;
         const int cat = D.slot_cat[sr];
@@ -1058,9 +1137,10 @@ __global__ void __launch_bounds__(256) k_scatter(const DevLayout *__restrict__ D
         if (a.inplace) {
           double *dst = D.slot_mass[sr] ? a.m_o[cat] : a.n_o[cat];
           dst[i] = x;
-        } else {
+        } else {                                     // :321-323, :354-355
+          const double *old = D.slot_mass[sr] ? a.m_p[cat] : a.n_p[cat];
           double *dst = D.slot_mass[sr] ? a.m_t[cat] : a.n_t[cat];
-          dst[i] = dst[i] + x;
+          dst[i] = dst[i] + (x - old[i]) / a.dt;
         }
       }
     }
@@ -1098,7 +1178,7 @@ int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom 
   G.it0 = hyd ? 0 : Lay.it_aero0;
   G.it1 = hyd ? Lay.ntarget : Lay.it_aero1;
   G.jmask = hyd ? ~0ull : (Lay.nb[0] >= 64 ? ~0ull : ((1ull << Lay.nb[0]) - 1ull));
-  G.tab_bytes = align_up(2 * G.ntab * G.ntab, 16);
+  G.tab_bytes = align_up(4 * Lay.nent_total, 16);
   int32_t o = 0;
   G.oN = o; o += 8 * Lay.nt;
   G.oV = o; o += 8 * Lay.nt * G.nsV;
@@ -1106,6 +1186,8 @@ int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom 
   o = align_up(o, 16);
   G.opr = o; o += 2 * G.ntab * 4 * kt_bytes;
   G.oK = o; o += G.npairs * kt_bytes;
+  // the condensation set-up reuses the particle-term and kernel regions
+  if (o - G.opr < 5 * 8 * Lay.nt) o = G.opr + 5 * 8 * Lay.nt;
   G.opf = o; o += hyd ? Lay.ncc : 0;
   o = align_up(o, 16);
   // keep neighbouring cells of a block off the same LDS banks

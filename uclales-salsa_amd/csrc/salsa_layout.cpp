@@ -221,6 +221,7 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
   L.eddy_dis = p.eddy_dis; L.Eiagg_max = p.Eiagg_max; L.Eiagg_min = p.Eiagg_min;
   L.ice_alpha = p.ice_alpha; L.ice_beta = p.ice_beta; L.ice_gamma = p.ice_gamma; L.ice_sigma = p.ice_sigma;
   L.charge_factor = p.lcharge ? 1. + p.charge_coll_enh : 1.;
+  L.eddy_075 = std::pow(p.eddy_dis, 0.75);
 
   const int ice = level == 5 && d.nice > 0;
   const int cgaa = p.lscgaa, cgcc = p.lscgcc, cgpp = p.lscgpp, cgca = p.lscgca, cgpa = p.lscgpa,
@@ -413,6 +414,31 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
     }
   }
   L.ntarget = nt;
+  // entry lists: every bin named by a mask of the target, with its roles
+  {
+    int ne = 0;
+    for (int it = 0; it < nt; ++it) {
+      DevTarget &T = L.tgt[it];
+      T.ent0 = ne;
+      const uint64_t m[10] = {T.sink, T.sink_eagg, T.src, T.src_eagg, T.src_pc_wa, T.src_pc_ri,
+                              T.pf_own, T.pf_oth, T.fix_own, T.fix_oth};
+      for (int pass = 0; pass < 2; ++pass) {        // aerosol collectors first
+        for (int j = 0; j < L.nt; ++j) {
+          if ((L.cat_of[j] == 0) != (pass == 0)) continue;
+          uint32_t role = 0;
+          for (int r = 0; r < 10; ++r)
+            if ((m[r] >> j) & 1ull) role |= 1u << r;
+          if (!role) continue;
+          const int pid = L.pidx[T.t][j];
+          if (pid < 0) continue;                    // kernel block switched off: K = 0
+          L.ent[ne++] = (uint32_t)j | (role << 6) | ((uint32_t)pid << 16);
+        }
+        if (pass == 0) T.nent_aero = ne - T.ent0;
+      }
+      T.nent = ne - T.ent0;
+    }
+    L.nent_total = ne;
+  }
   return SALSA_OK;
 }
 
