@@ -31,8 +31,11 @@ FP32_PEAK_TFLOPS = 157.3
 
 
 def _cpu_worker(args):
-    """Time the FP64 C oracle on a slice of the packed cells (spawned process)."""
-    shm_name, shapes, lo, hi = args
+    """Time one slice of the packed cells on one core (spawned process): the
+    compiled reference (oracle/_ref harness, its own clock around the
+    coagulation + condensation calls) or, where that is not built, the FP64 C
+    restatement in oracle/."""
+    shm_name, shapes, lo, hi, kind = args
     import numpy as np
     from multiprocessing import shared_memory
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -44,30 +47,37 @@ def _cpu_worker(args):
         n = int(np.prod(shp))
         arrs.append(np.ndarray(shp, dtype=np.float64, buffer=shm.buf, offset=off))
         off += 8 * n
-    T, p, rs, rsi, rv, numc, volc = arrs
     L = O.make_layout()
     P = O.make_params(level=4)
     sl = slice(lo, hi)
-    a = [np.ascontiguousarray(x[sl]) for x in (T, p, rs, rsi, rv, numc, volc)]
-    t0 = time.perf_counter()
-    O.step_batch(L, P, *a)
-    dt = time.perf_counter() - t0
-    del arrs, T, p, rs, rsi, rv, numc, volc
+    a = [np.array(x[sl], copy=True) for x in arrs]     # own copies: the shared block is unmapped next
+    del arrs
     shm.close()
+    if kind == "reference":
+        import refio
+        dt = refio.run(L, P, 4, *a)["compute_s"]
+    else:
+        t0 = time.perf_counter()
+        O.step_batch(L, P, *a)
+        dt = time.perf_counter() - t0
     return hi - lo, dt
 
 
 def cpu_baseline(desc, f, max_cores=16):
-    """FP64 C restatement (oracle/) of the step on the host cores, one process
-    per core over contiguous slices of the same 491,520 cells."""
+    """CPU leg on the host cores: one process per core over contiguous slices
+    of the same 491,520 cells, one step each.  Throughput = cells / the
+    slowest process's compute time (the processes run concurrently)."""
     import multiprocessing as mp
     from multiprocessing import shared_memory
 
     import numpy as np
     from salsa_amd import workload as W
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refio
+    kind = "reference" if refio.available() else "port"
     T, p, rs, rsi, rv, numc, volc = W.to_salsa_units(desc, f, level=4)
     n, nt0 = numc.shape
-    ntot = nt0 + desc.nice                      # oracle layout keeps the 7 inert ice bins
+    ntot = nt0 + desc.nice                      # the oracle layout keeps the 7 inert ice bins
     nc = np.zeros((n, ntot))
     vc = np.zeros((n, ntot, 9))
     nc[:, :nt0] = numc
@@ -89,17 +99,19 @@ def cpu_baseline(desc, f, max_cores=16):
         cores = min(max_cores, os.cpu_count() or 1)
     bounds = np.linspace(0, n, cores + 1).astype(int)
     ctx = mp.get_context("spawn")
-    with ctx.Pool(cores) as pool:
-        t0 = time.perf_counter()
-        res = pool.map(_cpu_worker, [(shm.name, shapes, bounds[i], bounds[i + 1]) for i in range(cores)])
-        wall = time.perf_counter() - t0
-    shm.close()
-    shm.unlink()
+    try:
+        with ctx.Pool(cores) as pool:
+            res = pool.map(_cpu_worker, [(shm.name, shapes, bounds[i], bounds[i + 1], kind) for i in range(cores)])
+    finally:
+        shm.close()
+        shm.unlink()
+    slowest = max(r[1] for r in res)
     per_core = sum(r[0] for r in res) / sum(r[1] for r in res)
-    return {"value": n / wall, "unit": "cell-steps/s", "cores": cores, "kind": "port",
-            "per_core": per_core,
-            "sample": "all %d cells of the 64x64x120 workload, 1 step, FP64 C restatement of the "
-                      "reference (oracle/salsa_oracle.c, gcc -O2), one process per core" % n}
+    what = ("the compiled reference (oracle/_ref/ref_harness: coagulation + condensation of "
+            "src/src_salsa built with amdflang -O2; its own clock, file I/O excluded)"
+            if kind == "reference" else "the FP64 C restatement (oracle/salsa_oracle.c, gcc -O2)")
+    return {"value": n / slowest, "unit": "cell-steps/s", "cores": cores, "kind": kind, "per_core": per_core,
+            "sample": "all %d cells of the 64x64x120 workload, 1 step, %s, one process per core" % (n, what)}
 
 
 def counter_run(kind, lanes, steps):

@@ -52,6 +52,34 @@ enum {
   SALSA_CELL_NONFINITE = 8     /* NaN/Inf produced                                                  */
 };
 
+/* Process-rate diagnostic fields written by a step when `rates` is given:
+ * rates[f*nzp*nxp*nyp + cell], interior cells overwritten, the other cells
+ * untouched (the Diag fields of mo_salsa_driver.f90:392-419 with the names
+ * given there).  Level 4 writes the first SALSA_NRATE_L4 fields, level 5 all
+ * SALSA_NRATE_L5.  Activation and ice nucleation are not part of the
+ * coagulation + condensation step; their fields are written as 0. */
+enum {
+  SALSA_RATE_AUTOC = 0,        /* s_m_autoc      Autoconversion%volc(iwa)*rhowa/dn            */
+  SALSA_RATE_AUTOC50,          /* s_m_autoc50                                                  */
+  SALSA_RATE_AUTOC80,          /* s_m_autoc80                                                  */
+  SALSA_RATE_ACCR,             /* s_m_accr       Accretion%volc(iwa)*rhowa/dn                 */
+  SALSA_RATE_ACCR50,           /* s_m_accr50                                                   */
+  SALSA_RATE_ACCR80,           /* s_m_accr80                                                   */
+  SALSA_RATE_ACCOLL,           /* s_m_ACcoll_dry SUM(ACcoll%volc(1:ndry)*rholiq)/dn           */
+  SALSA_RATE_APCOLL,           /* s_m_APcoll_dry                                               */
+  SALSA_RATE_ACTIV,            /* s_n_activ      (0)                                           */
+  SALSA_RATE_CONDA,            /* s_m_conda      Cond_a%volc(iwa)/dn                           */
+  SALSA_RATE_CONDC,            /* s_m_condc                                                    */
+  SALSA_RATE_CONDP,            /* s_m_condp                                                    */
+  SALSA_NRATE_L4,              /* 12 fields at level 4                                         */
+  SALSA_RATE_AICOLL = SALSA_NRATE_L4, /* s_m_AIcoll_dry                                        */
+  SALSA_RATE_ICEHOM,           /* s_n_icehom     (0)                                           */
+  SALSA_RATE_ICEDEP,           /* s_n_icedep     (0)                                           */
+  SALSA_RATE_ICEIMM,           /* s_n_iceimm     (0)                                           */
+  SALSA_RATE_CONDI,            /* s_m_condi      (Cond_i%volc(iwa)*rhoic+%volc(irim)*rhori)/dn */
+  SALSA_NRATE_L5               /* 17 fields at level 5                                         */
+};
+
 /* precision modes */
 enum {
   SALSA_FP64 = 0,              /* plain FP64 everywhere                                             */
@@ -147,7 +175,8 @@ void salsa_ctx_destroy(salsa_ctx *ctx);
  *   n_p[c], m_p[c] : category c number / mass mixing ratios (read-only unless
  *                    SALSA_STEP_INPLACE); n_t[c], m_t[c] : tendencies (+=).
  *   Category 3 (ice) pointers are ignored at level 4 and may be NULL.
- *   rates : NULL (process-rate diagnostics are not produced by this build)
+ *   rates : NULL, or the process-rate fields (SALSA_RATE_*), same memory
+ *           kind as the other pointers of the call
  *   time, istp : model time [s] and step number of run_SALSA; the process
  *           switch states (set_SALSA_runtime, mo_salsa_driver.f90:445-467) are
  *           taken from salsa_params as the caller resolved them, so these two

@@ -36,6 +36,8 @@ PROGRAM ref_harness
   REAL :: T(1,1), P(1,1), RS(1,1), RSI(1,1), RV(1,1), dt
   REAL :: g1(1,1), g2(1,1), g3(1,1), g4(1,1), g5(1,1)
   INTEGER :: pbl(1)
+  INTEGER(8) :: c0, c1, crate
+  REAL :: tcomp
   CHARACTER(len=3), PARAMETER :: names(7) = ['SO4','OC ','BC ','DU ','SS ','NO ','NH ']
   LOGICAL :: lcharge
 
@@ -103,6 +105,7 @@ PROGRAM ref_harness
   WRITE(11) (allSALSA(1,1,nb)%dmid, nb=1,nt)
   WRITE(11) precpbins(1:nprc)
 
+  tcomp = 0.
   DO ic = 1, ncell
      READ(10) cellin
      T(1,1) = cellin(1); P(1,1) = cellin(2); RS(1,1) = cellin(3)
@@ -117,6 +120,7 @@ PROGRAM ref_harness
      CALL rateDiag%Reset()
 
      IF (mode == 1) THEN
+        CALL SYSTEM_CLOCK(c0, crate)
         DO ist = 1, nstep
            IF (ist > 1) CALL rateDiag%Reset()
            IF (lscoag%state) CALL coagulation(1, 1, 1, dt, T, P, lcharge)
@@ -124,6 +128,8 @@ PROGRAM ref_harness
            IF (lscnd%state) CALL condensation(1, 1, 1, 1, g1, g2, g3, g4, g5, RV, RS, RSI, &
                                               T, P, dt, pbl)
         END DO
+        CALL SYSTEM_CLOCK(c1)
+        tcomp = tcomp + REAL(c1 - c0) / REAL(crate)
         cellout(1) = RV(1,1)
         DO nb = 1, nt
            cellout(1 + nb) = allSALSA(1,1,nb)%numc
@@ -161,6 +167,8 @@ PROGRAM ref_harness
   END DO
   CLOSE(10)
   CLOSE(11)
+  ! seconds spent in coagulation + condensation (file I/O excluded)
+  WRITE(*,'(A,ES20.12)') 'compute_s ', tcomp
 
 CONTAINS
 

@@ -53,8 +53,12 @@ def run(L, P, level, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, mode=1, lc
             dh.tofile(fh)
             sw.tofile(fh)
             cells.tofile(fh)
-        subprocess.run([HARNESS, fin, fout], check=True, cwd=td)
+        pr = subprocess.run([HARNESS, fin, fout], check=True, cwd=td, capture_output=True, text=True)
         raw = open(fout, "rb").read()
+    compute_s = None
+    for line in pr.stdout.splitlines():
+        if line.startswith("compute_s"):
+            compute_s = float(line.split()[1])
     off = 0
     ntr, nprc = np.frombuffer(raw, np.int32, 2, off)
     off += 8
@@ -72,4 +76,5 @@ def run(L, P, level, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, mode=1, lc
     else:
         out = np.frombuffer(raw, np.float64, n * nt * nt, off).reshape(n, nt, nt)
         res["kmat"] = np.transpose(out, (0, 2, 1)).copy()
+    res["compute_s"] = compute_s
     return res

@@ -19,7 +19,7 @@ void set_error(const char *msg);
 int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &L);
 int launch_step(int cap, int lanes, int precision, bool hyd, const DevLayout *d, const StepArgs &a,
                 const SmemGeom &G, hipStream_t s);
-int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom &G);
+int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, bool rates, SmemGeom &G);
 int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, double *rec, uint8_t *cls,
                   int32_t *ids, int32_t *counts, hipStream_t s);
 int launch_scatter(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, hipStream_t s);
@@ -198,7 +198,7 @@ extern "C" int salsa_set_launch_shape(salsa_ctx *c, int32_t cls, int32_t lanes) 
     return SALSA_ERR_ARG;
   }
   SmemGeom G;
-  if (make_geom(c->hostL, cls == 2, c->params.precision == SALSA_FP64 ? 8 : 4, lanes, G)) {
+  if (make_geom(c->hostL, cls == 2, c->params.precision == SALSA_FP64 ? 8 : 4, lanes, false, G)) {
     set_error("this lanes_per_cell needs more than 64 KiB of LDS per block for the layout");
     return SALSA_ERR_ARG;
   }
@@ -214,7 +214,7 @@ extern "C" int salsa_get_launch_shape(salsa_ctx *c, int32_t cls) {
 static StepArgs make_args(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
                           const double *rsi, const double *dn, const double *rv_p, double *rv_t,
                           const double *const n_p[4], double *const n_t[4], const double *const m_p[4],
-                          double *const m_t[4], double tstep, int32_t flags) {
+                          double *const m_t[4], double *rates, double tstep, int32_t flags) {
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.nzp = c->nzp; a.nxp = c->nxp; a.nyp = c->nyp; a.hz = c->params.hz; a.hxy = c->params.hxy;
@@ -235,6 +235,7 @@ static StepArgs make_args(salsa_ctx *c, const double *press, const double *temp,
   a.inplace = (flags & SALSA_STEP_INPLACE) ? 1 : 0;
   a.lsfreeRH = c->params.lsfreeRH;
   a.rhlim = c->params.rhlim;
+  a.rates = rates;
   a.rec_in = c->rec_in;
   a.rec_out = c->rec_out;
   a.cell_flags = c->flags;
@@ -247,7 +248,6 @@ static int check_ptrs(salsa_ctx *c, const double *press, const double *temp, con
                       const double *const n_p[4], double *const n_t[4], const double *const m_p[4],
                       double *const m_t[4], double *rates, double tstep, int32_t flags) {
   if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
-  if (rates) { set_error("process-rate diagnostics are not produced by this build (pass NULL)"); return SALSA_ERR_ARG; }
   if (!(tstep > 0.)) { set_error("tstep must be positive"); return SALSA_ERR_ARG; }
   if (!press || !temp || !rsl || !dn || !rv_p || !n_p || !m_p) { set_error("null field pointer"); return SALSA_ERR_ARG; }
   if (c->hostL.level == 5 && !rsi) { set_error("rsi required at level 5"); return SALSA_ERR_ARG; }
@@ -273,7 +273,7 @@ static int timing_mark(salsa_ctx *c) {
 static int launch_class(salsa_ctx *c, int k, const StepArgs &a) {
   const bool hyd = k == 2;
   SmemGeom G;
-  if (make_geom(c->hostL, hyd, c->params.precision == SALSA_FP64 ? 8 : 4, c->lanes[k], G)) {
+  if (make_geom(c->hostL, hyd, c->params.precision == SALSA_FP64 ? 8 : 4, c->lanes[k], a.rates != nullptr, G)) {
     set_error("launch shape exceeds 64 KiB of LDS per block");
     return SALSA_ERR_ARG;
   }
@@ -349,7 +349,7 @@ extern "C" int salsa_step_device(salsa_ctx *c, const double *press, const double
   (void)time; (void)istp;
   int rc = check_ptrs(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, rates, tstep, flags);
   if (rc) return rc;
-  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, tstep, flags);
+  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, rates, tstep, flags);
   return enqueue_step(c, a, flags);
 }
 
@@ -381,7 +381,8 @@ extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *
   const DevLayout &L = c->hostL;
   const bool inplace = flags & SALSA_STEP_INPLACE;
   const size_t n3 = (size_t)c->n3;
-  size_t nf = 7;                                       // press temp rsl rsi dn rv_p rv_t
+  const size_t nrate = rates ? (size_t)(L.level == 5 ? SALSA_NRATE_L5 : SALSA_NRATE_L4) : 0;
+  size_t nf = 7 + nrate;                               // press temp rsl rsi dn rv_p rv_t (+ rates)
   size_t nmass[4], nnum[4];
   for (int k = 0; k < 4; ++k) {
     nnum[k] = k < L.ncat ? L.nb[k] : 0;
@@ -406,6 +407,7 @@ extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *
   };
   double *dpress = put(press, n3), *dtemp = put(temp, n3), *drsl = put(rsl, n3), *drsi = put(rsi, n3);
   double *ddn = put(dn, n3), *drvp = put(rv_p, n3), *drvt = put(inplace ? nullptr : rv_t, n3);
+  double *drates = rates ? put(rates, nrate * n3) : nullptr;     // halo cells keep their values
   const double *dnp[4] = {0, 0, 0, 0}, *dmp[4] = {0, 0, 0, 0};
   double *dnt[4] = {0, 0, 0, 0}, *dmt[4] = {0, 0, 0, 0};
   for (int k = 0; k < L.ncat; ++k) {
@@ -416,7 +418,7 @@ extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *
   }
   if (!dpress || !drvt) { set_error("host to device copy failed"); return SALSA_ERR_HIP; }
   StepArgs a = make_args(c, dpress, dtemp, drsl, rsi ? drsi : nullptr, ddn, drvp, drvt, dnp, dnt, dmp, dmt,
-                         tstep, flags);
+                         drates, tstep, flags);
   rc = enqueue_step(c, a, flags);
   if (rc) return rc;
   if (inplace) {
@@ -432,6 +434,7 @@ extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *
       HIPCHK(hipMemcpyAsync(m_t[k], dmt[k], nmass[k] * n3 * 8, hipMemcpyDeviceToHost, s));
     }
   }
+  if (rates) HIPCHK(hipMemcpyAsync(rates, drates, nrate * n3 * 8, hipMemcpyDeviceToHost, s));
   (void)time; (void)istp;
   return salsa_ctx_sync(c);
 }
@@ -454,7 +457,7 @@ extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *t
     nt[k] = p; p += (size_t)L.nb[k] * c->n3;
     mt[k] = p; p += (size_t)L.nb[k] * (k < 3 ? L.nwet : L.nspec) * c->n3;
   }
-  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rvt, n_p, nt, m_p, mt, tstep, 0);
+  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rvt, n_p, nt, m_p, mt, nullptr, tstep, 0);
   int rc = enqueue_step(c, a, 0);                        // classification + warm-up
   if (rc) { hipFree(scr); return rc; }
   hipStream_t s = c->stream;
@@ -473,7 +476,7 @@ extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *t
     int bestl = keep;
     for (int lanes : {8, 16, 32, 64}) {
       SmemGeom G;
-      if (make_geom(c->hostL, k == 2, c->params.precision == SALSA_FP64 ? 8 : 4, lanes, G)) continue;
+      if (make_geom(c->hostL, k == 2, c->params.precision == SALSA_FP64 ? 8 : 4, lanes, false, G)) continue;
       c->lanes[k] = lanes;
       if (launch_class(c, k, b)) { c->lanes[k] = keep; continue; }   // warm
       hipEventRecord(e0, s);

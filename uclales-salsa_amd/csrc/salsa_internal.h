@@ -11,7 +11,8 @@ namespace salsa {
 
 constexpr int kMaxBin = 64;           // master bins (aerosol|cloud|precip|ice)
 constexpr int kMaxSpec = 9;           // Section%volc(9)
-constexpr int kThermo = 6;            // T, p, rs, rsi, dn, rv at the head of a cell record
+constexpr int kThermo = 6;
+constexpr int kRateLanes = 16;        // lanes per cell of the step variant that writes rate diagnostics            // T, p, rs, rsi, dn, rv at the head of a cell record
 constexpr int kMaxSlot = kThermo + kMaxBin * (1 + kMaxSpec);
 
 // Physical constants exactly as written in src/src_shared/mo_submctl.f90:253-338.
@@ -130,6 +131,7 @@ struct SmemGeom {
   int32_t nsV;         // volume slots per bin
   int32_t it0, it1;    // sweep entries visited
   uint64_t jmask;      // collector bins that can be active
+  int32_t orr;         // byte offset of the rate accumulators, -1 without rates
 };
 
 // Per-launch arguments (passed by value).
@@ -147,6 +149,7 @@ struct StepArgs {
   double *n_o[4];                     // in-place outputs (== n_p cast) or null
   double *m_o[4];
   double *rv_o;
+  double *rates;                      // process-rate fields [nrate][n3], or null
   const double *rec_in;               // cell records [ncells_all][nslot]
   double *rec_out;                    // step results [ncells_all][nout]
   double dt;

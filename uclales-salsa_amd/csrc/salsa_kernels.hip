@@ -344,6 +344,56 @@ __device__ __forceinline__ void ice_bin_terms(const DevLayout &D, const CellC &c
   *dst_twin = PT4<KT>{(KT)p1.D, (KT)p1.c, (KT)p1.w, (KT)p1.v};
 }
 
+// LES array index of interior cell `cell` (k fastest)
+__device__ __forceinline__ int64_t les_index(const StepArgs &a, int64_t cell) {
+  const int k = (int)(cell % a.nz);
+  const int64_t r = cell / a.nz;
+  const int i = (int)(r % a.nx), j = (int)(r / a.nx);
+  return (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
+}
+
+// Per-cell rate accumulators of the process-rate diagnostics
+// (classProcessRates.f90:60-98), as the volumes the LES fields are made from.
+enum RateAcc {
+  RA_AUTOC = 0, RA_AUTOC50, RA_AUTOC80,   // Autoconversion(50/80)%volc(iwa)
+  RA_ACCR, RA_ACCR50, RA_ACCR80,         // Accretion(50/80)%volc(iwa)
+  RA_ACCOLL, RA_APCOLL,                  // SUM(ACcoll/APcoll%volc(1:ndry)*rholiq)
+  RA_CONDA, RA_CONDC, RA_CONDP,          // Cond_a/c/p%volc(iwa)
+  RA_CONDI,                              // Cond_i%volc(iwa)*rhoic + %volc(irim)*rhori
+  RA_N
+};
+
+// Rate contributions of one source entry of a cloud (tcat 1) or precipitation
+// (tcat 2) target, accumulateSource with its 50/80 um diagnostics
+// (mo_salsa_coagulation_processes.f90:1031-1210).
+__device__ __forceinline__ void rate_source_entry(const DevLayout &D, int tcat, int t, int j, double kj,
+                                                  const double *Vj, const double *sDW, double &rdry,
+                                                  double &racc, double &a80, double &a50, double &u80,
+                                                  double &u50) {
+  const int jc = j < D.off[1] ? 0 : (j < D.off[2] ? 1 : 2);
+  if (jc == 0) {
+    double dd = 0.;
+    for (int s = 0; s < D.ndry; ++s) dd += Vj[s] * D.rholiq[s];
+    rdry += kj * dd;
+    return;
+  }
+  if (tcat != 2) return;
+  const double w = kj * Vj[D.iwa];
+  if (jc == 1) racc += w;
+  const int R = D.off[2];
+  const double Dt = sDW[t], Dj = sDW[j];
+  const double lims[2] = {80.e-6, 50.e-6};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const double lim = lims[q];
+    const bool okj = jc == 1 || D.precpbins[j - R] < lim;      // ex = MIN(iend, cnt-1) for precip collectors
+    double &accr = q == 0 ? a80 : a50;
+    double &au = q == 0 ? u80 : u50;
+    if (okj && D.precpbins[t - R] > lim) accr += w;
+    if (okj && Dt > 0. && Dt < lim && cubed(Dt) + cubed(Dj) > cubed(lim)) au += w;
+  }
+}
+
 template <int L>
 __device__ __forceinline__ uint64_t cell_or64(uint64_t x) {
   return (uint64_t)cell_or<L>((uint32_t)x) | ((uint64_t)cell_or<L>((uint32_t)(x >> 32)) << 32);
@@ -358,7 +408,7 @@ __device__ __forceinline__ uint64_t cell_or64(uint64_t x) {
 // the aerosol-aerosol pairs and the aerosol targets are evaluated.  The
 // condensation stage still visits every bin.
 // ---------------------------------------------------------------------------
-template <int L, class KT, int MB, int NS, bool HYD>
+template <int L, class KT, int MB, int NS, bool HYD, bool RATES>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   constexpr int CPW = 64 / L, NQ = (MB + L - 1) / L;
@@ -383,7 +433,13 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   double *sDW = reinterpret_cast<double *>(cb + G.odw);
   PT4<KT> *sPR = reinterpret_cast<PT4<KT> *>(cb + G.opr);   // [2][ntab]: bin, self-offset twin
   KT *sK = reinterpret_cast<KT *>(cb + G.oK);
-  int8_t *sPF = reinterpret_cast<int8_t *>(cb + G.opf);
+  // precipitation formation per cloud-cloud pair: bits 0-4 target precip bin + 1
+  // (0: none), bit 5 D_new > 50 um, bit 6 D_new > 80 um
+  uint8_t *sPF = reinterpret_cast<uint8_t *>(cb + G.opf);
+  constexpr bool R = RATES;                                    // process-rate diagnostics
+  double *rr = RATES ? reinterpret_cast<double *>(cb + G.orr) : nullptr;
+  if (R && sub == 0)
+    for (int k = 0; k < RA_N; ++k) rr[k] = 0.;
 #define VV(b, s) sV[(b) * nsV + (s)]
 
   int32_t icell = 0;
@@ -481,7 +537,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
         const uint32_t pw = D.pw[p];
         const int b1 = pw & 255u, b2 = (pw >> 8) & 255u, tw = (pw >> 16) & 1u;
         KT k = KT(0.);
-        int8_t pfb = -1;
+        uint32_t pfb = 0;
         if (live && ((abits >> b1) & (abits >> b2) & 1ull)) {
           // particle 1 is the twin of particle 2 when the bins' dmid agree within 1%
           const double dB = sDW[b2];
@@ -495,11 +551,12 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
             double dnew = cbrt_fast(cubed(sDW[b1]) + cubed(dB));
             int cnt = 0;
             for (int qq = 0; qq < D.nb[2]; ++qq) cnt += dnew > D.precpbins[qq];
-            pfb = dnew > D.precpbins[0] ? (int8_t)(cnt - 1) : (int8_t)-1;
+            pfb = dnew > D.precpbins[0] ? (uint32_t)cnt : 0u;     // COUNT(D_new > precpbins): bin + 1
+            pfb |= (dnew > 50.e-6 ? 32u : 0u) | (dnew > 80.e-6 ? 64u : 0u);
           }
         }
         sK[p] = k;
-        if (HYD && p >= pf0 && p < pf1) sPF[p - pf0] = pfb;
+        if (HYD && p >= pf0 && p < pf1) sPF[p - pf0] = (uint8_t)pfb;
       }
     }
     __syncthreads();
@@ -532,6 +589,9 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
 #pragma unroll
       for (int s = 0; s < NS; ++s) src[s] = 0.;
       uint32_t pmask = 0;
+      // rate diagnostics of this target (cloud and precipitation targets)
+      const bool RT = R && (TG.cat == 1 || TG.cat == 2);
+      double rdry = 0., racc = 0., a80 = 0., a50 = 0., u80 = 0., u50 = 0.;
       if (!HYD || !(fl & (TF_ICE | TF_PF))) {
         // plain collection (accumulateSink / accumulateSource, :707-1210)
         for (int e = sub; e < ne; e += L) {
@@ -545,6 +605,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
 #pragma unroll
           for (int s = 0; s < NS; ++s)
             if (s < nupd) src[s] += kv * Vj[s];
+          if (RT && (role & R_SRC)) rate_source_entry(D, TG.cat, t, j, kj, Vj, sDW, rdry, racc, a80, a50, u80, u50);
         }
       } else {
         for (int e = sub; e < ne; e += L) {
@@ -568,6 +629,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
 #pragma unroll
             for (int s = 0; s < NS; ++s)
               if (s < nupd) src[s] += kj * Vj[s];
+            if (RT) rate_source_entry(D, TG.cat, t, j, kj, Vj, sDW, rdry, racc, a80, a50, u80, u50);
           }
           if (role & (R_PC_WA | R_PC_RI)) {              // accumulateSourcePhaseChange (:1236-1266)
 #pragma unroll
@@ -577,7 +639,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
             else src[irim] += kj * Vj[iwa] * kRhowa / kRhori;
           }
           if (role & (R_PF_OWN | R_PF_OTH)) {            // accumulatePrecipFormation (:1458-1576)
-            const int pb = sPF[pid - pf0];
+            const int pb = (int)(sPF[pid - pf0] & 31u) - 1;
             if (pb >= 0) {
               sink += kj * Nj;
               pmask |= 1u << pb;
@@ -595,6 +657,24 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
 #pragma unroll
       for (int s = 0; s < NS; ++s)
         if (s < nu) src[s] = cell_sum<L>(src[s]);
+      if (RT) {
+        rdry = cell_sum<L>(rdry);
+        if (TG.cat == 2) {
+          racc = cell_sum<L>(racc);
+          a80 = cell_sum<L>(a80); a50 = cell_sum<L>(a50);
+          u80 = cell_sum<L>(u80); u50 = cell_sum<L>(u50);
+        }
+        if (act && sub == 0) {                            // rates scale with the target numc (:1170-1196)
+          if (TG.cat == 1) {
+            rr[RA_ACCOLL] += rdry * Nt;
+          } else {
+            rr[RA_APCOLL] += rdry * Nt;
+            rr[RA_ACCR] += racc * Nt;
+            rr[RA_ACCR80] += a80 * Nt; rr[RA_ACCR50] += a50 * Nt;
+            rr[RA_AUTOC80] += u80 * Nt; rr[RA_AUTOC50] += u50 * Nt;
+          }
+        }
+      }
       double fix_own = 1., fix_oth = 1.;
       if (HYD && (fl & TF_PF)) {                          // fix_coag (:1440-1456)
         so = cell_sum<L>(so);
@@ -617,7 +697,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
         if (fl & TF_SELF_HALF_EAGG) sink_self = (emax - rft * (emax - emin)) * 0.5 * kself * Nt;
         if (HYD && (fl & TF_PF_SELF)) {
           fixs = fmax(1. - dt * (0.5 * kself * (Nt * Nt) / Nt), 0.1);
-          pself = pid >= 0 ? sPF[pid - pf0] : -1;
+          pself = pid >= 0 ? (int)(sPF[pid - pf0] & 31u) - 1 : -1;
           sink_self = 0.5 * kself * Nt;
           if (pself >= 0) {
 #pragma unroll
@@ -629,10 +709,10 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
 
       // applyCoag / applyCoagIce / applyCoagPrecp (:539-645)
       double Vn[NS];
-      const double den = 1. + dt * sink;
+      const double rden = rcp_nr(1. + dt * sink);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) Vn[s] = s < nu ? (Vt[s] + dt * src[s] * Nt) / den : Vt[s];
-      double Nn = Nt / (1. + dt * (sink + sink_self));
+      for (int s = 0; s < NS; ++s) Vn[s] = s < nu ? (Vt[s] + dt * src[s] * Nt) * rden : Vt[s];
+      double Nn = Nt * rcp_nr(1. + dt * (sink + sink_self));
       if (HYD && (fl & TF_PRECIP_SELFCOLL)) {         // precipSelfCoag (:1588-1611)
         const double nslf = 0.5 * kself * Nt * Nt;
         if (act && sub == 0) {
@@ -666,7 +746,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
         while (pm) {
           const int pb = __builtin_ctz(pm);
           pm &= pm - 1;
-          double vp[NS], npp = 0.;
+          double vp[NS], npp = 0., p50 = 0., p80 = 0.;
 #pragma unroll
           for (int s = 0; s < NS; ++s) vp[s] = 0.;
           for (int e = sub; e < ne; e += L) {
@@ -674,7 +754,8 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
             const uint32_t role = (w >> 6) & 1023u;
             if (!(role & (R_PF_OWN | R_PF_OTH))) continue;
             const int j = w & 63u, pid = (int)(w >> 16);
-            if (sPF[pid - pf0] == pb) {
+            const uint32_t pfv = sPF[pid - pf0];
+            if ((int)(pfv & 31u) - 1 == pb) {
               const double kj = (double)sK[pid];
               const double fx = (role & R_PF_OWN) ? fix_own : fix_oth;
               const double Nj = sN[j];
@@ -682,12 +763,32 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
               for (int s = 0; s < NS; ++s)
                 if (s < nupd) vp[s] += (VV(j, s) * Nt + Vt[s] * Nj) * kj * fx;
               npp += kj * Nj * Nt * fx;
+              if (R) {                                      // Autoconversion50/80 (:1487-1500)
+                const double dvw = (VV(j, iwa) * Nt + Vt[iwa] * Nj) * kj * fx;
+                if (pfv & 32u) p50 += dvw;
+                if (pfv & 64u) p80 += dvw;
+              }
             }
           }
           npp = cell_sum<L>(npp);
 #pragma unroll
           for (int s = 0; s < NS; ++s)
             if (s < nupd) vp[s] = cell_sum<L>(vp[s]);
+          if (R) {
+            p50 = cell_sum<L>(p50);
+            p80 = cell_sum<L>(p80);
+            if (sub == 0) {
+              rr[RA_AUTOC] += vp[iwa];
+              rr[RA_AUTOC50] += p50;
+              rr[RA_AUTOC80] += p80;
+              if (pb == pself) {
+                const uint32_t pfs = sPF[D.pidx[t][t] - pf0];
+                rr[RA_AUTOC] += volsink[iwa];
+                if (pfs & 32u) rr[RA_AUTOC50] += volsink[iwa];
+                if (pfs & 64u) rr[RA_AUTOC80] += volsink[iwa];
+              }
+            }
+          }
           if (pb == pself) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) vp[s] += volsink[s];
@@ -962,24 +1063,45 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
       if (nsub > 100) flags |= SALSA_CELL_SUBSTEPS;
       rv = zcwint * kMwa / rhoair;
       // write-back (:1058-1117)
+      double dcat[4] = {0., 0., 0., 0.};     // Cond_a/c/p%volc(iwa), Cond_i composite
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int b = sub + q * L;
         if (b < nt) {
           if (!((iceq >> q) & 1u)) {
-            VV(b, iwa) = fmax(0., zcw[q] * kMwa / kRhowa);
+            const double nv = fmax(0., zcw[q] * kMwa / kRhowa);
+            if (R) {
+              const int c = D.cat_of[b];
+              const double dv = nv - VV(b, iwa);
+              dcat[0] += c == 0 ? dv : 0.;
+              dcat[1] += c == 1 ? dv : 0.;
+              dcat[2] += c == 2 ? dv : 0.;
+            }
+            VV(b, iwa) = nv;
           } else {
             const double org_i = VV(b, iwa) * kRhoic / kMwa, org_r = VV(b, irim) * kRhori / kMwa;
             const double orig = org_i + org_r;
             const double dvt = zcw[q] - orig;
             if (dvt > 0.) {
               VV(b, iwa) = VV(b, iwa) + dvt * kMwa / kRhoic;
+              dcat[3] += (dvt * kMwa / kRhoic) * kRhoic;
             } else if (dvt < 0.) {
               const double fr = (zcw[q] / orig) - 1.;
               VV(b, iwa) = VV(b, iwa) + (fr * org_i) * kMwa / kRhoic;
               VV(b, irim) = VV(b, irim) + (fr * org_r) * kMwa / kRhori;
+              dcat[3] += ((fr * org_i) * kMwa / kRhoic) * kRhoic + ((fr * org_r) * kMwa / kRhori) * kRhori;
             }
           }
+        }
+      }
+      if (R) {
+        const int nc4 = D.level == 5 ? 4 : 3;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k < nc4) dcat[k] = cell_sum<L>(dcat[k]);
+        if (sub == 0) {
+          rr[RA_CONDA] += dcat[0]; rr[RA_CONDC] += dcat[1]; rr[RA_CONDP] += dcat[2];
+          if (nc4 == 4) rr[RA_CONDI] += dcat[3];
         }
       }
     }
@@ -1015,6 +1137,29 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
       o[0] = rv;
     }
   }
+  // rate diagnostics into the LES Diag fields (mo_salsa_driver.f90:392-411)
+  if (R && live) {
+    const int nrf = D.level == 5 ? SALSA_NRATE_L5 : SALSA_NRATE_L4;
+    for (int f = sub; f < nrf; f += L) {
+      double v = 0.;
+      switch (f) {
+        case SALSA_RATE_AUTOC: v = rr[RA_AUTOC] * kRhowa / dn; break;
+        case SALSA_RATE_AUTOC50: v = rr[RA_AUTOC50] * kRhowa / dn; break;
+        case SALSA_RATE_AUTOC80: v = rr[RA_AUTOC80] * kRhowa / dn; break;
+        case SALSA_RATE_ACCR: v = rr[RA_ACCR] * kRhowa / dn; break;
+        case SALSA_RATE_ACCR50: v = rr[RA_ACCR50] * kRhowa / dn; break;
+        case SALSA_RATE_ACCR80: v = rr[RA_ACCR80] * kRhowa / dn; break;
+        case SALSA_RATE_ACCOLL: v = rr[RA_ACCOLL] / dn; break;
+        case SALSA_RATE_APCOLL: v = rr[RA_APCOLL] / dn; break;
+        case SALSA_RATE_CONDA: v = rr[RA_CONDA] / dn; break;
+        case SALSA_RATE_CONDC: v = rr[RA_CONDC] / dn; break;
+        case SALSA_RATE_CONDP: v = rr[RA_CONDP] / dn; break;
+        case SALSA_RATE_CONDI: v = rr[RA_CONDI] / dn; break;
+        default: v = 0.;      // activation and ice nucleation: not part of this step
+      }
+      a.rates[(int64_t)f * a.n3 + les_index(a, icell)] = v;
+    }
+  }
   flags = cell_or<L>(flags);
   if (live && sub == 0) {
     if (a.cell_flags) a.cell_flags[icell] = (uint8_t)flags;
@@ -1033,13 +1178,6 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
 // from the next).
 // ---------------------------------------------------------------------------
 constexpr int kTileCells = 32;
-
-__device__ __forceinline__ int64_t les_index(const StepArgs &a, int64_t cell) {
-  const int k = (int)(cell % a.nz);
-  const int64_t r = cell / a.nz;
-  const int i = (int)(r % a.nx), j = (int)(r / a.nx);
-  return (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
-}
 
 // LES rows -> cell records, plus the cost class of every cell: 2 = some cloud /
 // precipitation / ice bin at or above its activity limit (numc >= nlim: it
@@ -1170,7 +1308,8 @@ int launch_scatter(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, 
 // ---------------------------------------------------------------------------
 static inline int32_t align_up(int32_t x, int32_t a) { return (x + a - 1) / a * a; }
 
-int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom &G) {
+int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, bool rates, SmemGeom &G) {
+  if (rates) lanes = kRateLanes;            // the rate-diagnostic variant has one shape
   const int cpw = 64 / lanes;
   G.ntab = hyd ? Lay.nt : Lay.nb[0];
   G.npairs = hyd ? Lay.npair : Lay.npair_aa;
@@ -1190,6 +1329,8 @@ int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom 
   if (o - G.opr < 5 * 8 * Lay.nt) o = G.opr + 5 * 8 * Lay.nt;
   G.opf = o; o += hyd ? Lay.ncc : 0;
   o = align_up(o, 16);
+  G.orr = -1;
+  if (rates) { G.orr = o; o += 8 * RA_N; }
   // keep neighbouring cells of a block off the same LDS banks
   if (o % 256 == 0) o += 16;
   G.cell_bytes = o;
@@ -1197,23 +1338,24 @@ int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, SmemGeom 
   return G.total_bytes <= 65536 ? 0 : -1;
 }
 
-template <int L, class KT, int MB, int NS, bool HYD>
+template <int L, class KT, int MB, int NS, bool HYD, bool RATES>
 static hipError_t launch_t(const DevLayout *d, const StepArgs &a, const SmemGeom &G, hipStream_t s) {
   constexpr int CPW = 64 / L;
   if (a.ncells <= 0) return hipSuccess;
   const int64_t nb = (a.ncells + CPW - 1) / CPW;
-  hipLaunchKernelGGL((k_salsa_step<L, KT, MB, NS, HYD>), dim3((unsigned)nb), dim3(64), (size_t)G.total_bytes, s, d,
-                     a, G);
+  hipLaunchKernelGGL((k_salsa_step<L, KT, MB, NS, HYD, RATES>), dim3((unsigned)nb), dim3(64), (size_t)G.total_bytes,
+                     s, d, a, G);
   return hipGetLastError();
 }
 
 template <class KT, int MB, int NS, bool HYD>
 static int launch_l(int lanes, const DevLayout *d, const StepArgs &a, const SmemGeom &G, hipStream_t s) {
+  if (a.rates) return (int)launch_t<kRateLanes, KT, MB, NS, HYD, true>(d, a, G, s);
   switch (lanes) {
-    case 8: return (int)launch_t<8, KT, MB, NS, HYD>(d, a, G, s);
-    case 16: return (int)launch_t<16, KT, MB, NS, HYD>(d, a, G, s);
-    case 32: return (int)launch_t<32, KT, MB, NS, HYD>(d, a, G, s);
-    case 64: return (int)launch_t<64, KT, MB, NS, HYD>(d, a, G, s);
+    case 8: return (int)launch_t<8, KT, MB, NS, HYD, false>(d, a, G, s);
+    case 16: return (int)launch_t<16, KT, MB, NS, HYD, false>(d, a, G, s);
+    case 32: return (int)launch_t<32, KT, MB, NS, HYD, false>(d, a, G, s);
+    case 64: return (int)launch_t<64, KT, MB, NS, HYD, false>(d, a, G, s);
   }
   return -1;
 }

@@ -46,6 +46,13 @@ EXPORTS = [
 ]
 
 
+# process-rate fields (include/salsa_amd.h SALSA_RATE_*)
+RATE_NAMES = ["s_m_autoc", "s_m_autoc50", "s_m_autoc80", "s_m_accr", "s_m_accr50", "s_m_accr80",
+              "s_m_ACcoll_dry", "s_m_APcoll_dry", "s_n_activ", "s_m_conda", "s_m_condc", "s_m_condp",
+              "s_m_AIcoll_dry", "s_n_icehom", "s_n_icedep", "s_n_iceimm", "s_m_condi"]
+NRATE_L4, NRATE_L5 = 12, 17
+
+
 class ParBin(C.Structure):
     _fields_ = [("cur", C.c_int32), ("par", C.c_int32)]
 
@@ -240,16 +247,19 @@ class Context:
     def stream(self):
         return lib().salsa_ctx_stream(self._h)
 
-    def _call(self, fn, f, n_p, m_p, n_t, m_t, rv_t, tstep, time, istp, flags):
+    def _call(self, fn, f, n_p, m_p, n_t, m_t, rv_t, tstep, time, istp, flags, rates=None):
         P = lambda xs: _P4(*[_ptr(x) for x in (list(xs) + [None] * (4 - len(xs)))])
         return fn(self._h, _ptr(f["press"]), _ptr(f["temp"]), _ptr(f["rsl"]), _ptr(f.get("rsi")),
                   _ptr(f["dn"]), _ptr(f["rv"]), _ptr(rv_t), P(n_p), P(n_t or []), P(m_p), P(m_t or []),
-                  None, tstep, time, istp, flags)
+                  _ptr(rates), tstep, time, istp, flags)
 
     def step_host(self, f, n_p, m_p, n_t=None, m_t=None, rv_t=None, tstep=1.0, time=10.0, istp=2,
-                  flags=STEP_TENDENCY):
-        """run_SALSA on host numpy arrays (staged through device memory)."""
-        return _check(self._call(lib().salsa_step_host, f, n_p, m_p, n_t, m_t, rv_t, tstep, time, istp, flags))
+                  flags=STEP_TENDENCY, rates=None):
+        """run_SALSA on host numpy arrays (staged through device memory).
+        rates: optional float64 array [nrate, nyp, nxp, nzp] (Fortran field
+        order, C-contiguous) receiving the process-rate fields (RATE_*)."""
+        return _check(self._call(lib().salsa_step_host, f, n_p, m_p, n_t, m_t, rv_t, tstep, time, istp, flags,
+                                 rates))
 
     def _ordered(self, enqueue):
         """Run `enqueue` with the context stream ordered after the work already
@@ -265,10 +275,10 @@ class Context:
         return rc
 
     def step_device(self, f, n_p, m_p, n_t=None, m_t=None, rv_t=None, tstep=1.0, time=10.0, istp=2,
-                    flags=STEP_TENDENCY):
+                    flags=STEP_TENDENCY, rates=None):
         """run_SALSA on device-resident torch tensors; enqueues only (see sync())."""
         return _check(self._ordered(lambda: self._call(lib().salsa_step_device, f, n_p, m_p, n_t, m_t, rv_t,
-                                                       tstep, time, istp, flags)))
+                                                       tstep, time, istp, flags, rates)))
 
     def sync(self):
         return _check(lib().salsa_ctx_sync(self._h))
