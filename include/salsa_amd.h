@@ -143,6 +143,10 @@ typedef struct {
   double charge_coll_enh;      /* chargeCollEnh: kernel scaled by (1 + this)       */
   int32_t precision;           /* SALSA_FP64 or SALSA_FP32_MIXED                  */
   int32_t hz, hxy;             /* halo widths of the 3-D arrays (LES: 1, 2)       */
+  int32_t lscoag_mode;         /* lscoag%mode: 1 kernels every step; 2 kernels every
+                                  cgintvl seconds, stored per cell in between
+                                  (mo_salsa_driver.f90:469-472, :601-697)          */
+  double cgintvl;              /* kernel update interval [s] for lscoag_mode 2    */
 } salsa_params;
 
 typedef struct salsa_ctx salsa_ctx;
@@ -177,10 +181,11 @@ void salsa_ctx_destroy(salsa_ctx *ctx);
  *   Category 3 (ice) pointers are ignored at level 4 and may be NULL.
  *   rates : NULL, or the process-rate fields (SALSA_RATE_*), same memory
  *           kind as the other pointers of the call
- *   time, istp : model time [s] and step number of run_SALSA; the process
- *           switch states (set_SALSA_runtime, mo_salsa_driver.f90:445-467) are
- *           taken from salsa_params as the caller resolved them, so these two
- *           do not change the result of this build's step
+ *   time, istp : model time [s] and step number of run_SALSA.  With
+ *           lscoag_mode 2 they select the kernel-update steps exactly as
+ *           set_SALSA_runtime does (MOD(time,cgintvl) < tstep .OR. istp <= 1,
+ *           mo_salsa_driver.f90:469-472); the process switch states (%state)
+ *           are taken from salsa_params as the caller resolved them
  * salsa_step_host takes host pointers (copies in and out on the context
  * stream); salsa_step_device takes device pointers and only enqueues work on
  * the context stream (salsa_ctx_stream): work that produces its inputs must

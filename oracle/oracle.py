@@ -81,6 +81,8 @@ def lib():
         L.so_step_cell.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_double, C.c_double,
                                    C.c_double, C.c_double, dp, C.c_double, dp, dp, C.POINTER(Rate)]
         L.so_step_cell.restype = C.c_int
+        L.so_step_cell_ks.argtypes = L.so_step_cell.argtypes + [C.POINTER(C.c_double), C.c_int]
+        L.so_step_cell_ks.restype = C.c_int
         L.so_step_batch.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_long, dp, dp, dp, dp, dp,
                                     C.c_double, C.c_int, dp, dp]
         L.so_step_batch.restype = C.c_int
@@ -124,11 +126,13 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def step_cells(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, rates=False):
+def step_cells(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, rates=False, kstore=None, kmodes=None):
     """Run `nsteps` box-model steps for each cell (SALSA units).
 
     numc: [ncell, ntotal]; volc: [ncell, ntotal, 9].  Returns
     (rv, numc, volc, status[, rates[ncell, 20, 10]]) as new arrays.
+    kstore: optional [ncell, ntotal, ntotal] kernel store (lscoag%mode 2),
+    updated in place; kmodes: per-step kernel mode (1 update, 2 fetch).
     """
     lb = lib()
     n = len(T)
@@ -143,10 +147,17 @@ def step_cells(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, rates=Fals
         nc = numc[i]
         vc = volc[i]
         st = 0
-        for _ in range(nsteps):
-            st |= lb.so_step_cell(C.byref(L), C.byref(P), float(T[i]), float(p[i]), float(rs[i]),
-                                  float(rsi[i]), C.byref(x), dt, _dp(nc), _dp(vc),
-                                  rbuf if rates else None)
+        for k in range(nsteps):
+            if kstore is None:
+                st |= lb.so_step_cell(C.byref(L), C.byref(P), float(T[i]), float(p[i]), float(rs[i]),
+                                      float(rsi[i]), C.byref(x), dt, _dp(nc), _dp(vc),
+                                      rbuf if rates else None)
+            else:
+                ks = kstore[i]
+                assert ks.flags["C_CONTIGUOUS"] and ks.shape == (L.ntotal, L.ntotal)
+                st |= lb.so_step_cell_ks(C.byref(L), C.byref(P), float(T[i]), float(p[i]), float(rs[i]),
+                                         float(rsi[i]), C.byref(x), dt, _dp(nc), _dp(vc),
+                                         rbuf if rates else None, _dp(ks), int(kmodes[k]))
         rv[i] = x.value
         status[i] = st
         if rates:

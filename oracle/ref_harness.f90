@@ -9,14 +9,19 @@
 ! condensation (the enabled branch of salsa(), mo_salsa.f90:92-107).
 !
 ! Modes: 1 = full step(s); 2 = dump the coagulation kernel matrices built
-! by update_coagulation_kernels for the input state.
+! by update_coagulation_kernels for the input state.  With sw(14) = 1 the
+! steps of mode 1 use lscoag%mode = 2: the first step of each cell is an
+! update step (lcgupdt = .TRUE.), the later ones reuse the kernels left in
+! the zcc* arrays by it, which is what fetch_coag provides for a single cell
+! (mo_salsa_driver.f90:294-297, :420-422; mo_salsa_dynamics.f90:146-147).
 !
 ! Usage: ref_harness <input.bin> <output.bin>
 PROGRAM ref_harness
   USE mo_submctl, ONLY : nbin, reglim, bloPrc, bloIce, nspec_dry, listspec, lscoag, lscnd, &
                          lsauto, eddy_dis, lscgaa, lscgcc, lscgpp, lscgii, lscgca, lscgpa, &
                          lscgia, lscgpc, lscgic, lscgip, lscndh2oae, lscndh2ocl, lscndh2oic, &
-                         lscndgas, nsnucl, lsdistupdate, ntotal, nprc, precpbins, nbins, ncld, nice
+                         lscndgas, nsnucl, lsdistupdate, ntotal, nprc, precpbins, nbins, ncld, nice, &
+                         lcgupdt
   USE mo_salsa_init, ONLY : salsa_initialize, associate_master_switches, setDefaultBinLayouts
   USE mo_salsa_types, ONLY : allSALSA, rateDiag, zccaa, zcccc, zccpp, zccii, zccca, &
                              zccpa, zccia, zccpc, zccic, zccip, iaero, icloud, iprecp, iice
@@ -123,6 +128,10 @@ PROGRAM ref_harness
         CALL SYSTEM_CLOCK(c0, crate)
         DO ist = 1, nstep
            IF (ist > 1) CALL rateDiag%Reset()
+           IF (sw(14) == 1) THEN
+              lscoag%mode = 2
+              lcgupdt = (ist == 1)
+           END IF
            IF (lscoag%state) CALL coagulation(1, 1, 1, dt, T, P, lcharge)
            g1 = 0.; g2 = 0.; g3 = 0.; g4 = 0.; g5 = 0.; pbl = 1
            IF (lscnd%state) CALL condensation(1, 1, 1, 1, g1, g2, g3, g4, g5, RV, RS, RSI, &

@@ -949,13 +949,27 @@ static void coag_cloud(const so_layout *L, const so_params *P, so_cell *c, const
   }
 }
 
-/* coagulation, src/src_salsa/mo_salsa_dynamics.f90:92-202 */
+/* coagulation, src/src_salsa/mo_salsa_dynamics.f90:92-202.  kmode 0: kernels
+ * computed (lscoag%mode 1); 1: computed and copied to kstore (mode 2 update
+ * step, store_coag mo_salsa_driver.f90:649-673); 2: taken from kstore (mode 2
+ * other steps, fetch_coag :677-697, no update_coagulation_kernels :146-147).
+ * kstore is ntotal x ntotal, row-major k[i][j]. */
 static int coagulation(const so_layout *L, const so_params *P, so_cell *c,
-                       double dt, double T, double p) {
+                       double dt, double T, double p, double *kstore, int kmode) {
   int status = 0;
   so_kmat K;
+  const int nt = L->ntotal;
   for (int b = 0; b < L->ntotal; ++b) update_diameter(L, c, b);
-  update_kernels(L, P, c, &K, T, p);    /* lscoag%mode == 1 */
+  if (kmode == 2) {
+    memset(&K, 0, sizeof(K));
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j) K.k[i][j] = kstore[i * nt + j];
+  } else {
+    update_kernels(L, P, c, &K, T, p);
+    if (kmode == 1)
+      for (int i = 0; i < nt; ++i)
+        for (int j = 0; j < nt; ++j) kstore[i * nt + j] = K.k[i][j];
+  }
   int any_aero = P->lscgaa || P->lscgca || P->lscgpa || P->lscgia;
   int any_cloud = P->lscgcc || P->lscgca || P->lscgpc || P->lscgic;
   int any_precp = P->lscgpp || P->lscgpa || P->lscgpc || P->lscgip;
@@ -1306,10 +1320,10 @@ static int gpparth2o(const so_layout *L, const so_params *P, so_cell *c,
 /* One cell: salsa() with coagulation and condensation enabled,           */
 /* src/src_salsa/mo_salsa.f90:92-107                                      */
 /* ====================================================================== */
-int so_step_cell(const so_layout *L, const so_params *P,
-                 double ptemp, double ppres, double prs, double prsi,
-                 double *prv, double ptstep,
-                 double *numc, double *volc, so_rate *rates) {
+int so_step_cell_ks(const so_layout *L, const so_params *P,
+                    double ptemp, double ppres, double prs, double prsi,
+                    double *prv, double ptstep,
+                    double *numc, double *volc, so_rate *rates, double *kstore, int kmode) {
   so_cell c;
   memset(&c, 0, sizeof(c));
   c.rates = rates;
@@ -1321,7 +1335,7 @@ int so_step_cell(const so_layout *L, const so_params *P,
     update_rhomean(L, &c, b);   /* mo_salsa_driver.f90:272 */
   }
   int status = 0;
-  if (P->lscoag) status |= coagulation(L, P, &c, ptstep, ptemp, ppres);
+  if (P->lscoag) status |= coagulation(L, P, &c, ptstep, ptemp, ppres, kstore, kstore ? kmode : 0);
   else for (int b = 0; b < L->ntotal; ++b) update_diameter(L, &c, b);
   if (P->lscnd && (P->lscndh2ocl || P->lscndh2oae || P->lscndh2oic))
     status |= gpparth2o(L, P, &c, ptemp, ppres, prs, prsi, prv, ptstep);
@@ -1330,6 +1344,13 @@ int so_step_cell(const so_layout *L, const so_params *P,
     for (int s = 0; s < SO_MAXSPEC; ++s) volc[b * SO_MAXSPEC + s] = c.volc[b][s];
   }
   return status;
+}
+
+int so_step_cell(const so_layout *L, const so_params *P,
+                 double ptemp, double ppres, double prs, double prsi,
+                 double *prv, double ptstep,
+                 double *numc, double *volc, so_rate *rates) {
+  return so_step_cell_ks(L, P, ptemp, ppres, prs, prsi, prv, ptstep, numc, volc, rates, 0, 0);
 }
 
 /* ---- thermodynamics used to build inputs, src/src_LES/thrm.f90:345-387 ---- */

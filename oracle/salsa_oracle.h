@@ -118,6 +118,14 @@ int so_step_cell(const so_layout *L, const so_params *P,
                  double *prv, double ptstep,
                  double *numc, double *volc, so_rate *rates);
 
+/* The same with the kernel store of lscoag%mode = 2 (mo_salsa_driver.f90:
+ * 601-697): kmode 1 computes the kernels and copies them to kstore
+ * (ntotal*ntotal, row-major), kmode 2 uses kstore instead of computing them. */
+int so_step_cell_ks(const so_layout *L, const so_params *P,
+                    double ptemp, double ppres, double prs, double prsi,
+                    double *prv, double ptstep,
+                    double *numc, double *volc, so_rate *rates, double *kstore, int kmode);
+
 /* n cells (numc [n*ntotal], volc [n*ntotal*SO_MAXSPEC]), nsteps each. */
 int so_step_batch(const so_layout *L, const so_params *P, long n, const double *T, const double *p,
                   const double *rs, const double *rsi, double *rv, double ptstep, int nsteps,

@@ -23,5 +23,6 @@ PROGRAM layout_check
   rc = salsa_params_default(p, 5_C_INT32_T)
   WRITE(*,'(A,I0)') 'prc ', rc
   WRITE(*,'(A,6(1X,I0))') 'psw ', p%level, p%lscgii, p%lscndh2oic, p%lsfreeRH, p%precision, p%hxy
-  WRITE(*,'(A,3(1X,ES24.16))') 'pdb ', p%rhlim, p%ice_sigma, p%charge_coll_enh
+  WRITE(*,'(A,4(1X,ES24.16))') 'pdb ', p%rhlim, p%ice_sigma, p%charge_coll_enh, p%cgintvl
+  WRITE(*,'(A,1X,I0)') 'pmd ', p%lscoag_mode
 END PROGRAM layout_check

@@ -45,13 +45,14 @@ def pick(f, n, seed):
     return np.sort(np.concatenate(out))[:n]
 
 
-def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0):
+def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0, kstore=False):
     T, p, rs, rsi, rv, nc, vc = cells
-    R = refio.run(L, P, level, T, p, rs, rsi, rv, nc, vc, nsteps=nsteps, lcharge=lcharge, charge_enh=enh)
+    R = refio.run(L, P, level, T, p, rs, rsi, rv, nc, vc, nsteps=nsteps, lcharge=lcharge, charge_enh=enh,
+                  kstore=kstore)
     ns = L.nspec
     np.savez_compressed(
         os.path.join(HERE, name),
-        level=level, nsteps=nsteps, lcharge=lcharge, charge_enh=enh,
+        level=level, nsteps=nsteps, lcharge=lcharge, charge_enh=enh, kstore=int(kstore),
         lsauto=P.lsauto_state, nbin=np.array(list(L.nbin)), nprc=L.nprc, nice=L.nice,
         species=np.array(list(L.dry_species)[:L.nspec_dry]),
         T=T, p=p, rs=rs, rsi=rsi, rv=rv, numc=nc, volc=vc[:, :, :ns],
@@ -59,7 +60,22 @@ def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0):
     print("wrote", name, len(T), "cells")
 
 
+def kstore_case():
+    """lscoag%mode = 2: one update step then two steps on the stored kernels,
+    cloudy cells with precipitation (written separately: `--only kstore`)."""
+    L = O.make_layout()
+    P4 = O.make_params(level=4)
+    f4 = W.to_salsa_units(L, W.config_fields(L, 4, grid_override=(20, 8, 8)), level=4)
+    T, p, rs, rsi, rv, numc, volc = f4
+    nc, vc = pad(L, numc, volc)
+    cells = (T, p, rs, rsi, rv, nc, vc)
+    sel = pick(cells, 120, 4)[:60]
+    save_step("step_cfg4_kstore.npz", L, 4, P4, tuple(x[sel] for x in cells), nsteps=3, kstore=True)
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "kstore":
+        return kstore_case()
     if not refio.available():
         sys.exit("oracle/_ref/ref_harness missing: run `make -C oracle ref` first")
     # warm layout nbin=(3,4): aerosol 11, cloud 8, precip 7 (+ 7 inert ice bins)
@@ -100,6 +116,7 @@ def main():
     f5 = W.to_salsa_units(L5, W.config_fields(L5, 5, grid_override=(20, 8, 8)), level=5)
     sel = np.random.default_rng(5).choice(len(f5[0]), 120, replace=False)
     save_step("step_cfg5.npz", L5, 5, P5, tuple(x[np.sort(sel)] for x in f5))
+    kstore_case()
 
 
 if __name__ == "__main__":

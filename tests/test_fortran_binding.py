@@ -46,7 +46,9 @@ def test_bindc_layout_matches_c(tmp_path):
     assert np.allclose([float(x) for x in rows["dbl"]], [d.dmid[4], d.precpbins[2], d.rholiq[1]], rtol=1e-15)
     assert [int(x) for x in rows["ph"]] == [d.phase[11], d.phase[d.ntotal - 1]]
     assert [int(x) for x in rows["psw"]] == [p.level, p.lscgii, p.lscndh2oic, p.lsfreeRH, p.precision, p.hxy]
-    assert np.allclose([float(x) for x in rows["pdb"]], [p.rhlim, p.ice_sigma, p.charge_coll_enh], rtol=1e-15)
+    assert np.allclose([float(x) for x in rows["pdb"]], [p.rhlim, p.ice_sigma, p.charge_coll_enh, p.cgintvl],
+                       rtol=1e-15)
+    assert [int(x) for x in rows["pmd"]] == [p.lscoag_mode]
 
 
 REF_MODS = os.path.join(ROOT, "oracle", "_ref", "obj")

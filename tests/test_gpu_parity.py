@@ -401,3 +401,52 @@ def test_gpu_rates_match_oracle_and_reference(capi, oracle_mod, name, kw, prec):
         print("rate %-15s scale %.3e  vs oracle %.2e  vs reference %.2e" % (capi.RATE_NAMES[f], scale, e1, e2))
         assert e1 <= tol, (capi.RATE_NAMES[f], e1)
         assert e2 <= max(tol, 1e-7), (capi.RATE_NAMES[f], e2)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gpu_kernel_store_mode2(capi, oracle_mod, prec):
+    """lscoag%mode = 2 (kernels every cgintvl seconds, stored per cell in
+    between, mo_salsa_driver.f90:294-297, :420-422, :469-472, :601-697).
+    Steps at t = 0 (istp 1: update), 1, 2 (fetch), 10 (update: MOD(10,10) < dt),
+    each compared with the oracle on identical inputs and the same store
+    sequence; then the three-step run against the compiled reference."""
+    g = golden("step_cfg4_kstore.npz")
+    desc = capi.bin_desc()
+    nt = desc.nbins + desc.ncld + desc.nprc
+    n = len(g["T"])
+    L = oracle_layout_for(g)
+    P = oracle_params_for(g)
+    fc = FlatCells(desc, 4, g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"][:, :nt],
+                   full_volc(g["volc"])[:, :nt])
+    prm = capi.params(level=4, precision=prec, hz=0, hxy=0, lscoag_mode=2, cgintvl=10.0)
+    ctx = capi.Context(desc, prm, fc.n, 1, 1)
+    ks = np.zeros((n, L.ntotal, L.ntotal))
+    state = (np.array(g["rv"], dtype=np.float64), g["numc"].copy(), full_volc(g["volc"]).copy())
+    tol = TOL[prec]
+    for time, istp, kmode in ((0., 1, 1), (1., 2, 2), (2., 3, 2), (10., 11, 1)):
+        rv, nc, vc = state
+        f = FlatCells(desc, 4, g["T"], g["p"], g["rs"], g["rsi"], rv, nc[:, :nt], vc[:, :nt])
+        n_p = [x.copy() for x in f.n_p]
+        m_p = [x.copy() for x in f.m_p]
+        fd = dict(f.f)
+        fd["rv"] = fd["rv"].copy()
+        inp = f.unpack(f.n_p, f.m_p, f.f["rv"])
+        ctx.step_host(fd, n_p, m_p, flags=capi.STEP_INPLACE, time=time, istp=istp)
+        grv, gnc, gvc = f.unpack(n_p, m_p, fd["rv"])
+        irv, inc, ivc = inp
+        nco = np.zeros((n, L.ntotal))
+        vco = np.zeros((n, L.ntotal, 9))
+        nco[:, :nt] = inc
+        vco[:, :nt] = ivc
+        orv, onc, ovc, _ = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco,
+                                                 kstore=ks, kmodes=[kmode])
+        # conditioning of the step itself (kernels recomputed; the store only fixes which kernels)
+        srv, snc, svc = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco,
+                                           kernel_eps=KEPS[prec])
+        e = errors(grv, gnc, gvc, orv, onc[:, :nt], ovc[:, :nt], 2)
+        print("t=%g kmode %d prec %d: rv %.2e N %.2e V %.2e" % ((time, kmode, prec) + e))
+        _, nflag = check_condition_aware((grv, gnc, gvc), (orv, onc[:, :nt], ovc[:, :nt]),
+                                         (srv, snc[:, :nt], svc[:, :nt]), tol, 2, "kstore t=%g" % time)
+        assert nflag <= max(2, gvc[:, :, :2].size // 200)
+        state = (orv, onc, ovc)
+    ctx.close()

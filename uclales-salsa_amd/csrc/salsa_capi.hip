@@ -6,6 +6,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -51,6 +52,8 @@ struct salsa_ctx {
   int64_t n3 = 0, ncells = 0;
   // cell records: LES values gathered per cell, step results per cell
   double *rec_in = nullptr, *rec_out = nullptr;
+  // lscoag mode 2: collision kernels of every cell between update steps
+  void *kstore = nullptr;
   // per-cell fault flags and the global OR
   uint8_t *flags = nullptr;
   int32_t *fault = nullptr;
@@ -107,7 +110,7 @@ static void release(salsa_ctx *c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   (void)hipFree(c->devL); (void)hipFree(c->flags); (void)hipFree(c->fault);
-  (void)hipFree(c->rec_in); (void)hipFree(c->rec_out);
+  (void)hipFree(c->rec_in); (void)hipFree(c->rec_out); (void)hipFree(c->kstore);
   (void)hipFree(c->cls); (void)hipFree(c->cls_sorted);
   (void)hipFree(c->ids); (void)hipFree(c->ids_sorted); (void)hipFree(c->cls_count);
   (void)hipFree(c->cub_tmp);
@@ -129,6 +132,14 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
   }
   if (params->precision != SALSA_FP64 && params->precision != SALSA_FP32_MIXED) {
     set_error("precision must be SALSA_FP64 or SALSA_FP32_MIXED");
+    return SALSA_ERR_ARG;
+  }
+  if (params->lscoag_mode != 1 && params->lscoag_mode != 2) {
+    set_error("lscoag_mode must be 1 or 2");
+    return SALSA_ERR_ARG;
+  }
+  if (params->lscoag_mode == 2 && !(params->cgintvl > 0.)) {
+    set_error("cgintvl must be positive with lscoag_mode 2");
     return SALSA_ERR_ARG;
   }
   salsa_ctx *c = new (std::nothrow) salsa_ctx();
@@ -162,6 +173,18 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
       hipHostMalloc(&c->cls_count_h, 3 * sizeof(int32_t)) != hipSuccess) {
     set_error("device allocation failed");
     return fail(SALSA_ERR_NOMEM);
+  }
+  if (c->params.lscoag_mode == 2 && c->params.lscoag) {
+    const size_t kt = c->params.precision == SALSA_FP64 ? 8 : 4;
+    if (hipMalloc(&c->kstore, nc * (size_t)c->hostL.npair * kt) != hipSuccess) {
+      set_error("kernel store allocation failed");
+      return fail(SALSA_ERR_NOMEM);
+    }
+    // zero-filled like initialize_coagstorage (mo_salsa_driver.f90:601-648)
+    if (hipMemset(c->kstore, 0, nc * (size_t)c->hostL.npair * kt) != hipSuccess) {
+      set_error("kernel store initialisation failed");
+      return fail(SALSA_ERR_HIP);
+    }
   }
   if (hipMemcpy(c->devL, &c->hostL, sizeof(DevLayout), hipMemcpyHostToDevice) != hipSuccess) {
     set_error("copy of device tables failed"); return fail(SALSA_ERR_HIP);
@@ -214,7 +237,8 @@ extern "C" int salsa_get_launch_shape(salsa_ctx *c, int32_t cls) {
 static StepArgs make_args(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
                           const double *rsi, const double *dn, const double *rv_p, double *rv_t,
                           const double *const n_p[4], double *const n_t[4], const double *const m_p[4],
-                          double *const m_t[4], double *rates, double tstep, int32_t flags) {
+                          double *const m_t[4], double *rates, double tstep, int32_t flags,
+                          double time = 0., int32_t istp = 1) {
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.nzp = c->nzp; a.nxp = c->nxp; a.nyp = c->nyp; a.hz = c->params.hz; a.hxy = c->params.hxy;
@@ -236,6 +260,12 @@ static StepArgs make_args(salsa_ctx *c, const double *press, const double *temp,
   a.lsfreeRH = c->params.lsfreeRH;
   a.rhlim = c->params.rhlim;
   a.rates = rates;
+  a.kstore = c->kstore;
+  a.kmode = 0;
+  if (c->kstore) {     // set_SALSA_runtime, mo_salsa_driver.f90:469-472
+    const bool upd = std::fmod(time, c->params.cgintvl) < tstep || istp <= 1;
+    a.kmode = upd ? 1 : 2;
+  }
   a.rec_in = c->rec_in;
   a.rec_out = c->rec_out;
   a.cell_flags = c->flags;
@@ -346,10 +376,10 @@ extern "C" int salsa_step_device(salsa_ctx *c, const double *press, const double
                                  const double *const n_p[4], double *const n_t[4],
                                  const double *const m_p[4], double *const m_t[4], double *rates,
                                  double tstep, double time, int32_t istp, int32_t flags) {
-  (void)time; (void)istp;
   int rc = check_ptrs(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, rates, tstep, flags);
   if (rc) return rc;
-  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, rates, tstep, flags);
+  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, rates, tstep, flags, time,
+                         istp);
   return enqueue_step(c, a, flags);
 }
 
@@ -418,7 +448,7 @@ extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *
   }
   if (!dpress || !drvt) { set_error("host to device copy failed"); return SALSA_ERR_HIP; }
   StepArgs a = make_args(c, dpress, dtemp, drsl, rsi ? drsi : nullptr, ddn, drvp, drvt, dnp, dnt, dmp, dmt,
-                         drates, tstep, flags);
+                         drates, tstep, flags, time, istp);
   rc = enqueue_step(c, a, flags);
   if (rc) return rc;
   if (inplace) {
@@ -435,7 +465,6 @@ extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *
     }
   }
   if (rates) HIPCHK(hipMemcpyAsync(rates, drates, nrate * n3 * 8, hipMemcpyDeviceToHost, s));
-  (void)time; (void)istp;
   return salsa_ctx_sync(c);
 }
 
@@ -458,6 +487,7 @@ extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *t
     mt[k] = p; p += (size_t)L.nb[k] * (k < 3 ? L.nwet : L.nspec) * c->n3;
   }
   StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rvt, n_p, nt, m_p, mt, nullptr, tstep, 0);
+  a.kmode = 0;                                           // tuning neither reads nor writes the kernel store
   int rc = enqueue_step(c, a, 0);                        // classification + warm-up
   if (rc) { hipFree(scr); return rc; }
   hipStream_t s = c->stream;

@@ -150,6 +150,8 @@ struct StepArgs {
   double *m_o[4];
   double *rv_o;
   double *rates;                      // process-rate fields [nrate][n3], or null
+  void *kstore;                       // lscoag mode 2: kernels per cell [ncells_all][npair] (KT)
+  int32_t kmode;                      // 0: compute; 1: compute and store; 2: fetch from kstore
   const double *rec_in;               // cell records [ncells_all][nslot]
   double *rec_out;                    // step results [ncells_all][nout]
   double dt;

@@ -533,12 +533,26 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
       const KT vkin = (KT)cc.vkin, cshear = (KT)cc.cshear, cinert = (KT)cc.cinert;
       const KT cfac = (KT)D.charge_factor;
       const int np = G.npairs, pf0 = D.pf0, pf1 = D.pf0 + D.ncc;
+      // lscoag mode 2 (mo_salsa_driver.f90:601-697): kernels of this cell as
+      // last stored (fetch steps) or stored after this evaluation (update steps)
+      KT *ks = a.kmode && live ? reinterpret_cast<KT *>(a.kstore) + (int64_t)icell * D.npair : nullptr;
+      const bool fetch = a.kmode == 2;
       for (int p = sub; p < np; p += L) {
         const uint32_t pw = D.pw[p];
         const int b1 = pw & 255u, b2 = (pw >> 8) & 255u, tw = (pw >> 16) & 1u;
         KT k = KT(0.);
         uint32_t pfb = 0;
-        if (live && ((abits >> b1) & (abits >> b2) & 1ull)) {
+        if (fetch) {
+          if (ks) k = ks[p];
+          // every cloud pair: a stored kernel may be non-zero for a bin now below nlim
+          if (HYD && p >= pf0 && p < pf1 && live) {
+            const double dnew = cbrt_fast(cubed(sDW[b1]) + cubed(sDW[b2]));
+            int cnt = 0;
+            for (int qq = 0; qq < D.nb[2]; ++qq) cnt += dnew > D.precpbins[qq];
+            pfb = dnew > D.precpbins[0] ? (uint32_t)cnt : 0u;
+            pfb |= (dnew > 50.e-6 ? 32u : 0u) | (dnew > 80.e-6 ? 64u : 0u);
+          }
+        } else if (live && ((abits >> b1) & (abits >> b2) & 1ull)) {
           // particle 1 is the twin of particle 2 when the bins' dmid agree within 1%
           const double dB = sDW[b2];
           const double dA = tw ? 1.1 * dB : sDW[b1];
@@ -556,8 +570,12 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
           }
         }
         sK[p] = k;
+        if (ks && !fetch) ks[p] = k;
         if (HYD && p >= pf0 && p < pf1) sPF[p - pf0] = (uint8_t)pfb;
       }
+      // the aerosol-only variant stores zeros for the pairs it does not evaluate
+      if (!HYD && ks && !fetch)
+        for (int p = np + sub; p < D.npair; p += L) ks[p] = KT(0.);
     }
     __syncthreads();
 
@@ -1226,7 +1244,10 @@ __global__ void __launch_bounds__(256) k_gather(const DevLayout *__restrict__ Dp
     const double *t = tile + tid * ld;
     const double dn = t[4];
     bool hyd = false;
-    for (int b = D.off[1]; b < D.nt; ++b) hyd |= !(t[D.sbase[b]] * dn < D.nlim[b]);
+    if (a.kmode == 2)          // stored kernels may pair any bin with non-zero numc
+      for (int b = D.off[1]; b < D.nt; ++b) hyd |= t[D.sbase[b]] * dn != 0.;
+    else
+      for (int b = D.off[1]; b < D.nt; ++b) hyd |= !(t[D.sbase[b]] * dn < D.nlim[b]);
     double rv = t[5];
     const double rs = t[2];
     if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);

@@ -158,6 +158,7 @@ extern "C" int salsa_params_default(salsa_params *p, int32_t level) {
   p->lcharge = 0; p->charge_coll_enh = 0.;
   p->precision = SALSA_FP32_MIXED;
   p->hz = 1; p->hxy = 2;
+  p->lscoag_mode = 1; p->cgintvl = 10.;     // mo_submctl.f90:23-24, :90
   if (level == 4) {   // define_salsa, mo_salsa_init.f90:615-635
     p->lscgia = p->lscgic = p->lscgii = p->lscgip = 0;
     p->lscndh2oic = 0;

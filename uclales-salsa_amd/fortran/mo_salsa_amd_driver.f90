@@ -24,7 +24,7 @@ MODULE mo_salsa_amd_driver
   USE mo_submctl, ONLY : nbin, reglim, bloPrc, bloIce, ncld, spec, rhlim, eddy_dis, Eiagg_max, Eiagg_min,  &
                          lscoag, lscnd, lsauto, lsactiv, lsicenucl, lsicemelt, lssecice, lsfreeRH,         &
                          lscgaa, lscgcc, lscgpp, lscgii, lscgca, lscgpa, lscgia, lscgpc, lscgic, lscgip,   &
-                         lscndh2oae, lscndh2ocl, lscndh2oic, lscndgas, nsnucl, lsdistupdate
+                         lscndh2oae, lscndh2ocl, lscndh2oic, lscndgas, nsnucl, lsdistupdate, cgintvl
   USE mo_ice_shape, ONLY : iceShapeAlpha, iceShapeBeta, iceShapeGamma, iceShapeSigma
   USE mo_salsa_types, ONLY : allSALSA
   IMPLICIT NONE
@@ -178,6 +178,8 @@ CONTAINS
     CALL GET_ENVIRONMENT_VARIABLE('SALSA_AMD_PRECISION', prec)
     IF (TRIM(prec) == 'mixed') p%precision = SALSA_FP32_MIXED
     p%hz = 1; p%hxy = 2                                   ! mo_salsa_driver.f90:180-182
+    p%lscoag_mode = INT(lscoag%mode, C_INT32_T)
+    p%cgintvl = REAL(cgintvl, C_DOUBLE)
     rc = salsa_ctx_create(desc, p, INT(nzp, C_INT32_T), INT(nxp, C_INT32_T), INT(nyp, C_INT32_T), ctx)
     IF (rc /= SALSA_OK) CALL fail('context')
   END SUBROUTINE create_context
@@ -192,7 +194,6 @@ CONTAINS
     IF (lscndgas) CALL outside('gas-phase condensation (lscndgas)')
     IF (nsnucl > 0) CALL outside('nucleation (nsnucl)')
     IF (lsdistupdate) CALL outside('size-distribution update (lsdistupdate)')
-    IF (lscoag%state .AND. lscoag%mode /= 1) CALL outside('reduced kernel update frequency (lscoag mode 2)')
   END SUBROUTINE check_switches
 
   SUBROUTINE outside(what)

@@ -58,6 +58,8 @@ MODULE mo_salsa_amd_cabi
      REAL(C_DOUBLE)     :: charge_coll_enh
      INTEGER(C_INT32_T) :: precision
      INTEGER(C_INT32_T) :: hz, hxy
+     INTEGER(C_INT32_T) :: lscoag_mode
+     REAL(C_DOUBLE)     :: cgintvl
   END TYPE salsa_params
 
   INTERFACE
