@@ -151,6 +151,35 @@ def counter_run(kind, lanes, steps):
     print("counter run: %s, lanes %s, %d steps done" % (kind, lanes, steps), flush=True)
 
 
+def gradient_leg(capi, W, desc, nsteps=20):
+    """Forward-mode gradient of the precipitation volume over a 20-step
+    trajectory of the 32x32x80 workload (81,920 cells, 30 % cloudy): dJ/dN0
+    for every bin of every cell and dJ/dE, timed per precision path."""
+    import numpy as np
+    import torch
+    f = W.config_fields(desc, 3)
+    g = f.grid
+    dev = torch.device("cuda")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fd = dict(press=T(f.press), temp=T(f.temp), rsl=T(f.rsl), rsi=T(f.rsi), dn=T(f.dn), rv=T(f.rv))
+    n_p = [T(x) for x in f.n[:3]]
+    m_p = [T(x) for x in f.m[:3]]
+    djdn = [torch.zeros_like(x) for x in n_p]
+    res = {"workload": "32x32x80 = %d cells, bins 11/8/7, %d steps, J = precipitation volume, "
+                       "%d directions per cell (every bin + E)" % (g.ncells, nsteps, 27),
+           "unit": "cell-steps/s"}
+    for prec, name in ((capi.SALSA_FP64, "fp64"), (capi.SALSA_FP32_MIXED, "mixed")):
+        ctx = capi.Context(desc, capi.params(level=4, precision=prec), g.nzp, g.nxp, g.nyp)
+        ctx.gradient_device(fd, n_p, m_p, djdn, nsteps=1)        # warm-up: code load, buffers
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J, dJdE = ctx.gradient_device(fd, n_p, m_p, djdn, nsteps=nsteps)
+        dt = time.perf_counter() - t0
+        ctx.close()
+        res[name] = {"value": g.ncells * nsteps / dt, "seconds": dt, "J": J, "dJdE": dJdE}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,11 +187,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--counter-run", choices=("mixed", "fp64", "copy"))
     ap.add_argument("--lanes", default="16,16,16")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the SALSA step has no cross-GPU work in this build "
+        print("bench.py measures one GPU: the SALSA step has no cross-GPU work "
               "(cells are independent; a multi-GPU run would shard the LES column blocks). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
@@ -272,6 +302,8 @@ def main():
              "V_SO4": rel(a[6][:, :, 0], b[6][:, :, 0], 1e-24), "V_H2O": rel(a[6][:, :, 1], b[6][:, :, 1], 1e-24)}
         out["accuracy_vs_fp64"] = {k: {"max": float(v.max()), "frac_gt_1e-5": float(np.mean(v > 1e-5))}
                                    for k, v in e.items()}
+    if not args.no_gradient:
+        out["gradient"] = gradient_leg(capi, W, desc)
     out["setup_s"] = gen_s
     print(json.dumps(out))
 

@@ -49,7 +49,11 @@ enum {
   SALSA_CELL_NEG_NUMC = 1,     /* numc < -1e-6 clamped after coagulation (mo_salsa_dynamics.f90:191) */
   SALSA_CELL_NOCONV = 2,       /* equilibration did not converge in 100 iterations                 */
   SALSA_CELL_SUBSTEPS = 4,     /* > 100 condensation substeps (mo_salsa_dynamics.f90:1051)         */
-  SALSA_CELL_NONFINITE = 8     /* NaN/Inf produced                                                  */
+  SALSA_CELL_NONFINITE = 8,    /* NaN/Inf produced                                                  */
+  SALSA_CELL_GRAD_UNSTABLE = 16 /* salsa_gradient_device: the linearised trajectory of the cell is
+                                   unstable -- a relative input change is amplified more than 1e8
+                                   times in J, or a derivative left the range of its type; the
+                                   derivative then describes no perturbation resolvable in FP64 */
 };
 
 /* Process-rate diagnostic fields written by a step when `rates` is given:
@@ -214,6 +218,38 @@ int salsa_ctx_sync(salsa_ctx *ctx);
  * interior order k fastest) to host memory. */
 int salsa_get_cell_flags(salsa_ctx *ctx, uint8_t *flags_host, int64_t n);
 
+/*
+ * Gradient of a multi-step trajectory (no counterpart in the reference: the
+ * sensitivity of precipitation formation to the initial size distribution
+ * and to the collision-kernel enhancement).  Starting from the given fields,
+ * the step of salsa_step_device is applied nsteps times in place (T, p, rs,
+ * rsi and dn fixed; N, masses and rv evolving; lscoag_mode 1) to a copy of
+ * the state, and
+ *     J = sum over interior cells and precipitation bins of the bin volume
+ *         concentration, all species [m^3 m^-3], after the last step
+ * is differentiated in forward mode:
+ *   djdn[c]   : dJ/dN0 for every bin of category c, in the layout of n_p[c]
+ *               (device; interior cells written, halo untouched); N0 is the
+ *               initial number mixing ratio [kg^-1], masses held fixed
+ *   j_cell    : per interior cell J (device, ncells; may be NULL)
+ *   djde_cell : per interior cell dJ/dE (device, ncells; may be NULL), E the
+ *               collision-kernel enhancement factor K = E * coagc, i.e.
+ *               1 + chargeCollEnh of mo_salsa_coagulation_kernels.f90:411-416
+ *               with chargeTime = chargeTimeMax, at the context's value
+ *   totals    : host double[2]: J summed over cells, and dJ/dE summed over
+ *               the cells without SALSA_CELL_GRAD_UNSTABLE, in a fixed order
+ * Per-cell fault bits (salsa_get_cell_flags) include SALSA_CELL_GRAD_UNSTABLE;
+ * that bit is not an error of the call.
+ * Inputs are device pointers and are not modified.  The call waits for the
+ * context stream and returns its fault status like salsa_ctx_sync.
+ */
+int salsa_gradient_device(salsa_ctx *ctx, const double *press, const double *temp,
+                          const double *rsl, const double *rsi, const double *dn,
+                          const double *rv, const double *const n_p[4],
+                          const double *const m_p[4], double tstep, int32_t nsteps,
+                          double *const djdn[4], double *j_cell, double *djde_cell,
+                          double totals[2]);
+
 /* Launch-shape selection.  lanes_per_cell in {8,16,32,64} for each cost class
  * (0 = clear and dry, 1 = clear and humid, 2 = cloudy); 0 keeps the current
  * value.  salsa_autotune times the candidates on the given device-resident
@@ -229,8 +265,8 @@ int salsa_autotune(salsa_ctx *ctx, const double *press, const double *temp,
  * resets and enables it: each step then records one event before its first and
  * one after its last step-kernel launch.  salsa_ctx_kernel_time waits for the
  * stream and returns the summed milliseconds and the number of steps timed.
- * (Instrumentation of this build; no counterpart in the reference, which
- * only prints cpu_time per LES step, step.f90:134-141.) */
+ * (Instrumentation with no counterpart in the reference, which only prints
+ * cpu_time per LES step, step.f90:134-141.) */
 int salsa_ctx_timing(salsa_ctx *ctx, int32_t enable);
 int salsa_ctx_kernel_time(salsa_ctx *ctx, double *ms, int64_t *nsteps);
 

@@ -1,0 +1,281 @@
+"""Forward-mode gradient of the multi-step trajectory (salsa_gradient_device):
+J = precipitation volume concentration summed over cells after n in-place
+steps; dJ/dN0 for every bin of every cell and dJ/dE (collision-kernel
+enhancement).  Checked against central finite differences, h = 1e-6 relative:
+of the FP64 oracle on small grids (both precision paths), and of the FP64
+GPU step at the 32x32x80 workload over 20 steps.  The value part of the
+derivative run is checked against the oracle's trajectory.
+
+Bar: |g - g_fd| <= 1e-4 |g_fd| + noise, where noise = 3e-7 V_cell / x0 is
+the rounding floor of a central difference with h = 1e-6 relative: J is a
+small remainder of the cell's condensed volume V_cell, whose relative
+rounding noise after the trajectory stays below about 5e-13.  Where the trajectory
+crosses a threshold of the algorithm (an activity limit, a precipitation-bin
+boundary, a clamp) within +-h, J is not differentiable there and central
+differences at h and 4h disagree; such (cell, direction) pairs are counted,
+reported and left out, and must stay below 2 % of the checked pairs; so are
+pairs where the derivative itself says that +-h is beyond the linear range
+(|dJ/dx| x0 / J > 100: the trajectory amplifies a relative input change more
+than 100 times, and its internal state, which J only partly reflects, more).  A few
+cells of the workload have a condensation state whose discrete trajectory is
+unstable (a perturbation grows about 100x per step and saturates near 1e-3):
+their derivative is finite but astronomically large, central differences
+cannot follow it.  The kernel flags such cells (CELL_GRAD_UNSTABLE: a
+relative input change amplified more than 1e8 times in J, or a derivative
+out of the range of its type); they are left out here and must stay below
+1 % of the cells."""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from helpers import FlatCells
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-6
+TOL = 1e-4
+NOISE = 3e-7
+MAX_KINK_FRAC = 0.02
+
+
+@pytest.fixture(scope="module")
+def capi():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need an MI355X"
+    from salsa_amd import capi as c
+    c.lib()
+    return c
+
+
+def objective(desc, volc):
+    r0 = desc.nbins + desc.ncld
+    return volc[:, r0:r0 + desc.nprc, :desc.nspec - 1].sum(axis=(1, 2))
+
+
+def oracle_J(O, L, P, desc, cells, nsteps):
+    T, p, rs, rsi, rv, numc, volc = cells
+    n, nt = numc.shape
+    nco = np.zeros((n, L.ntotal))
+    nco[:, :nt] = numc
+    vco = np.zeros((n, L.ntotal, 9))
+    vco[:, :nt] = volc
+    rvc = np.array(rv, dtype=np.float64)
+    st = O.step_batch(L, P, *(np.array(x, dtype=np.float64) for x in (T, p, rs, rsi)), rvc, nco, vco, 1.0, nsteps)
+    assert st == 0
+    return objective(desc, vco)
+
+
+def gpu_gradient(capi, desc, prec, cells, nsteps, level=4):
+    import torch
+    T, p, rs, rsi, rv, numc, volc = cells
+    fc = FlatCells(desc, level, T, p, rs, rsi, rv, numc, volc)
+    ctx = capi.Context(desc, capi.params(level=level, precision=prec, hz=0, hxy=0), fc.n, 1, 1)
+    dev = torch.device("cuda")
+    D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    f = {k: D(v) for k, v in fc.f.items()}
+    n_p = [D(x) for x in fc.n_p]
+    m_p = [D(x) for x in fc.m_p]
+    n0 = [x.clone() for x in n_p]
+    djdn = [torch.zeros_like(x) for x in n_p]
+    jc = torch.zeros(fc.n, dtype=torch.float64, device=dev)
+    je = torch.zeros_like(jc)
+    J, dJdE = ctx.gradient_device(f, n_p, m_p, djdn, tstep=1.0, nsteps=nsteps, j_cell=jc, djde_cell=je)
+    flags = ctx.cell_flags()
+    ctx.close()
+    for a, b in zip(n_p, n0):            # inputs are left untouched
+        assert torch.equal(a, b)
+    g = np.concatenate([x.cpu().numpy() for x in djdn], axis=0).T      # [cell, bin], per unit N0
+    return J, dJdE, jc.cpu().numpy(), je.cpu().numpy(), g, flags
+
+
+def check_fd(name, g, fd, fd4, vol, x0, J, tol=TOL):
+    """Per element bar on the pairs where central differences can resolve the
+    derivative: the trajectory amplifies a relative input change at most 100
+    times in J and the differences at h and 4h agree.  Returns the number of
+    pairs left out."""
+    noise = NOISE * np.abs(vol) / np.maximum(np.abs(x0), 1e-300) + 1e-300
+    bar = tol * np.abs(fd) + noise
+    linear = np.abs(g) * np.abs(x0) <= 100. * np.abs(J)
+    smooth = linear & (np.abs(fd4 - fd) <= 16 * tol * np.abs(fd) + 4 * noise)
+    err = np.where(smooth, np.abs(g - fd) / bar, 0.)
+    worst = float(err.max()) if err.size else 0.
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
+    big = smooth & (np.abs(fd) > 1e3 * noise)
+    nk = int((~smooth).sum())
+    print("%s: %d values, %d left out (%d amplifying), worst err/bar %.3f, max rel err where |fd| >> noise "
+          "%.2e (%d values)" % (name, err.size, nk, int((~linear).sum()), worst,
+                                float(rel[big].max()) if big.any() else 0., int(big.sum())), flush=True)
+    if worst > 1.0:
+        bad = np.argsort(-err)[:5]
+        for i in bad:
+            print("   idx %d: g %.6e fd %.6e fd4h %.6e noise %.3e J %.3e" % (i, g[i], fd[i], fd4[i], noise[i], J[i]))
+        d = os.environ.get("SALSA_TEST_DUMP_DIR")
+        if d:
+            os.makedirs(d, exist_ok=True)
+            np.savez(os.path.join(d, "grad_%s.npz" % name.replace(" ", "_").replace("/", "")), g=g, fd=fd,
+                     fd4=fd4, vol=vol, x0=x0, J=J)
+    assert worst <= 1.0, name
+    return nk
+
+
+def small_cells(desc, W, grid=(16, 4, 4)):
+    f = W.config_fields(desc, 3, grid_override=grid)
+    return W.to_salsa_units(desc, f)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gradient_against_oracle_fd(capi, oracle_mod, prec):
+    """dJ/dN0 for every bin of every cell and dJ/dE against central
+    differences of the FP64 oracle over a 10-step trajectory."""
+    from salsa_amd import workload as W
+    O = oracle_mod
+    desc = capi.bin_desc()
+    cells = small_cells(desc, W)
+    nsteps = 10
+    print("case gradient vs oracle FD, precision %d, %d cells, %d steps" % (prec, len(cells[0]), nsteps), flush=True)
+    J, dJdE, jc, je, g, flags = gpu_gradient(capi, desc, prec, cells, nsteps)
+    assert not np.any(flags & capi.CELL_NONFINITE)
+    ok = (flags & capi.CELL_GRAD_UNSTABLE) == 0
+    assert ok.mean() >= 0.99
+    assert np.all(np.isfinite(g[ok])) and np.all(np.isfinite(je[ok]))
+    L = O.make_layout()
+    P = O.make_params(level=4)
+    Jo = oracle_J(O, L, P, desc, cells, nsteps)
+    assert Jo.max() > 0, "no precipitation formed: the objective is identically zero"
+    # value part against the oracle's trajectory
+    tolv = 1e-9 if prec == 0 else 1e-4
+    assert np.all(np.abs(jc - Jo) <= tolv * np.abs(Jo) + 1e-30), np.max(np.abs(jc - Jo) / np.maximum(Jo, 1e-30))
+    assert abs(J - jc.sum()) <= 1e-12 * abs(J) + 1e-300
+    T, p, rs, rsi, rv, numc, volc = cells
+    vol = volc.sum(axis=(1, 2))
+    # dJ/dE: enhancement factor E = 1 + chargeCollEnh
+    def fdE(h):
+        Jp = oracle_J(O, L, O.make_params(level=4, lcharge=1, charge_coll_enh=h), desc, cells, nsteps)
+        Jm = oracle_J(O, L, O.make_params(level=4, lcharge=1, charge_coll_enh=-h), desc, cells, nsteps)
+        return (Jp - Jm) / (2 * h)
+    fe, fe4 = fdE(H), fdE(4 * H)
+    nk = check_fd("dJ/dE per cell (prec %d)" % prec, je[ok], fe[ok], fe4[ok], vol[ok], np.ones(ok.sum()), Jo[ok])
+    npair = int(ok.sum())
+    # the total leaves the unstable cells out
+    assert abs(dJdE - fe[ok].sum()) <= TOL * abs(fe[ok].sum()) + NOISE * vol[ok].sum()
+    # dJ/dN0, every bin with particles
+    nb_checked = 0
+
+    def fdN(b, h, live):
+        cp = list(cells)
+        cm = list(cells)
+        cp[5] = numc.copy()
+        cp[5][:, b] *= 1 + h
+        cm[5] = numc.copy()
+        cm[5][:, b] *= 1 - h
+        return (oracle_J(O, L, P, desc, cp, nsteps) - oracle_J(O, L, P, desc, cm, nsteps)) / \
+            (2 * h * np.where(live, numc[:, b], 1.))
+    for b in range(numc.shape[1]):
+        live = (numc[:, b] > 0) & ok
+        # a bin empty in every cell: a relative step is zero and an absolute
+        # one below the activity limit is lost in rounding, so only
+        # finiteness is checked there
+        if not live.any():
+            continue
+        fd, fd4 = fdN(b, H, live), fdN(b, 4 * H, live)
+        nk += check_fd("dJ/dN0 bin %d (prec %d)" % (b, prec), g[live, b], fd[live], fd4[live], vol[live],
+                       numc[live, b], Jo[live])
+        npair += int(live.sum())
+        nb_checked += 1
+    assert nb_checked >= 15
+    print("pairs at thresholds: %d of %d" % (nk, npair))
+    assert nk <= MAX_KINK_FRAC * npair
+
+
+def test_gradient_workload3_against_gpu_fd(capi):
+    """32x32x80 cells, 20 steps: dJ/dE and dJ/dN0 of an aerosol and a cloud
+    bin in every cell against central differences of the FP64 GPU step."""
+    import torch
+    from salsa_amd import workload as W
+    desc = capi.bin_desc()
+    f = W.config_fields(desc, 3)
+    g = f.grid
+    nsteps = 20
+    dev = torch.device("cuda")
+    D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fd = {k: D(getattr(f, k)) for k in ("press", "temp", "rsl", "rsi", "dn", "rv")}
+    n_p = [D(x) for x in f.n[:3]]
+    m_p = [D(x) for x in f.m[:3]]
+    idx = g.interior_index()
+    print("case workload 3 gradient, %d cells, %d steps" % (len(idx), nsteps), flush=True)
+
+    def forward_J(prm_kw, scale=None):
+        ctx = capi.Context(desc, capi.params(level=4, precision=capi.SALSA_FP64, **prm_kw), g.nzp, g.nxp, g.nyp)
+        ff = dict(fd)
+        ff["rv"] = fd["rv"].clone()
+        nn = [x.clone() for x in n_p]
+        mm = [x.clone() for x in m_p]
+        if scale is not None:
+            c, b, s = scale
+            nn[c][b] *= s
+        for _ in range(nsteps):
+            ctx.step_device(ff, nn, mm, flags=capi.STEP_INPLACE)
+        ctx.sync()
+        ctx.close()
+        fo = W.Fields(grid=g, level=4, press=f.press, temp=f.temp, rsl=f.rsl, rsi=f.rsi, dn=f.dn,
+                      rv=ff["rv"].cpu().numpy(), n=[x.cpu().numpy() for x in nn] + [None],
+                      m=[x.cpu().numpy() for x in mm] + [None])
+        return objective(desc, W.to_salsa_units(desc, fo)[6])
+
+    res = {}
+    for prec in (capi.SALSA_FP64, capi.SALSA_FP32_MIXED):
+        ctx = capi.Context(desc, capi.params(level=4, precision=prec), g.nzp, g.nxp, g.nyp)
+        djdn = [torch.zeros_like(x) for x in n_p]
+        jc = torch.zeros(len(idx), dtype=torch.float64, device=dev)
+        je = torch.zeros_like(jc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J, dJdE = ctx.gradient_device(fd, n_p, m_p, djdn, tstep=1.0, nsteps=nsteps, j_cell=jc, djde_cell=je)
+        print("precision %d: gradient of %d cells x %d steps in %.3f s" % (prec, len(idx), nsteps,
+                                                                          time.perf_counter() - t0), flush=True)
+        fl = ctx.cell_flags()
+        ctx.close()
+        assert not np.any(fl & capi.CELL_NONFINITE)
+        ok = (fl & capi.CELL_GRAD_UNSTABLE) == 0
+        print("precision %d: %d cells with a derivative out of range" % (prec, int((~ok).sum())))
+        assert ok.mean() >= 0.99
+        res[prec] = (J, dJdE, jc.cpu().numpy(), je.cpu().numpy(), [x.cpu().numpy() for x in djdn], ok)
+    J64, dJdE64, jc64, je64, djdn64, ok64 = res[capi.SALSA_FP64]
+    J0 = forward_J({})
+    assert J0.max() > 0
+    # value part against the FP64 step (cells with an unstable linearisation
+    # amplify the rounding differences of the two codes and are left out)
+    dv = np.abs(jc64 - J0) / np.maximum(np.abs(J0), 1e-30)
+    print("value part vs FP64 step: max rel diff %.2e (stable cells), %.2e (all)" % (dv[ok64].max(), dv.max()))
+    assert np.all(np.abs(jc64 - J0)[ok64] <= 1e-9 * np.abs(J0[ok64]) + 1e-30)
+    vol = W.to_salsa_units(desc, f)[6].sum(axis=(1, 2))
+
+    def fdE(h):
+        return (forward_J(dict(lcharge=1, charge_coll_enh=h)) - forward_J(dict(lcharge=1, charge_coll_enh=-h))) / (2 * h)
+    fe, fe4 = fdE(H), fdE(4 * H)
+    print("J = %.6e, dJ/dE = %.6e (FD %.6e)" % (J64, dJdE64, fe[ok64].sum()))
+    assert abs(dJdE64 - fe[ok64].sum()) <= TOL * abs(fe[ok64].sum()) + NOISE * vol[ok64].sum()
+    nk = 0
+    npair = 0
+    for prec in (capi.SALSA_FP64, capi.SALSA_FP32_MIXED):
+        ok = res[prec][5]
+        nk += check_fd("workload 3 dJ/dE per cell prec %d" % prec, res[prec][3][ok], fe[ok], fe4[ok], vol[ok],
+                       np.ones(ok.sum()), J0[ok])
+        npair += int(ok.sum())
+    for c, b in ((0, desc.in2a + 1), (1, 0), (1, 2)):
+        n0 = f.n[c][b, idx]
+        live = n0 > 0
+
+        def fdb(h):
+            return (forward_J({}, (c, b, 1 + h)) - forward_J({}, (c, b, 1 - h))) / (2 * h * np.where(live, n0, 1.))
+        fb, fb4 = fdb(H), fdb(4 * H)
+        for prec in (capi.SALSA_FP64, capi.SALSA_FP32_MIXED):
+            gb = res[prec][4][c][b, idx]
+            sel = live & res[prec][5]
+            nk += check_fd("workload 3 dJ/dN0 cat %d bin %d prec %d" % (c, b, prec), gb[sel], fb[sel], fb4[sel],
+                           vol[sel], n0[sel], J0[sel])
+            npair += int(sel.sum())
+    print("pairs at thresholds: %d of %d" % (nk, npair))
+    assert nk <= MAX_KINK_FRAC * npair

@@ -2,7 +2,7 @@
 
     python uclales-salsa_amd/build.py [--jobs N]
 
-The three translation units are compiled in parallel and linked into
+The translation units are compiled in parallel and linked into
 uclales-salsa_amd/salsa_amd/libsalsa_amd.so.  Packed FP32 instructions are
 disabled for all device code.
 """
@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SALSA_ARCH", "gfx950")
-SRCS = ["csrc/salsa_layout.cpp", "csrc/salsa_kernels.hip", "csrc/salsa_capi.hip"]
+SRCS = ["csrc/salsa_layout.cpp", "csrc/salsa_kernels.hip", "csrc/salsa_gradient.hip", "csrc/salsa_capi.hip"]
 OUT = os.path.join(HERE, "salsa_amd", "libsalsa_amd.so")
 FLAGS = ["-O3", "--offload-arch=" + ARCH, "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

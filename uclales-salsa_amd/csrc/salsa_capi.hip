@@ -1,9 +1,8 @@
 // salsa_capi.hip -- the extern "C" boundary (include/salsa_amd.h): context
 // lifecycle, host- and device-pointer step entry points, cost-class
-// compaction of cells, hipGraph capture of the per-step launch sequence and
-// the launch-shape autotuner.
+// compaction of cells, the gradient entry point and the launch-shape
+// autotuner.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <chrono>
 #include <cmath>
@@ -22,10 +21,13 @@ int launch_step(int cap, int lanes, int precision, bool hyd, const DevLayout *d,
                 const SmemGeom &G, hipStream_t s);
 int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, bool rates, SmemGeom &G);
 int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, double *rec, uint8_t *cls,
-                  int32_t *ids, int32_t *counts, hipStream_t s);
+                  int32_t *tcount, int32_t *toff, int32_t *lists, int32_t *counts, hipStream_t s);
+int64_t gather_tiles(int64_t ncells);
 int launch_scatter(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, hipStream_t s);
 int cap_bins(int cap);
 int cap_spec(int cap);
+int launch_gradient(int cap, int precision, const DevLayout *d, const GradArgs &g, double *sum_out,
+                    hipStream_t s);
 }  // namespace salsa
 
 using namespace salsa;
@@ -57,22 +59,18 @@ struct salsa_ctx {
   // per-cell fault flags and the global OR
   uint8_t *flags = nullptr;
   int32_t *fault = nullptr;
-  // compaction: cost class per cell, cell ids sorted by class, class counts
-  uint8_t *cls = nullptr, *cls_sorted = nullptr;
-  int32_t *ids = nullptr, *ids_sorted = nullptr;
+  // compaction: class per cell, per-tile class counts and offsets, one list
+  // of cell ids per cost class ([3][ncells]), class counts
+  uint8_t *cls = nullptr;
+  int32_t *tcount = nullptr, *toff = nullptr;
+  int32_t *lists = nullptr;
   int32_t *cls_count = nullptr;       // device [3]
   int32_t *cls_count_h = nullptr;     // pinned host [3]
-  void *cub_tmp = nullptr;
-  size_t cub_bytes = 0;
+  // salsa_gradient_device: per-cell J and dJ/dE, then the two sums
+  double *gbuf = nullptr;
   // staging for salsa_step_host
   double *stage = nullptr;
   size_t stage_bytes = 0;
-  // captured launch sequence
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  StepArgs gkey;
-  int glanes[3] = {0, 0, 0};
-  bool gvalid = false;
   // kernel timing
   bool timing = false;
   std::vector<hipEvent_t> ev;         // start/stop pairs of the timed steps
@@ -107,14 +105,12 @@ static void release(salsa_ctx *c) {
   if (!c) return;
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   c->ev.clear();
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
   (void)hipFree(c->devL); (void)hipFree(c->flags); (void)hipFree(c->fault);
   (void)hipFree(c->rec_in); (void)hipFree(c->rec_out); (void)hipFree(c->kstore);
-  (void)hipFree(c->cls); (void)hipFree(c->cls_sorted);
-  (void)hipFree(c->ids); (void)hipFree(c->ids_sorted); (void)hipFree(c->cls_count);
-  (void)hipFree(c->cub_tmp);
+  (void)hipFree(c->cls); (void)hipFree(c->tcount); (void)hipFree(c->toff);
+  (void)hipFree(c->lists); (void)hipFree(c->cls_count);
   (void)hipFree(c->stage);
+  (void)hipFree(c->gbuf);
   if (c->cls_count_h) (void)hipHostFree(c->cls_count_h);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -165,8 +161,9 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
   const size_t nc = (size_t)c->ncells;
   if (hipMalloc(&c->devL, sizeof(DevLayout)) != hipSuccess ||
       hipMalloc(&c->flags, nc) != hipSuccess || hipMalloc(&c->fault, sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&c->cls, nc) != hipSuccess || hipMalloc(&c->cls_sorted, nc) != hipSuccess ||
-      hipMalloc(&c->ids, nc * 4) != hipSuccess || hipMalloc(&c->ids_sorted, nc * 4) != hipSuccess ||
+      hipMalloc(&c->lists, 3 * nc * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->cls, nc) != hipSuccess ||
+      hipMalloc(&c->tcount, 3 * (size_t)gather_tiles(c->ncells) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->toff, 3 * (size_t)gather_tiles(c->ncells) * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->cls_count, 3 * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->rec_in, nc * c->hostL.nslot * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->rec_out, nc * c->hostL.nout * sizeof(double)) != hipSuccess ||
@@ -189,15 +186,6 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
   if (hipMemcpy(c->devL, &c->hostL, sizeof(DevLayout), hipMemcpyHostToDevice) != hipSuccess) {
     set_error("copy of device tables failed"); return fail(SALSA_ERR_HIP);
   }
-  // radix-sort temporary storage for the class compaction
-  size_t bytes = 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->cls, c->cls_sorted, c->ids, c->ids_sorted,
-                                         (int)nc, 0, 2, c->stream) != hipSuccess ||
-      hipMalloc(&c->cub_tmp, bytes ? bytes : 16) != hipSuccess) {
-    set_error("sort workspace allocation failed");
-    return fail(SALSA_ERR_NOMEM);
-  }
-  c->cub_bytes = bytes;
   *out = c;
   return SALSA_OK;
 }
@@ -300,45 +288,54 @@ static int timing_mark(salsa_ctx *c) {
   return SALSA_OK;
 }
 
-static int launch_class(salsa_ctx *c, int k, const StepArgs &a) {
+static int launch_class(salsa_ctx *c, int k, const StepArgs &a, hipStream_t s) {
   const bool hyd = k == 2;
   SmemGeom G;
   if (make_geom(c->hostL, hyd, c->params.precision == SALSA_FP64 ? 8 : 4, c->lanes[k], a.rates != nullptr, G)) {
     set_error("launch shape exceeds 64 KiB of LDS per block");
     return SALSA_ERR_ARG;
   }
-  return launch_step(c->cap, c->lanes[k], c->params.precision, hyd, c->devL, a, G, c->stream);
+  return launch_step(c->cap, c->lanes[k], c->params.precision, hyd, c->devL, a, G, s);
+}
+
+// Gather + classification, then the class counts to the host (they size the
+// launches).
+static int enqueue_gather(salsa_ctx *c, const StepArgs &a, bool compact) {
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
+  int rc = launch_gather(c->hostL, c->devL, a, c->rec_in, compact ? c->cls : nullptr, c->tcount, c->toff, c->lists,
+                         c->cls_count, s);
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "gather launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  if (compact) {
+    HIPCHK(hipMemcpyAsync(c->cls_count_h, c->cls_count, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return SALSA_OK;
+}
+
+static StepArgs class_args(salsa_ctx *c, const StepArgs &a, int k) {
+  StepArgs b = a;
+  b.cell_list = c->lists + (int64_t)k * c->ncells;
+  b.ncells = c->cls_count_h[k];
+  return b;
 }
 
 static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
   hipStream_t s = c->stream;
   const bool compact = !(flags & SALSA_STEP_NO_COMPACT);
   if (timing_mark(c)) return SALSA_ERR_HIP;
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
-  if (compact) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->cls_count, 3);
-  // LES rows -> cell records (+ cost classes)
-  int rc = launch_gather(c->hostL, c->devL, a, c->rec_in, compact ? c->cls : nullptr, c->ids, c->cls_count, s);
-  if (rc) { std::snprintf(g_err, sizeof(g_err), "gather launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  int rc = enqueue_gather(c, a, compact);
+  if (rc) return rc;
   if (!compact) {
     StepArgs b = a;
     b.cell_list = nullptr;
-    rc = launch_class(c, 2, b);
+    rc = launch_class(c, 2, b, s);
     if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
   } else {
-    // sort cell ids by class (stable: k-fastest order kept within a class)
-    size_t bytes = c->cub_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, bytes, c->cls, c->cls_sorted, c->ids, c->ids_sorted,
-                                              (int)c->ncells, 0, 2, s));
-    HIPCHK(hipMemcpyAsync(c->cls_count_h, c->cls_count, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    int64_t off = 0;
     for (int k = 0; k < 3; ++k) {
-      StepArgs b = a;
-      b.cell_list = c->ids_sorted + off;
-      b.ncells = c->cls_count_h[k];
-      off += b.ncells;
+      const StepArgs b = class_args(c, a, k);
       if (b.ncells == 0) continue;
-      rc = launch_class(c, k, b);
+      rc = launch_class(c, k, b, s);
       if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
     }
   }
@@ -468,6 +465,59 @@ extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *
   return salsa_ctx_sync(c);
 }
 
+// Forward-mode gradient of the precipitation volume after nsteps in-place
+// steps: gather the cell records, run the derivative kernel (one lane per
+// direction: the cell's bins, then E) and sum the per-cell results.
+extern "C" int salsa_gradient_device(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                                     const double *rsi, const double *dn, const double *rv,
+                                     const double *const n_p[4], const double *const m_p[4], double tstep,
+                                     int32_t nsteps, double *const djdn[4], double *j_cell, double *djde_cell,
+                                     double totals[2]) {
+  int rc = check_ptrs(c, press, temp, rsl, rsi, dn, rv, nullptr, n_p, nullptr, m_p, nullptr, nullptr, tstep,
+                      SALSA_STEP_INPLACE);
+  if (rc) return rc;
+  const DevLayout &L = c->hostL;
+  if (nsteps < 1) { set_error("nsteps must be at least 1"); return SALSA_ERR_ARG; }
+  if (!djdn || !totals) { set_error("null output pointer"); return SALSA_ERR_ARG; }
+  for (int k = 0; k < L.ncat; ++k)
+    if (!djdn[k]) { set_error("null dJ/dN0 field"); return SALSA_ERR_ARG; }
+  if (c->params.lscoag_mode != 1) {
+    set_error("the gradient needs lscoag_mode 1 (kernels evaluated every step)");
+    return SALSA_ERR_ARG;
+  }
+  const int64_t n = c->ncells;
+  if (!c->gbuf && hipMalloc(&c->gbuf, (size_t)(2 * n + 2) * sizeof(double)) != hipSuccess) {
+    set_error("gradient buffer allocation failed");
+    return SALSA_ERR_NOMEM;
+  }
+  hipStream_t s = c->stream;
+  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv, nullptr, n_p, nullptr, m_p, nullptr, nullptr, tstep,
+                         SALSA_STEP_INPLACE);
+  a.kmode = 0;
+  a.kstore = nullptr;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
+  rc = launch_gather(L, c->devL, a, c->rec_in, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "gather launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  GradArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.s = a;
+  g.nsteps = nsteps;
+  g.lanes = L.nt + 1 <= 32 ? 32 : 64;
+  for (int k = 0; k < 4; ++k) g.djdn[k] = k < L.ncat ? djdn[k] : nullptr;
+  g.j_cell = c->gbuf;
+  g.djde_cell = c->gbuf + n;
+  rc = launch_gradient(c->cap, c->params.precision, c->devL, g, c->gbuf + 2 * n, s);
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "gradient launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  if (j_cell) HIPCHK(hipMemcpyAsync(j_cell, c->gbuf, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  if (djde_cell) HIPCHK(hipMemcpyAsync(djde_cell, c->gbuf + n, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  double tot[2] = {0., 0.};
+  HIPCHK(hipMemcpyAsync(tot, c->gbuf + 2 * n, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  rc = salsa_ctx_sync(c);
+  totals[0] = tot[0];
+  totals[1] = tot[1];
+  return rc;
+}
+
 // Time each lanes-per-cell candidate per cost class on the caller's
 // device-resident inputs (tendencies go to scratch) and keep the fastest.
 extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
@@ -494,12 +544,8 @@ extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *t
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  int64_t off = 0;
   for (int k = 0; k < 3; ++k) {
-    StepArgs b = a;
-    b.cell_list = c->ids_sorted + off;
-    b.ncells = c->cls_count_h[k];
-    off += b.ncells;
+    const StepArgs b = class_args(c, a, k);
     if (b.ncells == 0) continue;
     float best = 1e30f;
     const int keep = c->lanes[k];
@@ -508,9 +554,9 @@ extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *t
       SmemGeom G;
       if (make_geom(c->hostL, k == 2, c->params.precision == SALSA_FP64 ? 8 : 4, lanes, false, G)) continue;
       c->lanes[k] = lanes;
-      if (launch_class(c, k, b)) { c->lanes[k] = keep; continue; }   // warm
+      if (launch_class(c, k, b, s)) { c->lanes[k] = keep; continue; }   // warm
       hipEventRecord(e0, s);
-      for (int r = 0; r < 3; ++r) launch_class(c, k, b);
+      for (int r = 0; r < 3; ++r) launch_class(c, k, b, s);
       hipEventRecord(e1, s);
       hipEventSynchronize(e1);
       float ms = 0.f;

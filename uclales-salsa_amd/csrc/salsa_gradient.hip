@@ -1,0 +1,841 @@
+// salsa_gradient.hip -- derivatives of a multi-step SALSA coagulation +
+// condensation trajectory by forward-mode (tangent-linear) differentiation.
+//
+// Objective: J = sum over interior cells of the precipitation-bin volume
+// concentration (all species, m^3 m^-3) after nsteps in-place steps of the
+// box model (fixed T, p, rs, rsi, dn; N, masses and rv evolve).  Cells are
+// independent inside a step, so dJ/dN0 of a cell only involves that cell's
+// trajectory: with one tangent direction per bin of the cell plus one for
+// the collision-kernel enhancement factor E (K = E * coagc, the charge factor
+// of mo_salsa_coagulation_kernels.f90:411-416 with chargeTime =
+// chargeTimeMax), nt + 1 directions give the complete gradient -- the same
+// numbers a reverse sweep would produce, without storing or recomputing the
+// trajectory.
+//
+// Layout: one lane per direction (L = 32 or 64 lanes per cell); the lanes of
+// a cell evaluate the same values and take the same branches, so the
+// wavefront stays converged; each lane carries the tangent of its direction.
+// The per-lane state (numbers, volumes, particle terms, pair kernels, the
+// condensation set-up) lives in the lane's private memory.  The physics is
+// the step kernel's, in dual arithmetic: pack (mo_salsa_driver.f90:236-279),
+// bin properties (classSection.f90:91-224), coagc
+// (mo_salsa_coagulation_kernels.f90:185-420), the sequential sweep
+// (mo_salsa_coagulation_processes.f90:24-1611), equilibration
+// (mo_salsa_properties.f90:73-380) and the predictor-corrector water
+// condensation (mo_salsa_dynamics.f90:691-1157); between steps the state
+// passes through LES units as in run_SALSA.
+//
+// KT is the arithmetic type of the collision-kernel coefficients, as in the
+// step kernel (double: FP64 path; float: FP32-mixed path).
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "salsa_dual.h"
+#include "salsa_internal.h"
+
+namespace salsa {
+
+namespace {
+
+using DD = Dual<double>;
+using DF = Dual<float>;
+
+__device__ __forceinline__ double cube(double x) { return x * x * x; }
+
+__device__ __forceinline__ int64_t g_les_index(const StepArgs &a, int64_t cell) {
+  const int k = (int)(cell % a.nz);
+  const int64_t r = cell / a.nz;
+  const int i = (int)(r % a.nx), j = (int)(r / a.nx);
+  return (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
+}
+
+// Per-cell constants of coagc (mo_salsa_coagulation_kernels.f90:264-266, 323-325, 389-391)
+struct GCellC {
+  double T, visc, mfp, zrhoa, vkin, cshear, cinert;
+};
+__device__ GCellC g_cell_consts(const DevLayout &D, double T, double P) {
+  GCellC c;
+  c.T = T;
+  c.visc = (7.44523e-3 * ::sqrt(T * T * T)) / (5093. * (T + 110.4));
+  c.mfp = (1.656e-10 * T + 1.828e-8) * kPstand / P;
+  c.zrhoa = P / (kRd * T);
+  c.vkin = c.visc / c.zrhoa;
+  c.cshear = ::sqrt(8. * kPi * D.eddy_dis / (15. * c.vkin));
+  c.cinert = kPi * D.eddy_075 / (kGrav * ::sqrt(::sqrt(c.vkin)));
+  return c;
+}
+
+template <class R>
+struct GPT { R D, c, w, v; };      // diffusion coefficient, mean speed, omega, fall speed
+
+// terminal_vel for liquid particles (Beard 1976), mo_particle_external_properties.f90:61-90
+__device__ DD g_tv_liquid(DD diam, double rhop, double rhoa, double visc, DD beta) {
+  if (diam.v <= 19.e-6) return (diam * diam) * (rhop - rhoa) * kGrav * beta / (18. * visc);
+  if (diam.v < 1.07e-3) {
+    const DD nda = 4. / 3. * dcube(diam) * kGrav * (rhop - rhoa) * rhoa / (visc * visc);
+    const DD X = dlog(nda);
+    const DD X2 = X * X;
+    const DD Y = -0.318657E1 + 0.992696 * X - 0.153193E-2 * X2 - 0.987059E-3 * X2 * X
+                 - 0.578878E-3 * X2 * X2 + 0.855176E-4 * X2 * X2 * X - 0.327815E-5 * X2 * X2 * X2;
+    return visc / rhoa / diam * (beta * dexp(Y));
+  }
+  const double np_ = (kSurfw0 * kSurfw0 * kSurfw0) * (rhoa * rhoa) /
+                     ((visc * visc) * (visc * visc) * (rhop - rhoa) * kGrav);
+  const DD bo = 4. / 3. * (rhop - rhoa) * kGrav / kSurfw0 * (diam * diam);
+  const double np6 = ::pow(np_, 1. / 6.);
+  const DD X3 = dlog(bo * np6);
+  const DD X2 = X3 * X3;
+  const DD Y3 = -0.500015E1 + 0.523778E1 * X3 - 0.204914E1 * X2 + 0.475294 * X2 * X3
+                - 0.542819E-1 * X2 * X2 + 0.238449E-2 * X2 * X2 * X3;
+  return dmin(visc / rhoa / diam * (np6 * dexp(Y3)), 9.2);
+}
+
+struct GShape { DD alpha, beta, gamma, sigma; };
+
+// getShapeCoefficients, mo_ice_shape.f90:145-191
+__device__ GShape g_shape(const DevLayout &D, DD mpri, DD mrim, DD numc) {
+  GShape s{dconst(kPi6 * kRhoic), dconst(3.), dconst(kPi / 4.), dconst(2.)};
+  if (numc.v < 1.e-6) return s;
+  const DD mtot = (mpri + mrim) / numc;
+  const DD fr = dmax(dmin(mrim / (mtot * numc), 0.9999), 0.0001);
+  const DD rho_b = fr * kRhori + (1. - fr) * kRhoic;
+  const double dth = ::pow(kPi6 * kRhoic / D.ice_alpha, 1. / (D.ice_beta - 3.));
+  const DD dgr = dpow(D.ice_alpha / (kPi6 * rho_b), 1. / (3. - D.ice_beta));
+  const DD dcr = dpow((1. / (1. - fr)) * D.ice_alpha / (kPi6 * rho_b), 1. / (3. - D.ice_beta));
+  const double mth = kRhoic * kPi6 * cube(dth);
+  const DD mgr = rho_b * kPi6 * dcube(dgr), mcr = rho_b * kPi6 * dcube(dcr);
+  if (mtot >= mth && mtot < mgr) {
+    s = GShape{dconst(D.ice_alpha), dconst(D.ice_beta), dconst(D.ice_gamma), dconst(D.ice_sigma)};
+  } else if (mtot > mcr) {
+    const DD h = 1. / (1. - fr);
+    s = GShape{D.ice_alpha * h, dconst(D.ice_beta), D.ice_gamma * (1. - fr) + kPi / 4. * fr,
+               D.ice_sigma * (1. - fr) + 2. * fr};
+  }
+  return s;
+}
+
+// terminal_vel for ice (Khvorostyanov & Curry 2002), :91-98 with kcVt..kcbre :103-158
+__device__ DD g_tv_ice(DD dsph, DD rhop, double rhoa, double visc, const GShape &sh, DD dnsp) {
+  const double c1 = 0.0902, delta0 = 9.06;
+  const DD vb = kPi6 * dcube(dsph);
+  const DD ap = sh.gamma * dpow(dnsp, sh.sigma);
+  const DD X = (2. * vb * (rhop - rhoa) * kGrav * dnsp * dnsp) / (ap * rhoa * visc * visc);
+  const DD sX = dsqrt(X);
+  const DD k1213 = dsqrt(1. + c1 * sX);
+  const DD bre = 0.5 * c1 * sX / (k1213 - 1.) / k1213;
+  const DD are = 0.25 * delta0 * delta0 * (k1213 - 1.) * (k1213 - 1.) / dpow(X, bre);
+  const DD av = are * dpow(visc, 1. - 2. * bre) * dpow((2. * sh.alpha * kGrav) / (rhoa * sh.gamma), bre);
+  const DD bv = bre * (sh.beta - sh.sigma + 2.) - 1.;
+  return av * dpow(dnsp, bv);
+}
+
+// Particle terms of one colliding particle (coagc :280-286, 327-339)
+__device__ GPT<DD> g_pterms(const GCellC &cc, DD diam, DD mass, DD rhop, int phase, DD dsph, const GShape &sh) {
+  GPT<DD> r;
+  const DD knud = 2. * cc.mfp / diam;
+  const DD beta = 1. + knud * (1.142 + 0.558 * dexp(-0.999 / knud));
+  r.D = beta * kBoltz * cc.T / (3. * kPi * cc.visc * diam);
+  r.c = dsqrt((8. * kBoltz * cc.T) / (kPi * mass));
+  r.w = 8. * r.D / (kPi * r.c);
+  r.v = phase < 4 ? g_tv_liquid(diam, rhop.v, cc.zrhoa, cc.visc, beta)
+                  : g_tv_ice(dsph, rhop, cc.zrhoa, cc.visc, sh, diam);
+  return r;
+}
+
+// The liquid terms in FP32 (mixed path), with the regime of the fall speed
+// chosen on the FP64 diameter and its polynomial regimes kept in FP64.
+__device__ GPT<DF> g_pterms_liq_f(const GCellC &cc, DD diam_d, DD mass_d, double rhop_d) {
+  const DF diam = dcast<float>(diam_d), mass = dcast<float>(mass_d);
+  const float rhop = (float)rhop_d;
+  const float T = (float)cc.T, mfp = (float)cc.mfp, visc = (float)cc.visc, rhoa = (float)cc.zrhoa;
+  const float pi = (float)kPi, g = (float)kGrav, kb = (float)kBoltz;
+  GPT<DF> r;
+  const DF knud = (2.f * mfp) / diam;
+  const DF beta = 1.f + knud * (1.142f + 0.558f * dexp(-0.999f / knud));
+  r.D = (beta * kb * T) / (3.f * pi * visc * diam);
+  r.c = dsqrt((8.f * kb * T) / (pi * mass));
+  r.w = (8.f * r.D) / (pi * r.c);
+  if (diam_d.v <= 19.e-6)
+    r.v = ((diam * diam) * (rhop - rhoa) * g * beta) / (18.f * visc);
+  else
+    r.v = dcast<float>(g_tv_liquid(diam_d, rhop_d, cc.zrhoa, cc.visc, dcast<double>(beta)));
+  return r;
+}
+
+// Collision kernel of one pair (coagc :288-407); FP64 in the reference form,
+// FP32 in the cancellation-free form of the step kernel's mixed path.
+template <class T>
+__device__ Dual<T> g_pair_coef(Dual<T> dA, Dual<T> DA, Dual<T> cA, Dual<T> wA, Dual<T> vA, Dual<T> dB, Dual<T> DB,
+                               Dual<T> cB, Dual<T> wB, Dual<T> vB, bool a_lrg, T vkin, T cshear, T cinert) {
+  using X = Dual<T>;
+  const T pi = T(kPi);
+  const X md = T(0.5) * (dA + dB);
+  const X flux1 = T(4.) * pi * md * (DA + DB);
+  const X flux2 = pi * dsqrt(cA * cA + cB * cB) * (md * md);
+  X tA, tB;
+  if constexpr (sizeof(T) == 8) {
+    const X a2 = md * md + wA * wA, b2 = md * md + wB * wB;
+    tA = ((md + wA) * (md + wA) * (md + wA) - a2 * dsqrt(a2)) / (T(3.) * md * wA) - md;
+    tB = ((md + wB) * (md + wB) * (md + wB) - b2 * dsqrt(b2)) / (T(3.) * md * wB) - md;
+  } else {
+    const X sA = dsqrt(md * md + wA * wA), sB = dsqrt(md * md + wB * wB);
+    tA = (wA * (md + T(2.) * sA + T(4.) * wA - (md * wA) / (md + sA))) / (T(3.) * (md + wA + sA));
+    tB = (wB * (md + T(2.) * sB + T(4.) * wB - (md * wB) / (md + sB))) / (T(3.) * (md + wB + sB));
+  }
+  const X fmd = dsqrt(tA * tA + tB * tB);
+  const X zbrown = flux1 / (md / (md + fmd) + flux1 / flux2);
+  const X d_l = a_lrg ? dA : dB, v_l = a_lrg ? vA : vB, v_s = a_lrg ? vB : vA, D_s = a_lrg ? DB : DA;
+  const X reyn = (d_l * v_l) / vkin;
+  const X schm = vkin / D_s;
+  const X dv = vA > vB ? vA - vB : vB - vA;
+  const X stok = (T(2.) * v_s * dv) / (d_l * T(kGrav));
+  X zbrconv = dconst(T(0.));
+  if (reyn <= T(1.)) zbrconv = T(0.45) * zbrown * dcbrt(reyn * schm);
+  else if (reyn > T(1.)) zbrconv = T(0.45) * zbrown * dsqrt(reyn) * dcbrt(schm);
+  const X ztshear = cshear * (md * md * md);
+  const X zturb = cinert * (md * md) * dv;
+  const X s1 = stok + T(0.5);
+  const X zea = (stok * stok) / (s1 * s1);
+  X zev = dconst(T(0.));
+  if (stok > T(1.214)) {
+    zev = T(0.75) * (dlog(T(2.) * stok) / (stok - T(1.214)));
+    zev = T(1.) / ((T(1.) + zev) * (T(1.) + zev));
+  }
+  const X zecoll = (T(60.) * zev + zea * reyn) / (T(60.) + reyn);
+  const X zgrav = zecoll * pi * (md * md) * dv;
+  X hyp;
+  if constexpr (sizeof(T) == 8) {
+    hyp = dsqrt(zgrav * zgrav + ztshear * ztshear + zturb * zturb);
+  } else {
+    // scaled so that the squares stay clear of the FP32 denormal range; the
+    // tangent (g dg + s ds + t dt) / hyp in the same scaled form
+    const T m = fmaxf(zgrav.v, fmaxf(ztshear.v, zturb.v));
+    const T r = m > 0.f ? DMath<T>::rcp_(m) : 0.f;
+    const T a = zgrav.v * r, b = ztshear.v * r, c = zturb.v * r;
+    const T q = DMath<T>::sqrt_(a * a + b * b + c * c);
+    hyp.v = m * q;
+    hyp.d = q > 0.f ? (a * zgrav.d + b * ztshear.d + c * zturb.d) * DMath<T>::rcp_(q) : 0.f;
+  }
+  return zbrown + zbrconv + hyp;
+}
+
+// rime fraction, classSection.f90:202-224
+__device__ DD g_rime_frac(const DevLayout &D, DD n, const DD *v, int b) {
+  if (D.phase[b] == 4 && v[D.iwa] > 1.e-23 && v[D.irim] > 1.e-23 && n > D.nlim[b])
+    return v[D.irim] * kRhori / (v[D.iwa] * kRhoic + v[D.irim] * kRhori);
+  return dconst(0.);
+}
+
+// acth2o, mo_salsa_dynamics.f90:1127-1157, dry-solute term precomputed
+__device__ __forceinline__ DD g_act_h2o(DD znw, DD zns) {
+  const DD den = dmax(kEpsR8, znw + zns);
+  return dmax(0.1, znw / den);
+}
+
+// Particle terms of an ice bin and of its self-offset twin; updateRhomean
+// (classSection.f90:169-195) and getShapeCoefficients (mo_ice_shape.f90:145-191).
+template <class KT>
+__device__ void g_ice_terms(const DevLayout &D, const GCellC &cc, DD n, double nlim, const DD *v, DD dw,
+                            GPT<Dual<KT>> &o0, GPT<Dual<KT>> &o1) {
+  DD rho;
+  if (n > nlim) {
+    const DD mp = kRhoic * v[D.iwa], mr = kRhori * v[D.irim];
+    rho = dmin((mp * kRhoic + mr * kRhori) / dmax(mp + mr, 7.e-25), kRhoic);
+  } else {
+    rho = dconst(kRhoic);
+  }
+  DD sdry = dconst(0.);
+  for (int s = 0; s < D.nspec - 1; ++s) sdry += v[s] * kRhoic;
+  const GShape sh = g_shape(D, sdry, v[D.nspec - 1] * kRhori, n);
+  const DD mass = rho * kPi6 * dcube(dw);
+  const GPT<DD> p0 = g_pterms(cc, dw, mass, rho, 4, dw, sh);
+  const GPT<DD> p1 = g_pterms(cc, 1.1 * dw, 1.33 * mass, rho, 4, dw, sh);
+  o0 = GPT<Dual<KT>>{dcast<KT>(p0.D), dcast<KT>(p0.c), dcast<KT>(p0.w), dcast<KT>(p0.v)};
+  o1 = GPT<Dual<KT>>{dcast<KT>(p1.D), dcast<KT>(p1.c), dcast<KT>(p1.w), dcast<KT>(p1.v)};
+}
+
+__device__ __forceinline__ double species_rho(const DevLayout &D, int c, int s) {
+  return c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the derivative kernel
+// ---------------------------------------------------------------------------
+// One cell, one direction (dir < nt: dN0 of bin dir; dir == nt: dE; larger:
+// no seed); returns the cell's fault bits.
+template <class KT, int MB, int NS, int NP>
+__device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t cell, int dir) {
+  using DK = Dual<KT>;
+  const StepArgs &a = g.s;
+  const int nt = D.nt;
+  const int nsV = D.level == 5 ? D.nspec : D.nwet;
+  const double dt = a.dt;
+  const int nspec = D.nspec, ndry = D.ndry, iwa = D.iwa, irim = D.irim;
+  {
+    const double *rec = a.rec_in + cell * D.nslot;
+    const double T = rec[0], P = rec[1], rs = rec[2], rsi = rec[3], dn = rec[4];
+    DD rv = dconst(rec[5]);
+    uint32_t flags = 0;
+
+    // state of the cell with the tangent of this lane's direction
+    DD N[MB], V[MB][NS], DW[MB];
+    for (int b = 0; b < nt; ++b) {
+      const int c = D.cat_of[b];
+      const int nsw = c < 3 ? D.nwet : D.nspec;
+      const double *r = rec + D.sbase[b];
+      N[b] = dvar(r[0] * dn, b == dir ? dn : 0.);       // direction b: dN_b = 1 in LES units
+#pragma unroll
+      for (int s = 0; s < NS; ++s) V[b][s] = dconst(s < nsw ? r[1 + s] * dn / species_rho(D, c, s) : 0.);
+    }
+    const DK E = dvar((KT)D.charge_factor, dir == nt ? KT(1) : KT(0));
+
+    for (int step = 0; step < g.nsteps; ++step) {
+      if (!a.lsfreeRH) rv = dmin(rv, rs * a.rhlim);       // mo_salsa_driver.f90:193-197
+
+      // ---- coagulation (mo_salsa_dynamics.f90:92-202) ----
+      if (D.lscoag) {
+        const GCellC cc = g_cell_consts(D, T, P);
+        uint64_t abits = 0;                 // bins with numc >= nlim take part in kernels
+        GPT<DK> PR0[MB], PR1[MB];           // bin and its self-offset twin
+        for (int b = 0; b < nt; ++b) {
+          const DD n = N[b];
+          DD vs = dconst(0.);
+          for (int s = 0; s < nsV; ++s) vs += V[b][s];
+          DD dw = n > D.nlim[b] ? dcbrt(vs / n / kPi6) : dconst(D.dmid[b]);
+          dw = dmin(dw, D.dlim[b]);
+          DW[b] = dw;
+          if (!(n < D.nlim[b])) abits |= 1ull << b;
+          if (D.phase[b] == 4) {
+            g_ice_terms<KT>(D, cc, n, D.nlim[b], V[b], dw, PR0[b], PR1[b]);
+          } else {
+            const DD mass = kRhowa * kPi6 * dcube(dw);
+            if constexpr (sizeof(KT) == 4) {
+              PR0[b] = g_pterms_liq_f(cc, dw, mass, kRhowa);
+              PR1[b] = g_pterms_liq_f(cc, 1.1 * dw, 1.33 * mass, kRhowa);
+            } else {
+              const GShape sh{};
+              PR0[b] = g_pterms(cc, dw, mass, dconst(kRhowa), D.phase[b], dw, sh);
+              PR1[b] = g_pterms(cc, 1.1 * dw, 1.33 * mass, dconst(kRhowa), D.phase[b], dw, sh);
+            }
+          }
+        }
+
+        // pair kernels (buildKernelSelf / buildKernel, kernels.f90:94-180)
+        DK K[NP];
+        uint8_t PF[NP];
+        const KT vkin = (KT)cc.vkin, cshear = (KT)cc.cshear, cinert = (KT)cc.cinert;
+        const int np = D.npair, pf0 = D.pf0, pf1 = D.pf0 + D.ncc;
+        for (int p = 0; p < np; ++p) {
+          const uint32_t pw = D.pw[p];
+          const int b1 = pw & 255u, b2 = (pw >> 8) & 255u, tw = (pw >> 16) & 1u;
+          DK k = dconst(KT(0));
+          uint8_t pfb = 0;
+          if ((abits >> b1) & (abits >> b2) & 1ull) {
+            const DD dB = DW[b2];
+            const DD dA = tw ? 1.1 * dB : DW[b1];
+            const GPT<DK> &A = tw ? PR1[b2] : PR0[b1];
+            const GPT<DK> &B = PR0[b2];
+            k = g_pair_coef<KT>(dcast<KT>(dA), A.D, A.c, A.w, A.v, dcast<KT>(dB), B.D, B.c, B.w, B.v, dA.v >= dB.v,
+                                vkin, cshear, cinert);
+            k = k * E;                                    // charge enhancement (:411-416)
+            if (p >= pf0 && p < pf1) {
+              // precip bin reached by coalescence (accumulatePrecipFormation, processes.f90:1461-1467)
+              const double dnew = ::cbrt(cube(DW[b1].v) + cube(dB.v));
+              int cnt = 0;
+              for (int q = 0; q < D.nb[2]; ++q) cnt += dnew > D.precpbins[q];
+              pfb = dnew > D.precpbins[0] ? (uint8_t)cnt : (uint8_t)0;
+            }
+          }
+          K[p] = k;
+          if (p >= pf0 && p < pf1) PF[p - pf0] = pfb;
+        }
+
+        // sweep: coag_ice, coag_precp, coag_aero, coag_cloud in sequence
+        const int Roff = D.off[2];
+        const double emax = D.Eiagg_max, emin = D.Eiagg_min;
+        for (int it = 0; it < D.ntarget; ++it) {
+          const DevTarget &TG = D.tgt[it];
+          const int t = TG.t;
+          const uint32_t fl = TG.flags;
+          const DD Nt = N[t];
+          DD Vt[NS];
+#pragma unroll
+          for (int s = 0; s < NS; ++s) Vt[s] = s < nsV ? V[t][s] : dconst(0.);
+          bool act = !(Nt < D.nlim[t]);
+          if (fl & TF_ICE) act = act && T < 273.15;      // mo_salsa_dynamics.f90:166
+          if (!act) continue;
+          const DD rft = (fl & TF_ICE) ? g_rime_frac(D, Nt, Vt, t) : dconst(0.);
+          const int nu = (fl & TF_ICE) ? nspec : TG.nupd;
+          const int nupd = TG.nupd;
+          const uint32_t *ent = D.ent + TG.ent0;
+          const int ne = TG.nent;
+
+          DD sink = dconst(0.), so = dconst(0.), sx = dconst(0.), src[NS];
+#pragma unroll
+          for (int s = 0; s < NS; ++s) src[s] = dconst(0.);
+          uint32_t pmask = 0;
+          for (int e = 0; e < ne; ++e) {
+            const uint32_t w = ent[e];
+            const int j = w & 63u, pid = (int)(w >> 16);
+            const uint32_t role = (w >> 6) & 1023u;
+            const DD kj = dcast<double>(K[pid]);
+            const DD Nj = N[j];
+            const DD *Vj = V[j];
+            if (role & R_SINK) sink += kj * Nj;
+            if (role & (R_SINK_EAGG | R_SRC_EAGG)) {      // ice aggregation efficiency (:758-769)
+              const DD eg = emax - dmax(g_rime_frac(D, Nj, Vj, j), rft) * (emax - emin);
+              if (role & R_SINK_EAGG) sink += eg * kj * Nj;
+              if (role & R_SRC_EAGG) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                  if (s < nspec) src[s] += eg * kj * Vj[s];
+              }
+            }
+            if (role & R_SRC) {
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < nupd) src[s] += kj * Vj[s];
+            }
+            if (role & (R_PC_WA | R_PC_RI)) {              // accumulateSourcePhaseChange (:1236-1266)
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < ndry) src[s] += kj * Vj[s];
+              if (role & R_PC_WA) src[iwa] += kj * Vj[iwa] * kRhowa / kRhoic;
+              else src[irim] += kj * Vj[iwa] * kRhowa / kRhori;
+            }
+            if (role & (R_PF_OWN | R_PF_OTH)) {            // accumulatePrecipFormation (:1458-1576)
+              const int pb = (int)PF[pid - D.pf0] - 1;
+              if (pb >= 0) {
+                sink += kj * Nj;
+                pmask |= 1u << pb;
+              } else {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                  if (s < nupd) src[s] += kj * Vj[s];
+              }
+            }
+            if (role & R_FIX_OWN) so += kj * Nj;
+            if (role & R_FIX_OTH) sx += kj * Nj;
+          }
+          DD fix_own = dconst(1.), fix_oth = dconst(1.);
+          if (fl & TF_PF) {                                // fix_coag (:1440-1456)
+            fix_own = dmax(1. - dt * so, 0.1);
+            fix_oth = dmax(1. - dt * sx, 0.1);
+          }
+
+          // self terms
+          DD sink_self = dconst(0.), volsink[NS];
+#pragma unroll
+          for (int s = 0; s < NS; ++s) volsink[s] = dconst(0.);
+          int pself = -1;
+          DD kself = dconst(0.), fixs = dconst(1.);
+          if (fl & (TF_SELF_HALF | TF_SELF_HALF_EAGG | TF_PF_SELF | TF_PRECIP_SELFCOLL)) {
+            const int pid = D.pidx[t][t];
+            kself = pid >= 0 ? dcast<double>(K[pid]) : dconst(0.);
+            if (fl & TF_SELF_HALF) sink_self = 0.5 * kself * Nt;
+            if (fl & TF_SELF_HALF_EAGG) sink_self = (emax - rft * (emax - emin)) * 0.5 * kself * Nt;
+            if (fl & TF_PF_SELF) {
+              fixs = dmax(1. - dt * (0.5 * kself * (Nt * Nt) / Nt), 0.1);
+              pself = pid >= 0 ? (int)PF[pid - D.pf0] - 1 : -1;
+              sink_self = 0.5 * kself * Nt;
+              if (pself >= 0) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                  if (s < nupd) volsink[s] = Vt[s] * Nt * kself * fixs;
+              }
+            }
+          }
+
+          // applyCoag / applyCoagIce / applyCoagPrecp (:539-645)
+          DD Vn[NS];
+          const DD rden = 1. / (1. + dt * sink);
+#pragma unroll
+          for (int s = 0; s < NS; ++s) Vn[s] = s < nu ? (Vt[s] + dt * src[s] * Nt) * rden : Vt[s];
+          DD Nn = Nt / (1. + dt * (sink + sink_self));
+          if (fl & TF_PRECIP_SELFCOLL) {                   // precipSelfCoag (:1588-1611)
+            const DD nslf = 0.5 * kself * Nt * Nt;
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+              if (s < nupd) {
+                const DD vslf = Vt[s] * Nt * kself;
+                V[t + 1][s] = V[t + 1][s] + dt * vslf;
+                Vn[s] = Vn[s] - dt * vslf;
+              }
+            N[t + 1] = N[t + 1] + dt * nslf;
+            Nn = Nn - dt * nslf;
+          }
+          if (fl & TF_PF_SELF) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) Vn[s] = Vn[s] - dt * volsink[s];
+          }
+          N[t] = Nn;
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+            if (s < nsV) V[t][s] = Vn[s];
+
+          // transfer of coalesced cloud drops into precipitation bins (applyCoagPrecipFormation :649-703)
+          if (fl & (TF_PF | TF_PF_SELF)) {
+            uint32_t pm = pmask;
+            if (pself >= 0) pm |= 1u << pself;
+            while (pm) {
+              const int pb = __builtin_ctz(pm);
+              pm &= pm - 1;
+              DD vp[NS], npp = dconst(0.);
+#pragma unroll
+              for (int s = 0; s < NS; ++s) vp[s] = dconst(0.);
+              for (int e = 0; e < ne; ++e) {
+                const uint32_t w = ent[e];
+                const uint32_t role = (w >> 6) & 1023u;
+                if (!(role & (R_PF_OWN | R_PF_OTH))) continue;
+                const int j = w & 63u, pid = (int)(w >> 16);
+                if ((int)PF[pid - D.pf0] - 1 == pb) {
+                  const DD kj = dcast<double>(K[pid]);
+                  const DD fx = (role & R_PF_OWN) ? fix_own : fix_oth;
+                  const DD Nj = N[j];
+#pragma unroll
+                  for (int s = 0; s < NS; ++s)
+                    if (s < nupd) vp[s] += (V[j][s] * Nt + Vt[s] * Nj) * kj * fx;
+                  npp += kj * Nj * Nt * fx;
+                }
+              }
+              if (pb == pself) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) vp[s] += volsink[s];
+                npp += 0.5 * kself * Nt * Nt * fixs;
+              }
+              const int r = Roff + pb;
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < nupd) V[r][s] = V[r][s] + dt * vp[s];
+              N[r] = N[r] + dt * npp;
+            }
+          }
+        }
+
+        // clamp (mo_salsa_dynamics.f90:189-198)
+        for (int b = 0; b < nt; ++b) {
+          if (N[b].v < -1.e-6) flags |= SALSA_CELL_NEG_NUMC;
+          N[b] = dmax(0., N[b]);
+          for (int s = 0; s < nsV; ++s) V[b][s] = dmax(0., V[b][s]);
+        }
+      } else {
+        for (int b = 0; b < nt; ++b) {
+          DD vs = dconst(0.);
+          for (int s = 0; s < nsV; ++s) vs += V[b][s];
+          const DD dw = N[b] > D.nlim[b] ? dcbrt(vs / N[b] / kPi6) : dconst(D.dmid[b]);
+          DW[b] = dmin(dw, D.dlim[b]);
+        }
+      }
+
+      // ---- condensation (mo_salsa_dynamics.f90:276-342, 691-1125) ----
+      if (D.lscnd && (D.lscndh2oae || D.lscndh2ocl || D.lscndh2oic)) {
+        const int A0 = D.off[0];
+        const DD zrh = rv / rs;
+        DD w1 = dconst(0.);
+        for (int b = 0; b < D.nb[0]; ++b) w1 += V[b][iwa];
+        w1 = w1 * kRhowa;
+
+        int isu = -1, ioc = -1;
+        for (int s = 0; s < ndry; ++s) {
+          if (D.dry_id[s] == 0) isu = s;
+          if (D.dry_id[s] == 1) ioc = s;
+        }
+        // equilibration regime 1 (1a), mo_salsa_properties.f90:165-269
+        {
+          DD zke = dconst(1.001);
+          for (int kk = D.in1a; kk <= D.fn1a; ++kk) {
+            const int b = A0 + kk;
+            const DD n = N[b];
+            if (n > kNlim) {
+              const DD v1 = isu >= 0 ? V[b][isu] : dconst(0.), v2 = ioc >= 0 ? V[b][ioc] : dconst(0.);
+              const DD zcore = v1 / n + v2 / n;
+              DD zdwet = DW[b], zdold = dconst(1.), zlwc = dconst(0.);
+              const DD zr = dmin(dmax(zrh, 0.05), 0.98);
+              int count = 0;
+              while (dabs(zdwet / zdold - 1.).v > 1.e-2) {
+                zdold = dmax(zdwet, 1.e-20);
+                const DD aw = zr / zke;
+                const DD bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * (aw * aw)
+                               - 3.1543839e+2 * (aw * aw * aw) + 6.770824e+1 * ((aw * aw) * (aw * aw));
+                const DD bm2 = 1. / (aw * kMwa) - 1. / kMwa;
+                zlwc = v1 * (kRhosu / kMsu) / bm1 + kEpsoc * v2 * (kRhooc / kMoc) / bm2;
+                zdwet = dcbrt(zlwc / n / kRhowa / kPi6 + zcore / kPi6);
+                zke = dexp(2. * kSurfw0 * kMvwa / (kBoltz * T * zdwet));
+                if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
+              }
+              V[b][iwa] = zlwc / kRhowa;
+            }
+          }
+        }
+        // regime 2 (2a/2b, init call), mo_salsa_properties.f90:275-378
+        if (zrh.v < 0.98 || !D.lscndh2oae) {
+          for (int b = A0 + D.in2a; b <= A0 + D.fn2b; ++b) {
+            const DD n = N[b];
+            if (n > kNlim) {
+              DD zv[5] = {dconst(0.), dconst(0.), dconst(0.), dconst(0.), dconst(0.)};
+              for (int s = 0; s < ndry; ++s)
+                if (D.dry_id[s] < 5) zv[D.dry_id[s]] = V[b][s];
+              const DD zcore = zv[0] / n + zv[1] / n + zv[2] / n + zv[3] / n + zv[4] / n;
+              DD zdwet = DW[b], zdold = dconst(1.), zlwc = dconst(0.), zke = dconst(1.02);
+              const DD zr = dmin(dmax(zrh, 0.37), 0.98);
+              int count = 0;
+              while (dabs(zdwet / zdold - 1.).v > 1.e-12) {
+                zdold = dmax(zdwet, 1.e-20);
+                const DD aw = zr / zke;
+                const DD bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * (aw * aw)
+                               - 3.1543839e+2 * (aw * aw * aw) + 6.770824e+1 * ((aw * aw) * (aw * aw));
+                const DD bm2 = 1. / (aw * kMwa) - 1. / kMwa;
+                const DD bm5 = 5.875248e+1 - 1.8781997e+2 * aw + 2.7211377e+2 * (aw * aw)
+                               - 1.8458287e+2 * (aw * aw * aw) + 4.153689e+1 * ((aw * aw) * (aw * aw));
+                zlwc = zv[0] * (kRhosu / kMsu) / bm1 + kEpsoc * (zv[1] * (kRhooc / kMoc)) / bm2 +
+                       zv[4] * (kRhoss / kMss) / bm5;
+                zdwet = dcbrt(zlwc / n / kRhowa / kPi6 + zcore / kPi6);
+                zke = dexp(2. * kSurfw0 * kMvwa / (kBoltz * T * zdwet));
+                if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
+              }
+              V[b][iwa] = zlwc / kRhowa;
+            }
+          }
+        }
+        DD w2 = dconst(0.);
+        for (int b = 0; b < D.nb[0]; ++b) w2 += V[b][iwa];
+        w2 = w2 * kRhowa;
+        rv = rv - (w2 - w1) / (P * kMair / (kRg * T));
+
+        // cell-skip predicate (:784-788; .AND. binds before .OR.)
+        uint32_t any = 0;
+        for (int b = 0; b < nt; ++b) {
+          const int c = D.cat_of[b];
+          const double n = N[b].v;
+          if (c == 0 && n > kNlim) any |= 8u;
+          if (c == 1 && n > kNlim) any |= 1u;
+          if (c == 2 && n > kPrlim) any |= 2u;
+          if (c == 3 && n > kPrlim) any |= 4u;
+        }
+        const bool pass = ((any & 1u) || ((any & 2u) && D.lscndh2ocl)) || ((any & 4u) && D.lscndh2oic) ||
+                          ((any & 8u) && zrh.v > 0.98 && D.lscndh2oae);
+        if (pass) {
+          const double rhoair = kMair * P / (kRg * T);
+          double zdf = (5. / (16. * kAvog * rhoair * 1.e-3 * (3.11e-8 * 3.11e-8))) *
+                       ::sqrt(kRg * 1.e7 * T * kMair * 1.e3 * (kMwa + kMair) * 1.e3 / (2. * kPi * kMwa * 1.e3));
+          zdf = zdf * 1.e-4;
+          const double zmfp = 3. * zdf * ::sqrt(kPi * kMwa / (8. * kRg * T));
+          const double zthc = 0.023807 + 7.1128e-5 * (T - 273.16);
+          const double hl = ((kAlv * kMwa) / (kRg * T)) - 1.;
+          const double hs = ((kAls * kMwa) / (kRg * T)) - 1.;
+          const double csul = rs * rhoair / kMwa, csui = rsi * rhoair / kMwa;
+          DD zcw[MB], zmt[MB], zkl[MB], zns[MB], zws[MB];
+          uint64_t upd = 0, pred = 0, iceq = 0;
+          for (int b = 0; b < nt; ++b) {
+            const int c = D.cat_of[b];
+            const DD n = N[b];
+            DD vs = dconst(0.);
+            for (int s = 0; s < nsV; ++s) vs += V[b][s];
+            DD dw = n > D.nlim[b] ? dcbrt(vs / n / kPi6) : dconst(D.dmid[b]);
+            dw = dmin(dw, D.dlim[b]);                     // updateDiameter (:803-806)
+            DD zn = dconst(0.);
+            for (int s = 0; s < ndry; ++s) zn += D.diss[s] * V[b][s] * D.rholiq[s] / D.mm[s];
+            DD zcw0, zmt0 = dconst(0.), zkl0 = dconst(1.), zws0 = dconst(0.);
+            if (c == 3) {
+              iceq |= 1ull << b;
+              zcw0 = V[b][iwa] * kRhoic / kMwa + V[b][irim] * kRhori / kMwa;
+            } else {
+              zcw0 = V[b][iwa] * kRhowa / kMwa;
+            }
+            const bool on = n > D.nlim[b] &&
+                            (c == 0 ? (zrh.v > 0.98 && D.lscndh2oae)
+                                    : c == 3 ? (D.lscndh2oic && T < 273.15) : D.lscndh2ocl);
+            if (on) {
+              const DD zknud = 2. * zmfp / dw;
+              const DD zbeta = (zknud + 1.) / (0.377 * zknud + 1. + 4. / 3. * (zknud + zknud * zknud));
+              if (c == 3) {                                   // :892-930
+                zkl0 = dexp(4. * kSurfi0 * kMwa / (kRg * T * kRhowa * dw));
+                zws0 = 1.0 * zkl0;
+                const DD h1 = n * 4. * kPi * (0.5 * dw) * zdf * zbeta;
+                const DD h2 = kMwa * zdf * kAls * zws0 * csui / (zthc * T);
+                zmt0 = h1 / (h2 * hs + 1.);
+              } else {                                        // :826-890, :932-955
+                const DD zact = g_act_h2o(V[b][iwa] * D.rholiq[iwa] / D.mm[iwa], zn);
+                zkl0 = dexp(4. * kSurfw0 * kMwa / (kRg * T * kRhowa * dw));
+                zws0 = zact * zkl0;
+                const DD h1 = n * 2. * kPi * dw * zdf * zbeta;
+                const DD h2 = kMwa * zdf * kAlv * zws0 * csul / (zthc * T);
+                zmt0 = h1 / (h2 * hl + 1.);
+              }
+            }
+            zcw[b] = zcw0; zmt[b] = zmt0; zkl[b] = zkl0; zns[b] = zn; zws[b] = zws0;
+            // aerosol 1a bins sit outside the predictor and are never updated (nstr = in2a)
+            const bool inpred = c != 0 || b >= A0 + D.in2a;
+            if (inpred) pred |= 1ull << b;
+            if (inpred && on) upd |= 1ull << b;
+          }
+          DD zcwc = rv * rhoair / kMwa;
+          DD stot = dconst(0.), s1 = dconst(0.), s2 = dconst(0.), sfix = dconst(0.);
+          for (int b = 0; b < nt; ++b) {
+            stot += zcw[b];
+            if ((pred >> b) & 1ull) {
+              s1 += zmt[b] * zws[b] * (((iceq >> b) & 1ull) ? csui : csul);
+              s2 += zmt[b];
+            }
+            if (!((upd >> b) & 1ull)) sfix += zcw[b];
+          }
+          const DD zcwtot = zcwc + stot;
+          const double adt = 2.e-2;
+          DD zcwint = dconst(0.);
+          for (int st = 0; st < a.nsub; ++st) {          // :986-1053
+            const DD zhlp1 = zcwc + adt * s1;
+            const DD zhlp2 = 1. + adt * s2;
+            zcwint = dmin(zhlp1 / zhlp2, zcwtot);
+            DD tsum = dconst(0.), s1n = dconst(0.);
+            for (int b = 0; b < nt; ++b) {
+              const bool ic = (iceq >> b) & 1ull;
+              if ((upd >> b) & 1ull) {
+                const double csu = ic ? csui : csul;
+                const DD d = adt * zmt[b] * (zcwint - zws[b] * csu);
+                DD nw = zcw[b] + dmin(dmax(d, -1.e-1 * zcw[b]), 1.e-1 * zcw[b]);
+                nw = dmax(nw, 0.);
+                zcw[b] = nw;
+                zws[b] = ic ? zkl[b] : g_act_h2o(nw, zns[b]) * zkl[b];
+                tsum += nw;
+              }
+              if ((pred >> b) & 1ull) s1n += zmt[b] * zws[b] * (ic ? csui : csul);
+            }
+            s1 = s1n;
+            zcwint = zcwtot - (sfix + tsum);
+            zcwc = zcwint;
+          }
+          if (a.nsub > 100) flags |= SALSA_CELL_SUBSTEPS;
+          rv = zcwint * kMwa / rhoair;
+          // write-back (:1058-1117)
+          for (int b = 0; b < nt; ++b) {
+            if (!((iceq >> b) & 1ull)) {
+              V[b][iwa] = dmax(0., zcw[b] * kMwa / kRhowa);
+            } else {
+              const DD org_i = V[b][iwa] * kRhoic / kMwa, org_r = V[b][irim] * kRhori / kMwa;
+              const DD orig = org_i + org_r;
+              const DD dvt = zcw[b] - orig;
+              if (dvt.v > 0.) {
+                V[b][iwa] = V[b][iwa] + dvt * kMwa / kRhoic;
+              } else if (dvt.v < 0.) {
+                const DD fr = (zcw[b] / orig) - 1.;
+                V[b][iwa] = V[b][iwa] + (fr * org_i) * kMwa / kRhoic;
+                V[b][irim] = V[b][irim] + (fr * org_r) * kMwa / kRhori;
+              }
+            }
+          }
+        }
+      }
+
+      // ---- the new state through LES units, as the next run_SALSA call sees it ----
+      for (int b = 0; b < nt; ++b) {
+        const int c = D.cat_of[b];
+        const int nsw = c < 3 ? D.nwet : D.nspec;
+        const DD nn = N[b] / dn;
+        if (!isfinite(nn.v)) flags |= SALSA_CELL_NONFINITE;
+        if (!isfinite(nn.d)) flags |= SALSA_CELL_GRAD_UNSTABLE;
+        N[b] = nn * dn;
+        for (int s = 0; s < nsw; ++s) {
+          const double rho = species_rho(D, c, s);
+          const DD mm = V[b][s] * rho / dn;
+          if (!isfinite(mm.v)) flags |= SALSA_CELL_NONFINITE;
+          if (!isfinite(mm.d)) flags |= SALSA_CELL_GRAD_UNSTABLE;
+          V[b][s] = mm * dn / rho;
+        }
+      }
+      if (!isfinite(rv.v)) flags |= SALSA_CELL_NONFINITE;
+      if (!isfinite(rv.d)) flags |= SALSA_CELL_GRAD_UNSTABLE;
+    }
+
+    // objective of this cell: precipitation volume concentration
+    DD J = dconst(0.);
+    for (int b = D.off[2]; b < D.off[2] + D.nb[2]; ++b)
+      for (int s = 0; s < D.nwet; ++s) J += V[b][s];
+    // amplification of a relative change of this direction's input in J:
+    // beyond 1e8 no perturbation resolvable in FP64 behaves linearly
+    const double x0 = dir < nt ? rec[D.sbase[dir]] : (dir == nt ? D.charge_factor : 0.);
+    if (J.v > 0. && fabs(J.d) * x0 > 1.e8 * J.v) flags |= SALSA_CELL_GRAD_UNSTABLE;
+    if (!isfinite(J.d)) flags |= SALSA_CELL_GRAD_UNSTABLE;
+    if (dir < nt) {
+      g.djdn[D.cat_of[dir]][g_les_index(a, cell) + a.n3 * D.loc_of[dir]] = J.d;
+    } else if (dir == nt) {
+      g.j_cell[cell] = J.v;
+      g.djde_cell[cell] = J.d;
+    }
+    return dir <= nt ? flags : 0u;
+  }
+}
+
+template <class KT, int MB, int NS, int NP>
+__global__ void __launch_bounds__(64) k_salsa_gradient(const DevLayout *__restrict__ Dp, GradArgs g) {
+  const DevLayout &D = *Dp;
+  const int LPC = g.lanes, cpb = 64 / LPC;
+  const int dir = (int)threadIdx.x % LPC;
+  for (int64_t grp = blockIdx.x; grp * cpb < g.s.ncells; grp += gridDim.x) {
+    const int64_t cell = grp * cpb + (int64_t)threadIdx.x / LPC;
+    if (cell >= g.s.ncells) break;
+    uint32_t flags = gradient_cell<KT, MB, NS, NP>(D, g, cell, dir);
+    for (int m = 1; m < LPC; m <<= 1) flags |= (uint32_t)__shfl_xor((int)flags, m, LPC);   // over the cell's lanes
+    if (dir == 0) {
+      if (g.s.cell_flags) g.s.cell_flags[cell] = (uint8_t)flags;
+      if (flags && g.s.fault_any) atomicOr(g.s.fault_any, (int)flags);
+    }
+  }
+}
+
+// Sums of the per-cell objective and of dJ/dE over the cells with a stable
+// linearisation, in a fixed order (one block).
+__global__ void __launch_bounds__(1024) k_gradient_sum(const double *__restrict__ x, const double *__restrict__ y,
+                                                       const uint8_t *__restrict__ fl, int64_t n, double *out) {
+  __shared__ double sx[1024], sy[1024];
+  const int t = threadIdx.x;
+  double ax = 0., ay = 0.;
+  for (int64_t i = t; i < n; i += 1024) {
+    ax += x[i];
+    if (!(fl[i] & SALSA_CELL_GRAD_UNSTABLE)) ay += y[i];
+  }
+  sx[t] = ax;
+  sy[t] = ay;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (t < s) {
+      sx[t] += sx[t + s];
+      sy[t] += sy[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = sx[0];
+    out[1] = sy[0];
+  }
+}
+
+template <class KT, int MB, int NS, int NP>
+static int launch_gradient_t(const DevLayout *d, const GradArgs &g, hipStream_t s) {
+  const int cpb = 64 / g.lanes;
+  const int64_t ngrp = (g.s.ncells + cpb - 1) / cpb;
+  // enough wavefronts to cover the machine a few times over; each loops over cell groups
+  const unsigned nblk = (unsigned)(ngrp < 4096 ? ngrp : 4096);
+  if (nblk == 0) return 0;
+  hipLaunchKernelGGL((k_salsa_gradient<KT, MB, NS, NP>), dim3(nblk), dim3(64), 0, s, d, g);
+  return (int)hipGetLastError();
+}
+
+int launch_gradient(int cap, int precision, const DevLayout *d, const GradArgs &g, double *sum_out,
+                    hipStream_t s) {
+  int rc = -1;
+  if (cap == 4)
+    rc = precision == SALSA_FP64 ? launch_gradient_t<double, 32, 3, 528>(d, g, s)
+                                 : launch_gradient_t<float, 32, 3, 528>(d, g, s);
+  else if (cap == 5)
+    rc = precision == SALSA_FP64 ? launch_gradient_t<double, 40, 4, 820>(d, g, s)
+                                 : launch_gradient_t<float, 40, 4, 820>(d, g, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_gradient_sum, dim3(1), dim3(1024), 0, s, g.j_cell, g.djde_cell, g.s.cell_flags, g.s.ncells,
+                     sum_out);
+  return (int)hipGetLastError();
+}
+
+}  // namespace salsa

@@ -163,4 +163,14 @@ struct StepArgs {
   int32_t *fault_any;                 // OR of all cell flags
 };
 
+// Arguments of the forward-mode derivative kernel: the trajectory starts from
+// the cell records in s.rec_in and runs nsteps in-place steps of s.dt.
+struct GradArgs {
+  StepArgs s;
+  int32_t nsteps;
+  int32_t lanes;                      // lanes per cell, one per direction: bins, then E
+  double *djdn[4];                    // dJ/dN0 per category, LES layout of the number fields
+  double *j_cell, *djde_cell;         // per interior cell: J and dJ/dE
+};
+
 }  // namespace salsa

@@ -451,6 +451,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
     T = rec[0]; P = rec[1]; rs = rec[2]; rsi = rec[3]; dn = rec[4]; rv = rec[5];
   }
   if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);     // mo_salsa_driver.f90:193-197
+  const double rv0 = rv;                             // the tendency is taken against the limited value
   uint32_t flags = 0;
 
   // ---- 1. pack (mo_salsa_driver.f90:236-279) from the cell record ----
@@ -1126,10 +1127,12 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   }
   __syncthreads();
 
-  // ---- 6. unpack (mo_salsa_driver.f90:310-388): the new state in LES units
-  //      into the cell's output record (the scatter forms the tendencies) ----
+  // ---- 6. unpack (mo_salsa_driver.f90:310-388): per slot of the cell's
+  //      output record the new state in LES units (in place) or its change
+  //      new - old (tendency mode; the scatter divides by dt and adds) ----
   if (live) {
     double *o = a.rec_out + (int64_t)icell * D.nout;
+    const bool inp = a.inplace;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int b = sub + q * L;
@@ -1138,7 +1141,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
         double *ob = o + (D.sbase[b] - kThermo + 1);
         const double nn = sN[b] / dn;
         if (!isfinite(nn)) flags |= SALSA_CELL_NONFINITE;
-        ob[0] = nn;
+        ob[0] = inp ? nn : nn - rec[D.sbase[b]];
         const int nsw = c < 3 ? D.nwet : D.nspec;
 #pragma unroll
         for (int s = 0; s < NS; ++s)
@@ -1146,13 +1149,13 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
             const double rho = c < 3 ? D.rholiq[s] : (s == D.irim ? kRhori : D.rhoice[s]);
             const double mm = VV(b, s) * rho / dn;
             if (!isfinite(mm)) flags |= SALSA_CELL_NONFINITE;
-            ob[1 + s] = mm;
+            ob[1 + s] = inp ? mm : mm - rec[D.sbase[b] + 1 + s];
           }
       }
     }
     if (sub == 0) {
       if (!isfinite(rv)) flags |= SALSA_CELL_NONFINITE;
-      o[0] = rv;
+      o[0] = inp ? rv : rv - rv0;
     }
   }
   // rate diagnostics into the LES Diag fields (mo_salsa_driver.f90:392-411)
@@ -1193,9 +1196,13 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
 // and the records pass through LDS so that both sides are coalesced.  Each
 // row visit thus covers 32 cells, which keeps the address-translation misses
 // per cell low (every bin of every field is its own row, 8*nzp*nxp*nyp bytes
-// from the next).
+// from the next).  The row base addresses are resolved once per block into
+// LDS, and every thread issues its row loads in batches of eight independent
+// loads before storing them, so that the block keeps many loads in flight.
 // ---------------------------------------------------------------------------
 constexpr int kTileCells = 32;
+constexpr int kRowBatch = 8;
+constexpr int kRowsPerPass = 256 / kTileCells;
 
 // LES rows -> cell records, plus the cost class of every cell: 2 = some cloud /
 // precipitation / ice bin at or above its activity limit (numc >= nlim: it
@@ -1203,36 +1210,49 @@ constexpr int kTileCells = 32;
 // and in the condensation predicate, mo_salsa_dynamics.f90:784-788); 1 = no
 // such bin and RH > 0.98 (aerosol substeps); 0 = no such bin and drier
 // (equilibrium water only).  Classes 0 and 1 run the step-kernel variant
-// without hydrometeor pairs, which is exact for them.
+// without hydrometeor pairs, which is exact for them.  The class of every cell
+// goes to cls, the tile's count per class to tcount[tile][3]; k_class_scan
+// and k_class_fill then build one list of cell ids per class, in ascending
+// cell order.
 __global__ void __launch_bounds__(256) k_gather(const DevLayout *__restrict__ Dp, StepArgs a, double *rec,
-                                                uint8_t *cls, int32_t *ids, int32_t *counts) {
+                                                uint8_t *cls, int32_t *tcount) {
   extern __shared__ __align__(16) char smem[];
-  __shared__ int32_t bc[3];
   const DevLayout &D = *Dp;
   const int ns = D.nslot, ld = ns + 1;
-  double *tile = reinterpret_cast<double *>(smem);                 // [kTileCells][ld]
-  int64_t *gi = reinterpret_cast<int64_t *>(tile + kTileCells * ld);
+  double *tile = reinterpret_cast<double *>(smem);                               // [kTileCells][ld]
+  const double **src = reinterpret_cast<const double **>(tile + kTileCells * ld); // [ns] row bases
+  int64_t *gi = reinterpret_cast<int64_t *>(src + ns);                            // [kTileCells]
   const int64_t c0 = (int64_t)blockIdx.x * kTileCells;
   const int nc = (int)(a.ncells - c0 < kTileCells ? a.ncells - c0 : kTileCells);
   const int tid = threadIdx.x;
   if (tid < kTileCells) gi[tid] = tid < nc ? les_index(a, c0 + tid) : 0;
-  if (tid < 3) bc[tid] = 0;
+  for (int r = tid; r < ns; r += 256) {
+    const int cat = D.slot_cat[r];
+    const double *p;
+    if (cat >= 0)
+      p = (D.slot_mass[r] ? a.m_p[cat] : a.n_p[cat]) + a.n3 * D.slot_row[r];
+    else
+      p = r == 0 ? a.temp : r == 1 ? a.press : r == 2 ? a.rsl
+        : r == 3 ? (D.level == 5 ? a.rsi : nullptr) : r == 4 ? a.dn : a.rv_p;
+    src[r] = p;
+  }
   __syncthreads();
-  for (int e = tid; e < ns * kTileCells; e += 256) {
-    const int r = e / kTileCells, c = e % kTileCells;
-    if (c < nc) {
-      const int64_t g = gi[c];
-      const int cat = D.slot_cat[r];
-      double v;
-      if (cat >= 0) {
-        const double *base = D.slot_mass[r] ? a.m_p[cat] : a.n_p[cat];
-        v = base[g + a.n3 * D.slot_row[r]];
-      } else {
-        const double *f = r == 0 ? a.temp : r == 1 ? a.press : r == 2 ? a.rsl
-                        : r == 3 ? (D.level == 5 ? a.rsi : nullptr) : r == 4 ? a.dn : a.rv_p;
-        v = f ? f[g] : 0.;
+  {
+    const int c = tid % kTileCells, r0 = tid / kTileCells;
+    const int64_t g = gi[c];
+    for (int j0 = r0; j0 < ns; j0 += kRowsPerPass * kRowBatch) {
+      double v[kRowBatch];
+#pragma unroll
+      for (int u = 0; u < kRowBatch; ++u) {
+        const int r = j0 + kRowsPerPass * u;
+        const double *p = r < ns ? src[r] : nullptr;
+        v[u] = (c < nc && p) ? p[g] : 0.;
       }
-      tile[c * ld + r] = v;
+#pragma unroll
+      for (int u = 0; u < kRowBatch; ++u) {
+        const int r = j0 + kRowsPerPass * u;
+        if (r < ns) tile[c * ld + r] = v[u];
+      }
     }
   }
   __syncthreads();
@@ -1240,67 +1260,148 @@ __global__ void __launch_bounds__(256) k_gather(const DevLayout *__restrict__ Dp
     const int c = e / ns, r = e % ns;
     rec[(c0 + c) * ns + r] = tile[c * ld + r];
   }
-  if (cls && tid < nc) {
-    const double *t = tile + tid * ld;
-    const double dn = t[4];
-    bool hyd = false;
-    if (a.kmode == 2)          // stored kernels may pair any bin with non-zero numc
-      for (int b = D.off[1]; b < D.nt; ++b) hyd |= t[D.sbase[b]] * dn != 0.;
-    else
-      for (int b = D.off[1]; b < D.nt; ++b) hyd |= !(t[D.sbase[b]] * dn < D.nlim[b]);
-    double rv = t[5];
-    const double rs = t[2];
-    if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);
-    const int cl = hyd ? 2 : (rv / rs > 0.98 ? 1 : 0);
-    cls[c0 + tid] = (uint8_t)cl;
-    ids[c0 + tid] = (int32_t)(c0 + tid);
-    atomicAdd(&bc[cl], 1);
+  if (cls && tid < kTileCells) {            // the first wavefront classifies the tile
+    int cl = -1;
+    if (tid < nc) {
+      const double *t = tile + tid * ld;
+      const double dn = t[4];
+      bool hyd = false;
+      if (a.kmode == 2)          // stored kernels may pair any bin with non-zero numc
+        for (int b = D.off[1]; b < D.nt; ++b) hyd |= t[D.sbase[b]] * dn != 0.;
+      else
+        for (int b = D.off[1]; b < D.nt; ++b) hyd |= !(t[D.sbase[b]] * dn < D.nlim[b]);
+      double rv = t[5];
+      const double rs = t[2];
+      if (!a.lsfreeRH) rv = fmin(rv, rs * a.rhlim);
+      cl = hyd ? 2 : (rv / rs > 0.98 ? 1 : 0);
+      cls[c0 + tid] = (uint8_t)cl;
+    }
+    const uint64_t m0 = __ballot(cl == 0), m1 = __ballot(cl == 1), m2 = __ballot(cl == 2);
+    if (tid == 0) {
+      int32_t *tc = tcount + 3 * (int64_t)blockIdx.x;
+      tc[0] = __popcll(m0);
+      tc[1] = __popcll(m1);
+      tc[2] = __popcll(m2);
+    }
   }
-  __syncthreads();
-  if (cls && tid < 3 && bc[tid]) atomicAdd(&counts[tid], bc[tid]);
 }
 
-// output records -> tendencies (+=) or, in place, the new state
+// Exclusive prefix over the tiles of each class's count (one block of 1024
+// threads): each thread sums a contiguous run of tiles (loads batched), the
+// run sums are scanned across lanes and wavefronts, and each thread writes
+// its tiles' first positions; counts receives the class totals.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+__global__ void __launch_bounds__(1024) k_class_scan(const int32_t *__restrict__ tcount, int32_t ntile,
+                                                     int32_t *toff, int32_t *counts) {
+  __shared__ int32_t wsum[3][16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int per = (ntile + 1023) / 1024;
+  const int lo = t * per, hi = lo + per < ntile ? lo + per : ntile;
+  int acc[3] = {0, 0, 0};
+#pragma unroll 8
+  for (int i = lo; i < hi; ++i) {
+    acc[0] += tcount[3 * i];
+    acc[1] += tcount[3 * i + 1];
+    acc[2] += tcount[3 * i + 2];
+  }
+  int incl[3];
+  for (int k = 0; k < 3; ++k) {
+    incl[k] = wave_incl_scan(acc[k]);
+    if (lane == 63) wsum[k][w] = incl[k];
+  }
+  __syncthreads();
+  if (t < 64) {
+    for (int k = 0; k < 3; ++k) {
+      const int v = t < 16 ? wsum[k][t] : 0;
+      const int sc = wave_incl_scan(v);
+      if (t < 16) wsum[k][t] = sc - v;            // exclusive prefix of the wavefront sums
+      if (t == 15) counts[k] = sc;
+    }
+  }
+  __syncthreads();
+  int run[3];
+  for (int k = 0; k < 3; ++k) run[k] = wsum[k][w] + incl[k] - acc[k];
+#pragma unroll 8
+  for (int i = lo; i < hi; ++i) {
+    const int c0 = tcount[3 * i], c1 = tcount[3 * i + 1], c2 = tcount[3 * i + 2];
+    toff[3 * i] = run[0];
+    toff[3 * i + 1] = run[1];
+    toff[3 * i + 2] = run[2];
+    run[0] += c0;
+    run[1] += c1;
+    run[2] += c2;
+  }
+}
+
+// Cell ids into the class lists (lists + class * ncells), tile by tile.
+__global__ void __launch_bounds__(64) k_class_fill(const uint8_t *__restrict__ cls, const int32_t *__restrict__ toff,
+                                                   int64_t ncells, int32_t *lists) {
+  const int tid = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * kTileCells;
+  const int nc = (int)(ncells - c0 < kTileCells ? ncells - c0 : kTileCells);
+  const int cl = tid < nc ? (int)cls[c0 + tid] : -1;
+  const uint64_t m0 = __ballot(cl == 0), m1 = __ballot(cl == 1), m2 = __ballot(cl == 2);
+  if (cl >= 0) {
+    const uint64_t below = (1ull << tid) - 1ull;
+    const uint64_t m = cl == 0 ? m0 : cl == 1 ? m1 : m2;
+    lists[(int64_t)cl * ncells + toff[3 * blockIdx.x + cl] + __popcll(m & below)] = (int32_t)(c0 + tid);
+  }
+}
+
+// output records -> tendencies (t += change / dt) or, in place, the new state
 __global__ void __launch_bounds__(256) k_scatter(const DevLayout *__restrict__ Dp, StepArgs a) {
   extern __shared__ __align__(16) char smem[];
   const DevLayout &D = *Dp;
   const int no = D.nout, ld = no + 1;
   double *tile = reinterpret_cast<double *>(smem);
-  int64_t *gi = reinterpret_cast<int64_t *>(tile + kTileCells * ld);
+  double **dst = reinterpret_cast<double **>(tile + kTileCells * ld);   // [no] row bases
+  int64_t *gi = reinterpret_cast<int64_t *>(dst + no);
   const int64_t c0 = (int64_t)blockIdx.x * kTileCells;
   const int nc = (int)(a.ncells - c0 < kTileCells ? a.ncells - c0 : kTileCells);
   const int tid = threadIdx.x;
+  const bool inp = a.inplace;
   if (tid < kTileCells) gi[tid] = tid < nc ? les_index(a, c0 + tid) : 0;
+  for (int r = tid; r < no; r += 256) {
+    double *p;
+    if (r == 0) {
+      p = inp ? a.rv_o : a.rv_t;                      // mo_salsa_driver.f90:387-388
+    } else {                                         // :321-323, :354-355
+      const int sr = r - 1 + kThermo;
+      const int cat = D.slot_cat[sr];
+      double *base = D.slot_mass[sr] ? (inp ? a.m_o[cat] : a.m_t[cat]) : (inp ? a.n_o[cat] : a.n_t[cat]);
+      p = base + a.n3 * D.slot_row[sr];
+    }
+    dst[r] = p;
+  }
   for (int e = tid; e < nc * no; e += 256) {
     const int c = e / no, r = e % no;
     tile[c * ld + r] = a.rec_out[(c0 + c) * no + r];
   }
   __syncthreads();
-  for (int e = tid; e < no * kTileCells; e += 256) {
-    const int r = e / kTileCells, c = e % kTileCells;
-    if (c < nc) {
-      const int64_t g = gi[c];
-      const double x = tile[c * ld + r];
-      if (r == 0) {
-        if (a.inplace) {
-          a.rv_o[g] = x;
-        } else {
-          double rv_old = a.rv_p[g];                 // mo_salsa_driver.f90:193-197, 387-388
-          if (!a.lsfreeRH) rv_old = fmin(rv_old, a.rsl[g] * a.rhlim);
-          a.rv_t[g] = a.rv_t[g] + (x - rv_old) / a.dt;
-        }
-      } else {
-        const int sr = r - 1 + kThermo;
-        const int cat = D.slot_cat[sr];
-        const int64_t i = g + a.n3 * D.slot_row[sr];
-        if (a.inplace) {
-          double *dst = D.slot_mass[sr] ? a.m_o[cat] : a.n_o[cat];
-          dst[i] = x;
-        } else {                                     // :321-323, :354-355
-          const double *old = D.slot_mass[sr] ? a.m_p[cat] : a.n_p[cat];
-          double *dst = D.slot_mass[sr] ? a.m_t[cat] : a.n_t[cat];
-          dst[i] = dst[i] + (x - old[i]) / a.dt;
-        }
+  const int c = tid % kTileCells, r0 = tid / kTileCells;
+  const int64_t g = gi[c];
+  for (int j0 = r0; j0 < no; j0 += kRowsPerPass * kRowBatch) {
+    double v[kRowBatch];
+#pragma unroll
+    for (int u = 0; u < kRowBatch; ++u) {
+      const int r = j0 + kRowsPerPass * u;
+      v[u] = (!inp && r < no && c < nc) ? dst[r][g] : 0.;
+    }
+#pragma unroll
+    for (int u = 0; u < kRowBatch; ++u) {
+      const int r = j0 + kRowsPerPass * u;
+      if (r < no && c < nc) {
+        const double x = tile[c * ld + r];
+        dst[r][g] = inp ? x : v[u] + x / a.dt;
       }
     }
   }
@@ -1308,13 +1409,21 @@ __global__ void __launch_bounds__(256) k_scatter(const DevLayout *__restrict__ D
 
 size_t record_tile_lds(const DevLayout &Lay) {
   const int n = Lay.nslot > Lay.nout ? Lay.nslot : Lay.nout;
-  return (size_t)kTileCells * (n + 1) * sizeof(double) + kTileCells * sizeof(int64_t);
+  return (size_t)kTileCells * (n + 1) * sizeof(double) + (size_t)n * sizeof(void *) + kTileCells * sizeof(int64_t);
 }
 
+int64_t gather_tiles(int64_t ncells) { return (ncells + kTileCells - 1) / kTileCells; }
+
+// Records of every cell; with cls != null also the class lists (tcount and
+// toff: [tiles][3] work arrays) and the class counts.
 int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, double *rec, uint8_t *cls,
-                  int32_t *ids, int32_t *counts, hipStream_t s) {
-  const unsigned nb = (unsigned)((a.ncells + kTileCells - 1) / kTileCells);
-  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), record_tile_lds(Lay), s, d, a, rec, cls, ids, counts);
+                  int32_t *tcount, int32_t *toff, int32_t *lists, int32_t *counts, hipStream_t s) {
+  const unsigned nb = (unsigned)gather_tiles(a.ncells);
+  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), record_tile_lds(Lay), s, d, a, rec, cls, tcount);
+  if (cls) {
+    hipLaunchKernelGGL(k_class_scan, dim3(1), dim3(1024), 0, s, tcount, (int32_t)nb, toff, counts);
+    hipLaunchKernelGGL(k_class_fill, dim3(nb), dim3(64), 0, s, cls, toff, a.ncells, lists);
+  }
   return (int)hipGetLastError();
 }
 

@@ -107,6 +107,20 @@ MODULE mo_salsa_amd_cabi
        INTEGER(C_INT32_T), VALUE :: istp, flags
      END FUNCTION salsa_step_host
 
+     ! Forward-mode gradient of the precipitation volume after nsteps in-place
+     ! steps (device pointers, e.g. from hipMalloc or OpenMP/OpenACC device data)
+     INTEGER(C_INT) FUNCTION salsa_gradient_device(ctx, press, temp, rsl, rsi, dn, rv, n_p, m_p, tstep, &
+                                                   nsteps, djdn, j_cell, djde_cell, totals) &
+                                                   BIND(C, name="salsa_gradient_device")
+       IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE        :: ctx, press, temp, rsl, rsi, dn, rv
+       TYPE(C_PTR), INTENT(in)   :: n_p(4), m_p(4), djdn(4)
+       REAL(C_DOUBLE), VALUE     :: tstep
+       INTEGER(C_INT32_T), VALUE :: nsteps
+       TYPE(C_PTR), VALUE        :: j_cell, djde_cell
+       REAL(C_DOUBLE), INTENT(out) :: totals(2)
+     END FUNCTION salsa_gradient_device
+
      INTEGER(C_INT) FUNCTION salsa_ctx_sync(ctx) BIND(C, name="salsa_ctx_sync")
        IMPORT :: C_INT, C_PTR
        TYPE(C_PTR), VALUE :: ctx

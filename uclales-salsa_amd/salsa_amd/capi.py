@@ -34,6 +34,7 @@ CELL_NEG_NUMC = 1
 CELL_NOCONV = 2
 CELL_SUBSTEPS = 4
 CELL_NONFINITE = 8
+CELL_GRAD_UNSTABLE = 16
 
 SPECIES = {"SO4": 0, "OC": 1, "BC": 2, "DU": 3, "SS": 4, "NO": 5, "NH": 6}
 
@@ -160,6 +161,9 @@ def lib():
     L.salsa_ctx_timing.restype = C.c_int
     L.salsa_ctx_kernel_time.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.salsa_ctx_kernel_time.restype = C.c_int
+    L.salsa_gradient_device.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [_P4, _P4, C.c_double, C.c_int32,
+                                                                          _P4, C.c_void_p, C.c_void_p, _dp]
+    L.salsa_gradient_device.restype = C.c_int
     L.salsa_last_error.argtypes = []
     L.salsa_last_error.restype = C.c_char_p
     _lib = L
@@ -283,6 +287,22 @@ class Context:
 
     def sync(self):
         return _check(lib().salsa_ctx_sync(self._h))
+
+    def gradient_device(self, f, n_p, m_p, djdn, tstep=1.0, nsteps=1, j_cell=None, djde_cell=None):
+        """Forward-mode gradient of J = precipitation volume concentration
+        summed over interior cells after `nsteps` in-place steps (inputs are
+        not modified).  djdn: per category a float64 tensor shaped like
+        n_p[c] receiving dJ/dN0 (interior cells); j_cell / djde_cell:
+        optional per-cell outputs.  Returns (J, dJ/dE) after waiting for the
+        context stream."""
+        P = lambda xs: _P4(*[_ptr(x) for x in (list(xs) + [None] * (4 - len(xs)))])
+        tot = (C.c_double * 2)()
+        rc = self._ordered(lambda: lib().salsa_gradient_device(
+            self._h, _ptr(f["press"]), _ptr(f["temp"]), _ptr(f["rsl"]), _ptr(f.get("rsi")), _ptr(f["dn"]),
+            _ptr(f["rv"]), P(n_p), P(m_p), tstep, nsteps, P(djdn), _ptr(j_cell), _ptr(djde_cell),
+            C.cast(tot, _dp)))
+        _check(rc)
+        return tot[0], tot[1]
 
     def cell_flags(self):
         out = np.zeros(self.ncells, dtype=np.uint8)

@@ -347,7 +347,7 @@ def generate(L, grid, seed, level=4, t_range=(275., 295.), p_range=(900e2, 1000e
 
 
 CONFIGS = {
-    # BASELINE configs at the reference-legal layout nbin=(3,4): 11/8/7 (+ice 7)
+    # benchmark workloads 1-5 at the reference-legal layout nbin=(3,4): 11/8/7 (+ice 7)
     1: dict(grid=(40, 8, 8), seed=1, level=4, cloud_mode="random", cloud_frac=0.3),
     2: dict(grid=(120, 64, 64), seed=2, level=4, cloud_mode="slab", cloud_slab=(60, 90)),
     3: dict(grid=(80, 32, 32), seed=3, level=4, cloud_mode="random", cloud_frac=0.3),
