@@ -865,69 +865,100 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
     }
     w1 = cell_sum<L>(w1) * kRhowa;
 
-    // equilibration regime 1 (1a bins, sequential: the Kelvin factor carries
-    // over between bins), mo_salsa_properties.f90:165-269
+    // Equilibration (mo_salsa_properties.f90:73-380), both regimes in one
+    // loop.  Regime 1 (1a bins, :165-269) is one sequential chain -- the
+    // Kelvin factor carries over from bin to bin -- and runs on lane 0 of the
+    // cell; the bins of regime 2 (2a/2b, init call, :275-378; only when
+    // zrh < 0.98 or without aerosol condensation) are independent and go to
+    // the cell's lanes 1, 2, ... (lane 0 takes what is left after its chain).
+    // Every lane iterates its current bin's Kelvin fixed point; the
+    // wavefront loops until all lanes are done, so the two regimes overlap.
     int isu = -1, ioc = -1;
     for (int s = 0; s < ndry; ++s) {
       if (D.dry_id[s] == 0) isu = s;
       if (D.dry_id[s] == 1) ioc = s;
     }
-    if (sub == 0 && live) {
-      double zke = 1.001;
-      for (int kk = D.in1a; kk <= D.fn1a; ++kk) {
-        const int b = A0 + kk;
-        const double n = sN[b];
-        if (n > kNlim) {
-          const double v1 = isu >= 0 ? VV(b, isu) : 0., v2 = ioc >= 0 ? VV(b, ioc) : 0.;
-          const double zcore = v1 / n + v2 / n;
-          double zdwet = sDW[b], zdold = 1., zlwc = 0.;
-          const double zr = fmin(fmax(zrh, 0.05), 0.98);
-          int count = 0;
-          while (fabs(zdwet / zdold - 1.) > 1.e-2) {
-            zdold = fmax(zdwet, 1.e-20);
-            const double aw = div_nr(zr, zke);
-            const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * (aw * aw)
-                               - 3.1543839e+2 * (aw * aw * aw) + 6.770824e+1 * ((aw * aw) * (aw * aw));
-            const double bm2 = rcp_nr(aw * kMwa) - 1. / kMwa;
-            zlwc = div_nr(v1 * (kRhosu / kMsu), bm1) + div_nr(kEpsoc * v2 * (kRhooc / kMoc), bm2);
-            zdwet = cbrt_fast(div_nr(div_nr(div_nr(zlwc, n), kRhowa), kPi6) + div_nr(zcore, kPi6));
-            zke = ::exp(div_nr(2. * kSurfw0 * kMvwa, kBoltz * T * zdwet));
-            if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
+    {
+      const bool reg2 = zrh < 0.98 || !D.lscndh2oae;
+      const int n1 = D.fn1a - D.in1a + 1;
+      const int n2 = reg2 ? D.fn2b - D.in2a + 1 : 0;
+      // this lane's queue: regime-1 bins (lane 0), then regime-2 bins j with
+      // (j + 1) % L == sub
+      int q1 = sub == 0 ? 0 : n1;           // next regime-1 position
+      int q2 = (sub + L - 1) % L;           // next regime-2 index
+      double zke1 = 1.001;                  // regime-1 Kelvin factor, carried over
+      int b = -1, reg = 0, count = 0;
+      double n = 1., zcore = 0., c1 = 0., c2 = 0., c5 = 0., zr = 0., tol = 0.;
+      double zdwet = 0., zdold = 1., zlwc = 0., zke = 1.;
+      auto next_task = [&]() {
+        reg = 0;
+        while (q1 < n1) {                   // regime 1: skip bins with numc <= nlim
+          const int bb = A0 + D.in1a + q1++;
+          if (sN[bb] > kNlim) {
+            b = bb; reg = 1; n = sN[bb];
+            const double v1 = isu >= 0 ? VV(bb, isu) : 0., v2 = ioc >= 0 ? VV(bb, ioc) : 0.;
+            zcore = v1 / n + v2 / n;
+            c1 = v1 * (kRhosu / kMsu);
+            c2 = kEpsoc * v2 * (kRhooc / kMoc);
+            c5 = 0.;
+            zr = fmin(fmax(zrh, 0.05), 0.98);
+            tol = 1.e-2;
+            zke = zke1;
+            zdwet = sDW[bb]; zdold = 1.; zlwc = 0.; count = 0;
+            return;
           }
-          VV(b, iwa) = zlwc / kRhowa;
         }
-      }
-    }
-    // regime 2 (2a/2b bins, init call), mo_salsa_properties.f90:275-378
-    if (zrh < 0.98 || !D.lscndh2oae) {
-#pragma unroll 1
-      for (int q = 0; q < NQ; ++q) {
-        const int b = sub + q * L;
-        if (live && b >= A0 + D.in2a && b <= A0 + D.fn2b) {
-          const double n = sN[b];
-          if (n > kNlim) {
+        while (q2 < n2) {                   // regime 2
+          const int bb = A0 + D.in2a + q2;
+          q2 += L;
+          if (sN[bb] > kNlim) {
+            b = bb; reg = 2; n = sN[bb];
             double zv[5] = {0., 0., 0., 0., 0.};
             for (int s = 0; s < ndry; ++s)
-              if (D.dry_id[s] < 5) zv[D.dry_id[s]] = VV(b, s);
-            const double zcore = zv[0] / n + zv[1] / n + zv[2] / n + zv[3] / n + zv[4] / n;
-            double zdwet = sDW[b], zdold = 1., zlwc = 0., zke = 1.02;
-            const double zr = fmin(fmax(zrh, 0.37), 0.98);
-            int count = 0;
-            while (fabs(zdwet / zdold - 1.) > 1.e-12) {
-              zdold = fmax(zdwet, 1.e-20);
-              const double aw = div_nr(zr, zke);
-              const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * (aw * aw)
-                                 - 3.1543839e+2 * (aw * aw * aw) + 6.770824e+1 * ((aw * aw) * (aw * aw));
-              const double bm2 = rcp_nr(aw * kMwa) - 1. / kMwa;
-              const double bm5 = 5.875248e+1 - 1.8781997e+2 * aw + 2.7211377e+2 * (aw * aw)
-                                 - 1.8458287e+2 * (aw * aw * aw) + 4.153689e+1 * ((aw * aw) * (aw * aw));
-              zlwc = div_nr(zv[0] * (kRhosu / kMsu), bm1) + div_nr(kEpsoc * (zv[1] * (kRhooc / kMoc)), bm2) +
-                     div_nr(zv[4] * (kRhoss / kMss), bm5);
-              zdwet = cbrt_fast(div_nr(div_nr(div_nr(zlwc, n), kRhowa), kPi6) + div_nr(zcore, kPi6));
-              zke = ::exp(div_nr(2. * kSurfw0 * kMvwa, kBoltz * T * zdwet));
-              if (++count > 100) { flags |= SALSA_CELL_NOCONV; break; }
-            }
-            VV(b, iwa) = zlwc / kRhowa;
+              if (D.dry_id[s] < 5) zv[D.dry_id[s]] = VV(bb, s);
+            zcore = zv[0] / n + zv[1] / n + zv[2] / n + zv[3] / n + zv[4] / n;
+            c1 = zv[0] * (kRhosu / kMsu);
+            c2 = kEpsoc * (zv[1] * (kRhooc / kMoc));
+            c5 = zv[4] * (kRhoss / kMss);
+            zr = fmin(fmax(zrh, 0.37), 0.98);
+            tol = 1.e-12;
+            zke = 1.02;
+            zdwet = sDW[bb]; zdold = 1.; zlwc = 0.; count = 0;
+            return;
+          }
+        }
+      };
+      if (live) next_task();
+      // a task is live while the reference's loop condition holds
+      auto finish = [&]() {
+        VV(b, iwa) = zlwc / kRhowa;
+        if (reg == 1) zke1 = zke;
+        next_task();
+      };
+      while (reg && !(fabs(zdwet / zdold - 1.) > tol)) finish();   // condition false on entry
+#pragma unroll 1
+      while (__any(reg != 0)) {
+        if (reg) {
+          zdold = fmax(zdwet, 1.e-20);
+          const double aw = div_nr(zr, zke);
+          const double aw2 = aw * aw;
+          const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * aw2
+                             - 3.1543839e+2 * (aw2 * aw) + 6.770824e+1 * (aw2 * aw2);
+          const double bm2 = rcp_nr(aw * kMwa) - 1. / kMwa;
+          double l = div_nr(c1, bm1) + div_nr(c2, bm2);
+          if (reg == 2) {
+            const double bm5 = 5.875248e+1 - 1.8781997e+2 * aw + 2.7211377e+2 * aw2
+                               - 1.8458287e+2 * (aw2 * aw) + 4.153689e+1 * (aw2 * aw2);
+            l = l + div_nr(c5, bm5);
+          }
+          zlwc = l;
+          zdwet = cbrt_fast(div_nr(div_nr(div_nr(zlwc, n), kRhowa), kPi6) + div_nr(zcore, kPi6));
+          zke = ::exp(div_nr(2. * kSurfw0 * kMvwa, kBoltz * T * zdwet));
+          bool done = !(fabs(zdwet / zdold - 1.) > tol);
+          if (++count > 100) { flags |= SALSA_CELL_NOCONV; done = true; }
+          if (done) {
+            finish();
+            while (reg && !(fabs(zdwet / zdold - 1.) > tol)) finish();
           }
         }
       }
@@ -977,7 +1008,6 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
       // the lane's bins in registers.
       double *cs = reinterpret_cast<double *>(cb + G.opr);   // [5][nt]
       uint32_t upd = 0;        // bit q: bin updated in the substeps
-      uint32_t pred = 0;       // bit q: bin in the vapour predictor sums
       uint32_t iceq = 0;       // bit q: ice bin
 #pragma unroll 1
       for (int q = 0; q < NQ; ++q) {
@@ -1024,28 +1054,69 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
           cs[b] = zcw0; cs[nt + b] = zmt0; cs[2 * nt + b] = zkl0; cs[3 * nt + b] = zn; cs[4 * nt + b] = zws0;
           // aerosol 1a bins sit outside the predictor and are never updated (nstr = in2a)
           const bool inpred = c != 0 || b >= A0 + D.in2a;
-          if (inpred) pred |= 1u << q;
           if (inpred && on) upd |= 1u << q;
         }
       }
-      double zcw[NQ], zmt[NQ], zkl[NQ], zns[NQ], zws[NQ];
+      double zcwc = rv * rhoair / kMwa;
+      // total and fixed water over every bin (bins not updated keep theirs)
+      double stot = 0., sfix = 0.;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int b = sub + q * L;
-        zcw[q] = 0.; zmt[q] = 0.; zkl[q] = 1.; zns[q] = 0.; zws[q] = 0.;
         if (b < nt) {
-          zcw[q] = cs[b]; zmt[q] = cs[nt + b]; zkl[q] = cs[2 * nt + b]; zns[q] = cs[3 * nt + b];
-          zws[q] = cs[4 * nt + b];
+          stot += cs[b];
+          if (!((upd >> q) & 1u)) sfix += cs[b];
         }
       }
-      double zcwc = rv * rhoair / kMwa;
-      double stot = 0., s1 = 0., s2 = 0., sfix = 0.;
+      // The substeps touch only the updated bins (a predictor bin that is not
+      // updated has zmt = 0 and adds nothing): deal them round-robin onto the
+      // cell's lanes in bin order, so that each lane iterates ceil(k/L) bins
+      // instead of every bin it owns.
+      double zcw[NQ], zmt[NQ], zkl[NQ], zns[NQ], zws[NQ];
+      int cb_[NQ];
+      uint32_t iceq_c = 0;
+      int nq = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) cb_[q] = -1;
+      if (live) {
+        int rank = 0;
+        for (int b = 0; b < nt; ++b) {
+          const int c = D.cat_of[b];
+          const bool on = sN[b] > D.nlim[b] &&
+                          (c == 0 ? (zrh > 0.98 && D.lscndh2oae)
+                                  : c == 3 ? (D.lscndh2oic && T < 273.15) : D.lscndh2ocl);
+          if (!(on && (c != 0 || b >= A0 + D.in2a))) continue;
+          if (rank % L == sub) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+              if (q == nq) cb_[q] = b;
+            ++nq;
+          }
+          ++rank;
+        }
+      }
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        stot += zcw[q];
-        if ((pred >> q) & 1u) { s1 += zmt[q] * zws[q] * (((iceq >> q) & 1u) ? csui : csul); s2 += zmt[q]; }
-        if (!((upd >> q) & 1u)) sfix += zcw[q];
+        const int b = cb_[q];
+        zcw[q] = 0.; zmt[q] = 0.; zkl[q] = 1.; zns[q] = 0.; zws[q] = 0.;
+        if (b >= 0) {
+          zcw[q] = cs[b]; zmt[q] = cs[nt + b]; zkl[q] = cs[2 * nt + b]; zns[q] = cs[3 * nt + b];
+          zws[q] = cs[4 * nt + b];
+          if (D.cat_of[b] == 3) iceq_c |= 1u << q;
+        }
       }
+      // slots in use anywhere in the wavefront (a uniform bound for the loops)
+      uint32_t qw = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (__any(q < nq)) qw |= 1u << q;
+      double s1 = 0., s2 = 0.;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if ((qw >> q) & 1u) {
+          s1 += zmt[q] * zws[q] * (((iceq_c >> q) & 1u) ? csui : csul);
+          s2 += zmt[q];
+        }
       stot = cell_sum<L>(stot);
       s1 = cell_sum<L>(s1);
       s2 = cell_sum<L>(s2);
@@ -1063,16 +1134,19 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
         double tsum = 0., s1n = 0.;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          if ((upd >> q) & 1u) {
-            const double csu = ((iceq >> q) & 1u) ? csui : csul;
-            const double d = adt * zmt[q] * (zcwint - zws[q] * csu);
-            double nw = zcw[q] + fmin(fmax(d, -1.e-1 * zcw[q]), 1.e-1 * zcw[q]);
-            nw = fmax(nw, 0.);
-            zcw[q] = nw;
-            zws[q] = ((iceq >> q) & 1u) ? zkl[q] : act_h2o<true>(nw, zns[q]) * zkl[q];
-            tsum += nw;
+          if ((qw >> q) & 1u) {
+            if (q < nq) {
+              const bool ic = (iceq_c >> q) & 1u;
+              const double csu = ic ? csui : csul;
+              const double d = adt * zmt[q] * (zcwint - zws[q] * csu);
+              double nw = zcw[q] + fmin(fmax(d, -1.e-1 * zcw[q]), 1.e-1 * zcw[q]);
+              nw = fmax(nw, 0.);
+              zcw[q] = nw;
+              zws[q] = ic ? zkl[q] : act_h2o<true>(nw, zns[q]) * zkl[q];
+              tsum += nw;
+              s1n += zmt[q] * zws[q] * csu;
+            }
           }
-          if ((pred >> q) & 1u) s1n += zmt[q] * zws[q] * (((iceq >> q) & 1u) ? csui : csul);
         }
         tsum = cell_sum<L>(tsum);
         s1 = cell_sum<L>(s1n);
@@ -1081,14 +1155,20 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
       }
       if (nsub > 100) flags |= SALSA_CELL_SUBSTEPS;
       rv = zcwint * kMwa / rhoair;
+      // updated water back to the per-bin table, then every bin is written back
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (q < nq) cs[cb_[q]] = zcw[q];
+      __syncthreads();
       // write-back (:1058-1117)
       double dcat[4] = {0., 0., 0., 0.};     // Cond_a/c/p%volc(iwa), Cond_i composite
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int b = sub + q * L;
         if (b < nt) {
+          const double zw = cs[b];
           if (!((iceq >> q) & 1u)) {
-            const double nv = fmax(0., zcw[q] * kMwa / kRhowa);
+            const double nv = fmax(0., zw * kMwa / kRhowa);
             if (R) {
               const int c = D.cat_of[b];
               const double dv = nv - VV(b, iwa);
@@ -1100,12 +1180,12 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
           } else {
             const double org_i = VV(b, iwa) * kRhoic / kMwa, org_r = VV(b, irim) * kRhori / kMwa;
             const double orig = org_i + org_r;
-            const double dvt = zcw[q] - orig;
+            const double dvt = zw - orig;
             if (dvt > 0.) {
               VV(b, iwa) = VV(b, iwa) + dvt * kMwa / kRhoic;
               dcat[3] += (dvt * kMwa / kRhoic) * kRhoic;
             } else if (dvt < 0.) {
-              const double fr = (zcw[q] / orig) - 1.;
+              const double fr = (zw / orig) - 1.;
               VV(b, iwa) = VV(b, iwa) + (fr * org_i) * kMwa / kRhoic;
               VV(b, irim) = VV(b, irim) + (fr * org_r) * kMwa / kRhori;
               dcat[3] += ((fr * org_i) * kMwa / kRhoic) * kRhoic + ((fr * org_r) * kMwa / kRhori) * kRhori;
