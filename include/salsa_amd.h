@@ -41,7 +41,8 @@ enum {
                                   (mo_salsa_properties.f90:238-243, :352-357)       */
   SALSA_ERR_NONFINITE = -4,    /* NaN/Inf in the result (check_arrays analogue,
                                   mo_salsa.f90:157-192)                              */
-  SALSA_ERR_NOMEM = -5
+  SALSA_ERR_NOMEM = -5,
+  SALSA_ERR_DISTR = -6         /* distr_update failed in some cell (SALSA_CELL_DISTR)   */
 };
 
 /* per-cell fault bits returned by salsa_get_cell_flags() */
@@ -50,10 +51,12 @@ enum {
   SALSA_CELL_NOCONV = 2,       /* equilibration did not converge in 100 iterations                 */
   SALSA_CELL_SUBSTEPS = 4,     /* > 100 condensation substeps (mo_salsa_dynamics.f90:1051)         */
   SALSA_CELL_NONFINITE = 8,    /* NaN/Inf produced                                                  */
-  SALSA_CELL_GRAD_UNSTABLE = 16 /* salsa_gradient_device: the linearised trajectory of the cell is
+  SALSA_CELL_GRAD_UNSTABLE = 16, /* salsa_gradient_device: the linearised trajectory of the cell is
                                    unstable -- a relative input change is amplified more than 1e8
                                    times in J, or a derivative left the range of its type; the
                                    derivative then describes no perturbation resolvable in FP64 */
+  SALSA_CELL_DISTR = 32        /* distr_update did not settle in 100 sweeps or moved a negative
+                                  fraction (the reference STOPs, mo_salsa_update.f90:137, 216, 311) */
 };
 
 /* Process-rate diagnostic fields written by a step when `rates` is given:
@@ -95,8 +98,7 @@ enum {
 enum {
   SALSA_STEP_TENDENCY = 0,     /* run_SALSA semantics: p-fields read-only, t-fields accumulated     */
   SALSA_STEP_INPLACE = 1,      /* box-model / device-resident: write the new state into the p-fields */
-  SALSA_STEP_NO_COMPACT = 2,   /* do not reorder cells by cost class (debug / comparison)          */
-  SALSA_STEP_NO_GRAPH = 4      /* launch directly instead of through the captured hipGraph          */
+  SALSA_STEP_NO_COMPACT = 2    /* do not reorder cells by cost class (debug / comparison)          */
 };
 
 /*
@@ -151,6 +153,11 @@ typedef struct {
                                   cgintvl seconds, stored per cell in between
                                   (mo_salsa_driver.f90:469-472, :601-697)          */
   double cgintvl;              /* kernel update interval [s] for lscoag_mode 2    */
+  int32_t lsdistupdate;        /* size-distribution update after condensation
+                                  (distr_update, mo_salsa.f90:148-149,
+                                  mo_salsa_update.f90:21-433); the reference
+                                  default is on, salsa_params_default leaves it
+                                  off (coagulation + condensation only)         */
 } salsa_params;
 
 typedef struct salsa_ctx salsa_ctx;

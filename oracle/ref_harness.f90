@@ -14,6 +14,7 @@
 ! update step (lcgupdt = .TRUE.), the later ones reuse the kernels left in
 ! the zcc* arrays by it, which is what fetch_coag provides for a single cell
 ! (mo_salsa_driver.f90:294-297, :420-422; mo_salsa_dynamics.f90:146-147).
+! With sw(15) = 1 each step ends with distr_update (mo_salsa.f90:148-149).
 !
 ! Usage: ref_harness <input.bin> <output.bin>
 PROGRAM ref_harness
@@ -26,6 +27,7 @@ PROGRAM ref_harness
   USE mo_salsa_types, ONLY : allSALSA, rateDiag, zccaa, zcccc, zccpp, zccii, zccca, &
                              zccpa, zccia, zccpc, zccic, zccip, iaero, icloud, iprecp, iice
   USE mo_salsa_dynamics, ONLY : coagulation, condensation
+  USE mo_salsa_update, ONLY : distr_update
   USE mo_salsa_coagulation_kernels, ONLY : update_coagulation_kernels
   USE classBinLayout, ONLY : BinLayout
   USE classProcessRates, ONLY : Rates
@@ -136,6 +138,7 @@ PROGRAM ref_harness
            g1 = 0.; g2 = 0.; g3 = 0.; g4 = 0.; g5 = 0.; pbl = 1
            IF (lscnd%state) CALL condensation(1, 1, 1, 1, g1, g2, g3, g4, g5, RV, RS, RSI, &
                                               T, P, dt, pbl)
+           IF (sw(15) == 1) CALL distr_update(1, 1, level)      ! mo_salsa.f90:148-149
         END DO
         CALL SYSTEM_CLOCK(c1)
         tcomp = tcomp + REAL(c1 - c0) / REAL(crate)

@@ -18,7 +18,7 @@ def available():
 
 
 def run(L, P, level, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, mode=1, lcharge=0,
-        charge_enh=0.0, kstore=False):
+        charge_enh=0.0, kstore=False, distr=False):
     """L, P: oracle Layout/Params (ctypes); numc [n, nt], volc [n, nt, 9] in
     SALSA units with nt = L.ntotal.  Returns dict with rv, numc, volc, rates
     (mode 1) or kmat [n, nt, nt] (mode 2), plus the reference bin table."""
@@ -36,6 +36,7 @@ def run(L, P, level, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, mode=1, lc
     sw[:13] = [P.lscgaa, P.lscgcc, P.lscgpp, P.lscgii, P.lscgca, P.lscgpa, P.lscgia, P.lscgpc,
                P.lscgic, P.lscgip, P.lscndh2oae, P.lscndh2ocl, P.lscndh2oic]
     sw[13] = 1 if kstore else 0
+    sw[14] = 1 if distr else 0
     sw[15] = mode
     cells = np.zeros((n, 5 + nt + 9 * nt))
     cells[:, 0] = T

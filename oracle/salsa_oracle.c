@@ -188,6 +188,7 @@ void so_params_default(so_params *P, int level) {
   P->ice_gamma = 0.55; P->ice_sigma = 1.97;
   P->lcharge = 0; P->charge_coll_enh = 0.;
   P->kpert = 0.; P->kseed = 0ull;
+  P->lsdistupdate = 0; P->level = level;
   if (level == 4) {
     P->lscgia = P->lscgic = P->lscgii = P->lscgip = 0;
     P->lscndh2oic = 0;
@@ -1317,6 +1318,215 @@ static int gpparth2o(const so_layout *L, const so_params *P, so_cell *c,
 }
 
 /* ====================================================================== */
+/* distr_update: moving-centre re-binning of one cell,                    */
+/* src/src_salsa/mo_salsa_update.f90:21-433.  Each category is swept from */
+/* its largest bin down, repeatedly, until the LAST bin examined in a     */
+/* sweep lies within its limits (within_bins is overwritten per bin, not  */
+/* accumulated, :119-122), at most 100 sweeps.  The bins whose test is    */
+/* skipped by CYCLE do not touch within_bins.                             */
+/* ====================================================================== */
+static int distr_update(const so_layout *L, so_cell *c, int level) {
+  const int ndry = L->ndry, nwet = L->nwet, nnsp = L->nspec;
+  int status = 0;
+  /* ---- aerosols, :50-140 ---- */
+  {
+    const int A = L->iaero;
+    int within = 0, count = 0;
+    while (!within) {
+      within = 1;
+      for (int kk = L->fn2b - 1; kk >= L->in1a; --kk) {
+        const int b = A + kk;
+        if (c->numc[b] > nlim_c) {
+          if (kk == L->fn2a) continue;
+          double sd = 0.;
+          for (int s = 0; s < ndry; ++s) sd += c->volc[b][s];
+          double zvpart = sd / c->numc[b];
+          if (L->vlolim[b] > zvpart && kk == L->in1a) continue;   /* smallest bin cannot decrease */
+          if (L->vlolim[b] > zvpart) {                           /* decreasing: whole bin down */
+            int mm = kk - 1;
+            if (kk == L->in2b) mm = L->fn1a;                     /* 2b goes to 1a */
+            const int m = A + mm;
+            c->numc[m] = c->numc[m] + c->numc[b];
+            c->numc[b] = 0.;
+            for (int s = 0; s < nwet; ++s) {
+              c->volc[m][s] = c->volc[m][s] + c->volc[b][s];
+              c->volc[b][s] = 0.;
+            }
+            continue;
+          }
+          if (zvpart <= pi6 * cube(L->dmid[b])) continue;       /* not grown */
+          const double zVrat = L->vhilim[b] / L->vlolim[b];
+          const double zVilo = 2. * zvpart / (1. + zVrat);
+          const double zVihi = zVrat * zVilo;
+          const double zVexc = 0.5 * (zVihi + L->vhilim[b]);
+          const double znfrac = fmin(1., (zVihi - L->vhilim[b]) / (zVihi - zVilo));
+          const double zvfrac = fmin(0.99, znfrac * zVexc / zvpart);
+          if (zvfrac < 0.) status |= SO_ERR_DISTR;
+          const int m = b + 1;
+          double add[SO_MAXSPEC];
+          for (int s = 0; s < nwet; ++s) add[s] = znfrac * c->numc[b] * zVexc * c->volc[b][s] / sd;
+          for (int s = 0; s < nwet; ++s) c->volc[m][s] = c->volc[m][s] + add[s];
+          for (int s = 0; s < nwet; ++s) c->volc[b][s] = c->volc[b][s] - add[s];
+          c->numc[m] = c->numc[m] + znfrac * c->numc[b];
+          c->numc[b] = c->numc[b] * (1. - znfrac);
+        }
+        if (c->numc[b] > nlim_c) {
+          double sd = 0.;
+          for (int s = 0; s < ndry; ++s) sd += c->volc[b][s];
+          const double zvpart = sd / c->numc[b];
+          within = L->vlolim[b] < zvpart && zvpart < L->vhilim[b];
+        }
+      }
+      if (++count > 100) { status |= SO_ERR_DISTR; break; }
+    }
+  }
+  /* ---- cloud droplets (dry volume), :145-225; the category ends stay ---- */
+  {
+    const int C = L->icloud;
+    int within = 0, count = 0;
+    while (!within) {
+      within = 1;
+      for (int kk = L->ncld - 1; kk >= L->ica.cur; --kk) {
+        const int b = C + kk;
+        double sd = 0.;
+        for (int s = 0; s < ndry; ++s) sd += c->volc[b][s];
+        if (c->numc[b] > nlim_c && sd > 1.e-30) {
+          const double zvpart = sd / c->numc[b];
+          const double zVrat = L->vhilim[b] / L->vlolim[b];
+          const double zVilo = 2. * zvpart / (1. + zVrat);
+          const double zVihi = zVrat * zVilo;
+          double zVexc, znfrac;
+          int mm;
+          if (zvpart < L->vlolim[b] && kk != L->ica.cur && kk != L->icb.cur) {
+            zVexc = 0.5 * (zVilo + L->vlolim[b]);
+            znfrac = fmin(1., (L->vlolim[b] - zVilo) / (zVihi - zVilo));
+            mm = kk - 1;
+          } else if (zvpart > L->vhilim[b] && kk != L->fca.cur && kk != L->fcb.cur) {
+            zVexc = 0.5 * (zVihi + L->vhilim[b]);
+            znfrac = fmin(1., (zVihi - L->vhilim[b]) / (zVihi - zVilo));
+            mm = kk + 1;
+          } else {
+            continue;
+          }
+          const double zvfrac = fmin(0.99, znfrac * zVexc / zvpart);
+          if (zvfrac < 0.) status |= SO_ERR_DISTR;
+          const int m = C + mm;
+          for (int s = 0; s < nwet; ++s) c->volc[m][s] = c->volc[m][s] + zvfrac * c->volc[b][s];
+          for (int s = 0; s < nwet; ++s) c->volc[b][s] = c->volc[b][s] - zvfrac * c->volc[b][s];
+          c->numc[m] = c->numc[m] + znfrac * c->numc[b];
+          c->numc[b] = c->numc[b] - znfrac * c->numc[b];
+        }
+        sd = 0.;
+        for (int s = 0; s < ndry; ++s) sd += c->volc[b][s];
+        if (c->numc[b] > nlim_c && sd > 1.e-30) {
+          const double zvpart = sd / c->numc[b];
+          within = L->vlolim[b] < zvpart && zvpart < L->vhilim[b];
+        }
+      }
+      if (++count > 100) { status |= SO_ERR_DISTR; break; }
+    }
+  }
+  /* ---- precipitation (wet volume), :227-320 ---- */
+  {
+    const int R = L->iprecp;
+    int within = 0, count = 0;
+    while (!within) {
+      within = 1;
+      for (int kk = L->nprc - 1; kk >= 0; --kk) {
+        const int b = R + kk;
+        if (c->numc[b] > prlim_c) {
+          double sw = 0.;
+          for (int s = 0; s < nwet; ++s) sw += c->volc[b][s];
+          const double zvpart = sw / c->numc[b];
+          const double zVrat = L->vhilim[b] / L->vlolim[b];
+          const double zVilo = 2. * zvpart / (1. + zVrat);
+          const double zVihi = zVrat * zVilo;
+          const double vmid = pi6 * cube(L->dmid[b]);
+          const double zvdec = vmid - 0.2 * (vmid - L->vlolim[b]);
+          double zVexc, znfrac;
+          int mm;
+          if (zvpart < zvdec && kk != 0) {
+            zVexc = 0.5 * (zVilo + L->vlolim[b]);
+            znfrac = fmin(1., (L->vlolim[b] - zVilo) / (zVihi - zVilo));
+            mm = kk - 1;
+          } else if (zvpart > vmid && kk != L->nprc - 1) {
+            zVexc = 0.5 * (zVihi + L->vhilim[b]);
+            znfrac = fmin(.99, (zVihi - L->vhilim[b]) / (zVihi - zVilo));
+            mm = kk + 1;
+          } else {
+            continue;
+          }
+          const double zvfrac = fmin(0.99, znfrac * zVexc / zvpart);
+          if (zvfrac < 0.) status |= SO_ERR_DISTR;
+          const int m = R + mm;
+          for (int s = 0; s < nwet; ++s) c->volc[m][s] = c->volc[m][s] + zvfrac * c->volc[b][s];
+          for (int s = 0; s < nwet; ++s) c->volc[b][s] = c->volc[b][s] - zvfrac * c->volc[b][s];
+          c->numc[m] = c->numc[m] + znfrac * c->numc[b];
+          c->numc[b] = c->numc[b] - znfrac * c->numc[b];
+        }
+        if (c->numc[b] > prlim_c) {
+          double sw = 0.;
+          for (int s = 0; s < nwet; ++s) sw += c->volc[b][s];
+          const double zvpart = sw / c->numc[b];
+          within = L->vlolim[b] < zvpart && zvpart < L->vhilim[b];
+        }
+      }
+      if (++count > 100) { status |= SO_ERR_DISTR; break; }
+    }
+  }
+  if (level < 5) return status;
+  /* ---- ice (all species incl. rime), :330-430 ---- */
+  {
+    const int I = L->iice;
+    int within = 0, count = 0;
+    while (!within) {
+      within = 1;
+      for (int kk = L->nice - 1; kk >= 0; --kk) {
+        const int b = I + kk;
+        if (c->numc[b] > L->nlim[b]) {
+          double st = 0.;
+          for (int s = 0; s < nnsp; ++s) st += c->volc[b][s];
+          const double zvpart = st / c->numc[b];
+          const double zVrat = L->vhilim[b] / L->vlolim[b];
+          const double zVilo = 2. * zvpart / (1. + zVrat);
+          const double zVihi = zVrat * zVilo;
+          const double vmid = pi6 * cube(L->dmid[b]);
+          const double zvdec = vmid - 0.2 * (vmid - L->vlolim[b]);
+          double zVexc, znfrac;
+          int mm;
+          if (zvpart < zvdec && kk != 0) {
+            zVexc = 0.5 * (zVilo + L->vlolim[b]);
+            znfrac = fmin(1.0, (L->vlolim[b] - zVilo) / (zVihi - zVilo));
+            mm = kk - 1;
+          } else if (zvpart > vmid && kk != L->nice - 1) {
+            zVexc = 0.5 * (zVihi + L->vhilim[b]);
+            znfrac = fmin(0.99, (zVihi - L->vhilim[b]) / (zVihi - zVilo));
+            mm = kk + 1;
+          } else {
+            continue;
+          }
+          const double zvfrac = fmin(0.99, znfrac * zVexc / zvpart);
+          if (zvfrac < 0.) status |= SO_ERR_DISTR;
+          const int m = I + mm;
+          for (int s = 0; s < nnsp; ++s) c->volc[m][s] = c->volc[m][s] + zvfrac * c->volc[b][s];
+          for (int s = 0; s < nnsp; ++s) c->volc[b][s] = c->volc[b][s] - zvfrac * c->volc[b][s];
+          c->numc[m] = c->numc[m] + znfrac * c->numc[b];
+          c->numc[b] = c->numc[b] - znfrac * c->numc[b];
+        }
+        double st = 0.;
+        for (int s = 0; s < nnsp; ++s) st += c->volc[b][s];
+        if (c->numc[b] > L->nlim[b] && st > 1.e-30) {
+          const double zvpart = st / c->numc[b];
+          within = L->vlolim[b] < zvpart && zvpart < L->vhilim[b];
+        }
+      }
+      if (++count > 100) { status |= SO_ERR_DISTR; break; }
+    }
+  }
+  return status;
+}
+
+/* ====================================================================== */
 /* One cell: salsa() with coagulation and condensation enabled,           */
 /* src/src_salsa/mo_salsa.f90:92-107                                      */
 /* ====================================================================== */
@@ -1339,6 +1549,7 @@ int so_step_cell_ks(const so_layout *L, const so_params *P,
   else for (int b = 0; b < L->ntotal; ++b) update_diameter(L, &c, b);
   if (P->lscnd && (P->lscndh2ocl || P->lscndh2oae || P->lscndh2oic))
     status |= gpparth2o(L, P, &c, ptemp, ppres, prs, prsi, prv, ptstep);
+  if (P->lsdistupdate) status |= distr_update(L, &c, P->level);
   for (int b = 0; b < L->ntotal; ++b) {
     numc[b] = c.numc[b];
     for (int s = 0; s < SO_MAXSPEC; ++s) volc[b * SO_MAXSPEC + s] = c.volc[b][s];

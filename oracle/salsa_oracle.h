@@ -75,6 +75,10 @@ typedef struct {
    * answer depends on rounding-sized kernel errors.  Default 0. */
   double kpert;
   unsigned long long kseed;
+  int lsdistupdate;            /* size distribution update after condensation
+                                  (mo_salsa.f90:148-149); default 0 here       */
+  int level;                   /* thermodynamic level (4 or 5): distr_update
+                                  skips the ice bins below 5 (:323)            */
 } so_params;
 
 /* One Rate (classProcessRates.f90:40-45): volc(9) and numc */
@@ -93,7 +97,9 @@ enum {
   SO_OK = 0,
   SO_WARN_NEG_NUMC = 1,       /* numc < -1e-6 after coagulation (mo_salsa_dynamics.f90:191-193) */
   SO_ERR_EQUIL_NOCONV = 2,    /* equilibration > 100 iterations (mo_salsa_properties.f90:238-243) */
-  SO_WARN_SUBSTEPS = 4        /* > 100 condensation substeps (mo_salsa_dynamics.f90:1051) */
+  SO_WARN_SUBSTEPS = 4,       /* > 100 condensation substeps (mo_salsa_dynamics.f90:1051) */
+  SO_ERR_DISTR = 8            /* distr_update: > 100 passes or a negative fraction (the
+                                 reference STOPs, mo_salsa_update.f90:109, 137, 216, 311) */
 };
 
 /* Fill the layout from nbin/reglim/bloPrc/bloIce/species; returns 0 on success */

@@ -45,14 +45,14 @@ def pick(f, n, seed):
     return np.sort(np.concatenate(out))[:n]
 
 
-def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0, kstore=False):
+def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0, kstore=False, distr=False):
     T, p, rs, rsi, rv, nc, vc = cells
     R = refio.run(L, P, level, T, p, rs, rsi, rv, nc, vc, nsteps=nsteps, lcharge=lcharge, charge_enh=enh,
-                  kstore=kstore)
+                  kstore=kstore, distr=distr)
     ns = L.nspec
     np.savez_compressed(
         os.path.join(HERE, name),
-        level=level, nsteps=nsteps, lcharge=lcharge, charge_enh=enh, kstore=int(kstore),
+        level=level, nsteps=nsteps, lcharge=lcharge, charge_enh=enh, kstore=int(kstore), distr=int(distr),
         lsauto=P.lsauto_state, nbin=np.array(list(L.nbin)), nprc=L.nprc, nice=L.nice,
         species=np.array(list(L.dry_species)[:L.nspec_dry]),
         T=T, p=p, rs=rs, rsi=rsi, rv=rv, numc=nc, volc=vc[:, :, :ns],
@@ -73,9 +73,34 @@ def kstore_case():
     save_step("step_cfg4_kstore.npz", L, 4, P4, tuple(x[sel] for x in cells), nsteps=3, kstore=True)
 
 
+def distr_cases():
+    """Steps ending with distr_update (mo_salsa.f90:148-149): config-1 cells
+    over three steps, cloudy cells with precipitation over two, mixed-phase
+    cells over one (written separately: `--only distr`)."""
+    L = O.make_layout()
+    P4 = O.make_params(level=4, lsdistupdate=1)
+    f1 = W.to_salsa_units(L, W.config_fields(L, 1), level=4)
+    nc, vc = pad(L, f1[5], f1[6])
+    cells = tuple(f1[:5]) + (nc, vc)
+    sel = pick(cells, 120, 11)
+    save_step("step_cfg1_distr.npz", L, 4, P4, tuple(x[sel] for x in cells), nsteps=3, distr=True)
+    f4 = W.to_salsa_units(L, W.config_fields(L, 4, grid_override=(20, 8, 8)), level=4)
+    nc, vc = pad(L, f4[5], f4[6])
+    cells = tuple(f4[:5]) + (nc, vc)
+    sel = pick(cells, 120, 14)
+    save_step("step_cfg4_distr.npz", L, 4, P4, tuple(x[sel] for x in cells), nsteps=2, distr=True)
+    L5 = O.make_layout(species=(0, 3))
+    P5 = O.make_params(level=5, lsdistupdate=1)
+    f5 = W.to_salsa_units(L5, W.config_fields(L5, 5, grid_override=(20, 8, 8)), level=5)
+    sel = np.sort(np.random.default_rng(15).choice(len(f5[0]), 80, replace=False))
+    save_step("step_cfg5_distr.npz", L5, 5, P5, tuple(x[sel] for x in f5), distr=True)
+
+
 def main():
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "kstore":
         return kstore_case()
+    if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "distr":
+        return distr_cases()
     if not refio.available():
         sys.exit("oracle/_ref/ref_harness missing: run `make -C oracle ref` first")
     # warm layout nbin=(3,4): aerosol 11, cloud 8, precip 7 (+ 7 inert ice bins)
@@ -117,6 +142,7 @@ def main():
     sel = np.random.default_rng(5).choice(len(f5[0]), 120, replace=False)
     save_step("step_cfg5.npz", L5, 5, P5, tuple(x[np.sort(sel)] for x in f5))
     kstore_case()
+    distr_cases()
 
 
 if __name__ == "__main__":

@@ -12,10 +12,13 @@ h = 1e-6 relative (J is a small remainder of the cell's condensed volume
 V_cell) and the last term the observed error of the difference itself;
 wherever the difference is precise to 1e-5 the plain 1e-4 relative bar is
 asserted.  Where the trajectory crosses a threshold of the algorithm (an
-activity limit, a precipitation-bin boundary, a clamp) within +-h, J is not
-differentiable there and central differences at h and 4h disagree by more
-than 10 %; such (cell, direction) pairs are counted, reported and left out,
-and must stay below 2 % of the checked pairs; so are
+activity limit, a precipitation-bin boundary, a clamp, a re-binning decision
+of the size-distribution update) within +-4h, J is not differentiable there:
+central differences at h and 4h disagree by more than 10 %, or the forward
+and backward differences at h do (a threshold on one side only, which can
+bend both central differences alike); such (cell, direction) pairs are
+counted, reported and left out, and must stay below 2 % of the checked
+pairs; so are
 pairs where the derivative itself says that +-h is beyond the linear range
 (|dJ/dx| x0 / J > 100: the trajectory amplifies a relative input change more
 than 100 times, and its internal state, which J only partly reflects, more).  A few
@@ -69,11 +72,11 @@ def oracle_J(O, L, P, desc, cells, nsteps):
     return objective(desc, vco)
 
 
-def gpu_gradient(capi, desc, prec, cells, nsteps, level=4):
+def gpu_gradient(capi, desc, prec, cells, nsteps, level=4, distr=0):
     import torch
     T, p, rs, rsi, rv, numc, volc = cells
     fc = FlatCells(desc, level, T, p, rs, rsi, rv, numc, volc)
-    ctx = capi.Context(desc, capi.params(level=level, precision=prec, hz=0, hxy=0), fc.n, 1, 1)
+    ctx = capi.Context(desc, capi.params(level=level, precision=prec, hz=0, hxy=0, lsdistupdate=distr), fc.n, 1, 1)
     dev = torch.device("cuda")
     D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     f = {k: D(v) for k, v in fc.f.items()}
@@ -92,16 +95,17 @@ def gpu_gradient(capi, desc, prec, cells, nsteps, level=4):
     return J, dJdE, jc.cpu().numpy(), je.cpu().numpy(), g, flags
 
 
-def check_fd(name, g, fd, fd4, vol, x0, J, tol=TOL):
+def check_fd(name, g, fd, fd4, vol, x0, J, asym, tol=TOL):
     """Per element bar on the pairs where central differences resolve the
-    derivative.  est = |fd - fd4| estimates the error of fd (rounding, and
-    thresholds of the algorithm within +-4h).  A pair is resolved when est
-    stays within 10 % of fd or within the rounding floor, and the trajectory
-    amplifies a relative input change at most 100 times in J.  Bar on
-    resolved pairs: |g - fd| <= tol |fd| + noise + 2 est.  Returns the number
-    of pairs left out."""
+    derivative.  est = max(|fd - fd4|, asym) estimates the error of fd
+    (rounding, and thresholds of the algorithm within +-4h); asym is the
+    difference of the forward and backward differences at h.  A pair is
+    resolved when est stays within 10 % of fd or within the rounding floor,
+    and the trajectory amplifies a relative input change at most 100 times
+    in J.  Bar on resolved pairs: |g - fd| <= tol |fd| + noise + 2 est.
+    Returns the number of pairs left out."""
     noise = NOISE * np.abs(vol) / np.maximum(np.abs(x0), 1e-300) + 1e-300
-    est = np.abs(fd4 - fd)
+    est = np.maximum(np.abs(fd4 - fd), np.abs(asym))
     linear = np.abs(g) * np.abs(x0) <= 100. * np.abs(J)
     resolved = linear & (est <= 0.1 * np.abs(fd) + noise)
     bar = tol * np.abs(fd) + noise + 2 * est
@@ -133,23 +137,26 @@ def small_cells(desc, W, grid=(16, 4, 4)):
     return W.to_salsa_units(desc, f)
 
 
+@pytest.mark.parametrize("distr", [0, 1])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_gradient_against_oracle_fd(capi, oracle_mod, prec):
+def test_gradient_against_oracle_fd(capi, oracle_mod, prec, distr):
     """dJ/dN0 for every bin of every cell and dJ/dE against central
-    differences of the FP64 oracle over a 10-step trajectory."""
+    differences of the FP64 oracle over a 10-step trajectory, without and
+    with the size-distribution update after each step."""
     from salsa_amd import workload as W
     O = oracle_mod
     desc = capi.bin_desc()
     cells = small_cells(desc, W)
     nsteps = 10
-    print("case gradient vs oracle FD, precision %d, %d cells, %d steps" % (prec, len(cells[0]), nsteps), flush=True)
-    J, dJdE, jc, je, g, flags = gpu_gradient(capi, desc, prec, cells, nsteps)
+    print("case gradient vs oracle FD, precision %d, distr_update %d, %d cells, %d steps"
+          % (prec, distr, len(cells[0]), nsteps), flush=True)
+    J, dJdE, jc, je, g, flags = gpu_gradient(capi, desc, prec, cells, nsteps, distr=distr)
     assert not np.any(flags & capi.CELL_NONFINITE)
     ok = (flags & capi.CELL_GRAD_UNSTABLE) == 0
     assert ok.mean() >= 0.99
     assert np.all(np.isfinite(g[ok])) and np.all(np.isfinite(je[ok]))
     L = O.make_layout()
-    P = O.make_params(level=4)
+    P = O.make_params(level=4, lsdistupdate=distr)
     Jo = oracle_J(O, L, P, desc, cells, nsteps)
     assert Jo.max() > 0, "no precipitation formed: the objective is identically zero"
     # value part against the oracle's trajectory
@@ -158,13 +165,17 @@ def test_gradient_against_oracle_fd(capi, oracle_mod, prec):
     assert abs(J - jc.sum()) <= 1e-12 * abs(J) + 1e-300
     T, p, rs, rsi, rv, numc, volc = cells
     vol = volc.sum(axis=(1, 2))
-    # dJ/dE: enhancement factor E = 1 + chargeCollEnh
+    # dJ/dE: enhancement factor E = 1 + chargeCollEnh.  Each difference
+    # returns the central quotient and |forward - backward| at its step.
     def fdE(h):
-        Jp = oracle_J(O, L, O.make_params(level=4, lcharge=1, charge_coll_enh=h), desc, cells, nsteps)
-        Jm = oracle_J(O, L, O.make_params(level=4, lcharge=1, charge_coll_enh=-h), desc, cells, nsteps)
-        return (Jp - Jm) / (2 * h)
-    fe, fe4 = fdE(H), fdE(4 * H)
-    nk = check_fd("dJ/dE per cell (prec %d)" % prec, je[ok], fe[ok], fe4[ok], vol[ok], np.ones(ok.sum()), Jo[ok])
+        Jp = oracle_J(O, L, O.make_params(level=4, lcharge=1, charge_coll_enh=h, lsdistupdate=distr), desc, cells,
+                      nsteps)
+        Jm = oracle_J(O, L, O.make_params(level=4, lcharge=1, charge_coll_enh=-h, lsdistupdate=distr), desc, cells,
+                      nsteps)
+        return (Jp - Jm) / (2 * h), np.abs(Jp - 2 * Jo + Jm) / h
+    (fe, ae), (fe4, _) = fdE(H), fdE(4 * H)
+    nk = check_fd("dJ/dE per cell (prec %d)" % prec, je[ok], fe[ok], fe4[ok], vol[ok], np.ones(ok.sum()), Jo[ok],
+                  ae[ok])
     npair = int(ok.sum())
     # the total leaves the unstable cells out
     assert abs(dJdE - fe[ok].sum()) <= TOL * abs(fe[ok].sum()) + NOISE * vol[ok].sum()
@@ -178,8 +189,9 @@ def test_gradient_against_oracle_fd(capi, oracle_mod, prec):
         cp[5][:, b] *= 1 + h
         cm[5] = numc.copy()
         cm[5][:, b] *= 1 - h
-        return (oracle_J(O, L, P, desc, cp, nsteps) - oracle_J(O, L, P, desc, cm, nsteps)) / \
-            (2 * h * np.where(live, numc[:, b], 1.))
+        Jp, Jm = oracle_J(O, L, P, desc, cp, nsteps), oracle_J(O, L, P, desc, cm, nsteps)
+        den = h * np.where(live, numc[:, b], 1.)
+        return (Jp - Jm) / (2 * den), np.abs(Jp - 2 * Jo + Jm) / den
     for b in range(numc.shape[1]):
         live = (numc[:, b] > 0) & ok
         # a bin empty in every cell: a relative step is zero and an absolute
@@ -187,9 +199,9 @@ def test_gradient_against_oracle_fd(capi, oracle_mod, prec):
         # finiteness is checked there
         if not live.any():
             continue
-        fd, fd4 = fdN(b, H, live), fdN(b, 4 * H, live)
+        (fd, ad), (fd4, _) = fdN(b, H, live), fdN(b, 4 * H, live)
         nk += check_fd("dJ/dN0 bin %d (prec %d)" % (b, prec), g[live, b], fd[live], fd4[live], vol[live],
-                       numc[live, b], Jo[live])
+                       numc[live, b], Jo[live], ad[live])
         npair += int(live.sum())
         nb_checked += 1
     assert nb_checked >= 15
@@ -261,8 +273,9 @@ def test_gradient_workload3_against_gpu_fd(capi):
     vol = W.to_salsa_units(desc, f)[6].sum(axis=(1, 2))
 
     def fdE(h):
-        return (forward_J(dict(lcharge=1, charge_coll_enh=h)) - forward_J(dict(lcharge=1, charge_coll_enh=-h))) / (2 * h)
-    fe, fe4 = fdE(H), fdE(4 * H)
+        Jp, Jm = forward_J(dict(lcharge=1, charge_coll_enh=h)), forward_J(dict(lcharge=1, charge_coll_enh=-h))
+        return (Jp - Jm) / (2 * h), np.abs(Jp - 2 * J0 + Jm) / h
+    (fe, ae), (fe4, _) = fdE(H), fdE(4 * H)
     print("J = %.6e, dJ/dE = %.6e (FD %.6e)" % (J64, dJdE64, fe[ok64].sum()))
     assert abs(dJdE64 - fe[ok64].sum()) <= TOL * abs(fe[ok64].sum()) + NOISE * vol[ok64].sum()
     nk = 0
@@ -270,20 +283,22 @@ def test_gradient_workload3_against_gpu_fd(capi):
     for prec in (capi.SALSA_FP64, capi.SALSA_FP32_MIXED):
         ok = res[prec][5]
         nk += check_fd("workload 3 dJ/dE per cell prec %d" % prec, res[prec][3][ok], fe[ok], fe4[ok], vol[ok],
-                       np.ones(ok.sum()), J0[ok])
+                       np.ones(ok.sum()), J0[ok], ae[ok])
         npair += int(ok.sum())
     for c, b in ((0, desc.in2a + 1), (1, 0), (1, 2)):
         n0 = f.n[c][b, idx]
         live = n0 > 0
 
         def fdb(h):
-            return (forward_J({}, (c, b, 1 + h)) - forward_J({}, (c, b, 1 - h))) / (2 * h * np.where(live, n0, 1.))
-        fb, fb4 = fdb(H), fdb(4 * H)
+            Jp, Jm = forward_J({}, (c, b, 1 + h)), forward_J({}, (c, b, 1 - h))
+            den = h * np.where(live, n0, 1.)
+            return (Jp - Jm) / (2 * den), np.abs(Jp - 2 * J0 + Jm) / den
+        (fb, ab), (fb4, _) = fdb(H), fdb(4 * H)
         for prec in (capi.SALSA_FP64, capi.SALSA_FP32_MIXED):
             gb = res[prec][4][c][b, idx]
             sel = live & res[prec][5]
             nk += check_fd("workload 3 dJ/dN0 cat %d bin %d prec %d" % (c, b, prec), gb[sel], fb[sel], fb4[sel],
-                           vol[sel], n0[sel], J0[sel])
+                           vol[sel], n0[sel], J0[sel], ab[sel])
             npair += int(sel.sum())
     print("pairs at thresholds: %d of %d" % (nk, npair))
     assert nk <= MAX_KINK_FRAC * npair

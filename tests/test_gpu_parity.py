@@ -55,6 +55,9 @@ GOLDEN_CASES = [
     ("step_cfg4.npz", {}),
     ("step_cfg4_noauto.npz", {"lsauto": 0}),
     ("step_cfg5.npz", {}),
+    ("step_cfg1_distr.npz", {"lsdistupdate": 1}),
+    ("step_cfg4_distr.npz", {"lsdistupdate": 1}),
+    ("step_cfg5_distr.npz", {"lsdistupdate": 1}),
 ]
 
 
@@ -72,8 +75,9 @@ def test_gpu_matches_reference_golden(capi, oracle_mod, name, kw, prec):
     numc = g["numc"][:, :nt]
     volc = full_volc(g["volc"])[:, :nt]
     cells = (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], numc, volc)
-    inp, out, fl = run_flat(capi, desc, level, prec, cells, params_kw=kw)
-    print("case %s prec %d" % (name, prec))
+    nsteps = int(g["nsteps"])
+    print("case %s prec %d, %d steps" % (name, prec, nsteps), flush=True)
+    inp, out, fl = run_flat(capi, desc, level, prec, cells, params_kw=kw, nsteps=nsteps)
     assert not np.any(fl & capi.CELL_NONFINITE)
     # against the compiled reference (inputs differ from the packed ones by <= 1 ulp)
     e_ref = errors(out[0], out[1], out[2], g["out_rv"], g["out_numc"][:, :nt], g["out_volc"][:, :nt], ns)
@@ -85,18 +89,78 @@ def test_gpu_matches_reference_golden(capi, oracle_mod, name, kw, prec):
     vco = np.zeros((len(irv), L.ntotal, 9))
     nco[:, :nt] = inc
     vco[:, :nt] = ivc
-    orv, onc, ovc, _ = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco)
+    orv, onc, ovc, _ = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco, nsteps=nsteps)
     e_orc = errors(out[0], out[1], out[2], orv, onc[:, :nt], ovc[:, :nt], ns)
     print("  vs reference rv %.2e N %.2e V %.2e | vs oracle rv %.2e N %.2e V %.2e" % (e_ref + e_orc))
     srv, snc, svc = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco,
-                                       kernel_eps=KEPS[prec])
+                                       kernel_eps=KEPS[prec], nsteps=nsteps)
     sens = (srv, snc[:, :nt], svc[:, :nt])
     _, nf1 = check_condition_aware(out, (orv, onc[:, :nt], ovc[:, :nt]), sens, TOL[prec], ns, "oracle")
     _, nf2 = check_condition_aware(out, (g["out_rv"], g["out_numc"][:, :nt], g["out_volc"][:, :nt]), sens,
                                    TOL[prec], ns, "reference")
     nel = out[2][:, :, :ns].size
-    print("  elements inside the conditioning allowance: %d / %d" % (nf1, nel))
-    assert nf1 <= max(2, nel // 200) and nf2 <= max(2, nel // 200)
+    # The allowance may be used on 0.5 % of the elements, or on as many as
+    # the oracle's own answer moves by more than tol/10 under 1e-15 relative
+    # input noise (a few ulps).  The size-distribution update needs the
+    # latter: a bin whose mean volume sits just past its mid point moves a
+    # fraction (vhi_i - vhi)/(vhi_i - vlo_i) of its particles that a few
+    # ulps change relatively by 1e-9 and more (mo_salsa_update.f90:96-107).
+    s15 = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco, eps=1e-15,
+                             nsteps=nsteps)
+    n_ill = int(np.sum(10 * s15[0] > TOL[prec]) + np.sum(10 * s15[1][:, :nt] > TOL[prec]) +
+                np.sum(10 * s15[2][:, :nt, :ns] > TOL[prec]))
+    lim = max(2, nel // 200, n_ill)
+    print("  elements inside the conditioning allowance: %d / %d (limit %d)" % (nf1, nel, lim))
+    assert nf1 <= lim and nf2 <= lim
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("name", ["step_cfg1.npz", "step_cfg4.npz", "step_cfg5.npz"])
+def test_gpu_distr_update_alone(capi, oracle_mod, name, prec):
+    """distr_update by itself (coagulation and condensation switched off,
+    mo_salsa.f90:92-149) on the states the compiled reference left after its
+    step, where a third to a half of the bins move: GPU against the oracle on
+    identical packed inputs, 1e-10 with the conditioning allowance (the
+    update runs in FP64 on both precision paths), number and volume of
+    every category conserved, vapour untouched."""
+    g = golden(name)
+    level = int(g["level"])
+    desc = capi.bin_desc(species=tuple(int(s) for s in g["species"]))
+    nt = desc.nbins + desc.ncld + desc.nprc + (desc.nice if level == 5 else 0)
+    ns = g["volc"].shape[2]
+    cells = (g["T"], g["p"], g["rs"], g["rsi"], g["out_rv"], g["out_numc"][:, :nt], full_volc(g["out_volc"])[:, :nt])
+    print("case distr_update alone on %s prec %d" % (name, prec), flush=True)
+    inp, out, fl = run_flat(capi, desc, level, prec, cells, params_kw=dict(lscoag=0, lscnd=0, lsdistupdate=1))
+    assert not np.any(fl & (capi.CELL_NONFINITE | capi.CELL_DISTR))
+    irv, inc, ivc = inp
+    assert np.array_equal(out[0], irv)
+    L = oracle_layout_for(g)
+    P = oracle_params_for(g)
+    P.lscoag, P.lscnd, P.lsdistupdate = 0, 0, 1
+    nco = np.zeros((len(irv), L.ntotal))
+    vco = np.zeros((len(irv), L.ntotal, 9))
+    nco[:, :nt] = inc
+    vco[:, :nt] = ivc
+    orv, onc, ovc, st = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco)
+    assert not np.any(st)
+    moved = rel(onc[:, :nt], inc, FLOOR_N) > 1e-6
+    assert moved.mean() > 0.1, moved.mean()
+    # conservation per category (the moves stay inside a category)
+    offs = np.cumsum([0, desc.nbins, desc.ncld, desc.nprc] + ([desc.nice] if level == 5 else []))
+    for c in range(len(offs) - 1):
+        sl = slice(offs[c], offs[c + 1])
+        for a0, a1 in ((inc[:, sl], out[1][:, sl]), (ivc[:, sl].sum(axis=1), out[2][:, sl].sum(axis=1))):
+            assert np.all(np.abs(a1.sum(axis=1) - a0.sum(axis=1)) <= 1e-13 * np.abs(a0).sum(axis=1) + 1e-300)
+    sens = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco)
+    sens = (sens[0], sens[1][:, :nt], sens[2][:, :nt])
+    e, nf = check_condition_aware(out, (orv, onc[:, :nt], ovc[:, :nt]), sens, 1e-10, ns, "oracle")
+    s15 = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco, eps=1e-15)
+    n_ill = int(np.sum(10 * s15[1][:, :nt] > 1e-10) + np.sum(10 * s15[2][:, :nt, :ns] > 1e-10))
+    nel = out[2][:, :, :ns].size
+    lim = max(2, nel // 200, n_ill)
+    print("  bins moved %.3f; max rel err N %.2e V %.2e; inside the allowance %d (limit %d) of %d"
+          % (moved.mean(), e[1], max(e[2:]), nf, lim, nel))
+    assert nf <= lim
 
 
 def config_cells(capi, cfg, grid=None):
@@ -377,8 +441,10 @@ def test_gpu_rates_match_oracle_and_reference(capi, oracle_mod, name, kw, prec):
     n = len(g["T"])
     nrf = capi.NRATE_L5 if level == 5 else capi.NRATE_L4
     rates = np.full((nrf, n), -7.0)
+    # the fixtures of several steps hold the rates of their last step
+    nsteps = int(g["nsteps"])
     inp, out, fl = run_flat(capi, desc, level, prec, (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], numc, volc),
-                            params_kw=kw, rates=rates)
+                            params_kw=kw, rates=rates, nsteps=nsteps)
     assert not np.any(fl & capi.CELL_NONFINITE)
     L = oracle_layout_for(g)
     P = oracle_params_for(g)
@@ -387,7 +453,7 @@ def test_gpu_rates_match_oracle_and_reference(capi, oracle_mod, name, kw, prec):
     vco = np.zeros((n, L.ntotal, 9))
     nco[:, :nt] = inc
     vco[:, :nt] = ivc
-    *_, rr = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco, rates=True)
+    *_, rr = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco, rates=True, nsteps=nsteps)
     # SALSA-unit air density of the flat cells (pack multiplies by dn; recover it)
     from helpers import FlatCells
     dn = FlatCells(desc, level, g["T"], g["p"], g["rs"], g["rsi"], g["rv"], numc, volc).f["dn"]

@@ -102,3 +102,37 @@ def test_condensation_conserves_total_water(oracle_mod):
     rv, nc, vc, _ = oracle_mod.step_cells(L, P, [T], [p], [rs], [0.9 * rs], [rv0], numc[None], volc[None])
     w1 = rv[0] * rhoair + vc[0, :, L.iwa].sum() * 1000.
     assert w1 == pytest.approx(w0, rel=1e-13)
+
+
+def test_distr_update_conserves_number_and_volume(oracle_mod):
+    """distr_update alone (coagulation and condensation off) only moves number
+    and volume between neighbouring bins of one category
+    (mo_salsa_update.f90:94-129, 196-209, 289-302): per category the totals
+    stay the same to rounding, and it changes the distribution."""
+    from salsa_amd import capi, workload as W
+    O = oracle_mod
+    for cfg, level, species in ((1, 4, (0,)), (4, 4, (0,)), (5, 5, (0, 3))):
+        desc = capi.bin_desc(species=species)
+        f = W.config_fields(desc, cfg, grid_override=(10, 4, 4))
+        T, p, rs, rsi, rv, numc, volc = W.to_salsa_units(desc, f)
+        L = O.make_layout(species=species)
+        n, nt = numc.shape
+        nc = np.zeros((n, L.ntotal))
+        nc[:, :nt] = numc
+        vc = np.zeros((n, L.ntotal, 9))
+        vc[:, :nt] = volc
+        # move the particle volumes off the bin centres: growth and shrinkage
+        vc[:, :nt] *= np.random.default_rng(cfg).uniform(0.4, 2.5, size=(n, nt, 1))
+        P = O.make_params(level=level, lscoag=0, lscnd=0, lsdistupdate=1)
+        _, nc2, vc2, st = O.step_cells(L, P, T, p, rs, rsi, rv, nc, vc)
+        assert not np.any(st & 8)
+        assert np.any(np.abs(nc2 - nc) > 1e-6 * np.abs(nc))
+        ncat = 4 if level == 5 else 3
+        lo = 0
+        for c, nb in enumerate([L.nbins, L.ncld, L.nprc, L.nice][:ncat]):
+            sl = slice(lo, lo + nb)
+            lo += nb
+            N0, N1 = nc[:, sl].sum(axis=1), nc2[:, sl].sum(axis=1)
+            assert np.all(np.abs(N1 - N0) <= 1e-13 * np.abs(N0) + 1e-300), (cfg, c)
+            V0, V1 = vc[:, sl].sum(axis=(1, 2)), vc2[:, sl].sum(axis=(1, 2))
+            assert np.all(np.abs(V1 - V0) <= 1e-13 * np.abs(V0) + 1e-300), (cfg, c)

@@ -3,7 +3,8 @@ tests/golden/make_golden.py from oracle/_ref)."""
 import numpy as np
 import pytest
 
-from helpers import FLOOR_N, FLOOR_RV, FLOOR_V, full_volc, golden, oracle_layout_for, oracle_params_for, rel
+from helpers import FLOOR_N, FLOOR_RV, FLOOR_V, check_condition_aware, full_volc, golden, oracle_layout_for, \
+    oracle_params_for, oracle_sensitivity, rel
 
 CASES = ["step_cfg1.npz", "step_cfg1_charge.npz", "step_cfg4.npz", "step_cfg4_noauto.npz", "step_cfg5.npz"]
 
@@ -87,3 +88,29 @@ def test_oracle_kernel_store_matches_reference(oracle_mod):
     rv1, nc1, vc1, _ = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"],
                                              full_volc(g["volc"]), nsteps=3)
     assert rel(nc1, g["out_numc"], FLOOR_N).max() > 1e-6
+
+
+DISTR_CASES = ["step_cfg1_distr.npz", "step_cfg4_distr.npz", "step_cfg5_distr.npz"]
+
+
+@pytest.mark.parametrize("name", DISTR_CASES)
+def test_oracle_distr_update_matches_reference(oracle_mod, name):
+    """Steps ending with distr_update (mo_salsa_update.f90:21-433) against the
+    compiled reference.  The moved number fraction enters as numc*(1 - znfrac)
+    (:129), which cancels when znfrac is close to one, so each element is held
+    to max(1e-12, 10 x the oracle's own spread under 1e-13 input noise)."""
+    g = golden(name)
+    assert int(g["distr"]) == 1
+    L = oracle_layout_for(g)
+    P = oracle_params_for(g)
+    assert P.lsdistupdate == 1
+    nsteps = int(g["nsteps"])
+    args = (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"], full_volc(g["volc"]))
+    rv, nc, vc, st = oracle_mod.step_cells(L, P, *args, nsteps=nsteps)
+    assert not np.any(st & (2 | 8))
+    sens = oracle_sensitivity(oracle_mod, L, P, *args, nsteps=nsteps)
+    ns = g["volc"].shape[2]
+    out, nflag = check_condition_aware((rv, nc, vc[:, :, :ns]), (g["out_rv"], g["out_numc"], g["out_volc"]),
+                                       (sens[0], sens[1], sens[2][:, :, :ns]), 1e-12, ns, name)
+    print(name, "max rel", out, "elements needing the conditioning allowance:", nflag)
+    assert nflag <= 0.002 * nc.size

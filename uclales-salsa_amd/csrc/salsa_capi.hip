@@ -387,6 +387,7 @@ extern "C" int salsa_ctx_sync(salsa_ctx *c) {
   HIPCHK(hipMemcpy(&f, c->fault, sizeof(int32_t), hipMemcpyDeviceToHost));
   if (f & SALSA_CELL_NONFINITE) { set_error("non-finite values produced in some cell"); return SALSA_ERR_NONFINITE; }
   if (f & SALSA_CELL_NOCONV) { set_error("equilibration did not converge in some cell"); return SALSA_ERR_NOCONV; }
+  if (f & SALSA_CELL_DISTR) { set_error("size-distribution update did not settle in some cell"); return SALSA_ERR_DISTR; }
   return SALSA_OK;
 }
 

@@ -31,6 +31,7 @@
 
 #include <stdint.h>
 
+#include "salsa_distr.h"
 #include "salsa_dual.h"
 #include "salsa_internal.h"
 
@@ -728,6 +729,9 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
           }
         }
       }
+
+      // ---- size-distribution update (distr_update, mo_salsa.f90:148-149) ----
+      if (D.lsdistupdate) flags |= distr_update_cell<DD>(D, N, &V[0][0], NS);
 
       // ---- the new state through LES units, as the next run_SALSA call sees it ----
       for (int b = 0; b < nt; ++b) {

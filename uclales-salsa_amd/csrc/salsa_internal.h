@@ -94,6 +94,8 @@ struct DevLayout {
   int32_t slot_row[kMaxSlot];         // row of the slot inside its category array (units of n3)
   uint8_t slot_mass[kMaxSlot];        // 1: mass slot, 0: number slot
   double dmid[kMaxBin], nlim[kMaxBin], dlim[kMaxBin];
+  double vlo[kMaxBin], vhi[kMaxBin];  // bin volume limits (distr_update)
+  int32_t cld_a0, cld_a1, cld_b0, cld_b1;   // first/last cloud bin of series a and b (cloud index)
   double precpbins[kMaxBin];
   // pairs: canonical orientation (b1 is "pp1") and the self-offset predicate
   int32_t npair;
@@ -111,6 +113,7 @@ struct DevLayout {
   int32_t nent_total;
   // switches and constants
   int32_t level, lscoag, lscnd, lscndh2oae, lscndh2ocl, lscndh2oic;
+  int32_t lsdistupdate;
   int32_t any_ice_coag;               // any of lscgii/lscgia/lscgic/lscgip
   double eddy_dis, Eiagg_max, Eiagg_min;
   double ice_alpha, ice_beta, ice_gamma, ice_sigma;

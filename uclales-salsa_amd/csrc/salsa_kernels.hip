@@ -24,6 +24,7 @@
 
 #include <stdint.h>
 
+#include "salsa_distr.h"
 #include "salsa_internal.h"
 
 namespace salsa {
@@ -1206,6 +1207,13 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
     }
   }
   __syncthreads();
+
+  // ---- 5b. size-distribution update (distr_update, mo_salsa.f90:148-149):
+  //      serial over the bins, on one lane of the cell ----
+  if (D.lsdistupdate) {
+    if (sub == 0 && live) flags |= distr_update_cell<double>(D, sN, sV, nsV);
+    __syncthreads();
+  }
 
   // ---- 6. unpack (mo_salsa_driver.f90:310-388): per slot of the cell's
   //      output record the new state in LES units (in place) or its change

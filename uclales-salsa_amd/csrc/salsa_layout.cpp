@@ -159,6 +159,7 @@ extern "C" int salsa_params_default(salsa_params *p, int32_t level) {
   p->precision = SALSA_FP32_MIXED;
   p->hz = 1; p->hxy = 2;
   p->lscoag_mode = 1; p->cgintvl = 10.;     // mo_submctl.f90:23-24, :90
+  p->lsdistupdate = 0;                       // reference default on (mo_submctl.f90:118): opt in
   if (level == 4) {   // define_salsa, mo_salsa_init.f90:615-635
     p->lscgia = p->lscgic = p->lscgii = p->lscgip = 0;
     p->lscndh2oic = 0;
@@ -195,6 +196,7 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
   for (int s = 0; s < d.nspec_dry; ++s) L.dry_id[s] = d.dry_species[s];
   for (int b = 0; b < L.nt; ++b) {
     L.phase[b] = d.phase[b]; L.dmid[b] = d.dmid[b]; L.nlim[b] = d.nlim[b]; L.dlim[b] = d.dlim[b];
+    L.vlo[b] = d.vlolim[b]; L.vhi[b] = d.vhilim[b];
     int c = b < L.off[1] ? 0 : b < L.off[2] ? 1 : b < L.off[3] ? 2 : 3;
     L.cat_of[b] = c; L.loc_of[b] = b - L.off[c];
   }
@@ -216,7 +218,9 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
     L.nout = sl - kThermo + 1;
   }
 
+  L.cld_a0 = d.ica.cur; L.cld_a1 = d.fca.cur; L.cld_b0 = d.icb.cur; L.cld_b1 = d.fcb.cur;
   L.lscoag = p.lscoag; L.lscnd = p.lscnd;
+  L.lsdistupdate = p.lsdistupdate;
   L.lscndh2oae = p.lscndh2oae; L.lscndh2ocl = p.lscndh2ocl;
   L.lscndh2oic = level == 5 ? p.lscndh2oic : 0;
   L.eddy_dis = p.eddy_dis; L.Eiagg_max = p.Eiagg_max; L.Eiagg_min = p.Eiagg_min;

@@ -180,6 +180,7 @@ CONTAINS
     p%hz = 1; p%hxy = 2                                   ! mo_salsa_driver.f90:180-182
     p%lscoag_mode = INT(lscoag%mode, C_INT32_T)
     p%cgintvl = REAL(cgintvl, C_DOUBLE)
+    p%lsdistupdate = MERGE(1_C_INT32_T, 0_C_INT32_T, lsdistupdate)   ! distr_update after condensation
     rc = salsa_ctx_create(desc, p, INT(nzp, C_INT32_T), INT(nxp, C_INT32_T), INT(nyp, C_INT32_T), ctx)
     IF (rc /= SALSA_OK) CALL fail('context')
   END SUBROUTINE create_context
@@ -193,7 +194,6 @@ CONTAINS
     IF (lsicemelt%state) CALL outside('ice melting (lsicemelt)')
     IF (lscndgas) CALL outside('gas-phase condensation (lscndgas)')
     IF (nsnucl > 0) CALL outside('nucleation (nsnucl)')
-    IF (lsdistupdate) CALL outside('size-distribution update (lsdistupdate)')
   END SUBROUTINE check_switches
 
   SUBROUTINE outside(what)

@@ -11,11 +11,12 @@ MODULE mo_salsa_amd_cabi
 
   ! status codes
   INTEGER(C_INT), PARAMETER :: SALSA_OK = 0, SALSA_ERR_ARG = -1, SALSA_ERR_HIP = -2, &
-                               SALSA_ERR_NOCONV = -3, SALSA_ERR_NONFINITE = -4, SALSA_ERR_NOMEM = -5
+                               SALSA_ERR_NOCONV = -3, SALSA_ERR_NONFINITE = -4, SALSA_ERR_NOMEM = -5, &
+                               SALSA_ERR_DISTR = -6
   ! precision modes and step flags
   INTEGER(C_INT32_T), PARAMETER :: SALSA_FP64 = 0, SALSA_FP32_MIXED = 1
   INTEGER(C_INT32_T), PARAMETER :: SALSA_STEP_TENDENCY = 0, SALSA_STEP_INPLACE = 1, &
-                                   SALSA_STEP_NO_COMPACT = 2, SALSA_STEP_NO_GRAPH = 4
+                                   SALSA_STEP_NO_COMPACT = 2
   ! process-rate fields
   INTEGER, PARAMETER :: SALSA_NRATE_L4 = 12, SALSA_NRATE_L5 = 17
 
@@ -60,6 +61,7 @@ MODULE mo_salsa_amd_cabi
      INTEGER(C_INT32_T) :: hz, hxy
      INTEGER(C_INT32_T) :: lscoag_mode
      REAL(C_DOUBLE)     :: cgintvl
+     INTEGER(C_INT32_T) :: lsdistupdate
   END TYPE salsa_params
 
   INTERFACE

@@ -28,13 +28,13 @@ SALSA_FP32_MIXED = 1
 STEP_TENDENCY = 0
 STEP_INPLACE = 1
 STEP_NO_COMPACT = 2
-STEP_NO_GRAPH = 4
 
 CELL_NEG_NUMC = 1
 CELL_NOCONV = 2
 CELL_SUBSTEPS = 4
 CELL_NONFINITE = 8
 CELL_GRAD_UNSTABLE = 16
+CELL_DISTR = 32
 
 SPECIES = {"SO4": 0, "OC": 1, "BC": 2, "DU": 3, "SS": 4, "NO": 5, "NH": 6}
 
@@ -98,7 +98,7 @@ class Params(C.Structure):
         ("ice_sigma", C.c_double),
         ("lcharge", C.c_int32), ("charge_coll_enh", C.c_double),
         ("precision", C.c_int32), ("hz", C.c_int32), ("hxy", C.c_int32),
-        ("lscoag_mode", C.c_int32), ("cgintvl", C.c_double),
+        ("lscoag_mode", C.c_int32), ("cgintvl", C.c_double), ("lsdistupdate", C.c_int32),
     ]
 
 
