@@ -21,14 +21,22 @@ counted, reported and left out, and must stay below 2 % of the checked
 pairs; so are
 pairs where the derivative itself says that +-h is beyond the linear range
 (|dJ/dx| x0 / J > 100: the trajectory amplifies a relative input change more
-than 100 times, and its internal state, which J only partly reflects, more).  A few
-cells of the workload have a condensation state whose discrete trajectory is
-unstable (a perturbation grows about 100x per step and saturates near 1e-3):
-their derivative is finite but astronomically large, central differences
-cannot follow it.  The kernel flags such cells (CELL_GRAD_UNSTABLE: a
-relative input change amplified more than 1e8 times in J, or a derivative
-out of the range of its type); they are left out here and must stay below
-1 % of the cells."""
+than 100 times, and its internal state, which J only partly reflects, more).
+
+Some cells have a condensation state whose discrete trajectory is unstable
+(a perturbation grows by orders of magnitude per step and saturates): about
+0.1 % of the level-4 workload, and about a fifth of the level-5 test cells,
+where near 273 K the water of the largest 2b aerosol bin oscillates under the
+0.02 s substeps (the compiled reference shows the same: a 1e-12 change of E
+moves that water by 10 %).  Their derivative is finite but astronomically
+large; differences cannot follow it.  The kernel flags a cell
+(CELL_GRAD_UNSTABLE) when a relative input change is amplified more than 1e8
+times in J, in the vapour or in the number or volume of an active bin, or a
+derivative leaves the range of its type.  Against the oracle every flagged
+cell must be one whose trajectory the oracle shows amplifying beyond 1e3;
+the cells the oracle shows amplifying are left out of the difference checks,
+and at least half of the cells must remain.  At the workload scale the
+flagged cells must stay below 1 %."""
 import os
 import time
 
@@ -132,81 +140,148 @@ def check_fd(name, g, fd, fd4, vol, x0, J, asym, tol=TOL):
     return nk
 
 
-def small_cells(desc, W, grid=(16, 4, 4)):
-    f = W.config_fields(desc, 3, grid_override=grid)
+def small_cells(desc, W, cfg=3, grid=(16, 4, 4)):
+    f = W.config_fields(desc, cfg, grid_override=grid)
     return W.to_salsa_units(desc, f)
 
 
-@pytest.mark.parametrize("distr", [0, 1])
-@pytest.mark.parametrize("prec", [0, 1])
-def test_gradient_against_oracle_fd(capi, oracle_mod, prec, distr):
-    """dJ/dN0 for every bin of every cell and dJ/dE against central
-    differences of the FP64 oracle over a 10-step trajectory, without and
-    with the size-distribution update after each step."""
-    from salsa_amd import workload as W
-    O = oracle_mod
-    desc = capi.bin_desc()
-    cells = small_cells(desc, W)
-    nsteps = 10
-    print("case gradient vs oracle FD, precision %d, distr_update %d, %d cells, %d steps"
-          % (prec, distr, len(cells[0]), nsteps), flush=True)
-    J, dJdE, jc, je, g, flags = gpu_gradient(capi, desc, prec, cells, nsteps, distr=distr)
-    assert not np.any(flags & capi.CELL_NONFINITE)
-    ok = (flags & capi.CELL_GRAD_UNSTABLE) == 0
-    assert ok.mean() >= 0.99
+def oracle_state(O, L, P, cells, nsteps):
+    T, p, rs, rsi, rv, numc, volc = cells
+    n, nt = numc.shape
+    nco = np.zeros((n, L.ntotal))
+    nco[:, :nt] = numc
+    vco = np.zeros((n, L.ntotal, 9))
+    vco[:, :nt] = volc
+    r, nc, vc, _ = O.step_cells(L, P, T, p, rs, rsi, np.array(rv, dtype=np.float64), nco, vco, nsteps=nsteps)
+    return r, nc, vc
+
+
+def oracle_amplification(O, L, level, distr, cells, nsteps, cats, seed=5):
+    """Per cell: the largest change of the final state over a relative input
+    change eps of E and of every bin's N0 (random signs), divided by eps; the
+    larger of eps = 1e-12 and 1e-13.  Changes are relative to the vapour and
+    to the category's total number and, per species, volume (a bin's content
+    against its category: the size-distribution update moves small fractions
+    between neighbouring bins that are ill-conditioned relative to the small
+    receiving contents, but not against the category).  A stable trajectory
+    gives O(1)-O(10); the reference's unstable condensation trajectories give
+    1e3 and more (a perturbation that grows until it saturates)."""
+    P = O.make_params(level=level, lsdistupdate=distr)
+    base = oracle_state(O, L, P, cells, nsteps)
+    rng = np.random.default_rng(seed)
+    amp = np.zeros(len(cells[0]))
+    for eps in (1e-12, 1e-13):
+        cp = list(cells)
+        cp[5] = cells[5] * (1 + eps * rng.choice([-1., 1.], size=cells[5].shape))
+        Pp = O.make_params(level=level, lcharge=1, charge_coll_enh=eps, lsdistupdate=distr)
+        r, nc, vc = oracle_state(O, L, Pp, cp, nsteps)
+        d = np.abs(r - base[0]) / np.abs(base[0])
+        for c0, c1 in cats:
+            tn = np.abs(base[1][:, c0:c1]).sum(axis=1) + 1e-300
+            tv = np.abs(base[2][:, c0:c1]).sum(axis=1) + 1e-300
+            d = np.maximum(d, (np.abs(nc - base[1])[:, c0:c1].max(axis=1)) / tn)
+            d = np.maximum(d, (np.abs(vc - base[2])[:, c0:c1] / tv[:, None, :]).max(axis=(1, 2)))
+        amp = np.maximum(amp, d / eps)
+    return amp
+
+
+class OracleFD:
+    """Central differences of the oracle's J for dJ/dE and for dJ/dN0 of every
+    bin with particles, at h and 4h, with |forward - backward| at h."""
+
+    def __init__(self, O, level, distr, desc, cells, nsteps):
+        self.L = L = O.make_layout(species=(0, 3) if level == 5 else (0,))
+        P = O.make_params(level=level, lsdistupdate=distr)
+        self.Jo = Jo = oracle_J(O, L, P, desc, cells, nsteps)
+        numc = cells[5]
+
+        def fdE(h):
+            Jp = oracle_J(O, L, O.make_params(level=level, lcharge=1, charge_coll_enh=h, lsdistupdate=distr), desc,
+                          cells, nsteps)
+            Jm = oracle_J(O, L, O.make_params(level=level, lcharge=1, charge_coll_enh=-h, lsdistupdate=distr), desc,
+                          cells, nsteps)
+            return (Jp - Jm) / (2 * h), np.abs(Jp - 2 * Jo + Jm) / h
+
+        def fdN(b, h, live):
+            cp = list(cells)
+            cm = list(cells)
+            cp[5] = numc.copy()
+            cp[5][:, b] *= 1 + h
+            cm[5] = numc.copy()
+            cm[5][:, b] *= 1 - h
+            Jp, Jm = oracle_J(O, L, P, desc, cp, nsteps), oracle_J(O, L, P, desc, cm, nsteps)
+            den = h * np.where(live, numc[:, b], 1.)
+            return (Jp - Jm) / (2 * den), np.abs(Jp - 2 * Jo + Jm) / den
+
+        (self.fe, self.ae), (self.fe4, _) = fdE(H), fdE(4 * H)
+        # a bin empty in every cell: a relative step is zero and an absolute
+        # one below the activity limit is lost in rounding, so only finiteness
+        # is checked there
+        self.bins = [b for b in range(numc.shape[1]) if (numc[:, b] > 0).any()]
+        self.fdn = {b: (fdN(b, H, numc[:, b] > 0), fdN(b, 4 * H, numc[:, b] > 0)) for b in self.bins}
+        nb = [desc.nbins, desc.ncld, desc.nprc] + ([desc.nice] if level == 5 else [])
+        off = np.cumsum([0] + nb)
+        self.amp = oracle_amplification(O, L, level, distr, cells, nsteps, list(zip(off[:-1], off[1:])))
+
+
+def check_against_oracle(ref, cells, prec, J, dJdE, jc, je, g, flags, unstable_bit):
+    """The checks of one precision path against the oracle's differences.
+    Cells that the kernel flags unstable must be cells whose trajectory the
+    oracle shows amplifying (> 1e3); cells the oracle shows amplifying are
+    left out of the comparison with differences (a step of h = 1e-6 moves
+    their state beyond its linear range), so are the flagged ones."""
+    numc, vol = cells[5], cells[6].sum(axis=(1, 2))
+    Jo = ref.Jo
+    ok = (flags & unstable_bit) == 0
+    chaotic = ref.amp > 1e3
+    print("prec %d: %d cells flagged unstable, %d amplifying beyond 1e3 in the oracle, %d of them flagged"
+          % (prec, int((~ok).sum()), int(chaotic.sum()), int((chaotic & ~ok).sum())), flush=True)
+    assert np.all(chaotic[~ok]), np.nonzero(~ok & ~chaotic)[0]
     assert np.all(np.isfinite(g[ok])) and np.all(np.isfinite(je[ok]))
-    L = O.make_layout()
-    P = O.make_params(level=4, lsdistupdate=distr)
-    Jo = oracle_J(O, L, P, desc, cells, nsteps)
-    assert Jo.max() > 0, "no precipitation formed: the objective is identically zero"
+    sel = ok & ~chaotic
+    assert sel.mean() >= 0.5
     # value part against the oracle's trajectory
     tolv = 1e-9 if prec == 0 else 1e-4
-    assert np.all(np.abs(jc - Jo) <= tolv * np.abs(Jo) + 1e-30), np.max(np.abs(jc - Jo) / np.maximum(Jo, 1e-30))
+    assert np.all(np.abs(jc - Jo)[sel] <= tolv * np.abs(Jo[sel]) + 1e-30), \
+        np.max(np.abs(jc - Jo)[sel] / np.maximum(Jo[sel], 1e-30))
     assert abs(J - jc.sum()) <= 1e-12 * abs(J) + 1e-300
-    T, p, rs, rsi, rv, numc, volc = cells
-    vol = volc.sum(axis=(1, 2))
-    # dJ/dE: enhancement factor E = 1 + chargeCollEnh.  Each difference
-    # returns the central quotient and |forward - backward| at its step.
-    def fdE(h):
-        Jp = oracle_J(O, L, O.make_params(level=4, lcharge=1, charge_coll_enh=h, lsdistupdate=distr), desc, cells,
-                      nsteps)
-        Jm = oracle_J(O, L, O.make_params(level=4, lcharge=1, charge_coll_enh=-h, lsdistupdate=distr), desc, cells,
-                      nsteps)
-        return (Jp - Jm) / (2 * h), np.abs(Jp - 2 * Jo + Jm) / h
-    (fe, ae), (fe4, _) = fdE(H), fdE(4 * H)
-    nk = check_fd("dJ/dE per cell (prec %d)" % prec, je[ok], fe[ok], fe4[ok], vol[ok], np.ones(ok.sum()), Jo[ok],
-                  ae[ok])
-    npair = int(ok.sum())
-    # the total leaves the unstable cells out
-    assert abs(dJdE - fe[ok].sum()) <= TOL * abs(fe[ok].sum()) + NOISE * vol[ok].sum()
-    # dJ/dN0, every bin with particles
-    nb_checked = 0
-
-    def fdN(b, h, live):
-        cp = list(cells)
-        cm = list(cells)
-        cp[5] = numc.copy()
-        cp[5][:, b] *= 1 + h
-        cm[5] = numc.copy()
-        cm[5][:, b] *= 1 - h
-        Jp, Jm = oracle_J(O, L, P, desc, cp, nsteps), oracle_J(O, L, P, desc, cm, nsteps)
-        den = h * np.where(live, numc[:, b], 1.)
-        return (Jp - Jm) / (2 * den), np.abs(Jp - 2 * Jo + Jm) / den
-    for b in range(numc.shape[1]):
-        live = (numc[:, b] > 0) & ok
-        # a bin empty in every cell: a relative step is zero and an absolute
-        # one below the activity limit is lost in rounding, so only
-        # finiteness is checked there
-        if not live.any():
-            continue
-        (fd, ad), (fd4, _) = fdN(b, H, live), fdN(b, 4 * H, live)
+    # the dJ/dE total is the sum over the cells not flagged
+    assert abs(dJdE - je[ok].sum()) <= 1e-12 * np.abs(je[ok]).sum() + 1e-300
+    assert abs(je[sel].sum() - ref.fe[sel].sum()) <= TOL * abs(ref.fe[sel].sum()) + NOISE * vol[sel].sum()
+    nk = check_fd("dJ/dE per cell (prec %d)" % prec, je[sel], ref.fe[sel], ref.fe4[sel], vol[sel],
+                  np.ones(sel.sum()), Jo[sel], ref.ae[sel])
+    npair = int(sel.sum())
+    for b in ref.bins:
+        live = (numc[:, b] > 0) & sel
+        (fd, ad), (fd4, _) = ref.fdn[b]
         nk += check_fd("dJ/dN0 bin %d (prec %d)" % (b, prec), g[live, b], fd[live], fd4[live], vol[live],
                        numc[live, b], Jo[live], ad[live])
         npair += int(live.sum())
-        nb_checked += 1
-    assert nb_checked >= 15
-    print("pairs at thresholds: %d of %d" % (nk, npair))
+    print("prec %d: pairs at thresholds: %d of %d" % (prec, nk, npair))
     assert nk <= MAX_KINK_FRAC * npair
+
+
+@pytest.mark.parametrize("level,distr", [(4, 0), (4, 1), (5, 0), (5, 1)])
+def test_gradient_against_oracle_fd(capi, oracle_mod, level, distr):
+    """dJ/dN0 for every bin of every cell and dJ/dE against central
+    differences of the FP64 oracle over a 10-step trajectory, both precision
+    paths: level 4 on workload-3 cells (SO4), level 5 on workload-5 cells
+    (SO4 + dust, ice bins with rime, T 240-275 K), each without and with the
+    size-distribution update after every step."""
+    from salsa_amd import workload as W
+    species = ("SO4", "DU") if level == 5 else ("SO4",)
+    desc = capi.bin_desc(species=species)
+    cells = small_cells(desc, W, cfg=5 if level == 5 else 3)
+    nsteps = 10
+    print("case gradient vs oracle FD, level %d, distr_update %d, %d cells, %d steps"
+          % (level, distr, len(cells[0]), nsteps), flush=True)
+    ref = OracleFD(oracle_mod, level, distr, desc, cells, nsteps)
+    assert ref.Jo.max() > 0, "no precipitation formed: the objective is identically zero"
+    assert len(ref.bins) >= 15
+    for prec in (0, 1):
+        J, dJdE, jc, je, g, flags = gpu_gradient(capi, desc, prec, cells, nsteps, level=level, distr=distr)
+        assert not np.any(flags & capi.CELL_NONFINITE)
+        check_against_oracle(ref, cells, prec, J, dJdE, jc, je, g, flags, capi.CELL_GRAD_UNSTABLE)
 
 
 def test_gradient_workload3_against_gpu_fd(capi):

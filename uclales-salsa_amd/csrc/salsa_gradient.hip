@@ -292,6 +292,9 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
       for (int s = 0; s < NS; ++s) V[b][s] = dconst(s < nsw ? r[1 + s] * dn / species_rho(D, c, s) : 0.);
     }
     const DK E = dvar((KT)D.charge_factor, dir == nt ? KT(1) : KT(0));
+    // this direction's input: a relative change x0 * eps moves a quantity X
+    // by |dX/dx| x0 eps
+    const double x0 = dir < nt ? rec[D.sbase[dir]] : (dir == nt ? D.charge_factor : 0.);
 
     for (int step = 0; step < g.nsteps; ++step) {
       if (!a.lsfreeRH) rv = dmin(rv, rs * a.rhlim);       // mo_salsa_driver.f90:193-197
@@ -734,15 +737,24 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
       if (D.lsdistupdate) flags |= distr_update_cell<DD>(D, N, &V[0][0], NS);
 
       // ---- the new state through LES units, as the next run_SALSA call sees it ----
+      // The cell's linearisation is unstable when the state of an active bin
+      // (numc > nlim) or the vapour amplifies a relative input change more
+      // than 1e8 times: no perturbation resolvable in FP64 follows it then
+      // (the reference's discrete condensation trajectory is unstable there).
       for (int b = 0; b < nt; ++b) {
         const int c = D.cat_of[b];
         const int nsw = c < 3 ? D.nwet : D.nspec;
         const DD nn = N[b] / dn;
         if (!isfinite(nn.v)) flags |= SALSA_CELL_NONFINITE;
         if (!isfinite(nn.d)) flags |= SALSA_CELL_GRAD_UNSTABLE;
+        const bool active = N[b].v > D.nlim[b];
+        if (active && fabs(N[b].d) * x0 > 1.e8 * N[b].v) flags |= SALSA_CELL_GRAD_UNSTABLE;
         N[b] = nn * dn;
+        double vtot = 0.;
+        for (int s = 0; s < nsw; ++s) vtot += V[b][s].v;
         for (int s = 0; s < nsw; ++s) {
           const double rho = species_rho(D, c, s);
+          if (active && fabs(V[b][s].d) * x0 > 1.e8 * vtot) flags |= SALSA_CELL_GRAD_UNSTABLE;
           const DD mm = V[b][s] * rho / dn;
           if (!isfinite(mm.v)) flags |= SALSA_CELL_NONFINITE;
           if (!isfinite(mm.d)) flags |= SALSA_CELL_GRAD_UNSTABLE;
@@ -751,15 +763,14 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
       }
       if (!isfinite(rv.v)) flags |= SALSA_CELL_NONFINITE;
       if (!isfinite(rv.d)) flags |= SALSA_CELL_GRAD_UNSTABLE;
+      if (fabs(rv.d) * x0 > 1.e8 * rv.v) flags |= SALSA_CELL_GRAD_UNSTABLE;
     }
 
     // objective of this cell: precipitation volume concentration
     DD J = dconst(0.);
     for (int b = D.off[2]; b < D.off[2] + D.nb[2]; ++b)
       for (int s = 0; s < D.nwet; ++s) J += V[b][s];
-    // amplification of a relative change of this direction's input in J:
-    // beyond 1e8 no perturbation resolvable in FP64 behaves linearly
-    const double x0 = dir < nt ? rec[D.sbase[dir]] : (dir == nt ? D.charge_factor : 0.);
+    // the same bound for the objective
     if (J.v > 0. && fabs(J.d) * x0 > 1.e8 * J.v) flags |= SALSA_CELL_GRAD_UNSTABLE;
     if (!isfinite(J.d)) flags |= SALSA_CELL_GRAD_UNSTABLE;
     if (dir < nt) {
