@@ -423,6 +423,116 @@ def oracle_rate_fields(oracle_mod, L, rr, dn, level):
     return np.array(f)
 
 
+@pytest.mark.parametrize("cfg", [4, 5])
+def test_gpu_full_size_configs_4_and_5(capi, oracle_mod, cfg):
+    """128x128x100 (1,638,400 cells).  Config 4: level 4, 15 % of the cells
+    cloudy with precipitation; config 5: level 5, SO4 + dust, ice bins with
+    rime, T 240-275 K.  Ten FP64 steps in place with autotuned launch shapes:
+    finite and non-negative state; the tenth step once more in FP32-mixed
+    from the same state, against FP64; the FP64 tenth step against the oracle
+    on a 2,000-cell sample with the conditioning allowance, and the per-cell
+    totals of every dry species (and of water at level 4) of both paths
+    against the oracle's.  (The semi-implicit coagulation update of the
+    reference does not conserve volume exactly -- a few 1e-6 per step in
+    cells with precipitation -- so totals are compared with the oracle's,
+    not with the inputs'.)"""
+    import time
+    import torch
+    from salsa_amd import workload as W
+    t0 = time.time()
+    print("case full-size config %d: generating the fields" % cfg, flush=True)
+    desc, f = config_cells(capi, cfg)
+    level = 5 if cfg == 5 else 4
+    ncat = 4 if level == 5 else 3
+    g = f.grid
+    print("  %d cells generated in %.0f s" % (g.nz * g.nx * g.ny, time.time() - t0), flush=True)
+    dev = torch.device("cuda")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fd = dict(press=T(f.press), temp=T(f.temp), rsl=T(f.rsl), rsi=T(f.rsi), dn=T(f.dn), rv=T(f.rv))
+    n_p = [T(x) for x in f.n[:ncat]]
+    m_p = [T(x) for x in f.m[:ncat]]
+
+    def salsa_state():
+        fo = W.Fields(grid=g, level=level, press=f.press, temp=f.temp, rsl=f.rsl, rsi=f.rsi, dn=f.dn,
+                      rv=fd["rv"].cpu().numpy(), n=[x.cpu().numpy() for x in n_p] + [None] * (4 - ncat),
+                      m=[x.cpu().numpy() for x in m_p] + [None] * (4 - ncat))
+        return W.to_salsa_units(desc, fo)
+
+    Tc, pc, rsc, rsic, rv0, nc0, vc0 = salsa_state()
+    ns = desc.nspec - 1 if level == 4 else desc.nspec
+    ndry = desc.nspec - 2
+    ctx = capi.Context(desc, capi.params(level=level, precision=0), g.nzp, g.nxp, g.nyp)
+    ctx.autotune(fd, n_p, m_p)
+    for _ in range(9):
+        ctx.step_device(fd, n_p, m_p, flags=capi.STEP_INPLACE)
+    ctx.sync()
+    s9 = dict(rv=fd["rv"].clone(), n=[x.clone() for x in n_p], m=[x.clone() for x in m_p])
+    _, _, _, _, rv9, nc9, vc9 = salsa_state()
+    ctx.step_device(fd, n_p, m_p, flags=capi.STEP_INPLACE)
+    ctx.sync()
+    fl = ctx.cell_flags()
+    ctx.close()
+    assert not np.any(fl & (capi.CELL_NONFINITE | capi.CELL_NOCONV | capi.CELL_DISTR))
+    _, _, _, _, rv64, nc64, vc64 = salsa_state()
+    assert np.all(np.isfinite(nc64)) and np.all(np.isfinite(vc64)) and np.all(np.isfinite(rv64))
+    assert np.all(nc64 >= 0) and np.all(vc64 >= 0)
+    for s_ in range(ndry):
+        a0, a1 = vc0[:, :, s_].sum(axis=1), vc64[:, :, s_].sum(axis=1)
+        print("  dry species %d: largest per-cell volume change over 10 steps %.2e"
+              % (s_, np.max(np.abs(a1 - a0) / a0)))
+    # the tenth step in FP32-mixed from the same state
+    fd["rv"].copy_(s9["rv"])
+    for a, b in zip(n_p + m_p, s9["n"] + s9["m"]):
+        a.copy_(b)
+    ctx = capi.Context(desc, capi.params(level=level, precision=1), g.nzp, g.nxp, g.nyp)
+    ctx.autotune(fd, n_p, m_p)
+    ctx.step_device(fd, n_p, m_p, flags=capi.STEP_INPLACE)
+    ctx.sync()
+    ctx.close()
+    _, _, _, _, rv32, nc32, vc32 = salsa_state()
+    e = errors(rv32, nc32, vc32, rv64, nc64, vc64, ns)
+    fr = [np.mean(rel(nc32, nc64, FLOOR_N) > 1e-5)] + \
+         [np.mean(rel(vc32[:, :, s_], vc64[:, :, s_], FLOOR_V) > 1e-5) for s_ in range(ns)]
+    print("  tenth step, mixed vs FP64: rv %.2e N %.2e V %.2e; fraction beyond 1e-5: N %.1e, V %s"
+          % (e + (fr[0], " ".join("%.1e" % x for x in fr[1:]))), flush=True)
+    assert e[0] < 1e-5
+    assert max(fr) < 1e-3
+    # the FP64 tenth step against the oracle on a sample
+    sel = np.sort(np.random.default_rng(cfg).choice(len(Tc), 2000, replace=False))
+    L = oracle_mod.make_layout(species=(0, 3) if level == 5 else (0,))
+    P = oracle_mod.make_params(level=level)
+    nt = nc9.shape[1]
+    nco = np.zeros((len(sel), L.ntotal))
+    vco = np.zeros((len(sel), L.ntotal, 9))
+    nco[:, :nt] = nc9[sel]
+    vco[:, :nt] = vc9[sel]
+    args = (Tc[sel], pc[sel], rsc[sel], rsic[sel], rv9[sel], nco, vco)
+    orv, onc, ovc, _ = oracle_mod.step_cells(L, P, *args)
+    sens = oracle_sensitivity(oracle_mod, L, P, *args)
+    got = (rv64[sel], nc64[sel], vc64[sel])
+    out, nf = check_condition_aware(got, (orv, onc[:, :nt], ovc[:, :nt]), (sens[0], sens[1][:, :nt],
+                                    sens[2][:, :nt]), 1e-10, ns, "oracle")
+    nel = vc64[sel][:, :, :ns].size
+    print("  FP64 vs oracle on %d cells: max rel %s; inside the conditioning allowance %d of %d"
+          % (len(sel), " ".join("%.1e" % x for x in out), nf, nel))
+    assert nf <= max(2, nel // 200)
+    # per-cell totals against the oracle's
+    rhoair = 28.97e-3 * pc[sel] / (8.314 * Tc[sel])
+    for name, vx, rvx, tol_dry in (("fp64", vc64, rv64, 1e-12), ("mixed", vc32, rv32, 1e-9)):
+        for s_ in range(ndry):
+            a, b = vx[sel, :, s_].sum(axis=1), ovc[:, :nt, s_].sum(axis=1)
+            ed = np.max(np.abs(a - b) / b)
+            print("  %s: dry species %d total vs oracle %.2e" % (name, s_, ed))
+            assert ed < tol_dry
+        if level == 4:
+            w = rvx[sel] * rhoair + vx[sel, :, ndry].sum(axis=1) * 1000.
+            wo = orv * rhoair + ovc[:, :nt, ndry].sum(axis=1) * 1000.
+            ew = np.max(np.abs(w - wo) / wo)
+            print("  %s: water total vs oracle %.2e" % (name, ew))
+            assert ew < 1e-12
+    print("  %.0f s" % (time.time() - t0))
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("name,kw", GOLDEN_CASES)
 def test_gpu_rates_match_oracle_and_reference(capi, oracle_mod, name, kw, prec):
