@@ -15,6 +15,7 @@
 ! the zcc* arrays by it, which is what fetch_coag provides for a single cell
 ! (mo_salsa_driver.f90:294-297, :420-422; mo_salsa_dynamics.f90:146-147).
 ! With sw(15) = 1 each step ends with distr_update (mo_salsa.f90:148-149).
+! With lsauto%mode = 2 autoconv2 follows condensation (mo_salsa.f90:110-112).
 !
 ! Usage: ref_harness <input.bin> <output.bin>
 PROGRAM ref_harness
@@ -28,6 +29,7 @@ PROGRAM ref_harness
                              zccpa, zccia, zccpc, zccic, zccip, iaero, icloud, iprecp, iice
   USE mo_salsa_dynamics, ONLY : coagulation, condensation
   USE mo_salsa_update, ONLY : distr_update
+  USE mo_salsa_cloud, ONLY : autoconv2
   USE mo_salsa_coagulation_kernels, ONLY : update_coagulation_kernels
   USE classBinLayout, ONLY : BinLayout
   USE classProcessRates, ONLY : Rates
@@ -138,6 +140,7 @@ PROGRAM ref_harness
            g1 = 0.; g2 = 0.; g3 = 0.; g4 = 0.; g5 = 0.; pbl = 1
            IF (lscnd%state) CALL condensation(1, 1, 1, 1, g1, g2, g3, g4, g5, RV, RS, RSI, &
                                               T, P, dt, pbl)
+           IF (lsauto%state .AND. lsauto%mode == 2) CALL autoconv2(1, 1, 1, dt)   ! mo_salsa.f90:110-112
            IF (sw(15) == 1) CALL distr_update(1, 1, level)      ! mo_salsa.f90:148-149
         END DO
         CALL SYSTEM_CLOCK(c1)

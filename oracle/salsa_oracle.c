@@ -1527,6 +1527,53 @@ static int distr_update(const so_layout *L, so_cell *c, int level) {
 }
 
 /* ====================================================================== */
+/* autoconv2: cloud drops beyond 50 um into the first precipitation bin,  */
+/* src/src_salsa/mo_salsa_cloud.f90:1401-1477.  Each cloud bin, largest   */
+/* first, is read as a lognormal of width sigmag = 1.2 around its mean    */
+/* volume; the number and water volume above zd0 = 50 um move to precip   */
+/* bin 1 (the dry species with the number fraction).  cumlognorm:         */
+/* src/src_shared/math_functions.f90:64-76.  The loop over bins stops     */
+/* once the converted number exceeds 1e10 m^-3 s^-1 * dt (:1459-1462).    */
+/* ====================================================================== */
+static double cumlognorm(double dg, double sigmag, double dpart) {
+  const double hlp1 = log(dpart) - log(dg);
+  const double hlp2 = sqrt(2.) * log(sigmag);
+  return 0.5 + 0.5 * erf(hlp1 / hlp2);
+}
+
+static void autoconv2(const so_layout *L, so_cell *c, double ptstep) {
+  const double zd0 = 50.e-6, sigmag = 1.2, max_rate_autoc = 1.0e10;
+  const int nwet = L->nwet, ndry = L->ndry, iwa = L->iwa;
+  const int P1 = L->iprecp;
+  const double ls = log(sigmag);
+  double tot = 0.;
+  for (int cc = L->ncld - 1; cc >= 0; --cc) {
+    const int b = L->icloud + cc;
+    const double Ntot = c->numc[b];
+    double Vtot = 0.;
+    for (int s = 0; s < nwet; ++s) Vtot += c->volc[b][s];
+    if (Ntot > L->nlim[b] && Vtot > 0.) {
+      const double dvg = pow(Vtot / Ntot / pi6, 1. / 3.) * exp((3. * (ls * ls)) / 2.);
+      const double dg = dvg * exp(-(3. * (ls * ls)));
+      const double Vrem = Vtot * (1. - cumlognorm(dvg, sigmag, zd0));
+      const double Nrem = Ntot * (1. - cumlognorm(dg, sigmag, zd0));
+      if (Vrem > 0. && Nrem > L->nlim[P1]) {
+        for (int s = 0; s < ndry; ++s) {
+          c->volc[P1][s] = c->volc[P1][s] + c->volc[b][s] * (Nrem / Ntot);
+          c->volc[b][s] = c->volc[b][s] * (1. - (Nrem / Ntot));
+        }
+        c->volc[P1][iwa] = c->volc[P1][iwa] + c->volc[b][iwa] * (Vrem / Vtot);
+        c->volc[b][iwa] = c->volc[b][iwa] * (1. - (Vrem / Vtot));
+        c->numc[P1] = c->numc[P1] + Nrem;
+        c->numc[b] = c->numc[b] - Nrem;
+        tot = tot + Nrem;
+        if (tot > max_rate_autoc * ptstep) break;
+      }
+    }
+  }
+}
+
+/* ====================================================================== */
 /* One cell: salsa() with coagulation and condensation enabled,           */
 /* src/src_salsa/mo_salsa.f90:92-107                                      */
 /* ====================================================================== */
@@ -1549,6 +1596,7 @@ int so_step_cell_ks(const so_layout *L, const so_params *P,
   else for (int b = 0; b < L->ntotal; ++b) update_diameter(L, &c, b);
   if (P->lscnd && (P->lscndh2ocl || P->lscndh2oae || P->lscndh2oic))
     status |= gpparth2o(L, P, &c, ptemp, ppres, prs, prsi, prv, ptstep);
+  if (P->lsauto_state && P->lsauto_mode == 2) autoconv2(L, &c, ptstep);   /* mo_salsa.f90:110-112 */
   if (P->lsdistupdate) status |= distr_update(L, &c, P->level);
   for (int b = 0; b < L->ntotal; ++b) {
     numc[b] = c.numc[b];

@@ -53,7 +53,7 @@ def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0, kstore=Fal
     np.savez_compressed(
         os.path.join(HERE, name),
         level=level, nsteps=nsteps, lcharge=lcharge, charge_enh=enh, kstore=int(kstore), distr=int(distr),
-        lsauto=P.lsauto_state, nbin=np.array(list(L.nbin)), nprc=L.nprc, nice=L.nice,
+        lsauto=P.lsauto_state, lsauto_mode=P.lsauto_mode, nbin=np.array(list(L.nbin)), nprc=L.nprc, nice=L.nice,
         species=np.array(list(L.dry_species)[:L.nspec_dry]),
         T=T, p=p, rs=rs, rsi=rsi, rv=rv, numc=nc, volc=vc[:, :, :ns],
         out_rv=R["rv"], out_numc=R["numc"], out_volc=R["volc"][:, :, :ns], out_rates=R["rates"])
@@ -96,11 +96,38 @@ def distr_cases():
     save_step("step_cfg5_distr.npz", L5, 5, P5, tuple(x[sel] for x in f5), distr=True)
 
 
+def autoconv2_cases():
+    """lsauto%mode = 2 (autoconv2 after condensation, mo_salsa.f90:110-112, and
+    no precipitation formation inside coagulation): config-1 cells over three
+    steps, cloudy cells with precipitation over two with distr_update, and
+    mixed-phase cells over one (written separately: `--only autoconv2`)."""
+    L = O.make_layout()
+    P4 = O.make_params(level=4, lsauto_mode=2)
+    f1 = W.to_salsa_units(L, W.config_fields(L, 1), level=4)
+    nc, vc = pad(L, f1[5], f1[6])
+    cells = tuple(f1[:5]) + (nc, vc)
+    sel = pick(cells, 120, 21)
+    save_step("step_cfg1_auto2.npz", L, 4, P4, tuple(x[sel] for x in cells), nsteps=3)
+    f4 = W.to_salsa_units(L, W.config_fields(L, 4, grid_override=(20, 8, 8)), level=4)
+    nc, vc = pad(L, f4[5], f4[6])
+    cells = tuple(f4[:5]) + (nc, vc)
+    sel = pick(cells, 120, 24)
+    save_step("step_cfg4_auto2_distr.npz", L, 4, O.make_params(level=4, lsauto_mode=2, lsdistupdate=1),
+              tuple(x[sel] for x in cells), nsteps=2, distr=True)
+    L5 = O.make_layout(species=(0, 3))
+    P5 = O.make_params(level=5, lsauto_mode=2)
+    f5 = W.to_salsa_units(L5, W.config_fields(L5, 5, grid_override=(20, 8, 8)), level=5)
+    sel = np.sort(np.random.default_rng(25).choice(len(f5[0]), 80, replace=False))
+    save_step("step_cfg5_auto2.npz", L5, 5, P5, tuple(x[sel] for x in f5))
+
+
 def main():
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "kstore":
         return kstore_case()
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "distr":
         return distr_cases()
+    if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "autoconv2":
+        return autoconv2_cases()
     if not refio.available():
         sys.exit("oracle/_ref/ref_harness missing: run `make -C oracle ref` first")
     # warm layout nbin=(3,4): aerosol 11, cloud 8, precip 7 (+ 7 inert ice bins)

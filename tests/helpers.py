@@ -41,6 +41,8 @@ def oracle_params_for(g):
         kw.update(lcharge=1, charge_coll_enh=float(g["charge_enh"]))
     if not int(g["lsauto"]):
         kw.update(lsauto_state=0)
+    if "lsauto_mode" in g:
+        kw.update(lsauto_mode=int(g["lsauto_mode"]))
     if "distr" in g and int(g["distr"]):
         kw.update(lsdistupdate=1)
     return O.make_params(level=int(g["level"]), **kw)

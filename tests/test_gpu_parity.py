@@ -58,6 +58,9 @@ GOLDEN_CASES = [
     ("step_cfg1_distr.npz", {"lsdistupdate": 1}),
     ("step_cfg4_distr.npz", {"lsdistupdate": 1}),
     ("step_cfg5_distr.npz", {"lsdistupdate": 1}),
+    ("step_cfg1_auto2.npz", {"lsauto_mode": 2}),
+    ("step_cfg4_auto2_distr.npz", {"lsauto_mode": 2, "lsdistupdate": 1}),
+    ("step_cfg5_auto2.npz", {"lsauto_mode": 2}),
 ]
 
 

@@ -136,3 +136,49 @@ def test_distr_update_conserves_number_and_volume(oracle_mod):
             assert np.all(np.abs(N1 - N0) <= 1e-13 * np.abs(N0) + 1e-300), (cfg, c)
             V0, V1 = vc[:, sl].sum(axis=(1, 2)), vc2[:, sl].sum(axis=(1, 2))
             assert np.all(np.abs(V1 - V0) <= 1e-13 * np.abs(V0) + 1e-300), (cfg, c)
+
+
+def test_autoconv2_against_lognormal_quadrature(oracle_mod):
+    """autoconv2 alone (coagulation and condensation off, lsauto mode 2,
+    mo_salsa_cloud.f90:1401-1477) on one cloud bin read as a lognormal number
+    distribution of geometric width 1.2 whose third moment is the bin's mean
+    volume: the number above 50 um and the water volume above 50 um move to
+    precipitation bin 1, the sulphate with the number fraction.  Expected
+    values from numerical quadrature of the lognormal (scipy), not from the
+    closed forms the code uses; number and volume are conserved."""
+    from scipy import integrate
+    O = oracle_mod
+    L = O.make_layout()
+    P = O.make_params(level=4, lscoag=0, lscnd=0, lsauto_mode=2)
+    s = np.log(1.2)
+    b = L.icloud + 7
+    for dbar in (30e-6, 42e-6, 55e-6):           # diameter of the mean volume
+        N0, vmean = 5.e6, 0.5235988 * dbar ** 3
+        numc = np.zeros(L.ntotal)
+        volc = np.zeros((L.ntotal, 9))
+        numc[b] = N0
+        volc[b, 0] = 0.02 * N0 * vmean               # sulphate
+        volc[b, L.iwa] = 0.98 * N0 * vmean           # water
+        # lognormal number density whose third moment gives vmean, integrated
+        # over x = ln d: find its median by bisection
+        phi = lambda x, mu: np.exp(-(x - mu) ** 2 / (2 * s * s)) / (s * np.sqrt(2 * np.pi))
+        mom = lambda k, mu, a=-np.inf: integrate.quad(lambda x: np.exp(k * (x - mu)) * phi(x, mu),
+                                                      max(a, mu - 14 * s), mu + 14 * s, epsabs=0, epsrel=1e-13,
+                                                      limit=400)[0] * np.exp(k * mu)
+        lo, hi = np.log(dbar) - 1., np.log(dbar) + 1.
+        for _ in range(100):
+            mid = 0.5 * (lo + hi)
+            lo, hi = (mid, hi) if mom(3, mid) < dbar ** 3 else (lo, mid)
+        mu = 0.5 * (lo + hi)
+        x0 = np.log(50e-6)
+        fn = mom(0, mu, x0) / mom(0, mu)
+        fv = mom(3, mu, x0) / mom(3, mu)
+        rv, nc, vc, st = O.step_cells(L, P, [285.], [9.e4], [0.01], [0.01], [0.01], numc[None], volc[None])
+        p1 = L.iprecp
+        assert nc[0, p1] == pytest.approx(N0 * fn, rel=1e-7)
+        assert vc[0, p1, L.iwa] == pytest.approx(volc[b, L.iwa] * fv, rel=1e-7)
+        assert vc[0, p1, 0] == pytest.approx(volc[b, 0] * fn, rel=1e-7)
+        assert nc[0, b] + nc[0, p1] == pytest.approx(N0, rel=1e-15)
+        for sp in (0, L.iwa):
+            assert vc[0, b, sp] + vc[0, p1, sp] == pytest.approx(volc[b, sp], rel=1e-15)
+        assert fn > 1e-12

@@ -91,6 +91,7 @@ def test_oracle_kernel_store_matches_reference(oracle_mod):
 
 
 DISTR_CASES = ["step_cfg1_distr.npz", "step_cfg4_distr.npz", "step_cfg5_distr.npz"]
+AUTO2_CASES = ["step_cfg1_auto2.npz", "step_cfg4_auto2_distr.npz", "step_cfg5_auto2.npz"]
 
 
 @pytest.mark.parametrize("name", DISTR_CASES)
@@ -108,6 +109,36 @@ def test_oracle_distr_update_matches_reference(oracle_mod, name):
     args = (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"], full_volc(g["volc"]))
     rv, nc, vc, st = oracle_mod.step_cells(L, P, *args, nsteps=nsteps)
     assert not np.any(st & (2 | 8))
+    sens = oracle_sensitivity(oracle_mod, L, P, *args, nsteps=nsteps)
+    ns = g["volc"].shape[2]
+    out, nflag = check_condition_aware((rv, nc, vc[:, :, :ns]), (g["out_rv"], g["out_numc"], g["out_volc"]),
+                                       (sens[0], sens[1], sens[2][:, :, :ns]), 1e-12, ns, name)
+    print(name, "max rel", out, "elements needing the conditioning allowance:", nflag)
+    assert nflag <= 0.002 * nc.size
+
+
+@pytest.mark.parametrize("name", AUTO2_CASES)
+def test_oracle_autoconv2_matches_reference(oracle_mod, name):
+    """Steps with lsauto%mode = 2: no precipitation formation inside
+    coagulation, autoconv2 after condensation (mo_salsa.f90:110-112,
+    mo_salsa_cloud.f90:1401-1477), against the compiled reference; each
+    element within max(1e-12, 10 x the oracle's own spread under 1e-13 input
+    noise), as for distr_update."""
+    g = golden(name)
+    assert int(g["lsauto_mode"]) == 2
+    L = oracle_layout_for(g)
+    P = oracle_params_for(g)
+    assert P.lsauto_mode == 2
+    nsteps = int(g["nsteps"])
+    args = (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"], full_volc(g["volc"]))
+    rv, nc, vc, st = oracle_mod.step_cells(L, P, *args, nsteps=nsteps)
+    assert not np.any(st & (2 | 8))
+    # autoconv2 acted: without autoconversion precipitation bin 1 ends otherwise
+    P0 = type(P).from_buffer_copy(P)
+    P0.lsauto_state = 0
+    _, nc0, _, _ = oracle_mod.step_cells(L, P0, *args, nsteps=nsteps)
+    p1 = L.iprecp
+    assert np.count_nonzero(np.abs(nc[:, p1] - nc0[:, p1]) > 1e-6 * np.abs(nc[:, p1])) > 10
     sens = oracle_sensitivity(oracle_mod, L, P, *args, nsteps=nsteps)
     ns = g["volc"].shape[2]
     out, nflag = check_condition_aware((rv, nc, vc[:, :, :ns]), (g["out_rv"], g["out_numc"], g["out_volc"]),

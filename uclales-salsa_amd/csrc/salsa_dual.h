@@ -97,6 +97,10 @@ template <class T> __device__ __forceinline__ Dual<T> dexp(Dual<T> x) {
 template <class T> __device__ __forceinline__ Dual<T> dlog(Dual<T> x) {
   return {DMath<T>::log_(x.v), x.d * DMath<T>::rcp_(x.v)};
 }
+// erf, FP64 only (d/dx erf x = 2/sqrt(pi) exp(-x^2))
+__device__ __forceinline__ Dual<double> derf(Dual<double> x) {
+  return {::erf(x.v), x.d * (1.1283791670955126 * ::exp(-(x.v * x.v)))};
+}
 template <class T> __device__ __forceinline__ Dual<T> dcbrt(Dual<T> x) {
   const T c = DMath<T>::cbrt_(x.v);
   return {c, c != T(0) ? x.d * DMath<T>::rcp_(T(3) * c * c) : T(0)};

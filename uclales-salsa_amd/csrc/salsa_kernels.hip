@@ -24,6 +24,7 @@
 
 #include <stdint.h>
 
+#include "salsa_autoconv.h"
 #include "salsa_distr.h"
 #include "salsa_internal.h"
 
@@ -1208,6 +1209,13 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   }
   __syncthreads();
 
+  // ---- 5a. autoconversion by lognormal fitting (autoconv2, lsauto%mode 2,
+  //      mo_salsa.f90:110-112): serial over the cloud bins on one lane ----
+  if (D.autoconv2) {
+    if (sub == 0 && live) autoconv2_cell<double>(D, sN, sV, nsV, dt);
+    __syncthreads();
+  }
+
   // ---- 5b. size-distribution update (distr_update, mo_salsa.f90:148-149):
   //      serial over the bins, on one lane of the cell ----
   if (D.lsdistupdate) {
@@ -1375,9 +1383,10 @@ __global__ void __launch_bounds__(256) k_gather(const DevLayout *__restrict__ Dp
 }
 
 // Exclusive prefix over the tiles of each class's count (one block of 1024
-// threads): each thread sums a contiguous run of tiles (loads batched), the
-// run sums are scanned across lanes and wavefronts, and each thread writes
-// its tiles' first positions; counts receives the class totals.
+// threads).  Wavefront w takes a contiguous run of tiles in chunks of 64, one
+// tile per lane, so that every load is coalesced: pass 1 sums the run, the run
+// sums are scanned across the wavefronts, pass 2 scans each chunk with lane
+// shuffles and carries the chunk total.  counts receives the class totals.
 __device__ __forceinline__ int wave_incl_scan(int x) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -1392,19 +1401,18 @@ __global__ void __launch_bounds__(1024) k_class_scan(const int32_t *__restrict__
                                                      int32_t *toff, int32_t *counts) {
   __shared__ int32_t wsum[3][16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int per = (ntile + 1023) / 1024;
-  const int lo = t * per, hi = lo + per < ntile ? lo + per : ntile;
+  const int per = ((ntile + 15) / 16 + 63) / 64 * 64;       // tiles per wavefront, whole chunks
+  const int lo = w * per, hi = lo + per < ntile ? lo + per : ntile;
   int acc[3] = {0, 0, 0};
-#pragma unroll 8
-  for (int i = lo; i < hi; ++i) {
+#pragma unroll 4
+  for (int i = lo + lane; i < hi; i += 64) {
     acc[0] += tcount[3 * i];
     acc[1] += tcount[3 * i + 1];
     acc[2] += tcount[3 * i + 2];
   }
-  int incl[3];
   for (int k = 0; k < 3; ++k) {
-    incl[k] = wave_incl_scan(acc[k]);
-    if (lane == 63) wsum[k][w] = incl[k];
+    const int tot = wave_incl_scan(acc[k]);
+    if (lane == 63) wsum[k][w] = tot;
   }
   __syncthreads();
   if (t < 64) {
@@ -1416,17 +1424,15 @@ __global__ void __launch_bounds__(1024) k_class_scan(const int32_t *__restrict__
     }
   }
   __syncthreads();
-  int run[3];
-  for (int k = 0; k < 3; ++k) run[k] = wsum[k][w] + incl[k] - acc[k];
-#pragma unroll 8
-  for (int i = lo; i < hi; ++i) {
-    const int c0 = tcount[3 * i], c1 = tcount[3 * i + 1], c2 = tcount[3 * i + 2];
-    toff[3 * i] = run[0];
-    toff[3 * i + 1] = run[1];
-    toff[3 * i + 2] = run[2];
-    run[0] += c0;
-    run[1] += c1;
-    run[2] += c2;
+  int base[3] = {wsum[0][w], wsum[1][w], wsum[2][w]};
+  for (int i0 = lo; i0 < hi; i0 += 64) {
+    const int i = i0 + lane;
+    for (int k = 0; k < 3; ++k) {
+      const int x = i < hi ? tcount[3 * i + k] : 0;
+      const int inc = wave_incl_scan(x);
+      if (i < hi) toff[3 * i + k] = base[k] + inc - x;
+      base[k] += __shfl(inc, 63);
+    }
   }
 }
 

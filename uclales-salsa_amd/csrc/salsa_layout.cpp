@@ -221,6 +221,7 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
   L.cld_a0 = d.ica.cur; L.cld_a1 = d.fca.cur; L.cld_b0 = d.icb.cur; L.cld_b1 = d.fcb.cur;
   L.lscoag = p.lscoag; L.lscnd = p.lscnd;
   L.lsdistupdate = p.lsdistupdate;
+  L.autoconv2 = p.lsauto && p.lsauto_mode == 2;
   L.lscndh2oae = p.lscndh2oae; L.lscndh2ocl = p.lscndh2ocl;
   L.lscndh2oic = level == 5 ? p.lscndh2oic : 0;
   L.eddy_dis = p.eddy_dis; L.Eiagg_max = p.Eiagg_max; L.Eiagg_min = p.Eiagg_min;

@@ -9,10 +9,11 @@
 !> pointers; the library adds the tendencies (Prog tlev 2) and overwrites the
 !> process-rate Diag fields of the interior cells.
 !>
-!> Only the processes of this step are provided.  A configuration that also
-!> switches on secondary ice, activation, autoconv2, ice nucleation, melting,
-!> gas-phase condensation, nucleation or distr_update stops with a message
-!> naming the process (mo_salsa.f90:92-153 calls them in the same step).
+!> Only the processes of this step are provided: coagulation, condensation,
+!> autoconv2 (lsauto mode 2) and distr_update.  A configuration that also
+!> switches on secondary ice, activation, ice nucleation, melting, gas-phase
+!> condensation or nucleation stops with a message naming the process
+!> (mo_salsa.f90:92-153 calls them in the same step).
 !>
 !> The precision mode is taken from the environment variable
 !> SALSA_AMD_PRECISION ("fp64", the default, or "mixed").
@@ -188,7 +189,6 @@ CONTAINS
   ! Processes that salsa() would run in the same step but this library does not
   SUBROUTINE check_switches()
     IF (lssecice%state) CALL outside('secondary ice production (lssecice)')
-    IF (lsauto%state .AND. lsauto%mode == 2) CALL outside('autoconv2 (lsauto mode 2)')
     IF (lsactiv%state) CALL outside('cloud activation (lsactiv)')
     IF (lsicenucl%state .OR. lsicenucl%mode == 2) CALL outside('ice nucleation (lsicenucl)')
     IF (lsicemelt%state) CALL outside('ice melting (lsicemelt)')
