@@ -63,8 +63,8 @@ enum {
  * rates[f*nzp*nxp*nyp + cell], interior cells overwritten, the other cells
  * untouched (the Diag fields of mo_salsa_driver.f90:392-419 with the names
  * given there).  Level 4 writes the first SALSA_NRATE_L4 fields, level 5 all
- * SALSA_NRATE_L5.  Activation and ice nucleation are not part of the
- * coagulation + condensation step; their fields are written as 0. */
+ * SALSA_NRATE_L5.  Ice nucleation is not part of the step; its fields are
+ * written as 0, and so is the activation field unless lsactiv is on. */
 enum {
   SALSA_RATE_AUTOC = 0,        /* s_m_autoc      Autoconversion%volc(iwa)*rhowa/dn            */
   SALSA_RATE_AUTOC50,          /* s_m_autoc50                                                  */
@@ -74,7 +74,7 @@ enum {
   SALSA_RATE_ACCR80,           /* s_m_accr80                                                   */
   SALSA_RATE_ACCOLL,           /* s_m_ACcoll_dry SUM(ACcoll%volc(1:ndry)*rholiq)/dn           */
   SALSA_RATE_APCOLL,           /* s_m_APcoll_dry                                               */
-  SALSA_RATE_ACTIV,            /* s_n_activ      (0)                                           */
+  SALSA_RATE_ACTIV,            /* s_n_activ      Activation%numc/dn (0 without lsactiv)        */
   SALSA_RATE_CONDA,            /* s_m_conda      Cond_a%volc(iwa)/dn                           */
   SALSA_RATE_CONDC,            /* s_m_condc                                                    */
   SALSA_RATE_CONDP,            /* s_m_condp                                                    */
@@ -158,6 +158,10 @@ typedef struct {
                                   mo_salsa_update.f90:21-433); the reference
                                   default is on, salsa_params_default leaves it
                                   off (coagulation + condensation only)         */
+  int32_t lsactiv;             /* cloud activation after condensation: 0 off, 1
+                                  interstitial (lsactiv%mode 1, mo_salsa.f90:115-119,
+                                  actInterst mo_salsa_cloud.f90:859-1195); cloud-base
+                                  activation (mode 2) is not on this path        */
 } salsa_params;
 
 typedef struct salsa_ctx salsa_ctx;

@@ -15,7 +15,9 @@
 ! the zcc* arrays by it, which is what fetch_coag provides for a single cell
 ! (mo_salsa_driver.f90:294-297, :420-422; mo_salsa_dynamics.f90:146-147).
 ! With sw(15) = 1 each step ends with distr_update (mo_salsa.f90:148-149).
-! With lsauto%mode = 2 autoconv2 follows condensation (mo_salsa.f90:110-112).
+! With lsauto%mode = 2 autoconv2 follows condensation (mo_salsa.f90:110-112);
+! with dh(12) = 1 interstitial cloud activation (lsactiv%mode 1) follows it
+! (mo_salsa.f90:115-119).
 !
 ! Usage: ref_harness <input.bin> <output.bin>
 PROGRAM ref_harness
@@ -23,13 +25,14 @@ PROGRAM ref_harness
                          lsauto, eddy_dis, lscgaa, lscgcc, lscgpp, lscgii, lscgca, lscgpa, &
                          lscgia, lscgpc, lscgic, lscgip, lscndh2oae, lscndh2ocl, lscndh2oic, &
                          lscndgas, nsnucl, lsdistupdate, ntotal, nprc, precpbins, nbins, ncld, nice, &
-                         lcgupdt
+                         lcgupdt, lsactiv
   USE mo_salsa_init, ONLY : salsa_initialize, associate_master_switches, setDefaultBinLayouts
   USE mo_salsa_types, ONLY : allSALSA, rateDiag, zccaa, zcccc, zccpp, zccii, zccca, &
                              zccpa, zccia, zccpc, zccic, zccip, iaero, icloud, iprecp, iice
   USE mo_salsa_dynamics, ONLY : coagulation, condensation
   USE mo_salsa_update, ONLY : distr_update
-  USE mo_salsa_cloud, ONLY : autoconv2
+  USE mo_salsa_cloud, ONLY : autoconv2, cloud_activation
+  USE classSection, ONLY : Section
   USE mo_salsa_coagulation_kernels, ONLY : update_coagulation_kernels
   USE classBinLayout, ONLY : BinLayout
   USE classProcessRates, ONLY : Rates
@@ -43,7 +46,8 @@ PROGRAM ref_harness
   INTEGER :: ic, ist, nb, k, i, j, ntot_in
   REAL, ALLOCATABLE :: cellin(:), cellout(:), kmat(:,:)
   REAL :: T(1,1), P(1,1), RS(1,1), RSI(1,1), RV(1,1), dt
-  REAL :: g1(1,1), g2(1,1), g3(1,1), g4(1,1), g5(1,1)
+  REAL :: g1(1,1), g2(1,1), g3(1,1), g4(1,1), g5(1,1), W(1,1)
+  TYPE(Section), ALLOCATABLE :: pactd(:,:,:)
   INTEGER :: pbl(1)
   INTEGER(8) :: c0, c1, crate
   REAL :: tcomp
@@ -75,6 +79,7 @@ PROGRAM ref_harness
   lscoag%switch = (hdr(9) /= 0);  lscoag%state = lscoag%switch;  lscoag%mode = 1
   lscnd%switch = (hdr(10) /= 0);  lscnd%state = lscnd%switch
   lsauto%switch = (hdr(11) /= 0); lsauto%state = lsauto%switch; lsauto%mode = hdr(12)
+  lsactiv%switch = (dh(12) > 0.5); lsactiv%state = lsactiv%switch; lsactiv%mode = 1
   lch = hdr(13)
   lcharge = (lch /= 0)
   nstep = hdr(14)
@@ -99,7 +104,8 @@ PROGRAM ref_harness
   IF (nt /= ntot_in) STOP 'ntotal mismatch'
   IF (lcharge) allSALSA(:,:,:)%chargeTime = 1000.
 
-  ALLOCATE(cellin(5 + nt + 9*nt))
+  ALLOCATE(cellin(5 + nt + 9*nt), pactd(1,1,ncld))
+  W = 0.
   IF (mode == 1) THEN
      ALLOCATE(cellout(1 + nt + 9*nt + 20*10))
   ELSE
@@ -141,6 +147,7 @@ PROGRAM ref_harness
            IF (lscnd%state) CALL condensation(1, 1, 1, 1, g1, g2, g3, g4, g5, RV, RS, RSI, &
                                               T, P, dt, pbl)
            IF (lsauto%state .AND. lsauto%mode == 2) CALL autoconv2(1, 1, 1, dt)   ! mo_salsa.f90:110-112
+           IF (lsactiv%state) CALL cloud_activation(1, 1, 1, T, P, RV, RS, W, pactd)   ! :115-119
            IF (sw(15) == 1) CALL distr_update(1, 1, level)      ! mo_salsa.f90:148-149
         END DO
         CALL SYSTEM_CLOCK(c1)

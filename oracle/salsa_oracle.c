@@ -188,7 +188,7 @@ void so_params_default(so_params *P, int level) {
   P->ice_gamma = 0.55; P->ice_sigma = 1.97;
   P->lcharge = 0; P->charge_coll_enh = 0.;
   P->kpert = 0.; P->kseed = 0ull;
-  P->lsdistupdate = 0; P->level = level;
+  P->lsdistupdate = 0; P->level = level; P->lsactiv = 0;
   if (level == 4) {
     P->lscgia = P->lscgic = P->lscgii = P->lscgip = 0;
     P->lscndh2oic = 0;
@@ -1574,6 +1574,155 @@ static void autoconv2(const so_layout *L, so_cell *c, double ptstep) {
 }
 
 /* ====================================================================== */
+/* Interstitial cloud activation (cloud_activation with lsactiv%mode 1,   */
+/* src/src_salsa/mo_salsa_cloud.f90:46-106, actInterst :859-1195).  When  */
+/* the cell is supersaturated, each 2a/2b aerosol bin is given piecewise  */
+/* linear profiles of wet volume and of number density over its dry       */
+/* volume, from its own and its neighbours' mean values; the part of the  */
+/* bin whose wet volume exceeds the critical volume of the cell's         */
+/* supersaturation moves to the parallel cloud bin.  intgN / intgV:       */
+/* :1381-1397.  The chargeTime and INdef bookkeeping (:1161-1187) is not  */
+/* carried: the kernels assume chargeTime = chargeTimeMax, and ice        */
+/* nucleation is not on the path.                                         */
+/* ====================================================================== */
+static double intgN(double ikk, double icc, double ilow, double ihigh) {
+  return 0.5 * ikk * fmax(ihigh * ihigh - ilow * ilow, 0.) + icc * fmax(ihigh - ilow, 0.);
+}
+static double intgV(double ikk, double icc, double ilow, double ihigh) {
+  return (1. / 3.) * ikk * fmax(ihigh * ihigh * ihigh - ilow * ilow * ilow, 0.) +
+         0.5 * icc * fmax(ihigh * ihigh - ilow * ilow, 0.);
+}
+
+static void act_interst(const so_layout *L, so_cell *c, double prv, double prs, double temp) {
+  const double THvol = 1.e-28;
+  const int ndry = L->ndry, nwet = L->nwet, iwa = L->iwa;
+  if (prv / prs <= 1.000) return;
+  const double zkelvin = 4. * mwa * surfw0 / (rg * rhowa * temp);
+  const double zdcstar = 2. * zkelvin / (3. * ((prv / prs) - 1.));
+  const double zvcstar = pi6 * (zdcstar * zdcstar * zdcstar);
+  const int A = L->iaero, C = L->icloud;
+  double act_n[SO_MAXBIN], act_v[SO_MAXBIN][SO_MAXSPEC];
+  for (int cb = L->ica.cur; cb <= L->fcb.cur; ++cb) {
+    const int ab = cb <= L->fca.cur ? L->ica.par + (cb - L->ica.cur) : L->icb.par + (cb - L->icb.cur);
+    act_n[cb] = 0.;
+    for (int s = 0; s < SO_MAXSPEC; ++s) act_v[cb][s] = 0.;
+    const int a = A + ab;
+    double vol_sol = 0., vol_insol = 0.;
+    for (int s = 0; s < ndry; ++s) {
+      const int id = L->spec_id[s];
+      if (id == 2 || id == 3) vol_insol += c->volc[a][s];       /* BC, DU (classSpecies.f90:99-100) */
+      else vol_sol += c->volc[a][s];
+    }
+    (void)vol_insol;
+    if (c->numc[a] < L->nlim[a]) continue;
+    if (vol_sol < c->numc[a] * THvol) continue;
+    if (c->volc[a][iwa] < c->numc[a] * THvol) continue;
+    const double vrlo = L->vlolim[a] / (pi6 * cube(L->dmid[a]));   /* mo_salsa_init.f90:186-227 */
+    const double vrhi = L->vhilim[a] / (pi6 * cube(L->dmid[a]));
+    const double Nmid = c->numc[a];
+    double sw = 0., sd = 0.;
+    for (int s = 0; s < nwet; ++s) sw += c->volc[a][s];
+    for (int s = 0; s < ndry; ++s) sd += c->volc[a][s];
+    const double Vwmid = sw / Nmid, Vmid = sd / Nmid;
+    const double Vlo = Vmid * vrlo, Vhi = Vmid * vrhi;
+    double Nim1, Vim1, Vlom1, Vhim1, Vwim1, Nip1, Vip1, Vlop1, Vhip1, Vwip1;
+    if (ab == L->in1a || ab == L->in2b) {                 /* first bin of a regime, :982-988 */
+      Nim1 = L->nlim[a]; Vim1 = Vlo / 2.; Vlom1 = 0.; Vhim1 = Vlo; Vwim1 = Vim1;
+    } else {
+      const int m = a - 1;
+      Nim1 = c->numc[m];
+      if (Nim1 > L->nlim[m]) {
+        double d = 0., w = 0.;
+        for (int s = 0; s < ndry; ++s) d += c->volc[m][s];
+        for (int s = 0; s < nwet; ++s) w += c->volc[m][s];
+        Vim1 = d / Nim1; Vwim1 = w / Nim1;
+      } else {
+        Vim1 = pi6 * cube(L->dmid[m]); Vwim1 = pi6 * cube(L->dmid[m]);
+      }
+      Vlom1 = Vim1 * (L->vlolim[m] / (pi6 * cube(L->dmid[m])));
+      Vhim1 = Vim1 * (L->vhilim[m] / (pi6 * cube(L->dmid[m])));
+    }
+    if (ab == L->fn2a || ab == L->fn2b) {                 /* last bin of a regime, :1001-1007 */
+      Nip1 = L->nlim[a]; Vip1 = Vhi + 0.5 * (Vhi - Vlo); Vlop1 = Vhi; Vhip1 = Vhi + (Vhi - Vlo); Vwip1 = Vip1;
+    } else {
+      const int m = a + 1;
+      Nip1 = c->numc[m];
+      if (Nip1 > L->nlim[m]) {
+        double d = 0., w = 0.;
+        for (int s = 0; s < ndry; ++s) d += c->volc[m][s];
+        for (int s = 0; s < nwet; ++s) w += c->volc[m][s];
+        Vip1 = d / Nip1; Vwip1 = w / Nip1;
+      } else {
+        Vip1 = pi6 * cube(L->dmid[m]); Vwip1 = pi6 * cube(L->dmid[m]);
+      }
+      Vlop1 = Vip1 * (L->vlolim[m] / (pi6 * cube(L->dmid[m])));
+      Vhip1 = Vip1 * (L->vhilim[m] / (pi6 * cube(L->dmid[m])));
+    }
+    Vip1 = fmax(Vhi, Vip1);
+    Vim1 = fmin(Vlo, Vim1);
+    /* wet volume profile, :1027-1037 */
+    double zs1 = (Vwmid - fmax(Vwim1, 0.)) / (Vmid - Vim1);
+    double zs2 = (fmax(Vwip1, 0.) - Vwmid) / (Vip1 - Vmid);
+    const double V01 = Vwmid - zs1 * Vmid, V02 = Vwmid - zs2 * Vmid;
+    const double Vwlo = fmax(V01 + zs1 * Vlo, Vlo), Vwhi = fmax(V02 + zs2 * Vhi, Vhi);
+    /* integration limits, :1050-1097 */
+    double vint1, vint2;
+    int ir[4];
+    if (zvcstar < Vwlo && zvcstar < Vwmid && zvcstar < Vwhi) {
+      vint1 = Vlo; vint2 = Vhi; ir[0] = ir[1] = ir[2] = ir[3] = 1;
+    } else if (zvcstar > Vwlo && zvcstar > Vwmid && zvcstar > Vwhi) {
+      vint1 = 999.; vint2 = 999.; ir[0] = ir[1] = ir[2] = ir[3] = 0;
+    } else {
+      double vcut = (zvcstar - V01) / zs1;
+      vint1 = (vcut < Vlo || vcut > Vmid) ? Vlo : vcut;
+      vcut = (zvcstar - V02) / zs2;
+      vint2 = (vcut < Vmid || vcut > Vhi) ? Vmid : vcut;
+      ir[0] = Vwlo > zvcstar; ir[1] = Vwmid > zvcstar; ir[2] = Vwmid > zvcstar; ir[3] = Vwhi > zvcstar;
+    }
+    /* number density profile and the integrals, :1099-1139 */
+    const double dNim1 = Nim1 / (Vhim1 - Vlom1), dNip1 = Nip1 / (Vhip1 - Vlop1), dNmid = Nmid / (Vhi - Vlo);
+    zs1 = (dNmid - dNim1) / (Vmid - Vim1);
+    zs2 = (dNip1 - dNmid) / (Vip1 - Vmid);
+    const double N01 = dNmid - zs1 * Vmid, N02 = dNmid - zs2 * Vmid;
+    const double Nnorm = intgN(zs1, N01, Vlo, Vmid) + intgN(zs2, N02, Vmid, Vhi);
+    const double Vnorm = intgV(zs1, N01, Vlo, Vmid) + intgV(zs2, N02, Vmid, Vhi);
+    double zactvol = 0., Nact = 0.;
+    if (ir[0]) {
+      Nact = Nact + (Nmid / Nnorm) * intgN(zs1, N01, Vlo, vint1);
+      zactvol = zactvol + (Nmid * Vmid / Vnorm) * intgV(zs1, N01, Vlo, vint1);
+    }
+    if (ir[1]) {
+      Nact = Nact + (Nmid / Nnorm) * intgN(zs1, N01, vint1, Vmid);
+      zactvol = zactvol + (Nmid * Vmid / Vnorm) * intgV(zs1, N01, vint1, Vmid);
+    }
+    if (ir[2]) {
+      Nact = Nact + (Nmid / Nnorm) * intgN(zs2, N02, Vmid, vint2);
+      zactvol = zactvol + (Nmid * Vmid / Vnorm) * intgV(zs2, N02, Vmid, vint2);
+    }
+    if (ir[3]) {
+      Nact = Nact + (Nmid / Nnorm) * intgN(zs2, N02, vint2, Vhi);
+      zactvol = zactvol + (Nmid * Vmid / Vnorm) * intgV(zs2, N02, vint2, Vhi);
+    }
+    act_n[cb] = fmin(Nact, Nmid);
+    for (int s = 0; s < nwet; ++s) act_v[cb][s] = fmin(zactvol * (c->volc[a][s] / (Vmid * Nmid)), c->volc[a][s]);
+    if (c->rates) {                                       /* rateDiag%Activation, :1143-1144 */
+      c->rates[SO_R_ACTIV].numc += act_n[cb];
+      for (int s = 0; s < nwet; ++s) c->rates[SO_R_ACTIV].volc[s] += act_v[cb][s];
+    }
+  }
+  /* apply, :1150-1189 (cloud bin k and aerosol bin ica%par + k) */
+  for (int k = 0; k <= L->fcb.cur - L->ica.cur; ++k) {
+    const int a = A + L->ica.par + k, cl = C + L->ica.cur + k, cb = L->ica.cur + k;
+    c->numc[a] = fmax(0., c->numc[a] - act_n[cb]);
+    c->numc[cl] = c->numc[cl] + act_n[cb];
+    for (int s = 0; s < nwet; ++s) {
+      c->volc[a][s] = fmax(0., c->volc[a][s] - act_v[cb][s]);
+      c->volc[cl][s] = c->volc[cl][s] + act_v[cb][s];
+    }
+  }
+}
+
+/* ====================================================================== */
 /* One cell: salsa() with coagulation and condensation enabled,           */
 /* src/src_salsa/mo_salsa.f90:92-107                                      */
 /* ====================================================================== */
@@ -1597,6 +1746,7 @@ int so_step_cell_ks(const so_layout *L, const so_params *P,
   if (P->lscnd && (P->lscndh2ocl || P->lscndh2oae || P->lscndh2oic))
     status |= gpparth2o(L, P, &c, ptemp, ppres, prs, prsi, prv, ptstep);
   if (P->lsauto_state && P->lsauto_mode == 2) autoconv2(L, &c, ptstep);   /* mo_salsa.f90:110-112 */
+  if (P->lsactiv == 1) act_interst(L, &c, *prv, prs, ptemp);                /* :115-119, mode 1 */
   if (P->lsdistupdate) status |= distr_update(L, &c, P->level);
   for (int b = 0; b < L->ntotal; ++b) {
     numc[b] = c.numc[b];

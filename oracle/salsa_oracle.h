@@ -79,6 +79,8 @@ typedef struct {
                                   (mo_salsa.f90:148-149); default 0 here       */
   int level;                   /* thermodynamic level (4 or 5): distr_update
                                   skips the ice bins below 5 (:323)            */
+  int lsactiv;                 /* cloud activation: 0 off, 1 interstitial
+                                  (lsactiv%mode 1, mo_salsa.f90:115-119)       */
 } so_params;
 
 /* One Rate (classProcessRates.f90:40-45): volc(9) and numc */

@@ -25,5 +25,6 @@ PROGRAM layout_check
   WRITE(*,'(A,6(1X,I0))') 'psw ', p%level, p%lscgii, p%lscndh2oic, p%lsfreeRH, p%precision, p%hxy
   WRITE(*,'(A,4(1X,ES24.16))') 'pdb ', p%rhlim, p%ice_sigma, p%charge_coll_enh, p%cgintvl
   p%lsdistupdate = 7
-  WRITE(*,'(A,2(1X,I0))') 'pmd ', p%lscoag_mode, p%lsdistupdate
+  p%lsactiv = 9
+  WRITE(*,'(A,3(1X,I0))') 'pmd ', p%lscoag_mode, p%lsdistupdate, p%lsactiv
 END PROGRAM layout_check

@@ -45,14 +45,15 @@ def pick(f, n, seed):
     return np.sort(np.concatenate(out))[:n]
 
 
-def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0, kstore=False, distr=False):
+def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0, kstore=False, distr=False, activ=False):
     T, p, rs, rsi, rv, nc, vc = cells
     R = refio.run(L, P, level, T, p, rs, rsi, rv, nc, vc, nsteps=nsteps, lcharge=lcharge, charge_enh=enh,
-                  kstore=kstore, distr=distr)
+                  kstore=kstore, distr=distr, activ=activ)
     ns = L.nspec
     np.savez_compressed(
         os.path.join(HERE, name),
         level=level, nsteps=nsteps, lcharge=lcharge, charge_enh=enh, kstore=int(kstore), distr=int(distr),
+        lsactiv=int(activ),
         lsauto=P.lsauto_state, lsauto_mode=P.lsauto_mode, nbin=np.array(list(L.nbin)), nprc=L.nprc, nice=L.nice,
         species=np.array(list(L.dry_species)[:L.nspec_dry]),
         T=T, p=p, rs=rs, rsi=rsi, rv=rv, numc=nc, volc=vc[:, :, :ns],
@@ -121,6 +122,32 @@ def autoconv2_cases():
     save_step("step_cfg5_auto2.npz", L5, 5, P5, tuple(x[sel] for x in f5))
 
 
+def activation_cases():
+    """Interstitial cloud activation (lsactiv%mode 1, mo_salsa.f90:115-119):
+    config-1 cells over three steps, cloudy cells with precipitation over two
+    with autoconv2 and distr_update as well, and mixed-phase cells over one
+    (written separately: `--only activation`)."""
+    L = O.make_layout()
+    f1 = W.to_salsa_units(L, W.config_fields(L, 1), level=4)
+    nc, vc = pad(L, f1[5], f1[6])
+    cells = tuple(f1[:5]) + (nc, vc)
+    sel = pick(cells, 120, 31)
+    save_step("step_cfg1_activ.npz", L, 4, O.make_params(level=4, lsactiv=1), tuple(x[sel] for x in cells),
+              nsteps=3, activ=True)
+    f4 = W.to_salsa_units(L, W.config_fields(L, 4, grid_override=(20, 8, 8)), level=4)
+    nc, vc = pad(L, f4[5], f4[6])
+    cells = tuple(f4[:5]) + (nc, vc)
+    sel = pick(cells, 120, 34)
+    save_step("step_cfg4_activ_auto2_distr.npz", L, 4,
+              O.make_params(level=4, lsactiv=1, lsauto_mode=2, lsdistupdate=1), tuple(x[sel] for x in cells),
+              nsteps=2, distr=True, activ=True)
+    L5 = O.make_layout(species=(0, 3))
+    f5 = W.to_salsa_units(L5, W.config_fields(L5, 5, grid_override=(20, 8, 8)), level=5)
+    sel = np.sort(np.random.default_rng(35).choice(len(f5[0]), 80, replace=False))
+    save_step("step_cfg5_activ.npz", L5, 5, O.make_params(level=5, lsactiv=1), tuple(x[sel] for x in f5),
+              activ=True)
+
+
 def main():
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "kstore":
         return kstore_case()
@@ -128,6 +155,8 @@ def main():
         return distr_cases()
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "autoconv2":
         return autoconv2_cases()
+    if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "activation":
+        return activation_cases()
     if not refio.available():
         sys.exit("oracle/_ref/ref_harness missing: run `make -C oracle ref` first")
     # warm layout nbin=(3,4): aerosol 11, cloud 8, precip 7 (+ 7 inert ice bins)

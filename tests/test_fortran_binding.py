@@ -48,7 +48,7 @@ def test_bindc_layout_matches_c(tmp_path):
     assert [int(x) for x in rows["psw"]] == [p.level, p.lscgii, p.lscndh2oic, p.lsfreeRH, p.precision, p.hxy]
     assert np.allclose([float(x) for x in rows["pdb"]], [p.rhlim, p.ice_sigma, p.charge_coll_enh, p.cgintvl],
                        rtol=1e-15)
-    assert [int(x) for x in rows["pmd"]] == [p.lscoag_mode, 7]
+    assert [int(x) for x in rows["pmd"]] == [p.lscoag_mode, 7, 9]
 
 
 REF_MODS = os.path.join(ROOT, "oracle", "_ref", "obj")

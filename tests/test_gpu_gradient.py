@@ -80,12 +80,12 @@ def oracle_J(O, L, P, desc, cells, nsteps):
     return objective(desc, vco)
 
 
-def gpu_gradient(capi, desc, prec, cells, nsteps, level=4, distr=0, auto_mode=1):
+def gpu_gradient(capi, desc, prec, cells, nsteps, level=4, distr=0, auto_mode=1, activ=0):
     import torch
     T, p, rs, rsi, rv, numc, volc = cells
     fc = FlatCells(desc, level, T, p, rs, rsi, rv, numc, volc)
     ctx = capi.Context(desc, capi.params(level=level, precision=prec, hz=0, hxy=0, lsdistupdate=distr,
-                                         lsauto_mode=auto_mode), fc.n, 1, 1)
+                                         lsauto_mode=auto_mode, lsactiv=activ), fc.n, 1, 1)
     dev = torch.device("cuda")
     D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     f = {k: D(v) for k, v in fc.f.items()}
@@ -157,7 +157,7 @@ def oracle_state(O, L, P, cells, nsteps):
     return r, nc, vc
 
 
-def oracle_amplification(O, L, level, distr, cells, nsteps, cats, auto_mode=1, seed=5):
+def oracle_amplification(O, L, level, distr, cells, nsteps, cats, auto_mode=1, activ=0, seed=5):
     """Per cell: the largest change of the final state over a relative input
     change eps of E and of every bin's N0 (random signs), divided by eps; the
     larger of eps = 1e-12 and 1e-13.  Changes are relative to the vapour and
@@ -167,14 +167,15 @@ def oracle_amplification(O, L, level, distr, cells, nsteps, cats, auto_mode=1, s
     receiving contents, but not against the category).  A stable trajectory
     gives O(1)-O(10); the reference's unstable condensation trajectories give
     1e3 and more (a perturbation that grows until it saturates)."""
-    P = O.make_params(level=level, lsdistupdate=distr, lsauto_mode=auto_mode)
+    P = O.make_params(level=level, lsdistupdate=distr, lsauto_mode=auto_mode, lsactiv=activ)
     base = oracle_state(O, L, P, cells, nsteps)
     rng = np.random.default_rng(seed)
     amp = np.zeros(len(cells[0]))
     for eps in (1e-12, 1e-13):
         cp = list(cells)
         cp[5] = cells[5] * (1 + eps * rng.choice([-1., 1.], size=cells[5].shape))
-        Pp = O.make_params(level=level, lcharge=1, charge_coll_enh=eps, lsdistupdate=distr, lsauto_mode=auto_mode)
+        Pp = O.make_params(level=level, lcharge=1, charge_coll_enh=eps, lsdistupdate=distr, lsauto_mode=auto_mode,
+                           lsactiv=activ)
         r, nc, vc = oracle_state(O, L, Pp, cp, nsteps)
         d = np.abs(r - base[0]) / np.abs(base[0])
         for c0, c1 in cats:
@@ -190,14 +191,14 @@ class OracleFD:
     """Central differences of the oracle's J for dJ/dE and for dJ/dN0 of every
     bin with particles, at h and 4h, with |forward - backward| at h."""
 
-    def __init__(self, O, level, distr, desc, cells, nsteps, auto_mode=1):
+    def __init__(self, O, level, distr, desc, cells, nsteps, auto_mode=1, activ=0):
         self.L = L = O.make_layout(species=(0, 3) if level == 5 else (0,))
-        P = O.make_params(level=level, lsdistupdate=distr, lsauto_mode=auto_mode)
+        P = O.make_params(level=level, lsdistupdate=distr, lsauto_mode=auto_mode, lsactiv=activ)
         self.Jo = Jo = oracle_J(O, L, P, desc, cells, nsteps)
         numc = cells[5]
 
         def fdE(h):
-            kw = dict(level=level, lcharge=1, lsdistupdate=distr, lsauto_mode=auto_mode)
+            kw = dict(level=level, lcharge=1, lsdistupdate=distr, lsauto_mode=auto_mode, lsactiv=activ)
             Jp = oracle_J(O, L, O.make_params(charge_coll_enh=h, **kw), desc, cells, nsteps)
             Jm = oracle_J(O, L, O.make_params(charge_coll_enh=-h, **kw), desc, cells, nsteps)
             return (Jp - Jm) / (2 * h), np.abs(Jp - 2 * Jo + Jm) / h
@@ -221,7 +222,8 @@ class OracleFD:
         self.fdn = {b: (fdN(b, H, numc[:, b] > 0), fdN(b, 4 * H, numc[:, b] > 0)) for b in self.bins}
         nb = [desc.nbins, desc.ncld, desc.nprc] + ([desc.nice] if level == 5 else [])
         off = np.cumsum([0] + nb)
-        self.amp = oracle_amplification(O, L, level, distr, cells, nsteps, list(zip(off[:-1], off[1:])), auto_mode)
+        self.amp = oracle_amplification(O, L, level, distr, cells, nsteps, list(zip(off[:-1], off[1:])), auto_mode,
+                                        activ)
 
 
 def check_against_oracle(ref, cells, prec, J, dJdE, jc, je, g, flags, unstable_bit):
@@ -261,27 +263,29 @@ def check_against_oracle(ref, cells, prec, J, dJdE, jc, je, g, flags, unstable_b
     assert nk <= MAX_KINK_FRAC * npair
 
 
-@pytest.mark.parametrize("level,distr,auto_mode", [(4, 0, 1), (4, 1, 1), (5, 0, 1), (5, 1, 1), (4, 0, 2)])
-def test_gradient_against_oracle_fd(capi, oracle_mod, level, distr, auto_mode):
+@pytest.mark.parametrize("level,distr,auto_mode,activ",
+                         [(4, 0, 1, 0), (4, 1, 1, 0), (5, 0, 1, 0), (5, 1, 1, 0), (4, 0, 2, 0), (4, 0, 1, 1)])
+def test_gradient_against_oracle_fd(capi, oracle_mod, level, distr, auto_mode, activ):
     """dJ/dN0 for every bin of every cell and dJ/dE against central
     differences of the FP64 oracle over a 10-step trajectory, both precision
     paths: level 4 on workload-3 cells (SO4), level 5 on workload-5 cells
     (SO4 + dust, ice bins with rime, T 240-275 K), each without and with the
     size-distribution update after every step; and level 4 with lsauto mode 2
-    (autoconv2 instead of precipitation formation inside coagulation)."""
+    (autoconv2 instead of precipitation formation inside coagulation), and
+    with interstitial cloud activation."""
     from salsa_amd import workload as W
     species = ("SO4", "DU") if level == 5 else ("SO4",)
     desc = capi.bin_desc(species=species)
     cells = small_cells(desc, W, cfg=5 if level == 5 else 3)
     nsteps = 10
-    print("case gradient vs oracle FD, level %d, distr_update %d, lsauto mode %d, %d cells, %d steps"
-          % (level, distr, auto_mode, len(cells[0]), nsteps), flush=True)
-    ref = OracleFD(oracle_mod, level, distr, desc, cells, nsteps, auto_mode)
+    print("case gradient vs oracle FD, level %d, distr_update %d, lsauto mode %d, lsactiv %d, %d cells, %d steps"
+          % (level, distr, auto_mode, activ, len(cells[0]), nsteps), flush=True)
+    ref = OracleFD(oracle_mod, level, distr, desc, cells, nsteps, auto_mode, activ)
     assert ref.Jo.max() > 0, "no precipitation formed: the objective is identically zero"
     assert len(ref.bins) >= 15
     for prec in (0, 1):
         J, dJdE, jc, je, g, flags = gpu_gradient(capi, desc, prec, cells, nsteps, level=level, distr=distr,
-                                                 auto_mode=auto_mode)
+                                                 auto_mode=auto_mode, activ=activ)
         assert not np.any(flags & capi.CELL_NONFINITE)
         check_against_oracle(ref, cells, prec, J, dJdE, jc, je, g, flags, capi.CELL_GRAD_UNSTABLE)
 

@@ -4,8 +4,8 @@ cells, through the C ABI.  Bars: FP64 path 1e-10 relative; FP32-mixed path
 import numpy as np
 import pytest
 
-from helpers import FLOOR_N, FLOOR_RV, FLOOR_V, KERNEL_EPS_FP32, FlatCells, check_condition_aware, full_volc, \
-    golden, oracle_layout_for, oracle_params_for, oracle_sensitivity, rel
+from helpers import FLOOR_N, FLOOR_RV, FLOOR_V, KERNEL_EPS_FP32, FlatCells, check_condition_aware, content_scale, \
+    full_volc, golden, oracle_layout_for, oracle_params_for, oracle_sensitivity, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +61,9 @@ GOLDEN_CASES = [
     ("step_cfg1_auto2.npz", {"lsauto_mode": 2}),
     ("step_cfg4_auto2_distr.npz", {"lsauto_mode": 2, "lsdistupdate": 1}),
     ("step_cfg5_auto2.npz", {"lsauto_mode": 2}),
+    ("step_cfg1_activ.npz", {"lsactiv": 1}),
+    ("step_cfg4_activ_auto2_distr.npz", {"lsactiv": 1, "lsauto_mode": 2, "lsdistupdate": 1}),
+    ("step_cfg5_activ.npz", {"lsactiv": 1}),
 ]
 
 
@@ -98,9 +101,12 @@ def test_gpu_matches_reference_golden(capi, oracle_mod, name, kw, prec):
     srv, snc, svc = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco,
                                        kernel_eps=KEPS[prec], nsteps=nsteps)
     sens = (srv, snc[:, :nt], svc[:, :nt])
-    _, nf1 = check_condition_aware(out, (orv, onc[:, :nt], ovc[:, :nt]), sens, TOL[prec], ns, "oracle")
+    nbs = [desc.nbins, desc.ncld, desc.nprc] + ([desc.nice] if level == 5 else [])
+    cats = list(zip(np.cumsum([0] + nbs)[:-1], np.cumsum(nbs)))
+    scale = content_scale(inp, (orv, onc[:, :nt], ovc[:, :nt]), cats)
+    _, nf1 = check_condition_aware(out, (orv, onc[:, :nt], ovc[:, :nt]), sens, TOL[prec], ns, "oracle", inputs=scale)
     _, nf2 = check_condition_aware(out, (g["out_rv"], g["out_numc"][:, :nt], g["out_volc"][:, :nt]), sens,
-                                   TOL[prec], ns, "reference")
+                                   TOL[prec], ns, "reference", inputs=scale)
     nel = out[2][:, :, :ns].size
     # The allowance may be used on 0.5 % of the elements, or on as many as
     # the oracle's own answer moves by more than tol/10 under 1e-15 relative
@@ -540,10 +546,11 @@ def test_gpu_full_size_configs_4_and_5(capi, oracle_mod, cfg):
 @pytest.mark.parametrize("name,kw", GOLDEN_CASES)
 def test_gpu_rates_match_oracle_and_reference(capi, oracle_mod, name, kw, prec):
     """Process-rate diagnostics (autoconversion, accretion with the 50/80 um
-    variants, aerosol collection, condensation) against the oracle on the
-    same packed inputs and against the compiled reference's golden rates;
-    each field is compared on its own scale (the condensation rates are
-    differences of round-tripped volumes)."""
+    variants, aerosol collection, activation, condensation) against the
+    oracle on the same packed inputs and against the compiled reference's
+    golden rates; each field is compared on its own scale (the condensation
+    rates are differences of round-tripped volumes), per cell within
+    max(tol x scale, 10 x the oracle's own spread under 1e-13 input noise)."""
     g = golden(name)
     level = int(g["level"])
     species = tuple(int(s) for s in g["species"])
@@ -572,14 +579,26 @@ def test_gpu_rates_match_oracle_and_reference(capi, oracle_mod, name, kw, prec):
     dn = FlatCells(desc, level, g["T"], g["p"], g["rs"], g["rsi"], g["rv"], numc, volc).f["dn"]
     want = oracle_rate_fields(oracle_mod, L, rr, dn, level)
     ref = oracle_rate_fields(oracle_mod, L, g["out_rates"], dn, level)
+    # the oracle's own spread of each rate under 1e-13 relative input noise
+    # (two seeds): near-equilibrium bins make some condensation rates
+    # ill-conditioned in the reference algorithm itself
+    rng = np.random.default_rng(7)
+    spread = np.zeros_like(want)
+    for _ in range(2):
+        pert = lambda x: x * (1 + 1e-13 * rng.standard_normal(np.shape(x)))
+        *_, rp = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], pert(irv), pert(nco), pert(vco),
+                                       rates=True, nsteps=nsteps)
+        spread = np.maximum(spread, np.abs(oracle_rate_fields(oracle_mod, L, rp, dn, level) - want))
     tol = {0: 1e-9, 1: 1e-4}[prec]
     for f in range(nrf):
         scale = max(np.abs(want[f]).max(), 1e-300)
+        allow = np.maximum(tol * scale, 10. * spread[f])
         e1 = np.abs(rates[f] - want[f]).max() / scale
         e2 = np.abs(rates[f] - ref[f]).max() / max(np.abs(ref[f]).max(), 1e-300)
-        print("rate %-15s scale %.3e  vs oracle %.2e  vs reference %.2e" % (capi.RATE_NAMES[f], scale, e1, e2))
-        assert e1 <= tol, (capi.RATE_NAMES[f], e1)
-        assert e2 <= max(tol, 1e-7), (capi.RATE_NAMES[f], e2)
+        print("rate %-15s scale %.3e  vs oracle %.2e  vs reference %.2e  (oracle spread %.2e)"
+              % (capi.RATE_NAMES[f], scale, e1, e2, spread[f].max() / scale))
+        assert np.all(np.abs(rates[f] - want[f]) <= allow), (capi.RATE_NAMES[f], e1)
+        assert np.all(np.abs(rates[f] - ref[f]) <= np.maximum(allow, 1e-7 * scale)), (capi.RATE_NAMES[f], e2)
 
 
 @pytest.mark.parametrize("prec", [0, 1])

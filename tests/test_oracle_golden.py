@@ -92,6 +92,7 @@ def test_oracle_kernel_store_matches_reference(oracle_mod):
 
 DISTR_CASES = ["step_cfg1_distr.npz", "step_cfg4_distr.npz", "step_cfg5_distr.npz"]
 AUTO2_CASES = ["step_cfg1_auto2.npz", "step_cfg4_auto2_distr.npz", "step_cfg5_auto2.npz"]
+ACTIV_CASES = ["step_cfg1_activ.npz", "step_cfg4_activ_auto2_distr.npz", "step_cfg5_activ.npz"]
 
 
 @pytest.mark.parametrize("name", DISTR_CASES)
@@ -139,6 +140,34 @@ def test_oracle_autoconv2_matches_reference(oracle_mod, name):
     _, nc0, _, _ = oracle_mod.step_cells(L, P0, *args, nsteps=nsteps)
     p1 = L.iprecp
     assert np.count_nonzero(np.abs(nc[:, p1] - nc0[:, p1]) > 1e-6 * np.abs(nc[:, p1])) > 10
+    sens = oracle_sensitivity(oracle_mod, L, P, *args, nsteps=nsteps)
+    ns = g["volc"].shape[2]
+    out, nflag = check_condition_aware((rv, nc, vc[:, :, :ns]), (g["out_rv"], g["out_numc"], g["out_volc"]),
+                                       (sens[0], sens[1], sens[2][:, :, :ns]), 1e-12, ns, name)
+    print(name, "max rel", out, "elements needing the conditioning allowance:", nflag)
+    assert nflag <= 0.002 * nc.size
+
+
+@pytest.mark.parametrize("name", ACTIV_CASES)
+def test_oracle_activation_matches_reference(oracle_mod, name):
+    """Steps with interstitial cloud activation (lsactiv%mode 1,
+    mo_salsa.f90:115-119, actInterst mo_salsa_cloud.f90:859-1195) against the
+    compiled reference, state and the activation rate (rateDiag%Activation);
+    each element within max(1e-12, 10 x the oracle's own spread under 1e-13
+    input noise)."""
+    g = golden(name)
+    assert int(g["lsactiv"]) == 1
+    L = oracle_layout_for(g)
+    P = oracle_params_for(g)
+    assert P.lsactiv == 1
+    nsteps = int(g["nsteps"])
+    args = (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"], full_volc(g["volc"]))
+    rv, nc, vc, st, rr = oracle_mod.step_cells(L, P, *args, nsteps=nsteps, rates=True)
+    assert not np.any(st & (2 | 8))
+    # activation acted: the reference's activation rate of the last step, and ours
+    ra, ra_ref = rr[:, 9, 9], g["out_rates"][:, 9, 9]
+    assert np.count_nonzero(ra_ref) >= 5
+    assert np.max(np.abs(ra - ra_ref)) <= 1e-12 * np.max(np.abs(ra_ref))
     sens = oracle_sensitivity(oracle_mod, L, P, *args, nsteps=nsteps)
     ns = g["volc"].shape[2]
     out, nflag = check_condition_aware((rv, nc, vc[:, :, :ns]), (g["out_rv"], g["out_numc"], g["out_volc"]),

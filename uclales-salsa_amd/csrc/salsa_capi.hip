@@ -134,6 +134,10 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
     set_error("lscoag_mode must be 1 or 2");
     return SALSA_ERR_ARG;
   }
+  if (params->lsactiv != 0 && params->lsactiv != 1) {
+    set_error("lsactiv must be 0 or 1 (interstitial activation; cloud-base activation is not provided)");
+    return SALSA_ERR_ARG;
+  }
   if (params->lscoag_mode == 2 && !(params->cgintvl > 0.)) {
     set_error("cgintvl must be positive with lscoag_mode 2");
     return SALSA_ERR_ARG;

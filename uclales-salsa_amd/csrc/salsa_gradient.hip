@@ -31,6 +31,7 @@
 
 #include <stdint.h>
 
+#include "salsa_activation.h"
 #include "salsa_autoconv.h"
 #include "salsa_distr.h"
 #include "salsa_dual.h"
@@ -734,9 +735,10 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
         }
       }
 
-      // ---- autoconversion (autoconv2) and size-distribution update
-      //      (distr_update), mo_salsa.f90:110-112, 148-149 ----
+      // ---- autoconversion (autoconv2), cloud activation and size-distribution
+      //      update (distr_update), mo_salsa.f90:110-119, 148-149 ----
       if (D.autoconv2) autoconv2_cell<DD>(D, N, &V[0][0], NS, dt);           // mo_salsa.f90:110-112
+      if (D.lsactiv) act_interst_cell<DD>(D, N, &V[0][0], NS, rv, rs, T);    // :115-119
       if (D.lsdistupdate) flags |= distr_update_cell<DD>(D, N, &V[0][0], NS);
 
       // ---- the new state through LES units, as the next run_SALSA call sees it ----

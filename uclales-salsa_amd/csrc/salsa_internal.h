@@ -115,6 +115,7 @@ struct DevLayout {
   int32_t level, lscoag, lscnd, lscndh2oae, lscndh2ocl, lscndh2oic;
   int32_t lsdistupdate;
   int32_t autoconv2;                  // lsauto%state .AND. lsauto%mode == 2 (mo_salsa.f90:110-112)
+  int32_t lsactiv;                    // 1: interstitial cloud activation (mo_salsa.f90:115-119)
   int32_t any_ice_coag;               // any of lscgii/lscgia/lscgic/lscgip
   double eddy_dis, Eiagg_max, Eiagg_min;
   double ice_alpha, ice_beta, ice_gamma, ice_sigma;

@@ -24,6 +24,7 @@
 
 #include <stdint.h>
 
+#include "salsa_activation.h"
 #include "salsa_autoconv.h"
 #include "salsa_distr.h"
 #include "salsa_internal.h"
@@ -362,6 +363,7 @@ enum RateAcc {
   RA_ACCOLL, RA_APCOLL,                  // SUM(ACcoll/APcoll%volc(1:ndry)*rholiq)
   RA_CONDA, RA_CONDC, RA_CONDP,          // Cond_a/c/p%volc(iwa)
   RA_CONDI,                              // Cond_i%volc(iwa)*rhoic + %volc(irim)*rhori
+  RA_ACTIV,                              // Activation%numc
   RA_N
 };
 
@@ -1162,8 +1164,34 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
       for (int q = 0; q < NQ; ++q)
         if (q < nq) cs[cb_[q]] = zcw[q];
       __syncthreads();
+      // Cond_i in the reference's order (:1086-1110): vrate is zeroed once
+      // before the ice bins, so a depositing bin also adds the rime entry that
+      // the last sublimating bin before it left there (before the write-back
+      // below changes the volumes)
+      if (R && D.level == 5 && sub == 0) {
+        double vw = 0., vri = 0., sw = 0., sr = 0.;
+        for (int cc = 0; cc < D.nb[3]; ++cc) {
+          const int b = D.off[3] + cc;
+          const double org_i = VV(b, iwa) * kRhoic / kMwa, org_r = VV(b, irim) * kRhori / kMwa;
+          const double orig = org_i + org_r;
+          const double dvt = cs[b] - orig;
+          if (dvt > 0.) {
+            vw = dvt * kMwa / kRhoic;
+          } else if (dvt < 0.) {
+            const double fr = (cs[b] / orig) - 1.;
+            vw = (fr * org_i) * kMwa / kRhoic;
+            vri = (fr * org_r) * kMwa / kRhori;
+          } else {
+            continue;
+          }
+          sw += vw;
+          sr += vri;
+        }
+        rr[RA_CONDI] += sw * kRhoic + sr * kRhori;
+      }
+      __syncthreads();
       // write-back (:1058-1117)
-      double dcat[4] = {0., 0., 0., 0.};     // Cond_a/c/p%volc(iwa), Cond_i composite
+      double dcat[3] = {0., 0., 0.};         // Cond_a/c/p%volc(iwa)
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int b = sub + q * L;
@@ -1185,24 +1213,19 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
             const double dvt = zw - orig;
             if (dvt > 0.) {
               VV(b, iwa) = VV(b, iwa) + dvt * kMwa / kRhoic;
-              dcat[3] += (dvt * kMwa / kRhoic) * kRhoic;
             } else if (dvt < 0.) {
               const double fr = (zw / orig) - 1.;
               VV(b, iwa) = VV(b, iwa) + (fr * org_i) * kMwa / kRhoic;
               VV(b, irim) = VV(b, irim) + (fr * org_r) * kMwa / kRhori;
-              dcat[3] += ((fr * org_i) * kMwa / kRhoic) * kRhoic + ((fr * org_r) * kMwa / kRhori) * kRhori;
             }
           }
         }
       }
       if (R) {
-        const int nc4 = D.level == 5 ? 4 : 3;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (k < nc4) dcat[k] = cell_sum<L>(dcat[k]);
+        for (int k = 0; k < 3; ++k) dcat[k] = cell_sum<L>(dcat[k]);
         if (sub == 0) {
           rr[RA_CONDA] += dcat[0]; rr[RA_CONDC] += dcat[1]; rr[RA_CONDP] += dcat[2];
-          if (nc4 == 4) rr[RA_CONDI] += dcat[3];
         }
       }
     }
@@ -1213,6 +1236,16 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   //      mo_salsa.f90:110-112): serial over the cloud bins on one lane ----
   if (D.autoconv2) {
     if (sub == 0 && live) autoconv2_cell<double>(D, sN, sV, nsV, dt);
+    __syncthreads();
+  }
+
+  // ---- 5a2. interstitial cloud activation (cloud_activation, lsactiv%mode 1,
+  //      mo_salsa.f90:115-119): serial over the 2a/2b bins on one lane ----
+  if (D.lsactiv) {
+    if (sub == 0 && live) {
+      const double na = act_interst_cell<double>(D, sN, sV, nsV, rv, rs, T);
+      if (R) rr[RA_ACTIV] += na;
+    }
     __syncthreads();
   }
 
@@ -1272,7 +1305,8 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
         case SALSA_RATE_CONDC: v = rr[RA_CONDC] / dn; break;
         case SALSA_RATE_CONDP: v = rr[RA_CONDP] / dn; break;
         case SALSA_RATE_CONDI: v = rr[RA_CONDI] / dn; break;
-        default: v = 0.;      // activation and ice nucleation: not part of this step
+        case SALSA_RATE_ACTIV: v = rr[RA_ACTIV] / dn; break;
+        default: v = 0.;      // ice nucleation: not part of this step
       }
       a.rates[(int64_t)f * a.n3 + les_index(a, icell)] = v;
     }

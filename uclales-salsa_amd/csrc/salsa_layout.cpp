@@ -160,6 +160,7 @@ extern "C" int salsa_params_default(salsa_params *p, int32_t level) {
   p->hz = 1; p->hxy = 2;
   p->lscoag_mode = 1; p->cgintvl = 10.;     // mo_submctl.f90:23-24, :90
   p->lsdistupdate = 0;                       // reference default on (mo_submctl.f90:118): opt in
+  p->lsactiv = 0;
   if (level == 4) {   // define_salsa, mo_salsa_init.f90:615-635
     p->lscgia = p->lscgic = p->lscgii = p->lscgip = 0;
     p->lscndh2oic = 0;
@@ -222,6 +223,7 @@ int build_dev_layout(const salsa_bin_desc &d, const salsa_params &p, DevLayout &
   L.lscoag = p.lscoag; L.lscnd = p.lscnd;
   L.lsdistupdate = p.lsdistupdate;
   L.autoconv2 = p.lsauto && p.lsauto_mode == 2;
+  L.lsactiv = p.lsactiv;
   L.lscndh2oae = p.lscndh2oae; L.lscndh2ocl = p.lscndh2ocl;
   L.lscndh2oic = level == 5 ? p.lscndh2oic : 0;
   L.eddy_dis = p.eddy_dis; L.Eiagg_max = p.Eiagg_max; L.Eiagg_min = p.Eiagg_min;

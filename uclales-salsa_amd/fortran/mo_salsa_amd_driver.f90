@@ -10,10 +10,11 @@
 !> process-rate Diag fields of the interior cells.
 !>
 !> Only the processes of this step are provided: coagulation, condensation,
-!> autoconv2 (lsauto mode 2) and distr_update.  A configuration that also
-!> switches on secondary ice, activation, ice nucleation, melting, gas-phase
-!> condensation or nucleation stops with a message naming the process
-!> (mo_salsa.f90:92-153 calls them in the same step).
+!> autoconv2 (lsauto mode 2), interstitial activation (lsactiv mode 1) and
+!> distr_update.  A configuration that also switches on secondary ice,
+!> cloud-base activation, ice nucleation, melting, gas-phase condensation or
+!> nucleation stops with a message naming the process (mo_salsa.f90:92-153
+!> calls them in the same step).
 !>
 !> The precision mode is taken from the environment variable
 !> SALSA_AMD_PRECISION ("fp64", the default, or "mixed").
@@ -182,6 +183,7 @@ CONTAINS
     p%lscoag_mode = INT(lscoag%mode, C_INT32_T)
     p%cgintvl = REAL(cgintvl, C_DOUBLE)
     p%lsdistupdate = MERGE(1_C_INT32_T, 0_C_INT32_T, lsdistupdate)   ! distr_update after condensation
+    p%lsactiv = MERGE(1_C_INT32_T, 0_C_INT32_T, lsactiv%state .AND. lsactiv%mode == 1)   ! interstitial activation
     rc = salsa_ctx_create(desc, p, INT(nzp, C_INT32_T), INT(nxp, C_INT32_T), INT(nyp, C_INT32_T), ctx)
     IF (rc /= SALSA_OK) CALL fail('context')
   END SUBROUTINE create_context
@@ -189,7 +191,7 @@ CONTAINS
   ! Processes that salsa() would run in the same step but this library does not
   SUBROUTINE check_switches()
     IF (lssecice%state) CALL outside('secondary ice production (lssecice)')
-    IF (lsactiv%state) CALL outside('cloud activation (lsactiv)')
+    IF (lsactiv%state .AND. lsactiv%mode /= 1) CALL outside('cloud-base activation (lsactiv mode 2)')
     IF (lsicenucl%state .OR. lsicenucl%mode == 2) CALL outside('ice nucleation (lsicenucl)')
     IF (lsicemelt%state) CALL outside('ice melting (lsicemelt)')
     IF (lscndgas) CALL outside('gas-phase condensation (lscndgas)')

@@ -62,6 +62,7 @@ MODULE mo_salsa_amd_cabi
      INTEGER(C_INT32_T) :: lscoag_mode
      REAL(C_DOUBLE)     :: cgintvl
      INTEGER(C_INT32_T) :: lsdistupdate
+     INTEGER(C_INT32_T) :: lsactiv
   END TYPE salsa_params
 
   INTERFACE

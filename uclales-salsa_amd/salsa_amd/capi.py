@@ -98,7 +98,7 @@ class Params(C.Structure):
         ("ice_sigma", C.c_double),
         ("lcharge", C.c_int32), ("charge_coll_enh", C.c_double),
         ("precision", C.c_int32), ("hz", C.c_int32), ("hxy", C.c_int32),
-        ("lscoag_mode", C.c_int32), ("cgintvl", C.c_double), ("lsdistupdate", C.c_int32),
+        ("lscoag_mode", C.c_int32), ("cgintvl", C.c_double), ("lsdistupdate", C.c_int32), ("lsactiv", C.c_int32),
     ]
 
 
