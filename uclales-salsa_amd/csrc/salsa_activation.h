@@ -14,7 +14,9 @@
 // from the state before activation.  The chargeTime and INdef
 // bookkeeping of the reference (:1161-1187) is not carried: the collision
 // kernels assume chargeTime = chargeTimeMax, and ice nucleation is not on the
-// path.  Returns the activated number (rateDiag%Activation%numc).
+// path.  tmp holds 2 * kMaxSpec values of R (work space: LDS in the step
+// kernel, which keeps the kernel free of private arrays).  Returns the
+// activated number (rateDiag%Activation%numc).
 #pragma once
 
 #include "salsa_dual.h"
@@ -47,7 +49,7 @@ template <class R> __device__ __forceinline__ R act_intg_v(R k, R c, R lo, R hi)
 }
 
 template <class R>
-__device__ double act_interst_cell(const DevLayout &D, R *N, R *V, int vs, R rv, double rs, double T) {
+__device__ double act_interst_cell(const DevLayout &D, R *N, R *V, int vs, R rv, double rs, double T, R *tmp) {
   using O = ActOps<R>;
   const double THvol = 1.e-28;
   const int ndry = D.ndry, nwet = D.nwet, iwa = D.iwa;
@@ -61,7 +63,8 @@ __device__ double act_interst_cell(const DevLayout &D, R *N, R *V, int vs, R rv,
   // of it is applied (:1142-1189).  Bin k reads aerosol bins k-1, k and k+1,
   // so bin k's transfer is applied after bin k+1 has been computed.
   int pk = -1;
-  R pn = O::c(0.), pv[kMaxSpec];
+  R pn = O::c(0.);
+  R *pv = tmp, *av = tmp + kMaxSpec;
   auto apply = [&]() {
     if (pk < 0) return;
     const int a = A + D.in2a + pk, cl = C + pk;
@@ -170,7 +173,6 @@ __device__ double act_interst_cell(const DevLayout &D, R *N, R *V, int vs, R rv,
     }
     // this bin's transfer, :1142-1144; the previous bin's is applied now
     const R an = O::min_(Nact, Nmid);
-    R av[kMaxSpec];
     for (int s = 0; s < nwet; ++s) av[s] = O::min_(zactvol * (V[a * vs + s] / (Vmid * Nmid)), V[a * vs + s]);
     ntot += O::val(an);
     apply();

@@ -21,14 +21,17 @@ template <class R> struct AcMath;
 template <> struct AcMath<double> {
   static __device__ __forceinline__ double log_(double x) { return ::log(x); }
   static __device__ __forceinline__ double erf_(double x) { return ::erf(x); }
-  static __device__ __forceinline__ double cbrt3_(double x) { return ::pow(x, 1. / 3.); }
+  // x**(1./3.) of the reference as a cube root: the two differ by a few ulp
+  // (1./3. is not exact), and the device pow routine would give the whole
+  // step kernel a stack frame
+  static __device__ __forceinline__ double cbrt3_(double x) { return ::cbrt(x); }
   static __device__ __forceinline__ double val(double x) { return x; }
 };
 template <> struct AcMath<Dual<double>> {
   using D = Dual<double>;
   static __device__ __forceinline__ D log_(D x) { return dlog(x); }
   static __device__ __forceinline__ D erf_(D x) { return derf(x); }
-  static __device__ __forceinline__ D cbrt3_(D x) { return dpow(x, 1. / 3.); }
+  static __device__ __forceinline__ D cbrt3_(D x) { return dcbrt(x); }
   static __device__ __forceinline__ double val(D x) { return x.v; }
 };
 

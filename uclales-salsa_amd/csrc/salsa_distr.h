@@ -9,6 +9,7 @@
 // overwritten per bin, not accumulated (:119-122) -- at most 100 sweeps.
 // Bins left by CYCLE do not touch within_bins.  Returns SALSA_CELL_DISTR
 // where the reference STOPs (no convergence, or a negative moved fraction).
+// tmp holds kMaxSpec values of R (work space: LDS in the step kernel).
 #pragma once
 
 #include "salsa_dual.h"
@@ -22,7 +23,7 @@ template <> struct DistrConst<Dual<double>> {
 };
 
 template <class R>
-__device__ uint32_t distr_update_cell(const DevLayout &D, R *N, R *V, int vs) {
+__device__ uint32_t distr_update_cell(const DevLayout &D, R *N, R *V, int vs, R *tmp) {
   const int ndry = D.ndry, nwet = D.nwet, nnsp = D.nspec;
   uint32_t flags = 0;
   auto vsum = [&](int b, int ns) {
@@ -66,7 +67,7 @@ __device__ uint32_t distr_update_cell(const DevLayout &D, R *N, R *V, int vs) {
           const R zvfrac = rmin(znfrac * zVexc / zvpart, 0.99);
           if (zvfrac < 0.) flags |= SALSA_CELL_DISTR;
           const int m = b + 1;
-          R add[kMaxSpec];
+          R *add = tmp;
           for (int s = 0; s < nwet; ++s) add[s] = znfrac * N[b] * zVexc * V[b * vs + s] / sd;
           for (int s = 0; s < nwet; ++s) V[m * vs + s] = V[m * vs + s] + add[s];
           for (int s = 0; s < nwet; ++s) V[b * vs + s] = V[b * vs + s] - add[s];

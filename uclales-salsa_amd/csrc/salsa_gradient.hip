@@ -285,6 +285,7 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
 
     // state of the cell with the tangent of this lane's direction
     DD N[MB], V[MB][NS], DW[MB];
+    DD tmpd[2 * kMaxSpec];              // work space of the activation and the size-distribution update
     for (int b = 0; b < nt; ++b) {
       const int c = D.cat_of[b];
       const int nsw = c < 3 ? D.nwet : D.nspec;
@@ -738,8 +739,8 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
       // ---- autoconversion (autoconv2), cloud activation and size-distribution
       //      update (distr_update), mo_salsa.f90:110-119, 148-149 ----
       if (D.autoconv2) autoconv2_cell<DD>(D, N, &V[0][0], NS, dt);           // mo_salsa.f90:110-112
-      if (D.lsactiv) act_interst_cell<DD>(D, N, &V[0][0], NS, rv, rs, T);    // :115-119
-      if (D.lsdistupdate) flags |= distr_update_cell<DD>(D, N, &V[0][0], NS);
+      if (D.lsactiv) act_interst_cell<DD>(D, N, &V[0][0], NS, rv, rs, T, tmpd);   // :115-119
+      if (D.lsdistupdate) flags |= distr_update_cell<DD>(D, N, &V[0][0], NS, tmpd);
 
       // ---- the new state through LES units, as the next run_SALSA call sees it ----
       // The cell's linearisation is unstable when the state of an active bin

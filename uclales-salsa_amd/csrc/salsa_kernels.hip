@@ -1243,7 +1243,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   //      mo_salsa.f90:115-119): serial over the 2a/2b bins on one lane ----
   if (D.lsactiv) {
     if (sub == 0 && live) {
-      const double na = act_interst_cell<double>(D, sN, sV, nsV, rv, rs, T);
+      const double na = act_interst_cell<double>(D, sN, sV, nsV, rv, rs, T, reinterpret_cast<double *>(cb + G.opr));
       if (R) rr[RA_ACTIV] += na;
     }
     __syncthreads();
@@ -1252,7 +1252,7 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   // ---- 5b. size-distribution update (distr_update, mo_salsa.f90:148-149):
   //      serial over the bins, on one lane of the cell ----
   if (D.lsdistupdate) {
-    if (sub == 0 && live) flags |= distr_update_cell<double>(D, sN, sV, nsV);
+    if (sub == 0 && live) flags |= distr_update_cell<double>(D, sN, sV, nsV, reinterpret_cast<double *>(cb + G.opr));
     __syncthreads();
   }
 
