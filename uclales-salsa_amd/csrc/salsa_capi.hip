@@ -23,6 +23,7 @@ int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, bool rate
 int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, double *rec, uint8_t *cls,
                   int32_t *tcount, int32_t *toff, int32_t *lists, int32_t *counts, hipStream_t s);
 int64_t gather_tiles(int64_t ncells);
+int64_t class_work_ints(int64_t ncells);
 int launch_scatter(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, hipStream_t s);
 int cap_bins(int cap);
 int cap_spec(int cap);
@@ -167,7 +168,7 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
       hipMalloc(&c->flags, nc) != hipSuccess || hipMalloc(&c->fault, sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->lists, 3 * nc * sizeof(int32_t)) != hipSuccess || hipMalloc(&c->cls, nc) != hipSuccess ||
       hipMalloc(&c->tcount, 3 * (size_t)gather_tiles(c->ncells) * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&c->toff, 3 * (size_t)gather_tiles(c->ncells) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->toff, (size_t)class_work_ints(c->ncells) * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->cls_count, 3 * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->rec_in, nc * c->hostL.nslot * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->rec_out, nc * c->hostL.nout * sizeof(double)) != hipSuccess ||

@@ -1331,6 +1331,15 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
 // loads before storing them, so that the block keeps many loads in flight.
 // ---------------------------------------------------------------------------
 constexpr int kTileCells = 32;
+
+// Tile of this block: blocks are dealt to the 8 XCDs round-robin, so give
+// each XCD a contiguous range of tiles; neighbouring tiles share the cache
+// lines at their ends, which then meet in one L2.
+__device__ __forceinline__ int64_t xcd_tile() {
+  const uint32_t nblk = gridDim.x, bid = blockIdx.x;
+  const uint32_t xcd = bid & 7u, rnk = bid >> 3, per = nblk >> 3, rem = nblk & 7u;
+  return (int64_t)(xcd * per + (xcd < rem ? xcd : rem) + rnk);
+}
 constexpr int kRowBatch = 8;
 constexpr int kRowsPerPass = 256 / kTileCells;
 
@@ -1352,7 +1361,8 @@ __global__ void __launch_bounds__(256) k_gather(const DevLayout *__restrict__ Dp
   double *tile = reinterpret_cast<double *>(smem);                               // [kTileCells][ld]
   const double **src = reinterpret_cast<const double **>(tile + kTileCells * ld); // [ns] row bases
   int64_t *gi = reinterpret_cast<int64_t *>(src + ns);                            // [kTileCells]
-  const int64_t c0 = (int64_t)blockIdx.x * kTileCells;
+  const int64_t tile_id = xcd_tile();
+  const int64_t c0 = tile_id * kTileCells;
   const int nc = (int)(a.ncells - c0 < kTileCells ? a.ncells - c0 : kTileCells);
   const int tid = threadIdx.x;
   if (tid < kTileCells) gi[tid] = tid < nc ? les_index(a, c0 + tid) : 0;
@@ -1408,7 +1418,7 @@ __global__ void __launch_bounds__(256) k_gather(const DevLayout *__restrict__ Dp
     }
     const uint64_t m0 = __ballot(cl == 0), m1 = __ballot(cl == 1), m2 = __ballot(cl == 2);
     if (tid == 0) {
-      int32_t *tc = tcount + 3 * (int64_t)blockIdx.x;
+      int32_t *tc = tcount + 3 * tile_id;
       tc[0] = __popcll(m0);
       tc[1] = __popcll(m1);
       tc[2] = __popcll(m2);
@@ -1416,11 +1426,13 @@ __global__ void __launch_bounds__(256) k_gather(const DevLayout *__restrict__ Dp
   }
 }
 
-// Exclusive prefix over the tiles of each class's count (one block of 1024
-// threads).  Wavefront w takes a contiguous run of tiles in chunks of 64, one
-// tile per lane, so that every load is coalesced: pass 1 sums the run, the run
-// sums are scanned across the wavefronts, pass 2 scans each chunk with lane
-// shuffles and carries the chunk total.  counts receives the class totals.
+// Exclusive prefix over the tiles of each class's count, in two levels:
+// k_class_scan scans 256 tiles per block (wavefront shuffle scans, one tile
+// per thread, coalesced loads) and leaves each block's totals in bsum;
+// k_class_fill adds the totals of the blocks before its tile (one wavefront
+// sums them) and block 0 also forms the class totals.
+constexpr int kScanTiles = 256;
+
 __device__ __forceinline__ int wave_incl_scan(int x) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -1431,57 +1443,68 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
   return x;
 }
 
-__global__ void __launch_bounds__(1024) k_class_scan(const int32_t *__restrict__ tcount, int32_t ntile,
-                                                     int32_t *toff, int32_t *counts) {
-  __shared__ int32_t wsum[3][16];
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+__global__ void __launch_bounds__(kScanTiles) k_class_scan(const int32_t *__restrict__ tcount, int32_t ntile,
+                                                           int32_t *toff, int32_t *bsum) {
+  __shared__ int32_t ws[3][kScanTiles / 64];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int per = ((ntile + 15) / 16 + 63) / 64 * 64;       // tiles per wavefront, whole chunks
-  const int lo = w * per, hi = lo + per < ntile ? lo + per : ntile;
-  int acc[3] = {0, 0, 0};
-#pragma unroll 4
-  for (int i = lo + lane; i < hi; i += 64) {
-    acc[0] += tcount[3 * i];
-    acc[1] += tcount[3 * i + 1];
-    acc[2] += tcount[3 * i + 2];
-  }
+  const int64_t i = (int64_t)blockIdx.x * kScanTiles + t;
+  int x[3], inc[3];
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const int tot = wave_incl_scan(acc[k]);
-    if (lane == 63) wsum[k][w] = tot;
+    x[k] = i < ntile ? tcount[3 * i + k] : 0;
+    inc[k] = wave_incl_scan(x[k]);
+    if (lane == 63) ws[k][w] = inc[k];
   }
   __syncthreads();
-  if (t < 64) {
-    for (int k = 0; k < 3; ++k) {
-      const int v = t < 16 ? wsum[k][t] : 0;
-      const int sc = wave_incl_scan(v);
-      if (t < 16) wsum[k][t] = sc - v;            // exclusive prefix of the wavefront sums
-      if (t == 15) counts[k] = sc;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int off = 0, tot = 0;
+    for (int v = 0; v < kScanTiles / 64; ++v) {
+      off += v < w ? ws[k][v] : 0;
+      tot += ws[k][v];
     }
-  }
-  __syncthreads();
-  int base[3] = {wsum[0][w], wsum[1][w], wsum[2][w]};
-  for (int i0 = lo; i0 < hi; i0 += 64) {
-    const int i = i0 + lane;
-    for (int k = 0; k < 3; ++k) {
-      const int x = i < hi ? tcount[3 * i + k] : 0;
-      const int inc = wave_incl_scan(x);
-      if (i < hi) toff[3 * i + k] = base[k] + inc - x;
-      base[k] += __shfl(inc, 63);
-    }
+    if (i < ntile) toff[3 * i + k] = off + inc[k] - x[k];
+    if (t == 0) bsum[3 * blockIdx.x + k] = tot;
   }
 }
 
 // Cell ids into the class lists (lists + class * ncells), tile by tile.
 __global__ void __launch_bounds__(64) k_class_fill(const uint8_t *__restrict__ cls, const int32_t *__restrict__ toff,
-                                                   int64_t ncells, int32_t *lists) {
+                                                   const int32_t *__restrict__ bsum, int32_t nsb, int64_t ncells,
+                                                   int32_t *lists, int32_t *counts) {
   const int tid = threadIdx.x;
-  const int64_t c0 = (int64_t)blockIdx.x * kTileCells;
+  const int64_t tile = blockIdx.x;
+  const int B = (int)(tile / kScanTiles);
+  const int nread = tile == 0 ? nsb : B;
+  int p[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+  for (int b = tid; b < nread; b += 64)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int v = bsum[3 * b + k];
+      p[k] += b < B ? v : 0;
+      q[k] += v;
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    p[k] = wave_sum(p[k]);
+    q[k] = wave_sum(q[k]);
+  }
+  if (tile == 0 && tid == 0) { counts[0] = q[0]; counts[1] = q[1]; counts[2] = q[2]; }
+  const int64_t c0 = tile * kTileCells;
   const int nc = (int)(ncells - c0 < kTileCells ? ncells - c0 : kTileCells);
   const int cl = tid < nc ? (int)cls[c0 + tid] : -1;
   const uint64_t m0 = __ballot(cl == 0), m1 = __ballot(cl == 1), m2 = __ballot(cl == 2);
   if (cl >= 0) {
     const uint64_t below = (1ull << tid) - 1ull;
     const uint64_t m = cl == 0 ? m0 : cl == 1 ? m1 : m2;
-    lists[(int64_t)cl * ncells + toff[3 * blockIdx.x + cl] + __popcll(m & below)] = (int32_t)(c0 + tid);
+    const int base = cl == 0 ? p[0] : cl == 1 ? p[1] : p[2];
+    lists[(int64_t)cl * ncells + base + toff[3 * tile + cl] + __popcll(m & below)] = (int32_t)(c0 + tid);
   }
 }
 
@@ -1493,7 +1516,7 @@ __global__ void __launch_bounds__(256) k_scatter(const DevLayout *__restrict__ D
   double *tile = reinterpret_cast<double *>(smem);
   double **dst = reinterpret_cast<double **>(tile + kTileCells * ld);   // [no] row bases
   int64_t *gi = reinterpret_cast<int64_t *>(dst + no);
-  const int64_t c0 = (int64_t)blockIdx.x * kTileCells;
+  const int64_t c0 = xcd_tile() * kTileCells;
   const int nc = (int)(a.ncells - c0 < kTileCells ? a.ncells - c0 : kTileCells);
   const int tid = threadIdx.x;
   const bool inp = a.inplace;
@@ -1542,6 +1565,12 @@ size_t record_tile_lds(const DevLayout &Lay) {
 
 int64_t gather_tiles(int64_t ncells) { return (ncells + kTileCells - 1) / kTileCells; }
 
+// ints of the toff work array: the tiles' offsets, then the scan blocks' totals
+int64_t class_work_ints(int64_t ncells) {
+  const int64_t nt = gather_tiles(ncells);
+  return 3 * (nt + (nt + kScanTiles - 1) / kScanTiles);
+}
+
 // Records of every cell; with cls != null also the class lists (tcount and
 // toff: [tiles][3] work arrays) and the class counts.
 int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, double *rec, uint8_t *cls,
@@ -1549,8 +1578,11 @@ int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, d
   const unsigned nb = (unsigned)gather_tiles(a.ncells);
   hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), record_tile_lds(Lay), s, d, a, rec, cls, tcount);
   if (cls) {
-    hipLaunchKernelGGL(k_class_scan, dim3(1), dim3(1024), 0, s, tcount, (int32_t)nb, toff, counts);
-    hipLaunchKernelGGL(k_class_fill, dim3(nb), dim3(64), 0, s, cls, toff, a.ncells, lists);
+    const unsigned nsb = (nb + kScanTiles - 1) / kScanTiles;
+    int32_t *bsum = toff + 3 * (int64_t)nb;
+    hipLaunchKernelGGL(k_class_scan, dim3(nsb), dim3(kScanTiles), 0, s, tcount, (int32_t)nb, toff, bsum);
+    hipLaunchKernelGGL(k_class_fill, dim3(nb), dim3(64), 0, s, cls, toff, bsum, (int32_t)nsb, a.ncells, lists,
+                       counts);
   }
   return (int)hipGetLastError();
 }
