@@ -53,8 +53,11 @@ enum {
   SALSA_CELL_NONFINITE = 8,    /* NaN/Inf produced                                                  */
   SALSA_CELL_GRAD_UNSTABLE = 16, /* salsa_gradient_device: the linearised trajectory of the cell is
                                    unstable -- a relative input change is amplified more than 1e8
-                                   times in J, or a derivative left the range of its type; the
-                                   derivative then describes no perturbation resolvable in FP64 */
+                                   times in J, the vapour or an active bin, the explicit water
+                                   update of one step amplifies a perturbation more than 1e4 times
+                                   past its +-10 % limiter, or a derivative left the range of its
+                                   type; the derivative then describes no perturbation resolvable
+                                   in FP64 */
   SALSA_CELL_DISTR = 32        /* distr_update did not settle in 100 sweeps or moved a negative
                                   fraction (the reference STOPs, mo_salsa_update.f90:137, 216, 311) */
 };

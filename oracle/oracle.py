@@ -58,6 +58,7 @@ class Params(C.Structure):
         ("lcharge", C.c_int), ("charge_coll_enh", C.c_double),
         ("kpert", C.c_double), ("kseed", C.c_ulonglong),
         ("lsdistupdate", C.c_int), ("level", C.c_int), ("lsactiv", C.c_int),
+        ("diag_growth", C.c_int),
     ]
 
 

@@ -188,7 +188,7 @@ void so_params_default(so_params *P, int level) {
   P->ice_gamma = 0.55; P->ice_sigma = 1.97;
   P->lcharge = 0; P->charge_coll_enh = 0.;
   P->kpert = 0.; P->kseed = 0ull;
-  P->lsdistupdate = 0; P->level = level; P->lsactiv = 0;
+  P->lsdistupdate = 0; P->level = level; P->lsactiv = 0; P->diag_growth = 0;
   if (level == 4) {
     P->lscgia = P->lscgic = P->lscgii = P->lscgip = 0;
     P->lscndh2oic = 0;
@@ -1000,6 +1000,22 @@ static double acth2o_bin(const so_layout *L, const double *volc, const double *p
   return fmax(0.1, znw / fmax(eps_r8, (znw + zns)));
 }
 
+/* Test hook (so_params.diag_growth), no counterpart in the reference: the
+ * factor by which one explicit substep multiplies a perturbation of bin b's
+ * water where the +-10 % limiter does not bind: the bin's own update,
+ * 1 - a csu w' with a = adt zmt, w' = d zwsat / d zcw (Raoult term of
+ * acth2o_bin times the Kelvin factor), times the share 1 - a / (1 + adt S)
+ * that the implicit vapour predictor leaves. */
+static double update_multiplier(const so_layout *L, const double *volc, double zcw, double d, double a,
+                                double csu, double zkelb, double zhlp2) {
+  if (!(fabs(d) < 1.e-1 * zcw)) return 1.;
+  double zns = 0.;
+  for (int nn = 0; nn < L->ndry; ++nn) zns += L->diss[nn] * volc[nn] * L->rholiq[nn] / L->mm[nn];
+  const double den = zcw + zns;
+  const double wp = den > eps_r8 && zcw / den > 0.1 ? zkelb * zns / (den * den) : 0.;
+  return fabs((1. - a * csu * wp) * (1. - a / zhlp2));
+}
+
 double so_acth2o(const so_layout *L, const double *volc_bin, const double *pcw) {
   return acth2o_bin(L, volc_bin, pcw);
 }
@@ -1218,6 +1234,7 @@ static int gpparth2o(const so_layout *L, const so_params *P, so_cell *c,
 
   double ttot = 0., zcwint = 0.;
   int counter = 0;
+  double grow = 1.;                         /* diag_growth */
   while (ttot < ptstep) {
     double adt = 2.e-2;
     /* vapour predictor, :992-1000 */
@@ -1234,10 +1251,13 @@ static int gpparth2o(const so_layout *L, const so_params *P, so_cell *c,
     double zhlp2 = 1. + adt * (s2a + s2c + s2p + s2i);
     zcwint = zhlp1 / zhlp2;
     zcwint = fmin(zcwint, zcwtot);
+    double mmax = 1.;
     /* per-bin correctors, :1002-1031 */
     if (act_a && zrh > 0.98) {
       for (int b = A + nstr; b < A + L->nbins; ++b) {
         double d = adt * zmt[b] * (zcwint - zwsat[b] * zcwsurf[b]);
+        if (P->diag_growth)
+          mmax = fmax(mmax, update_multiplier(L, c->volc[b], zcwc_b[b], d, adt * zmt[b], zcwsurf[b], zkel[b], zhlp2));
         zcwint_b[b] = zcwc_b[b] + fmin(fmax(d, -1.e-1 * zcwc_b[b]), 1.e-1 * zcwc_b[b]);
         zwsat[b] = acth2o_bin(L, c->volc[b], &zcwint_b[b]) * zkel[b];
       }
@@ -1245,6 +1265,8 @@ static int gpparth2o(const so_layout *L, const so_params *P, so_cell *c,
     if (act_c) {
       for (int b = C; b < C + L->ncld; ++b) {
         double d = adt * zmt[b] * (zcwint - zwsat[b] * zcwsurf[b]);
+        if (P->diag_growth)
+          mmax = fmax(mmax, update_multiplier(L, c->volc[b], zcwc_b[b], d, adt * zmt[b], zcwsurf[b], zkel[b], zhlp2));
         zcwint_b[b] = zcwc_b[b] + fmin(fmax(d, -1.e-1 * zcwc_b[b]), 1.e-1 * zcwc_b[b]);
         zwsat[b] = acth2o_bin(L, c->volc[b], &zcwint_b[b]) * zkel[b];
       }
@@ -1252,6 +1274,8 @@ static int gpparth2o(const so_layout *L, const so_params *P, so_cell *c,
     if (act_p) {
       for (int b = R; b < R + L->nprc; ++b) {
         double d = adt * zmt[b] * (zcwint - zwsat[b] * zcwsurf[b]);
+        if (P->diag_growth)
+          mmax = fmax(mmax, update_multiplier(L, c->volc[b], zcwc_b[b], d, adt * zmt[b], zcwsurf[b], zkel[b], zhlp2));
         zcwint_b[b] = zcwc_b[b] + fmin(fmax(d, -1.e-1 * zcwc_b[b]), 1.e-1 * zcwc_b[b]);
         zwsat[b] = acth2o_bin(L, c->volc[b], &zcwint_b[b]) * zkel[b];
       }
@@ -1277,7 +1301,10 @@ static int gpparth2o(const so_layout *L, const so_params *P, so_cell *c,
     ttot = ttot + adt;
     counter = counter + 1;
     if (counter > 100) status |= SO_WARN_SUBSTEPS;
+    grow = fmin(grow * mmax, 1.e300);
   }
+  if (P->diag_growth && grow > 1.e3) status |= SO_DIAG_GROWTH3;
+  if (P->diag_growth && grow > 1.e5) status |= SO_DIAG_GROWTH5;
   *prv = zcwint * mwa / rhoair;
 
   /* write-back and diagnostics, :1058-1117 */

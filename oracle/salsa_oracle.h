@@ -81,6 +81,13 @@ typedef struct {
                                   skips the ice bins below 5 (:323)            */
   int lsactiv;                 /* cloud activation: 0 off, 1 interstitial
                                   (lsactiv%mode 1, mo_salsa.f90:115-119)       */
+  /* Test hook with no counterpart in the reference: when set, the per-cell
+   * status carries SO_DIAG_GROWTH3 / SO_DIAG_GROWTH5 when, within one step,
+   * the explicit water update of the condensation substeps amplifies a
+   * perturbation more than 1e3 / 1e5 times where its +-10 % limiter does
+   * not bind (the product over the substeps of the largest per-bin
+   * multiplier).  Default 0. */
+  int diag_growth;
 } so_params;
 
 /* One Rate (classProcessRates.f90:40-45): volc(9) and numc */
@@ -100,8 +107,10 @@ enum {
   SO_WARN_NEG_NUMC = 1,       /* numc < -1e-6 after coagulation (mo_salsa_dynamics.f90:191-193) */
   SO_ERR_EQUIL_NOCONV = 2,    /* equilibration > 100 iterations (mo_salsa_properties.f90:238-243) */
   SO_WARN_SUBSTEPS = 4,       /* > 100 condensation substeps (mo_salsa_dynamics.f90:1051) */
-  SO_ERR_DISTR = 8            /* distr_update: > 100 passes or a negative fraction (the
+  SO_ERR_DISTR = 8,           /* distr_update: > 100 passes or a negative fraction (the
                                  reference STOPs, mo_salsa_update.f90:109, 137, 216, 311) */
+  SO_DIAG_GROWTH3 = 16,       /* diag_growth: explicit water update grew > 1e3 in a step */
+  SO_DIAG_GROWTH5 = 32        /* diag_growth: ... > 1e5 */
 };
 
 /* Fill the layout from nbin/reglim/bloPrc/bloIce/species; returns 0 on success */

@@ -42,3 +42,33 @@ def test_oracle_trajectory_unstable_cell(oracle_mod):
     grow = d[6:10] / np.maximum(d[5:9], 1e-300)
     assert np.all(grow > 30.)                # two orders of magnitude per step
     assert d.max() > 1e-5 and d.max() < 1e-1  # then bounded: the clamp keeps the values finite
+
+
+def test_oracle_update_growth_diagnostic(oracle_mod):
+    """The oracle's diag_growth hook: off by default (status unchanged); on,
+    it marks cell 3568 -- whose cloud water oscillates in the explicit
+    substep update -- as amplifying a perturbation beyond 1e3 within a step,
+    and only a small share of the workload with it."""
+    from salsa_amd import capi, workload as W
+    O = oracle_mod
+    desc = capi.bin_desc()
+    T, p, rs, rsi, rv, numc, volc = W.to_salsa_units(desc, W.config_fields(desc, 3))
+    L = O.make_layout()
+    nt = numc.shape[1]
+    sel = np.r_[3568, np.arange(0, 81920, 97)]
+    nc = np.zeros((len(sel), L.ntotal))
+    nc[:, :nt] = numc[sel]
+    vc = np.zeros((len(sel), L.ntotal, 9))
+    vc[:, :nt] = volc[sel]
+
+    def status(**kw):
+        return O.step_cells(L, O.make_params(level=4, **kw), T[sel], p[sel], rs[sel], rsi[sel], rv[sel], nc, vc,
+                            nsteps=20)[3]
+
+    assert not np.any(status())
+    st = status(diag_growth=1)
+    g3, g5 = (st & 16) != 0, (st & 32) != 0
+    assert g3[0]
+    assert np.all(g3[g5])
+    assert g5.mean() < 0.02
+    assert not np.any(st & ~48)

@@ -692,16 +692,33 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
           const DD zcwtot = zcwc + stot;
           const double adt = 2.e-2;
           DD zcwint = dconst(0.);
+          // Stability of the explicit water update.  Where the +-10 % limiter
+          // does not bind, a perturbation of bin b's water is multiplied per
+          // substep by m_b = (1 - a_b csu w'_b) (1 - a_b / (1 + adt s2)), with
+          // a_b = adt zmt_b and w'_b = d zws_b / d zcw_b (the vapour predictor
+          // takes the rest).  A haze bin near activation can have |m_b| >> 1:
+          // its water then alternates between the limits and every substep the
+          // limiter misses multiplies the tangent, while finite perturbations
+          // stay within the limits.  grow is the product over the substeps of
+          // the largest such |m_b| (the value path, alike on every lane).
+          double grow = 1.;
           for (int st = 0; st < a.nsub; ++st) {          // :986-1053
             const DD zhlp1 = zcwc + adt * s1;
             const DD zhlp2 = 1. + adt * s2;
             zcwint = dmin(zhlp1 / zhlp2, zcwtot);
             DD tsum = dconst(0.), s1n = dconst(0.);
+            double mmax = 1.;
             for (int b = 0; b < nt; ++b) {
               const bool ic = (iceq >> b) & 1ull;
               if ((upd >> b) & 1ull) {
                 const double csu = ic ? csui : csul;
                 const DD d = adt * zmt[b] * (zcwint - zws[b] * csu);
+                if (!ic && fabs(d.v) < 1.e-1 * zcw[b].v) {
+                  const double znw = zcw[b].v, zs = zns[b].v, den = znw + zs;
+                  const double wp = den > kEpsR8 && znw / den > 0.1 ? zkl[b].v * zs / (den * den) : 0.;
+                  const double ab = adt * zmt[b].v;
+                  mmax = fmax(mmax, fabs((1. - ab * csu * wp) * (1. - ab / zhlp2.v)));
+                }
                 DD nw = zcw[b] + dmin(dmax(d, -1.e-1 * zcw[b]), 1.e-1 * zcw[b]);
                 nw = dmax(nw, 0.);
                 zcw[b] = nw;
@@ -713,7 +730,11 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
             s1 = s1n;
             zcwint = zcwtot - (sfix + tsum);
             zcwc = zcwint;
+            grow = fmin(grow * mmax, 1.e300);
           }
+          // a growth beyond 1e4 in one step: the linearisation holds only for
+          // perturbations far below any that a difference can resolve
+          if (grow > 1.e4) flags |= SALSA_CELL_GRAD_UNSTABLE;
           if (a.nsub > 100) flags |= SALSA_CELL_SUBSTEPS;
           rv = zcwint * kMwa / rhoair;
           // write-back (:1058-1117)
