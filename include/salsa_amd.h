@@ -268,6 +268,33 @@ int salsa_gradient_device(salsa_ctx *ctx, const double *press, const double *tem
  * (0 = clear and dry, 1 = clear and humid, 2 = cloudy); 0 keeps the current
  * value.  salsa_autotune times the candidates on the given device-resident
  * inputs and keeps the fastest per class. */
+/*
+ * The LES side of a SALSA step for the SALSA tracers, on device: what cfll
+ * runs after run_SALSA (step.f90:330-333) -- tend_constrain2
+ * (constrain_SALSA.f90:18-36), update_sclrs (step.f90:450-489: a += a_t*dtlt
+ * on the interior, sclrset 'mixd' util.f90:71-107, periodic lateral halos
+ * of a process that owns the whole grid), tend0 (step.f90:379-396) and
+ * SALSA_diagnostics(.FALSE., lcharge) (constrain_SALSA.f90:50-404).
+ *   dtlt       : time step [s]
+ *   dzt        : HOST array of nzp inverse level thicknesses (mo_aux_state
+ *                dzt); the top boundary extrapolates with dzt(nzp-2)/dzt(nzp-1)
+ *   temp, rh   : Diag temperature and relative humidity (a_temp, a_rh)
+ *   rhi        : relative humidity over ice (a_rhi), level 5; may be NULL at 4
+ *   rp         : water vapour mixing ratio (a_rp), for rtot
+ *   n_p, m_p   : number / mass fields, updated in place; n_t, m_t : their
+ *                tendencies, applied then zeroed (ice ignored at level 4)
+ *   charge_time: NULL, or a_chargeTimep [nbins+ncld+nprc][n3]: zeroed where
+ *                the reference zeroes it (lcharge)
+ *   diag       : [6][n3] rc, srp, snrp, rtot, ri, riri (rows 4-5 at level 5)
+ * Needs hz = 1, hxy = 2.  Not carried: the gas-phase transfer of tiny
+ * aerosols (lscndgas) and the a_indefp clamp (ice_theta_dist).  Device
+ * pointers; work is enqueued on the context stream like salsa_step_device.
+ */
+int salsa_les_update_device(salsa_ctx *ctx, double dtlt, const double *dzt, const double *temp,
+                            const double *rh, const double *rhi, const double *rp,
+                            double *const n_p[4], double *const n_t[4], double *const m_p[4],
+                            double *const m_t[4], double *charge_time, double *diag);
+
 int salsa_set_launch_shape(salsa_ctx *ctx, int32_t cls, int32_t lanes_per_cell);
 int salsa_get_launch_shape(salsa_ctx *ctx, int32_t cls);
 int salsa_autotune(salsa_ctx *ctx, const double *press, const double *temp,

@@ -99,6 +99,12 @@ def lib():
         L.so_rslf.restype = C.c_double
         L.so_rsif.argtypes = [C.c_double, C.c_double]
         L.so_rsif.restype = C.c_double
+        P4 = C.c_void_p * 4
+        L.so_les_update.argtypes = [C.POINTER(Layout), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    dp, dp, dp, dp, P4, P4, P4, P4, C.c_void_p, dp]
+        L.so_les_update.restype = C.c_int
+        L.so_les_corr.argtypes = [dp, dp, C.c_int]
+        L.so_les_corr.restype = C.c_double
         _lib = L
     return _lib
 
@@ -185,3 +191,40 @@ def step_batch(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1):
         assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return lib().so_step_batch(C.byref(L), C.byref(P), len(T), _dp(T), _dp(p), _dp(rs), _dp(rsi), _dp(rv),
                                dt, nsteps, _dp(numc), _dp(volc))
+
+
+def les_update(L, level, shape, dtlt, dzf2, temp, rh, rhi, rp, n_p, n_t, m_p, m_t, charge_time=None):
+    """tend_constrain2 + update_sclrs + tend0 + SALSA_diagnostics on LES
+    arrays (so_les_update).  n_p/n_t/m_p/m_t: lists of 4 float64 arrays in
+    the LES layout ([rows, n3], n3 in Fortran order), None for the ice
+    category at level 4; updated in place.  Returns diag[6, n3]."""
+    nzp, nxp, nyp = shape
+    n3 = nzp * nxp * nyp
+    P4 = C.c_void_p * 4
+
+    def ptrs(lst):
+        out = []
+        for a in lst:
+            if a is None:
+                out.append(None)
+            else:
+                assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+                out.append(a.ctypes.data)
+        return P4(*out)
+    diag = np.zeros((6, n3))
+    ct = None
+    if charge_time is not None:
+        assert charge_time.dtype == np.float64 and charge_time.flags["C_CONTIGUOUS"]
+        ct = charge_time.ctypes.data
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (temp, rh, rhi, rp)]
+    rc = lib().so_les_update(C.byref(L), level, nzp, nxp, nyp, dtlt, dzf2, *(_dp(a) for a in arrs),
+                             ptrs(n_p), ptrs(n_t), ptrs(m_p), ptrs(m_t), ct, _dp(diag))
+    if rc != 0:
+        raise ValueError("so_les_update failed: %d" % rc)
+    return diag
+
+
+def les_corr(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().so_les_corr(_dp(x), _dp(y), len(x))

@@ -1874,3 +1874,271 @@ int so_step_batch(const so_layout *L, const so_params *P, long n, const double *
                          numc + i * L->ntotal, volc + i * L->ntotal * SO_MAXSPEC, 0);
   return st;
 }
+
+/* ====================================================================== */
+/* LES coupling after run_SALSA for the SALSA tracers of one process that  */
+/* owns the whole (nzp,nxp,nyp) grid (halo 1 in z, 2 laterally):          */
+/*   tend_constrain2   constrain_SALSA.f90:18-36                           */
+/*   update_sclrs      step.f90:450-489 (update; sclrset 'mixd',           */
+/*                     util.f90:71-107, with periodic lateral halos)       */
+/*   tend0             step.f90:379-396                                    */
+/*   SALSA_diagnostics constrain_SALSA.f90:50-404, findDry4Wet :406-469,   */
+/*                     calc_correlation util.f90:1005-1024                 */
+/* in the order of cfll (step.f90:330-333).  Arrays are the LES ones:      */
+/* n(c)[b][n3], m(c)[s*nb+b][n3], n3 index k + nzp*(i + nxp*j).            */
+/* ====================================================================== */
+static double les_corr(const double *x, const double *y, int n) {
+  /* calc_correlation: sxy keeps the last product only (util.f90:1017) */
+  double sx = 0., sy = 0., sx2 = 0., sy2 = 0., sxy = 0.;
+  for (int i = 0; i < n; ++i) {
+    sx = sx + x[i];
+    sy = sy + y[i];
+    sx2 = sx2 + x[i] * x[i];
+    sy2 = sy2 + y[i] * y[i];
+    sxy = x[i] * y[i];
+  }
+  if (sx2 * n - sx * sx < eps_r8 || sy2 * n - sy * sy < eps_r8) return 0.;
+  return (sxy * n - sx * sy) / (sqrt(sx2 * n - sx * sx) * sqrt(sy2 * n - sy * sy));
+}
+
+double so_les_corr(const double *x, const double *y, int n) { return les_corr(x, y, n); }
+
+/* findDry4Wet for an evaporating bin with number ne and dry masses md[ndry];
+ * na / ma: the cell's aerosol numbers and dry masses ma[s][nbins]. */
+static int les_dry4wet(const so_layout *L, double ne, const double *md, const double *na, double ma[][SO_MAXBIN]) {
+  const int ndry = L->ndry, A = L->iaero;
+  double cd = 0.;
+  for (int s = 0; s < ndry; ++s) cd = cd + md[s] / L->rholiq[s];
+  cd = cd / ne;
+  cd = pow(cd / pi6, 1. / 3.);
+  int cnt = 0;
+  for (int b = L->in2a; b <= L->fn2a; ++b) cnt += cd > pow(L->vlolim[A + b] / pi6, 1. / 3.);
+  int ba = L->in2a + cnt - 1;
+  ba = ba < L->in2a ? L->in2a : ba;
+  ba = ba > L->fn2a ? L->fn2a : ba;
+  const int bb = L->fn2a + (ba - L->in2a);     /* as written, :443 */
+  if (na[bb] <= nlim_c) return ba;
+  if (na[ba] <= nlim_c) return bb;
+  double xa[SO_MAXSPEC], xb[SO_MAXSPEC];
+  for (int s = 0; s < ndry; ++s) { xa[s] = ma[s][ba]; xb[s] = ma[s][bb]; }
+  const double ra = les_corr(xa, md, ndry), rb = les_corr(xb, md, ndry);
+  return ra < rb ? bb : ba;
+}
+
+int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, double dtlt, double dzf2,
+                  const double *temp, const double *rh, const double *rhi, const double *rp,
+                  double *const n_p[4], double *const n_t[4], double *const m_p[4], double *const m_t[4],
+                  double *charge_time, double *diag) {
+  const long n3 = (long)nzp * nxp * nyp;
+  const int ncat = level == 5 ? 4 : 3;
+  const int nb[4] = {L->nbins, L->ncld, L->nprc, L->nice};
+  const int nwet = L->nwet, ndry = L->ndry, iwa = L->iwa, irim = L->irim;
+  const int nsw[4] = {nwet, nwet, nwet, L->nspec};
+  const int nx = nxp - 4, ny = nyp - 4;
+  const double massTH = 1.e-25;
+  if (nzp < 4 || nx < 1 || ny < 1) return -1;
+#define IDX(k, i, j) ((long)(k) + (long)nzp * ((long)(i) + (long)nxp * (long)(j)))
+  /* tend_constrain2, update (interior), tend0 */
+  for (int c = 0; c < ncat; ++c) {
+    const int nrow = nb[c] * (1 + nsw[c]);
+    for (int r = 0; r < nrow; ++r) {
+      double *p = r < nb[c] ? n_p[c] + (long)r * n3 : m_p[c] + (long)(r - nb[c]) * n3;
+      double *t = r < nb[c] ? n_t[c] + (long)r * n3 : m_t[c] + (long)(r - nb[c]) * n3;
+      for (int j = 0; j < nyp; ++j)
+        for (int i = 0; i < nxp; ++i)
+          for (int k = 0; k < nzp; ++k) {
+            const long x = IDX(k, i, j);
+            const double tc = fmax((-1. + 1.e-8) * p[x] / dtlt, t[x]);
+            if (k >= 1 && k <= nzp - 2 && i >= 2 && i <= nxp - 3 && j >= 2 && j <= nyp - 3) p[x] = p[x] + tc * dtlt;
+            t[x] = 0.;
+          }
+      /* sclrset 'mixd' on every column, then the periodic halos */
+      for (int j = 0; j < nyp; ++j)
+        for (int i = 0; i < nxp; ++i) {
+          p[IDX(0, i, j)] = p[IDX(1, i, j)] - 0. * (p[IDX(2, i, j)] - p[IDX(1, i, j)]);
+          p[IDX(nzp - 1, i, j)] = p[IDX(nzp - 2, i, j)] + dzf2 * (p[IDX(nzp - 2, i, j)] - p[IDX(nzp - 3, i, j)]);
+        }
+      for (int j = 0; j < nyp; ++j)
+        for (int i = 0; i < nxp; ++i) {
+          if (i >= 2 && i <= nxp - 3 && j >= 2 && j <= nyp - 3) continue;
+          const int ii = i < 2 ? i + nx : (i > nxp - 3 ? i - nx : i);
+          const int jj = j < 2 ? j + ny : (j > nyp - 3 ? j - ny : j);
+          for (int k = 0; k < nzp; ++k) p[IDX(k, i, j)] = p[IDX(k, ii, jj)];
+        }
+    }
+  }
+  /* SALSA_diagnostics: remove negative values everywhere */
+  for (int c = 0; c < ncat; ++c) {
+    for (long x = 0; x < (long)nb[c] * n3; ++x) n_p[c][x] = fmax(0., n_p[c][x]);
+    for (long x = 0; x < (long)nb[c] * nsw[c] * n3; ++x) m_p[c][x] = fmax(0., m_p[c][x]);
+  }
+  const int A = 0;
+  (void)A;
+  for (int j = 2; j <= nyp - 3; ++j)
+    for (int i = 2; i <= nxp - 3; ++i)
+      for (int k = 0; k < nzp; ++k) {
+        const long x = IDX(k, i, j);
+#define NP(c, b) n_p[c][(long)(b) * n3 + x]
+#define MP(c, s, b) m_p[c][((long)(s) * nb[c] + (b)) * n3 + x]
+        /* particles with number but no mass (:94-146) */
+        for (int b = 0; b < nb[0]; ++b) {
+          double sm = 0.;
+          for (int s = 0; s < nwet; ++s) sm = sm + MP(0, s, b);
+          if (NP(0, b) > 0. && sm <= 0.) {
+            NP(0, b) = 0.;
+            for (int s = 0; s < nwet; ++s) MP(0, s, b) = 0.;
+            if (charge_time) charge_time[(long)b * n3 + x] = 0.;
+          }
+        }
+        for (int b = 0; b < nb[1]; ++b) {
+          double sm = 0.;
+          for (int s = 0; s < nwet; ++s) sm = sm + MP(1, s, b);
+          if (NP(1, b) > 0. && sm <= 0.) {
+            NP(1, b) = 0.;
+            for (int s = 0; s < nwet; ++s) MP(1, s, b) = 0.;
+            if (charge_time) charge_time[(long)(nb[0] + b) * n3 + x] = 0.;
+          }
+        }
+        for (int b = 0; b < nb[2]; ++b) {
+          if (NP(2, b) > 0. && MP(2, iwa, b) <= 0.) {
+            NP(2, b) = 0.;
+            for (int s = 0; s < nwet; ++s) MP(2, s, b) = 0.;
+            if (charge_time) charge_time[(long)(nb[0] + nb[1] + b) * n3 + x] = 0.;
+          }
+        }
+        if (level == 5)
+          for (int b = 0; b < nb[3]; ++b) {
+            const double mt = MP(3, iwa, b) + MP(3, irim, b);
+            if (NP(3, b) > 0. && mt < massTH) {
+              NP(3, b) = 0.;
+              for (int s = 0; s <= irim; ++s) MP(3, s, b) = 0.;
+            }
+          }
+        /* aerosol numbers and dry masses of the cell for findDry4Wet */
+        double na[SO_MAXBIN], ma[SO_MAXSPEC][SO_MAXBIN];
+        /* evaporating cloud droplets (:152-203) */
+        for (int bc = L->ica.cur; bc <= L->fcb.cur; ++bc) {
+          if (!(NP(1, bc) > nlim_c && rh[x] < 0.999 && MP(1, iwa, bc) < 1.e-5)) continue;
+          double ns = 0.;
+          for (int s = 0; s < ndry; ++s) ns = ns + L->diss[s] * MP(1, s, bc) / L->mm[s];
+          ns = ns / NP(1, bc);
+          const double zbb = 6. * mwa * ns / (pi * rhowa);
+          const double zaa = 4. * mwa * surfw0 / (rg * rhowa * temp[x]);
+          const double cdcld = sqrt(3. * zbb / zaa);
+          double zvol = 0.;
+          for (int s = 0; s < nwet; ++s) zvol = zvol + MP(1, s, bc) / L->rholiq[s];
+          zvol = zvol / NP(1, bc);
+          const double zdh2o = pow(zvol / pi6, 1. / 3.);
+          if (zdh2o < fmin(0.7 * cdcld, 1.3e-5) || MP(1, iwa, bc) < massTH * NP(1, bc)) {
+            const int ba = bc <= L->fca.cur ? L->ica.par + (bc - L->ica.cur) : L->icb.par + (bc - L->icb.cur);
+            NP(0, ba) = NP(0, ba) + NP(1, bc);
+            NP(1, bc) = 0.;
+            for (int s = 0; s < nwet; ++s) {
+              MP(0, s, ba) = MP(0, s, ba) + MP(1, s, bc);
+              MP(1, s, bc) = 0.;
+            }
+            if (charge_time) charge_time[(long)(nb[0] + bc) * n3 + x] = 0.;
+          }
+        }
+        /* evaporating precipitation (:205-257) */
+        for (int bc = 0; bc < nb[2]; ++bc) {
+          if (!(NP(2, bc) > prlim_c && rh[x] < 0.999 && MP(2, iwa, bc) < 1.e-6)) continue;
+          double ns = 0.;
+          for (int s = 0; s < ndry; ++s) ns = ns + L->diss[s] * MP(2, s, bc) / L->mm[s];
+          ns = ns / NP(2, bc);
+          const double zbb = 6. * mwa * ns / (pi * rhowa);
+          const double zaa = 4. * mwa * surfw0 / (rg * rhowa * temp[x]);
+          const double cdprc = sqrt(3. * zbb / zaa);
+          double zvol = 0.;
+          for (int s = 0; s < nwet; ++s) zvol = zvol + MP(2, s, bc) / L->rholiq[s];
+          zvol = zvol / NP(2, bc);
+          const double zdh2o = pow(zvol / pi6, 1. / 3.);
+          if (zdh2o < fmin(0.2 * cdprc, 10.e-6) || MP(2, iwa, bc) < massTH * NP(2, bc)) {
+            double md[SO_MAXSPEC];
+            for (int s = 0; s < ndry; ++s) md[s] = MP(2, s, bc);
+            for (int b = 0; b < nb[0]; ++b) {
+              na[b] = NP(0, b);
+              for (int s = 0; s < ndry; ++s) ma[s][b] = MP(0, s, b);
+            }
+            const int ba = les_dry4wet(L, NP(2, bc), md, na, ma);
+            NP(0, ba) = NP(0, ba) + NP(2, bc);
+            NP(2, bc) = 0.;
+            for (int s = 0; s < nwet; ++s) {
+              MP(0, s, ba) = MP(0, s, ba) + MP(2, s, bc);
+              MP(2, s, bc) = 0.;
+            }
+            if (charge_time) charge_time[(long)(nb[0] + nb[1] + bc) * n3 + x] = 0.;
+          }
+        }
+        /* sublimating ice (:259-309) */
+        if (level == 5)
+          for (int bc = 0; bc < nb[3]; ++bc) {
+            const double mt = MP(3, iwa, bc) + MP(3, irim, bc);
+            double vt = 0.;
+            for (int s = 0; s <= irim; ++s) vt = vt + MP(3, s, bc) / L->rhoice[s];
+            if (!(NP(3, bc) > prlim_c && rhi[x] < 0.999 && mt < 1.e-15)) continue;
+            double cdice = vt / NP(3, bc);
+            cdice = pow(cdice / pi6, 1. / 3.);
+            double zdrms = 0., zwams = 0.;
+            for (int s = 0; s < ndry; ++s) zdrms = zdrms + MP(3, s, bc);
+            for (int s = 0; s <= irim; ++s) zwams = zwams + MP(3, s, bc);
+            const double zvol = zdrms / zwams;
+            if (zvol > 0.5 || cdice < 2.e-6) {
+              double md[SO_MAXSPEC];
+              for (int s = 0; s < ndry; ++s) md[s] = MP(3, s, bc);
+              for (int b = 0; b < nb[0]; ++b) {
+                na[b] = NP(0, b);
+                for (int s = 0; s < ndry; ++s) ma[s][b] = MP(0, s, b);
+              }
+              const int ba = les_dry4wet(L, NP(3, bc), md, na, ma);
+              NP(0, ba) = NP(0, ba) + NP(3, bc);
+              NP(3, bc) = 0.;
+              for (int s = 0; s < nwet; ++s) {
+                MP(0, s, ba) = MP(0, s, ba) + MP(3, s, bc);
+                MP(3, s, bc) = 0.;
+              }
+              MP(0, iwa, ba) = MP(0, iwa, ba) + MP(3, irim, bc);
+              MP(3, irim, bc) = 0.;
+            }
+          }
+        /* aerosols below 0.1 nm dry diameter (:312-353); the gas-phase
+           transfer of lscndgas is not carried */
+        for (int b = 0; b < nb[0]; ++b) {
+          if (!(NP(0, b) > nlim_c)) continue;
+          double zvol = 0.;
+          for (int s = 0; s < ndry; ++s) zvol = zvol + MP(0, s, b) / L->rholiq[s];
+          zvol = zvol / NP(0, b);
+          if (zvol < pi6 * 1.e-30) {
+            for (int s = 0; s < nwet; ++s) MP(0, s, b) = 0.;
+            NP(0, b) = 0.;
+          }
+        }
+#undef NP
+#undef MP
+      }
+  /* diagnostic tracers over the whole grid (:357-385):
+     diag[0] rc, [1] srp, [2] snrp, [3] rtot, [4] ri, [5] riri */
+  for (long x = 0; x < n3; ++x) {
+    double sa = 0., sc = 0., sp = 0., sn = 0.;
+    for (int b = L->in1a; b <= L->fn2b; ++b) sa = sa + m_p[0][((long)iwa * nb[0] + b) * n3 + x];
+    for (int b = L->ica.cur; b <= L->fcb.cur; ++b) sc = sc + m_p[1][((long)iwa * nb[1] + b) * n3 + x];
+    for (int b = 0; b < nb[2]; ++b) sp = sp + m_p[2][((long)iwa * nb[2] + b) * n3 + x];
+    for (int b = 0; b < nb[2]; ++b) sn = sn + n_p[2][(long)b * n3 + x];
+    const double rc = sa + sc;
+    double rtot = rc + sp + rp[x];
+    diag[x] = rc;
+    diag[n3 + x] = sp;
+    diag[2 * n3 + x] = sn;
+    if (level == 5) {
+      double si = 0., sr = 0.;
+      for (int b = 0; b < nb[3]; ++b) si = si + m_p[3][((long)iwa * nb[3] + b) * n3 + x];
+      for (int b = 0; b < nb[3]; ++b) sr = sr + m_p[3][((long)irim * nb[3] + b) * n3 + x];
+      rtot = rtot + si + sr;
+      diag[4 * n3 + x] = si;
+      diag[5 * n3 + x] = sr;
+    }
+    diag[3 * n3 + x] = rtot;
+  }
+#undef IDX
+  return 0;
+}

@@ -170,6 +170,19 @@ void so_unpack_tend(const so_layout *L, int level, double dn, double tstep,
                     const double *n_cat[4], const double *m_cat[4],
                     double *nt_cat[4], double *mt_cat[4]);
 
+/* LES coupling after run_SALSA for one process owning the whole grid:
+ * tend_constrain2, update_sclrs (update, sclrset 'mixd', periodic halos),
+ * tend0 and SALSA_diagnostics (constrain_SALSA.f90:18-404, step.f90:330-333,
+ * 379-489, util.f90:71-107, 1005-1024).  Updates n_p/m_p in place, zeroes
+ * n_t/m_t, and writes diag[6][n3] = rc, srp, snrp, rtot, ri, riri (the last
+ * two at level 5 only).  charge_time: NULL or [nbins+ncld+nprc][n3].
+ * dzf2 = dzt(nzp-2)/dzt(nzp-1).  Returns 0, or -1 for a grid too small. */
+int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, double dtlt, double dzf2,
+                  const double *temp, const double *rh, const double *rhi, const double *rp,
+                  double *const n_p[4], double *const n_t[4], double *const m_p[4], double *const m_t[4],
+                  double *charge_time, double *diag);
+double so_les_corr(const double *x, const double *y, int n);
+
 #ifdef __cplusplus
 }
 #endif

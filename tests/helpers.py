@@ -195,3 +195,65 @@ def check_condition_aware(got, ref, sens, tol, ns, label="", inputs=None):
     for s in range(ns):
         nflag += one(gvc[:, :, s], ovc[:, :, s], svc[:, :, s], ins[2][:, :, s], FLOOR_V, "V%d" % s)
     return out, nflag
+
+
+def les_case(desc, level, grid, seed, dt=1.0):
+    """LES fields for the coupling after a step (tend_constrain2,
+    update_sclrs, SALSA_diagnostics): a workload state (config 3 at level 4,
+    config 5 at level 5) with random tendencies, some pushing below zero,
+    and cells planted with every condition SALSA_diagnostics acts on --
+    subsaturation with cloud / precipitation / ice bins that lost their water,
+    bins with number but no mass, negative values and aerosols below 0.1 nm.
+    Returns (shape, dzt, f, n_p, n_t, m_p, m_t) with f = temp, rh, rhi, rp."""
+    from salsa_amd import workload as W
+    rng = np.random.default_rng(seed)
+    f = W.config_fields(desc, 5 if level == 5 else 3, grid_override=grid)
+    g = f.grid
+    shape = (g.nzp, g.nxp, g.nyp)
+    n3 = g.nzp * g.nxp * g.nyp
+    ncat = 4 if level == 5 else 3
+    nb = [desc.nbins, desc.ncld, desc.nprc, desc.nice][:ncat]
+    nwet = desc.iwa + 1
+    n_p = [np.array(f.n[c], dtype=np.float64) for c in range(ncat)]
+    m_p = [np.array(f.m[c], dtype=np.float64) for c in range(ncat)]
+    # precipitation (empty in the workload): drops in a third of the cells
+    cells = rng.random(n3) < 0.35
+    n_p[2][:, cells] = 10 ** rng.uniform(-7, 3, size=(nb[2], cells.sum()))
+    for s in range(nwet):
+        scale = 1e-3 if s == desc.iwa else 1e-12
+        m_p[2][s * nb[2]:(s + 1) * nb[2], cells] = scale * 10 ** rng.uniform(-8, 0, size=(nb[2], cells.sum()))
+    # cloud, precipitation and ice bins that lost most of their water
+    for c in range(1, ncat):
+        sel = rng.random((nb[c], n3)) < 0.3
+        w = m_p[c][desc.iwa * nb[c]:(desc.iwa + 1) * nb[c]]
+        w[sel] *= 10 ** rng.uniform(-16, -1, size=sel.sum())
+        if c == 3:
+            r = m_p[c][desc.irim * nb[c]:(desc.irim + 1) * nb[c]]
+            r[sel] *= 10 ** rng.uniform(-16, -1, size=sel.sum())
+    # number but no mass
+    for c in range(ncat):
+        nsw = nwet + (1 if c == 3 else 0)
+        sel = rng.random((nb[c], n3)) < 0.02
+        n_p[c][sel] = np.maximum(n_p[c][sel], 5.)
+        for s in range(nsw):
+            m_p[c][s * nb[c]:(s + 1) * nb[c]][sel] = 0.
+    # aerosols below 0.1 nm dry diameter
+    sel = rng.random((nb[0], n3)) < 0.02
+    n_p[0][sel] = np.maximum(n_p[0][sel], 1e3)
+    for s in range(desc.iwa):
+        m_p[0][s * nb[0]:(s + 1) * nb[0]][sel] *= 1e-30
+    # negative values
+    for c in range(ncat):
+        for a in (n_p[c], m_p[c]):
+            neg = rng.random(a.shape) < 0.01
+            a[neg] *= -1e-3
+    n_t = [a * rng.uniform(-1.5, 0.5, size=a.shape) / dt for a in n_p]
+    m_t = [a * rng.uniform(-1.5, 0.5, size=a.shape) / dt for a in m_p]
+    rh = np.array(f.rv / f.rsl)
+    rhi = np.array(f.rv / f.rsi)
+    sub = rng.random(n3) < 0.5
+    rh[sub] = rng.uniform(0.9, 0.998, size=sub.sum())
+    rhi[sub] = rng.uniform(0.9, 0.998, size=sub.sum())
+    fields = dict(temp=np.array(f.temp), rh=rh, rhi=rhi, rp=np.array(f.rv))
+    dzt = 1. / np.linspace(20., 35., g.nzp)
+    return shape, dzt, fields, n_p, n_t, m_p, m_t

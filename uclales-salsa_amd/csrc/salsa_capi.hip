@@ -29,6 +29,7 @@ int cap_bins(int cap);
 int cap_spec(int cap);
 int launch_gradient(int cap, int precision, const DevLayout *d, const GradArgs &g, double *sum_out,
                     hipStream_t s);
+int launch_les_update(const LesArgs &a, hipStream_t s);
 }  // namespace salsa
 
 using namespace salsa;
@@ -575,5 +576,63 @@ extern "C" int salsa_autotune(salsa_ctx *c, const double *press, const double *t
   hipEventDestroy(e1);
   hipStreamSynchronize(s);
   hipFree(scr);
+  return SALSA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// LES coupling after a step: tend_constrain2, update_sclrs, tend0 and
+// SALSA_diagnostics on the device-resident prognostic arrays
+// ---------------------------------------------------------------------------
+extern "C" int salsa_les_update_device(salsa_ctx *c, double dtlt, const double *dzt, const double *temp,
+                                       const double *rh, const double *rhi, const double *rp,
+                                       double *const n_p[4], double *const n_t[4], double *const m_p[4],
+                                       double *const m_t[4], double *charge_time, double *diag) {
+  if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  const salsa_params &P = c->params;
+  const salsa_bin_desc &d = c->desc;
+  if (P.hz != 1 || P.hxy != 2) {
+    set_error("the LES update needs the LES halo widths (hz 1, hxy 2)");
+    return SALSA_ERR_ARG;
+  }
+  if (c->nzp < 4 || c->nxp < 5 || c->nyp < 5) { set_error("grid too small for the LES update"); return SALSA_ERR_ARG; }
+  if (!(dtlt > 0.) || !dzt || !temp || !rh || !rp || !diag || !n_p || !n_t || !m_p || !m_t) {
+    set_error("null argument or non-positive dtlt");
+    return SALSA_ERR_ARG;
+  }
+  const int ncat = P.level == 5 ? 4 : 3;
+  if (P.level == 5 && !rhi) { set_error("rhi is needed at level 5"); return SALSA_ERR_ARG; }
+  for (int k = 0; k < ncat; ++k)
+    if (!n_p[k] || !n_t[k] || !m_p[k] || !m_t[k]) { set_error("null field pointer"); return SALSA_ERR_ARG; }
+  const double dz1 = dzt[c->nzp - 2];
+  if (!(dz1 != 0.)) { set_error("dzt(nzp-1) is zero"); return SALSA_ERR_ARG; }
+  LesArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nzp = c->nzp; a.nxp = c->nxp; a.nyp = c->nyp; a.level = P.level;
+  a.n3 = c->n3;
+  a.dtlt = dtlt;
+  a.dzf2 = dzt[c->nzp - 3] / dz1;                     // sclrset 'mixd' (util.f90:89-91)
+  const int nbc[4] = {d.nbins, d.ncld, d.nprc, d.nice};
+  const int nwet = d.iwa + 1;
+  for (int k = 0; k < 4; ++k) {
+    a.nb[k] = k < ncat ? nbc[k] : 0;
+    a.nsw[k] = k == 3 ? nwet + 1 : nwet;
+  }
+  a.ndry = d.iwa; a.nwet = nwet; a.iwa = d.iwa; a.irim = d.irim;
+  a.in1a = d.in1a; a.fn2b = d.fn2b; a.in2a = d.in2a; a.fn2a = d.fn2a;
+  a.ica_cur = d.ica.cur; a.ica_par = d.ica.par; a.fca_cur = d.fca.cur;
+  a.icb_cur = d.icb.cur; a.icb_par = d.icb.par; a.fcb_cur = d.fcb.cur;
+  for (int s = 0; s < SALSA_MAXSPEC; ++s) {
+    a.rholiq[s] = d.rholiq[s]; a.rhoice[s] = d.rhoice[s]; a.mm[s] = d.mm[s]; a.diss[s] = d.diss[s];
+  }
+  for (int b = 0; b < d.nbins; ++b) a.aerobins[b] = std::pow(d.vlolim[b] / 0.5235988, 1. / 3.);   // set_aerobins
+  a.temp = temp; a.rh = rh; a.rhi = rhi ? rhi : rh; a.rp = rp;
+  for (int k = 0; k < 4; ++k) {
+    a.n_p[k] = k < ncat ? n_p[k] : nullptr; a.n_t[k] = k < ncat ? n_t[k] : nullptr;
+    a.m_p[k] = k < ncat ? m_p[k] : nullptr; a.m_t[k] = k < ncat ? m_t[k] : nullptr;
+  }
+  a.charge_time = charge_time;
+  a.diag = diag;
+  const int rc = launch_les_update(a, c->stream);
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "LES update launch failed (%d)", rc); return SALSA_ERR_HIP; }
   return SALSA_OK;
 }

@@ -178,4 +178,24 @@ struct GradArgs {
   double *j_cell, *djde_cell;         // per interior cell: J and dJ/dE
 };
 
+// Arguments of the LES coupling passes after a step (salsa_les_update_device):
+// tend_constrain2, update_sclrs, tend0 and SALSA_diagnostics on the LES
+// arrays n(c)[b][n3], m(c)[s*nb+b][n3] of a grid with halo 1 in z and 2
+// laterally, owned whole by this process (periodic lateral halos).
+struct LesArgs {
+  int32_t nzp, nxp, nyp, level;
+  int64_t n3;
+  double dtlt, dzf2;
+  int32_t nb[4], nsw[4];              // bins and mass slots per category (ice: + rime)
+  int32_t ndry, nwet, iwa, irim;
+  int32_t in1a, fn2b, in2a, fn2a;
+  int32_t ica_cur, ica_par, fca_cur, icb_cur, icb_par, fcb_cur;
+  double rholiq[kMaxSpec], rhoice[kMaxSpec], mm[kMaxSpec], diss[kMaxSpec];
+  double aerobins[kMaxBin];           // lower dry-diameter limit of each aerosol bin
+  const double *temp, *rh, *rhi, *rp;
+  double *n_p[4], *n_t[4], *m_p[4], *m_t[4];
+  double *charge_time;                // [nbins+ncld+nprc][n3] or null
+  double *diag;                       // [6][n3]: rc, srp, snrp, rtot, ri, riri
+};
+
 }  // namespace salsa

@@ -164,6 +164,9 @@ def lib():
     L.salsa_gradient_device.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [_P4, _P4, C.c_double, C.c_int32,
                                                                           _P4, C.c_void_p, C.c_void_p, _dp]
     L.salsa_gradient_device.restype = C.c_int
+    L.salsa_les_update_device.argtypes = [C.c_void_p, C.c_double, _dp] + [C.c_void_p] * 4 + [_P4] * 4 + \
+        [C.c_void_p, C.c_void_p]
+    L.salsa_les_update_device.restype = C.c_int
     L.salsa_last_error.argtypes = []
     L.salsa_last_error.restype = C.c_char_p
     _lib = L
@@ -303,6 +306,20 @@ class Context:
             C.cast(tot, _dp)))
         _check(rc)
         return tot[0], tot[1]
+
+    def les_update_device(self, dtlt, dzt, f, n_p, n_t, m_p, m_t, diag, charge_time=None):
+        """tend_constrain2 + update_sclrs + tend0 + SALSA_diagnostics on the
+        device fields (salsa_les_update_device).  dzt: host float64[nzp];
+        f: dict with "temp", "rh", "rp" and (level 5) "rhi" tensors; n_p,
+        m_p updated in place, n_t, m_t zeroed; diag: float64 tensor [6, n3]."""
+        P = lambda xs: _P4(*[_ptr(x) for x in (list(xs) + [None] * (4 - len(xs)))])
+        dz = np.ascontiguousarray(dzt, dtype=np.float64)
+        if dz.size != self.shape[0]:
+            raise ValueError("dzt must have nzp values")
+        rc = self._ordered(lambda: lib().salsa_les_update_device(
+            self._h, dtlt, C.cast(dz.ctypes.data, _dp), _ptr(f["temp"]), _ptr(f["rh"]), _ptr(f.get("rhi")),
+            _ptr(f["rp"]), P(n_p), P(n_t), P(m_p), P(m_t), _ptr(charge_time), _ptr(diag)))
+        _check(rc)
 
     def cell_flags(self):
         out = np.zeros(self.ncells, dtype=np.uint8)
