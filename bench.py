@@ -180,6 +180,47 @@ def gradient_leg(capi, W, desc, nsteps=20):
     return res
 
 
+def les_leg(capi, desc, f, fd, n_p, m_p, calls=20):
+    """The LES side after a step on device (salsa_les_update_device:
+    tend_constrain2, update_sclrs, tend0, SALSA_diagnostics) on the same
+    fields: one step's tendencies, then `calls` updates timed with HIP events
+    on the context stream.  Algorithmic bytes per call: every tracer element
+    read and written with its tendency in the update pass (32 B) and read in
+    the diagnostics pass (8 B); per cell the temperature, humidity and vapour
+    read (24 B) and four diagnostic sums written (32 B)."""
+    import numpy as np
+    import torch
+    g = f.grid
+    n3 = g.nzp * g.nxp * g.nyp
+    np_ = [x.clone() for x in n_p]
+    mp_ = [x.clone() for x in m_p]
+    nt = [torch.zeros_like(x) for x in np_]
+    mt = [torch.zeros_like(x) for x in mp_]
+    rvt = torch.zeros_like(fd["rv"])
+    ctx = capi.Context(desc, capi.params(level=4), g.nzp, g.nxp, g.nyp)
+    ctx.step_device(fd, np_, mp_, nt, mt, rvt)
+    lf = dict(temp=fd["temp"], rh=fd["rv"] / fd["rsl"], rp=fd["rv"])
+    diag = torch.zeros((6, n3), dtype=torch.float64, device=fd["temp"].device)
+    dzt = 1. / np.full(g.nzp, 20.)
+    ctx.les_update_device(1.0, dzt, lf, np_, nt, mp_, mt, diag)
+    ctx.sync()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(calls):
+        ctx.les_update_device(1.0, dzt, lf, np_, nt, mp_, mt, diag)
+    e1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / calls
+    ctx.close()
+    elems = sum(x.numel() for x in np_ + mp_)
+    nbytes = 40 * elems + 56 * n3
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"ms_per_call": ms, "cells": n3, "tracer_elements": elems, "bytes_per_call": nbytes,
+            "achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS, "bound": "hbm"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +229,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--no-les", action="store_true")
     ap.add_argument("--counter-run", choices=("mixed", "fp64", "copy"))
     ap.add_argument("--lanes", default="16,16,16")
     args = ap.parse_args()
@@ -302,6 +344,8 @@ def main():
              "V_SO4": rel(a[6][:, :, 0], b[6][:, :, 0], 1e-24), "V_H2O": rel(a[6][:, :, 1], b[6][:, :, 1], 1e-24)}
         out["accuracy_vs_fp64"] = {k: {"max": float(v.max()), "frac_gt_1e-5": float(np.mean(v > 1e-5))}
                                    for k, v in e.items()}
+    if not args.no_les:
+        out["les_update"] = les_leg(capi, desc, f, fd, n_p, m_p)
     if not args.no_gradient:
         out["gradient"] = gradient_leg(capi, W, desc)
     out["setup_s"] = gen_s

@@ -286,7 +286,7 @@ int salsa_gradient_device(salsa_ctx *ctx, const double *press, const double *tem
  *   charge_time: NULL, or a_chargeTimep [nbins+ncld+nprc][n3]: zeroed where
  *                the reference zeroes it (lcharge)
  *   diag       : [6][n3] rc, srp, snrp, rtot, ri, riri (rows 4-5 at level 5)
- * Needs hz = 1, hxy = 2.  Not carried: the gas-phase transfer of tiny
+ * Needs hz = 1, hxy = 2, nzp >= 4, nxp and nyp >= 6.  Not carried: the gas-phase transfer of tiny
  * aerosols (lscndgas) and the a_indefp clamp (ice_theta_dist).  Device
  * pointers; work is enqueued on the context stream like salsa_step_device.
  */

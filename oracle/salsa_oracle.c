@@ -1936,7 +1936,7 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
   const int nsw[4] = {nwet, nwet, nwet, L->nspec};
   const int nx = nxp - 4, ny = nyp - 4;
   const double massTH = 1.e-25;
-  if (nzp < 4 || nx < 1 || ny < 1) return -1;
+  if (nzp < 4 || nx < 2 || ny < 2) return -1;     /* the periodic images need two interior columns */
 #define IDX(k, i, j) ((long)(k) + (long)nzp * ((long)(i) + (long)nxp * (long)(j)))
   /* tend_constrain2, update (interior), tend0 */
   for (int c = 0; c < ncat; ++c) {

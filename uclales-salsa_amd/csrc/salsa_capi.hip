@@ -594,7 +594,10 @@ extern "C" int salsa_les_update_device(salsa_ctx *c, double dtlt, const double *
     set_error("the LES update needs the LES halo widths (hz 1, hxy 2)");
     return SALSA_ERR_ARG;
   }
-  if (c->nzp < 4 || c->nxp < 5 || c->nyp < 5) { set_error("grid too small for the LES update"); return SALSA_ERR_ARG; }
+  if (c->nzp < 4 || c->nxp < 6 || c->nyp < 6) {      // two interior columns per halo width for the periodic images
+    set_error("grid too small for the LES update (nzp >= 4, nxp and nyp >= 6)");
+    return SALSA_ERR_ARG;
+  }
   if (!(dtlt > 0.) || !dzt || !temp || !rh || !rp || !diag || !n_p || !n_t || !m_p || !m_t) {
     set_error("null argument or non-positive dtlt");
     return SALSA_ERR_ARG;

@@ -54,24 +54,54 @@ __global__ void __launch_bounds__(256) k_les_update(LesArgs a, double *__restric
   t[e] = 0.;
 }
 
-// pass B: boundary rows and halo columns of one array of nrow rows
-__global__ void __launch_bounds__(256) k_les_bounds(LesArgs a, double *__restrict__ p, int nrow) {
-  const int64_t ncol = (int64_t)a.nxp * a.nyp;
-  const int nx = a.nxp - 4, ny = a.nyp - 4, nz = a.nzp;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)nrow * ncol;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e / ncol, col = e % ncol;
-    const int i = (int)(col % a.nxp), j = (int)(col / a.nxp);
-    const int ii = i < 2 ? i + nx : (i > a.nxp - 3 ? i - nx : i);
-    const int jj = j < 2 ? j + ny : (j > a.nyp - 3 ? j - ny : j);
-    double *dst = p + r * a.n3 + col * nz;
-    const double *src = p + r * a.n3 + ((int64_t)ii + (int64_t)a.nxp * jj) * nz;
-    if (!interior_col(a, i, j))
-      for (int k = 1; k <= nz - 2; ++k) dst[k] = src[k];
-    const double s1 = src[1], s2 = src[2], sm = src[nz - 2], sm2 = src[nz - 3];
-    dst[0] = s1 - 0. * (s2 - s1);
-    dst[nz - 1] = sm + a.dzf2 * (sm - sm2);
+// pass B, halo columns: one thread per (halo column, k), blockIdx.y the
+// row; k fastest, so a wavefront reads and writes along one column.  The
+// halo columns are the 2 rows of columns at each y edge (all x) followed by
+// the 2 columns at each x edge of the interior y range.
+__global__ void __launch_bounds__(256) k_les_halo(LesArgs a, double *__restrict__ p) {
+  const int nz = a.nzp, nxp = a.nxp, nyp = a.nyp, nx = nxp - 4, ny = nyp - 4;
+  const int nhc = 4 * nxp + 4 * ny;
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nhc * nz) return;
+  const int hc = h / nz, k = h % nz;
+  int i, j;
+  if (hc < 4 * nxp) {
+    const int jq = hc / nxp;
+    j = jq < 2 ? jq : nyp - 4 + jq;
+    i = hc % nxp;
+  } else {
+    const int q = (hc - 4 * nxp) % 4;
+    j = 2 + (hc - 4 * nxp) / 4;
+    i = q < 2 ? q : nxp - 4 + q;
   }
+  const int ii = i < 2 ? i + nx : (i > nxp - 3 ? i - nx : i);
+  const int jj = j < 2 ? j + ny : (j > nyp - 3 ? j - ny : j);
+  double *row = p + (int64_t)blockIdx.y * a.n3;
+  const double *src = row + ((int64_t)ii + (int64_t)nxp * jj) * nz;
+  double v;
+  if (k == 0) {
+    const double s1 = src[1], s2 = src[2];
+    v = s1 - 0. * (s2 - s1);
+  } else if (k == nz - 1) {
+    const double sm = src[nz - 2], sm2 = src[nz - 3];
+    v = sm + a.dzf2 * (sm - sm2);
+  } else {
+    v = src[k];
+  }
+  row[((int64_t)i + (int64_t)nxp * j) * nz + k] = v;
+}
+
+// pass B, interior columns: bottom and top rows (sclrset 'mixd'), one thread
+// per column, blockIdx.y the row
+__global__ void __launch_bounds__(256) k_les_rows(LesArgs a, double *__restrict__ p) {
+  const int nz = a.nzp, nx = a.nxp - 4, ny = a.nyp - 4;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = 2 + c % nx, j = 2 + c / nx;
+  double *col = p + (int64_t)blockIdx.y * a.n3 + ((int64_t)i + (int64_t)a.nxp * j) * nz;
+  const double s1 = col[1], s2 = col[2], sm = col[nz - 2], sm2 = col[nz - 3];
+  col[0] = s1 - 0. * (s2 - s1);
+  col[nz - 1] = sm + a.dzf2 * (sm - sm2);
 }
 
 // findDry4Wet for an evaporating bin (number ne, dry masses md[s * nbev])
@@ -119,17 +149,22 @@ __global__ void __launch_bounds__(256) k_les_diag(LesArgs a) {
   const int nwet = a.nwet, ndry = a.ndry, iwa = a.iwa, irim = a.irim;
   const int64_t n3 = a.n3;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n3; x += (int64_t)gridDim.x * blockDim.x) {
-    // remove negative values (:86-97)
-    for (int c = 0; c < ncat; ++c) {
-      for (int b = 0; b < a.nb[c]; ++b) {
-        double *q = a.n_p[c] + (int64_t)b * n3 + x;
-        *q = fmax(0., *q);
+    // remove negative values (:86-97); only values that change are written.
+    // Rows are loaded 8 at a time before any store so that the loads are in
+    // flight together.
+    for (int c = 0; c < ncat; ++c)
+      for (int q = 0; q < 2; ++q) {
+        double *base = (q ? a.m_p[c] : a.n_p[c]) + x;
+        const int nrow = q ? a.nb[c] * a.nsw[c] : a.nb[c];
+        for (int r0 = 0; r0 < nrow; r0 += 8) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = r0 + u < nrow ? base[(int64_t)(r0 + u) * n3] : 0.;
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (r0 + u < nrow && !(v[u] >= 0.)) base[(int64_t)(r0 + u) * n3] = fmax(0., v[u]);
+        }
       }
-      for (int r = 0; r < a.nb[c] * a.nsw[c]; ++r) {
-        double *q = a.m_p[c] + (int64_t)r * n3 + x;
-        *q = fmax(0., *q);
-      }
-    }
     const int64_t col = x / a.nzp;
     const int i = (int)(col % a.nxp), j = (int)(col / a.nxp);
     if (interior_col(a, i, j)) {
@@ -293,11 +328,15 @@ int launch_les_update(const LesArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_les_update, dim3(gx, a.nb[c]), dim3(256), 0, s, a, a.n_p[c], a.n_t[c]);
     hipLaunchKernelGGL(k_les_update, dim3(gx, a.nb[c] * a.nsw[c]), dim3(256), 0, s, a, a.m_p[c], a.m_t[c]);
   }
-  const int64_t ncol = (int64_t)a.nxp * a.nyp;
+  const unsigned gh = (unsigned)(((4 * a.nxp + 4 * (a.nyp - 4)) * a.nzp + 255) / 256);
+  const unsigned gr = (unsigned)(((a.nxp - 4) * (a.nyp - 4) + 255) / 256);
   for (int c = 0; c < ncat; ++c) {
-    hipLaunchKernelGGL(k_les_bounds, dim3(grid_for(a.nb[c] * ncol, 256)), dim3(256), 0, s, a, a.n_p[c], a.nb[c]);
-    hipLaunchKernelGGL(k_les_bounds, dim3(grid_for((int64_t)a.nb[c] * a.nsw[c] * ncol, 256)), dim3(256), 0, s, a,
-                       a.m_p[c], a.nb[c] * a.nsw[c]);
+    for (int q = 0; q < 2; ++q) {
+      double *arr = q ? a.m_p[c] : a.n_p[c];
+      const unsigned nrow = (unsigned)(q ? a.nb[c] * a.nsw[c] : a.nb[c]);
+      hipLaunchKernelGGL(k_les_halo, dim3(gh, nrow), dim3(256), 0, s, a, arr);
+      hipLaunchKernelGGL(k_les_rows, dim3(gr, nrow), dim3(256), 0, s, a, arr);
+    }
   }
   hipLaunchKernelGGL(k_les_diag, dim3(grid_for(a.n3, 256)), dim3(256), 0, s, a);
   return (int)hipGetLastError();
