@@ -35,9 +35,9 @@ def run(O, desc, level, case, charge_time=None):
 
 
 def one_cell(desc, level=4, nz=2):
-    """A 1x1 interior column (nzp = nz + 2, 5x5 columns incl. halos) with every
+    """2x2 interior columns (nzp = nz + 2, 6x6 columns incl. halos) with every
     field zero; rh 0.9, T 280 K."""
-    shape = (nz + 2, 5, 5)
+    shape = (nz + 2, 6, 6)
     n3 = int(np.prod(shape))
     ncat = 4 if level == 5 else 3
     nb = [desc.nbins, desc.ncld, desc.nprc, desc.nice][:ncat]
