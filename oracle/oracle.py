@@ -105,6 +105,12 @@ def lib():
         L.so_les_update.restype = C.c_int
         L.so_les_corr.argtypes = [dp, dp, C.c_int]
         L.so_les_corr.restype = C.c_double
+        P3 = C.c_void_p * 3
+        L.so_sedim.argtypes = [C.POINTER(Layout), C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, dp, dp, dp,
+                               P3, P3, P3, P3, dp, dp, dp, dp, C.c_int, C.c_int, C.c_int]
+        L.so_sedim.restype = C.c_int
+        L.so_terminal_vel.argtypes = [C.c_double] * 5
+        L.so_terminal_vel.restype = C.c_double
         _lib = L
     return _lib
 
@@ -228,3 +234,29 @@ def les_corr(x, y):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
     return lib().so_les_corr(_dp(x), _dp(y), len(x))
+
+
+def sedim(L, shape, dtlt, dzt, tk, th, adn, ustar, n_p, m_p, n_t, m_t, tlt, sed=(1, 1, 1)):
+    """so_sedim on LES arrays (level 4: aerosol, cloud, precipitation).
+    n_t, m_t, tlt are added to in place.  Returns (rrate[n3],
+    sfcrrate[nxp*nyp], vt_prc[nprc, n3])."""
+    nzp, nxp, nyp = shape
+    n3 = nzp * nxp * nyp
+    P3 = C.c_void_p * 3
+
+    def ptrs(lst):
+        for a in lst:
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        return P3(*[a.ctypes.data for a in lst])
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (dzt, tk, th, adn, ustar)]
+    assert tlt.dtype == np.float64 and tlt.flags["C_CONTIGUOUS"]
+    rr, sr, vt = np.zeros(n3), np.zeros(nxp * nyp), np.zeros((L.nprc, n3))
+    rc = lib().so_sedim(C.byref(L), nzp, nxp, nyp, dtlt, *(_dp(a) for a in arrs), ptrs(n_p[:3]), ptrs(m_p[:3]),
+                        ptrs(n_t[:3]), ptrs(m_t[:3]), _dp(tlt), _dp(rr), _dp(sr), _dp(vt), *sed)
+    if rc != 0:
+        raise ValueError("so_sedim failed: %d" % rc)
+    return rr, sr, vt
+
+
+def terminal_vel(diam, rhop, rhoa, visc, beta):
+    return lib().so_terminal_vel(diam, rhop, rhoa, visc, beta)

@@ -2142,3 +2142,224 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
 #undef IDX
   return 0;
 }
+
+/* ====================================================================== */
+/* Sedimentation of the liquid categories at level 4 (sedim_SALSA,        */
+/* mcrp.f90:745-905): DepositionSlow for aerosols and cloud droplets      */
+/* (:910-1041, with the dry deposition velocity at the lowest level) and  */
+/* DepositionFast for precipitation (:1045-1283, fall distance per bin    */
+/* and time step, with the reference's numerical-diffusion switch);       */
+/* calcDiamLES and terminal_vel (mo_particle_external_properties.f90:     */
+/* 43-100, 169-201) for liquid particles.  Constants of defs.f90: pi with */
+/* full precision, kb, g = 9.81, cp = 1005, alvl = 2.5e6; of mo_submctl:  */
+/* pi6, grav, surfw0.                                                     */
+/* ====================================================================== */
+static const double sd_pi = 3.14159265358979323846, sd_kb = 1.3807e-23, sd_g = 9.81, sd_cp = 1005.,
+                    sd_alvl = 2.5e6;
+
+static double sd_terminal_vel(double diam, double rhop, double rhoa, double visc, double beta) {
+  if (diam <= 19e-6) return (diam * diam) * (rhop - rhoa) * grav * beta / (18. * visc);
+  if (diam > 19e-6 && diam < 1.07e-3) {
+    const double NDa = 4. / 3. * pow(diam, 3.) * grav * (rhop - rhoa) * rhoa / pow(visc, 2.);
+    const double X = log(NDa);
+    const double X2 = X * X, X3 = X2 * X, X4 = X2 * X2, X5 = X4 * X, X6 = X3 * X3;
+    const double Y = -0.318657e1 + 0.992696 * X - 0.153193e-2 * X2 - 0.987059e-3 * X3 - 0.578878e-3 * X4 +
+                     0.855176e-4 * X5 - 0.327815e-5 * X6;
+    const double Re2 = beta * exp(Y);
+    return visc / rhoa / diam * Re2;
+  }
+  const double Np = pow(surfw0, 3.) * pow(rhoa, 2.) / (pow(visc, 4.) * (rhop - rhoa) * grav);
+  const double Bo = 4. / 3. * (rhop - rhoa) * grav / surfw0 * pow(diam, 2.);
+  const double X3 = log(Bo * pow(Np, 1. / 6.));
+  const double Z2 = X3 * X3, Z3 = Z2 * X3, Z4 = Z2 * Z2, Z5 = Z4 * X3;
+  const double Y3 = -0.500015e1 + 0.523778e1 * X3 - 0.204914e1 * Z2 + 0.475294 * Z3 - 0.542819e-1 * Z4 +
+                    0.238449e-2 * Z5;
+  const double Re3 = pow(Np, 1. / 6.) * exp(Y3);
+  return fmin(visc / rhoa / diam * Re3, 9.2);
+}
+
+double so_terminal_vel(double diam, double rhop, double rhoa, double visc, double beta) {
+  return sd_terminal_vel(diam, rhop, rhoa, visc, beta);
+}
+
+/* air properties of a level: viscosity (atm. modelling Eq. 4.54) and the
+   mean free path */
+static void sd_air(double tk, double adn, double *avis, double *lambda) {
+  *avis = 1.8325e-5 * (416.16 / (tk + 120.0)) * pow(tk / 296.16, 1.5);
+  const double va = sqrt(8 * sd_kb * tk / (sd_pi * 4.8096e-26));
+  *lambda = 2 * (*avis) / (adn * va);
+}
+
+/* wet diameter, bulk density and Cunningham-corrected terminal velocity
+   of one bin (the bin's ns masses at m[s*mstride]) */
+static double sd_vc(const so_layout *L, int ns, double numc, const double *m, long mstride, double adn, double avis,
+                    double lambda) {
+  double v = 0., ms = 0.;
+  for (int s = 0; s < ns; ++s) v = v + m[s * mstride] / L->rholiq[s];
+  for (int s = 0; s < ns; ++s) ms = ms + m[s * mstride];
+  const double dwet = numc < 1.e-15 ? 0. : pow(v / numc / pi6, 1. / 3.);
+  const double zdn = ms / numc / (pi6 * (dwet * dwet * dwet));
+  const double Kn = 2. * lambda / dwet;
+  const double GG = 1. + Kn * (1.249 + 0.42 * exp(-0.87 / Kn));
+  return sd_terminal_vel(dwet, zdn, adn, avis, GG);
+}
+
+int so_sedim(const so_layout *L, int nzp, int nxp, int nyp, double dtlt, const double *dzt, const double *tk,
+             const double *th, const double *adn, const double *ustar, double *const n_p[3], double *const m_p[3],
+             double *const n_t[3], double *const m_t[3], double *tlt, double *rrate, double *sfcrrate, double *vt_prc,
+             int sed_aero, int sed_cloud, int sed_precp) {
+  const long n3 = (long)nzp * nxp * nyp;
+  const int ns = L->nwet, iw = L->iwa;
+  const int nbs[3] = {L->nbins, L->ncld, L->nprc};
+  if (nzp < 4 || nxp < 5 || nyp < 5) return -1;
+#define IX(k, i, j) ((long)(k) + (long)nzp * ((long)(i) + (long)nxp * (long)(j)))
+  for (long x = 0; x < n3; ++x) rrate[x] = 0.;
+  for (long c = 0; c < (long)nxp * nyp; ++c) sfcrrate[c] = 0.;
+  for (long x = 0; x < (long)L->nprc * n3; ++x) vt_prc[x] = 0.;
+  /* slow deposition: aerosols (flag 1), cloud droplets (flag 2) */
+  for (int cat = 0; cat < 2; ++cat) {
+    if (!(cat == 0 ? sed_aero : sed_cloud)) continue;
+    const int nb = nbs[cat];
+    double *N = n_p[cat], *M = m_p[cat];
+    double rfln[SO_MAXBIN], rflm[SO_MAXBIN * SO_MAXSPEC], rfln1[SO_MAXBIN], rflm1[SO_MAXBIN * SO_MAXSPEC];
+    for (int j = 2; j <= nyp - 3; ++j)
+      for (int i = 2; i <= nxp - 3; ++i) {
+        for (int q = 0; q < nb * ns; ++q) rflm1[q] = 0.;
+        for (int b = 0; b < nb; ++b) rfln1[b] = 0.;
+        for (int k = nzp - 2; k >= 1; --k) {          /* 1-based k = nzp-1 .. 2 */
+          const long x = IX(k, i, j), x1 = IX(k + 1, i, j);
+          double avis, lambda;
+          sd_air(tk[x], adn[x], &avis, &lambda);
+          const double kvis = avis / adn[x];
+          for (int q = 0; q < nb * ns; ++q) rflm[q] = 0.;
+          for (int b = 0; b < nb; ++b) rfln[b] = 0.;
+          for (int b = 0; b < nb; ++b) {
+            const double zpn = N[(long)b * n3 + x];
+            if (zpn < nlim_c) continue;
+            const double *mb = M + (long)b * n3 + x;
+            double vc;
+            {
+              double v = 0., ms = 0.;
+              for (int s = 0; s < ns; ++s) v = v + mb[(long)s * nb * n3] / L->rholiq[s];
+              for (int s = 0; s < ns; ++s) ms = ms + mb[(long)s * nb * n3];
+              const double dwet = zpn < 1.e-15 ? 0. : pow(v / zpn / pi6, 1. / 3.);
+              const double zdn = ms / zpn / (pi6 * (dwet * dwet * dwet));
+              const double Kn = 2. * lambda / dwet;
+              const double GG = 1. + Kn * (1.249 + 0.42 * exp(-0.87 / Kn));
+              vc = sd_terminal_vel(dwet, zdn, adn[x], avis, GG);
+              vc = fmin(vc, fmin(0.5 * (1. / dzt[k]) / dtlt, 2.0));
+              if (k == 1) {                           /* the level just above the surface */
+                const double mdiff = (sd_kb * tk[x] * GG) / (3.0 * sd_pi * avis * dwet);
+                const double Sc = kvis / mdiff;
+                double St = vc * pow(ustar[(long)i + (long)nxp * j], 2.0) / sd_g * kvis;
+                if (St < 0.01) St = 0.01;
+                const double rt = 1.0 / fmax(eps_r8, (ustar[(long)i + (long)nxp * j] *
+                                                      (pow(Sc, -2.0 / 3.0) + pow(10., -3.0 / St))));
+                vc = (1. / rt) + vc;
+              }
+            }
+            for (int s = 0; s < ns; ++s) rflm[s * nb + b] = -mb[(long)s * nb * n3] * vc;
+            rfln[b] = -zpn * vc;
+          }
+          double sw = 0.;
+          for (int q = ns * nb - nb - 1; q < ns * nb; ++q) sw = sw + rflm[q];   /* (ns-1)*nb : ns*nb, as written */
+          rrate[x] = rrate[x] - sw * adn[x] * sd_alvl;
+          for (int b = 0; b < nb; ++b) {
+            const double dn = (rfln1[b] * adn[x1] - rfln[b] * adn[x]) * dzt[k] / adn[x];
+            n_t[cat][(long)b * n3 + x] = n_t[cat][(long)b * n3 + x] - dn;
+            for (int s = 0; s < ns; ++s) {
+              const double dm = (rflm1[s * nb + b] * adn[x1] - rflm[s * nb + b] * adn[x]) * dzt[k] / adn[x];
+              m_t[cat][((long)s * nb + b) * n3 + x] = m_t[cat][((long)s * nb + b) * n3 + x] - dm;
+            }
+          }
+          /* SUM over the water rows in bin order (mcrp.f90:808-813) */
+          {
+            double sdw = 0.;
+            for (int b = 0; b < nb; ++b)
+              sdw = sdw + (rflm1[iw * nb + b] * adn[x1] - rflm[iw * nb + b] * adn[x]) * dzt[k] / adn[x];
+            tlt[x] = tlt[x] + sdw * (sd_alvl / sd_cp) * th[x] / tk[x];
+          }
+          if (k == 1) {
+            double s2 = 0.;
+            for (int q = ns * nb - nb - 1; q < ns * nb; ++q) s2 = s2 + rflm[q];
+            sfcrrate[(long)i + (long)nxp * j] = sfcrrate[(long)i + (long)nxp * j] - s2 * adn[x] * sd_alvl;
+          }
+          for (int q = 0; q < nb * ns; ++q) rflm1[q] = rflm[q];
+          for (int b = 0; b < nb; ++b) rfln1[b] = rfln[b];
+        }
+      }
+  }
+  /* fast deposition: precipitation (flag 3) */
+  if (sed_precp) {
+    const int nb = nbs[2];
+    double *N = n_p[2], *M = m_p[2];
+    double prn[SO_MAXBIN][256], prv[SO_MAXBIN * SO_MAXSPEC][256], rem[SO_MAXBIN * SO_MAXSPEC];
+    if (nzp > 256) return -1;
+    for (int j = 2; j <= nyp - 3; ++j)
+      for (int i = 2; i <= nxp - 3; ++i) {
+        for (int b = 0; b < nb; ++b)
+          for (int k = 0; k < nzp; ++k) prn[b][k] = 0.;
+        for (int q = 0; q < nb * ns; ++q) {
+          for (int k = 0; k < nzp; ++k) prv[q][k] = 0.;
+          rem[q] = 0.;
+        }
+        for (int k = nzp - 2; k >= 1; --k) {
+          const long x = IX(k, i, j);
+          double avis, lambda;
+          sd_air(tk[x], adn[x], &avis, &lambda);
+          for (int b = 0; b < nb; ++b) {
+            const double zpn = N[(long)b * n3 + x];
+            if (zpn < prlim_c) continue;
+            const double *mb = M + (long)b * n3 + x;
+            const double vc = sd_vc(L, ns, zpn, mb, (long)nb * n3, adn[x], avis, lambda);
+            vt_prc[(long)b * n3 + x] = vc;
+            const double fdmax = dtlt * vc;
+            double fd = 0.;
+            int fi = 0, dep = 0;
+            while (fd < fdmax) {
+              fd = fd + (1. / dzt[k - fi]);
+              fi = fi + 1;
+              if (k - fi <= 0) { dep = 1; break; }     /* 1-based k-fi <= 1 */
+            }
+            fi = fi - 1;
+            const int kf = k - fi;
+            fd = fd - (1. / dzt[kf]);
+            const double fdos = fmin(fmax(fdmax - fd, 0.), 1. / dzt[kf]);
+            prn[b][k] = prn[b][k] - zpn;
+            for (int s = 0; s < ns; ++s) prv[s * nb + b][k] = prv[s * nb + b][k] - mb[(long)s * nb * n3];
+            if (dep)
+              for (int s = 0; s < ns; ++s) rem[s * nb + b] = rem[s * nb + b] + mb[(long)s * nb * n3] * adn[x] * vc;
+            rrate[x] = rrate[x] + mb[(long)iw * nb * n3] * adn[x] * vc * sd_alvl;
+            if (fdos * dzt[kf] > 0.5) {
+              prn[b][kf - 1] = prn[b][kf - 1] + zpn * dzt[kf - 1] / dzt[k];
+              for (int s = 0; s < ns; ++s)
+                prv[s * nb + b][kf - 1] = prv[s * nb + b][kf - 1] + mb[(long)s * nb * n3] * dzt[kf - 1] / dzt[k];
+            } else {
+              prn[b][kf - 1] = prn[b][kf - 1] + (fdos * dzt[kf]) * zpn * dzt[kf - 1] / dzt[kf];
+              prn[b][kf] = prn[b][kf] + (1. - fdos * dzt[kf]) * zpn;
+              for (int s = 0; s < ns; ++s) {
+                prv[s * nb + b][kf - 1] =
+                    prv[s * nb + b][kf - 1] + (fdos * dzt[kf]) * mb[(long)s * nb * n3] * dzt[kf - 1] / dzt[kf];
+                prv[s * nb + b][kf] = prv[s * nb + b][kf] + (1. - fdos * dzt[kf]) * mb[(long)s * nb * n3];
+              }
+            }
+          }
+        }
+        double s2 = 0.;
+        for (int q = ns * nb - nb - 1; q < ns * nb; ++q) s2 = s2 + rem[q];
+        sfcrrate[(long)i + (long)nxp * j] = sfcrrate[(long)i + (long)nxp * j] + s2 * sd_alvl;
+        for (int k = 0; k < nzp; ++k) {
+          const long x = IX(k, i, j);
+          for (int b = 0; b < nb; ++b) n_t[2][(long)b * n3 + x] = n_t[2][(long)b * n3 + x] + prn[b][k] / dtlt;
+          for (int q = 0; q < nb * ns; ++q) m_t[2][(long)q * n3 + x] = m_t[2][(long)q * n3 + x] + prv[q][k] / dtlt;
+          if (k <= nzp - 2) {                          /* 1-based k = 1 .. nzp-1 */
+            double sw = 0.;
+            for (int b = 0; b < nb; ++b) sw = sw + prv[iw * nb + b][k];
+            tlt[x] = tlt[x] - (sw / dtlt) * (sd_alvl / sd_cp) * th[x] / tk[x];
+          }
+        }
+      }
+  }
+#undef IX
+  return 0;
+}

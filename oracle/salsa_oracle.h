@@ -182,6 +182,17 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
                   double *const n_p[4], double *const n_t[4], double *const m_p[4], double *const m_t[4],
                   double *charge_time, double *diag);
 double so_les_corr(const double *x, const double *y, int n);
+/* Sedimentation at level 4 (sedim_SALSA, mcrp.f90:745-905, DepositionSlow
+ * :910-1041 for aerosols and cloud, DepositionFast :1045-1283 for
+ * precipitation): adds to the tendencies n_t/m_t and tlt, writes rrate
+ * [n3], sfcrrate [nxp*nyp] and vt_prc [nprc][n3] (zeroed first).  dzt, tk,
+ * th, adn: LES fields; ustar [nxp*nyp].  Returns 0, or -1 for a grid too
+ * small or nzp > 256. */
+int so_sedim(const so_layout *L, int nzp, int nxp, int nyp, double dtlt, const double *dzt, const double *tk,
+             const double *th, const double *adn, const double *ustar, double *const n_p[3], double *const m_p[3],
+             double *const n_t[3], double *const m_t[3], double *tlt, double *rrate, double *sfcrrate, double *vt_prc,
+             int sed_aero, int sed_cloud, int sed_precp);
+double so_terminal_vel(double diam, double rhop, double rhoa, double visc, double beta);
 
 #ifdef __cplusplus
 }

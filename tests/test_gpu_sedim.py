@@ -1,0 +1,77 @@
+"""salsa_sedim_device (level-4 sedimentation: DepositionSlow for aerosols and
+cloud droplets, DepositionFast for precipitation; mcrp.f90:745-1283) against
+the oracle's so_sedim on the same fields.  The fall speeds go through pow,
+exp and log, whose device and host results may differ in the last bit, and
+the tendencies are differences of level fluxes: the bar is 1e-10 of each
+quantity's scale in its column (the fields themselves for the tendencies).
+Fall-distance decisions compare such speeds with level depths; none of the
+cases here sits within rounding of a decision."""
+import numpy as np
+import pytest
+
+from test_oracle_sedim import sed_fields, run
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def capi():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need an MI355X"
+    from salsa_amd import capi as c
+    c.lib()
+    return c
+
+
+@pytest.mark.parametrize("seed,grid,sed", [(1, (20, 8, 8), (1, 1, 1)), (2, (40, 12, 10), (1, 1, 1)),
+                                           (3, (20, 8, 8), (0, 0, 1)), (4, (20, 8, 8), (1, 0, 0))])
+def test_sedim_matches_oracle(capi, oracle_mod, seed, grid, sed):
+    import torch
+    desc = capi.bin_desc()
+    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields(desc, grid, seed)
+    nzp, nxp, nyp = shape
+    n3 = nzp * nxp * nyp
+    print("case sedimentation, seed %d, grid %s, switches %s" % (seed, shape, sed), flush=True)
+    tlt = np.random.default_rng(seed).normal(size=n3) * 1e-4
+    ont, omt, otlt = [a.copy() for a in n_t], [a.copy() for a in m_t], tlt.copy()
+    rr, sr, vt = run(oracle_mod, desc, shape, dzt, fld, n_p, m_p, ont, omt, otlt, sed=sed)
+    dev = torch.device("cuda")
+    D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    dn, dm, dnt, dmt = ([D(a) for a in x] for x in (n_p, m_p, n_t, m_t))
+    df = {k: D(v) for k, v in fld.items()}
+    dtlt, drr, dsr, dvt = D(tlt), torch.zeros(n3, dtype=torch.float64, device=dev), \
+        torch.zeros(nxp * nyp, dtype=torch.float64, device=dev), torch.zeros((desc.nprc, n3), dtype=torch.float64,
+                                                                             device=dev)
+    ctx = capi.Context(desc, capi.params(level=4), nzp, nxp, nyp)
+    ctx.sedim_device(1.0, dzt, df, dn, dm, dnt, dmt, dtlt, drr, dsr, dvt, sed=sed)
+    ctx.sync()
+    ctx.close()
+
+    def close(name, got, want, scale):
+        err = np.abs(got - want)
+        bad = err > 1e-10 * scale + 1e-300
+        print("  %s: max err/scale %.2e" % (name, float((err / np.maximum(scale, 1e-300)).max())), flush=True)
+        assert not bad.any(), (name, np.argwhere(bad)[:5])
+
+    colmax = lambda a: np.broadcast_to(np.abs(a).reshape(-1, nyp * nxp, nzp).max(axis=2, keepdims=True),
+                                       (a.reshape(-1, nyp * nxp, nzp).shape)).reshape(a.shape)
+    for c in range(3):
+        close("n_t[%d]" % c, dnt[c].cpu().numpy(), ont[c], colmax(n_p[c]) + np.abs(n_t[c]))
+        close("m_t[%d]" % c, dmt[c].cpu().numpy(), omt[c], colmax(m_p[c]) + np.abs(m_t[c]))
+    close("tlt", dtlt.cpu().numpy(), otlt, np.abs(otlt) + np.abs(otlt - tlt).max())
+    close("rrate", drr.cpu().numpy(), rr, np.abs(rr).max() + 1e-300)
+    close("sfcrrate", dsr.cpu().numpy(), sr, np.abs(sr).max() + 1e-300)
+    close("VtPrc", dvt.cpu().numpy(), vt, np.abs(vt) + 1e-300)
+    if sed[2]:
+        assert vt.max() > 0.5 and np.abs(ont[2] - n_t[2]).max() > 0
+
+
+def test_sedim_refuses_level5(capi):
+    import torch
+    desc = capi.bin_desc(species=("SO4", "DU"))
+    ctx = capi.Context(desc, capi.params(level=5), 8, 6, 6)
+    z = lambda: torch.zeros(8 * 36 * 40, dtype=torch.float64, device="cuda")
+    f = {k: z() for k in ("tk", "th", "adn", "ustar")}
+    with pytest.raises(RuntimeError, match="level 4"):
+        ctx.sedim_device(1.0, np.ones(8), f, [z()] * 3, [z()] * 3, [z()] * 3, [z()] * 3, z(), z(), z(), z())
+    ctx.close()

@@ -1,0 +1,152 @@
+"""The oracle's level-4 sedimentation (so_sedim: sedim_SALSA,
+DepositionSlow, DepositionFast, terminal_vel; mcrp.f90:745-1283,
+mo_particle_external_properties.f90:43-100), pinned by:
+
+* fall speeds of water drops against the Gunn & Kinzer (1949) measurements
+  (20 C, 1013 hPa) within 8 %, the Stokes law at 10 um, and continuity of
+  the three regimes at 19 um and 1.07 mm;
+* the precipitation scheme conserving sum_k N/dzt and sum_k M/dzt per column
+  on a uniform grid (particles only move between levels, the level below
+  the surface included; on a stretched grid the reference's split between
+  two levels is not conservative, as written);
+* the slow scheme's flux form: per column sum_k dM_w/dt adn/dzt equals the
+  surface water flux that the surface rain rate reports;
+* the fall distance on a uniform grid worked by hand.
+Runs the FP64 oracle only."""
+import numpy as np
+import pytest
+
+from helpers import les_case
+
+ALVL = 2.5e6
+
+
+def sed_fields(desc, grid, seed):
+    shape, dzt, f, n_p, n_t, m_p, m_t = les_case(desc, 4, grid, seed)
+    from salsa_amd import workload as W
+    rng = np.random.default_rng(seed + 7)
+    n3 = int(np.prod(shape))
+    for a in n_p + m_p:
+        np.abs(a, out=a)
+    # the state SALSA_diagnostics leaves: no particles without mass
+    for c in range(3):
+        nb = n_p[c].shape[0]
+        tot = sum(m_p[c][s * nb:(s + 1) * nb] for s in range(desc.iwa + 1))
+        n_p[c][tot <= 1e-25 * np.maximum(n_p[c], 1.)] = 0.
+    tk = f["temp"]
+    fields = dict(tk=tk, th=tk * 1.02, adn=np.full(n3, 1.1) + 0.1 * rng.random(n3),
+                  ustar=0.1 + 0.4 * rng.random(shape[1] * shape[2]))
+    n_t = [rng.normal(size=a.shape) * 1e-3 * a for a in n_p]
+    m_t = [rng.normal(size=a.shape) * 1e-3 * a for a in m_p]
+    return shape, dzt, fields, n_p, m_p, n_t, m_t
+
+
+def run(O, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, tlt, sed=(1, 1, 1), dtlt=1.0):
+    L = O.make_layout(species=(0,))
+    return O.sedim(L, shape, dtlt, dzt, fld["tk"], fld["th"], fld["adn"], fld["ustar"], n_p, m_p, n_t, m_t, tlt,
+                   sed)
+
+
+def test_fall_speed_against_measurements(oracle_mod):
+    O = oracle_mod
+    mu, rho_a = 1.81e-5, 1.204                    # air at 20 C, 1013 hPa
+    for d, v in ((0.1e-3, 0.27), (0.2e-3, 0.72), (0.5e-3, 2.06), (1.0e-3, 4.03), (2.0e-3, 6.49), (4.0e-3, 8.83)):
+        assert O.terminal_vel(d, 998.2, rho_a, mu, 1.0) == pytest.approx(v, rel=0.08), d
+    d = 10e-6
+    assert O.terminal_vel(d, 998.2, rho_a, mu, 1.0) == pytest.approx(d * d * (998.2 - rho_a) * 9.81 / (18 * mu),
+                                                                      rel=1e-15)
+    for edge in (19e-6, 1.07e-3):
+        lo, hi = O.terminal_vel(edge * (1 - 1e-9), 998.2, rho_a, mu, 1.0), O.terminal_vel(edge * (1 + 1e-9), 998.2,
+                                                                                          rho_a, mu, 1.0)
+        assert abs(hi - lo) / lo < 0.12, edge
+
+
+def test_precipitation_conserved_per_column(oracle_mod):
+    from salsa_amd import capi
+    desc = capi.bin_desc()
+    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields(desc, (20, 4, 4), 5)
+    # a uniform grid: where the drops land between two levels, the reference
+    # moves the lower share without the dzt(kf)/dzt(k) factor (mcrp.f90:1255),
+    # which conserves sum_k N/dzt only when the levels are equally thick
+    dzt = np.full_like(dzt, 1. / 25.)
+    n_t = [np.zeros_like(a) for a in n_p]
+    m_t = [np.zeros_like(a) for a in m_p]
+    tlt = np.zeros(int(np.prod(shape)))
+    rr, sr, vt = run(oracle_mod, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, tlt, sed=(0, 0, 1))
+    nzp, nxp, nyp = shape
+    assert vt.max() > 0.5, "no fast drops: the case does not exercise the fall distance"
+    for a, src in ((n_t[2], n_p[2]), (m_t[2], m_p[2])):
+        s = a.reshape(a.shape[0], nyp, nxp, nzp)[:, 2:nyp - 2, 2:nxp - 2, :]
+        s0 = src.reshape(s.shape[0], nyp, nxp, nzp)[:, 2:nyp - 2, 2:nxp - 2, :]
+        col = (s / dzt[None, None, None, :]).sum(axis=-1)
+        # rounding: drops that stay mostly in their level leave N - (1 - f) N
+        scale = (np.abs(s0) / dzt[None, None, None, :]).sum(axis=-1) + 1e-300
+        assert np.all(np.abs(col) <= 1e-14 * scale)
+        assert (np.abs(s) > 0).sum() > 0
+
+
+def test_slow_flux_matches_surface_rate(oracle_mod):
+    from salsa_amd import capi
+    desc = capi.bin_desc()
+    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields(desc, (12, 4, 4), 6)
+    nzp, nxp, nyp = shape
+    nw = desc.iwa
+    for c, nb in ((0, desc.nbins), (1, desc.ncld)):
+        n_p[c][nb - 1] = 0.                       # the rate sums start one entry early (as written): keep it empty
+    for c in (0, 1):
+        n_t[c][:] = 0.
+        m_t[c][:] = 0.
+    tlt = np.zeros(int(np.prod(shape)))
+    rr, sr, vt = run(oracle_mod, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, tlt, sed=(1, 1, 0))
+    adn = fld["adn"].reshape(nyp, nxp, nzp)
+    tot = 0.
+    for c, nb in ((0, desc.nbins), (1, desc.ncld)):
+        w = m_t[c][nw * nb:(nw + 1) * nb].reshape(nb, nyp, nxp, nzp).sum(axis=0)
+        tot = tot + (w * adn / dzt[None, None, :]).sum(axis=-1)
+    surf = sr.reshape(nyp, nxp)
+    inner = (slice(2, nyp - 2), slice(2, nxp - 2))
+    # sum_k dM/dt adn / dzt = rfl(2) adn(2) = -(surface rate) / alvl; the sum
+    # telescopes level fluxes up to 1e4 times the surface term, hence 1e-9
+    np.testing.assert_allclose(tot[inner], -surf[inner] / ALVL, rtol=1e-9, atol=1e-30)
+    assert np.all(surf[inner] > 0)
+
+
+def test_fall_distance_uniform_grid(oracle_mod):
+    """One precipitation bin at level k of a uniform grid (thickness D): the
+    drops fall fdmax = vc dt; they land fi = floor(fdmax/D) levels down, and
+    the overshoot fraction f = fdmax/D - fi goes one level further (all of
+    it when f > 0.5)."""
+    from salsa_amd import capi
+    desc = capi.bin_desc()
+    nzp = 30
+    shape = (nzp, 5, 5)
+    n3 = nzp * 25
+    D = 20.
+    dzt = np.full(nzp, 1. / D)
+    fld = dict(tk=np.full(n3, 285.), th=np.full(n3, 290.), adn=np.full(n3, 1.15), ustar=np.full(25, 0.3))
+    for dtlt in (3.0, 7.0, 11.0):
+        n_p = [np.zeros((desc.nbins, n3)), np.zeros((desc.ncld, n3)), np.zeros((desc.nprc, n3))]
+        m_p = [np.zeros((desc.nbins * 2, n3)), np.zeros((desc.ncld * 2, n3)), np.zeros((desc.nprc * 2, n3))]
+        n_t = [np.zeros_like(a) for a in n_p]
+        m_t = [np.zeros_like(a) for a in m_p]
+        k = 25
+        x = k + nzp * (2 + 5 * 2)
+        b = 3
+        N, d = 100., 1.0e-3
+        n_p[2][b, x] = N
+        m_p[2][1 * desc.nprc + b, x] = N * 1000. * 0.5235988 * d ** 3
+        rr, sr, vt = run(oracle_mod, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, np.zeros(n3), sed=(0, 0, 1),
+                         dtlt=dtlt)
+        vc = vt[b, x]
+        fdmax = vc * dtlt
+        fi = int(np.floor(fdmax / D))
+        f = fdmax / D - fi
+        got = n_t[2][b, x - k:x - k + nzp] * dtlt
+        expect = np.zeros(nzp)
+        expect[k] -= N
+        if f > 0.5:
+            expect[k - fi - 1] += N
+        else:
+            expect[k - fi - 1] += f * N
+            expect[k - fi] += (1 - f) * N
+        np.testing.assert_allclose(got, expect, rtol=1e-12, atol=1e-9 * N)

@@ -30,6 +30,7 @@ int cap_spec(int cap);
 int launch_gradient(int cap, int precision, const DevLayout *d, const GradArgs &g, double *sum_out,
                     hipStream_t s);
 int launch_les_update(const LesArgs &a, hipStream_t s);
+int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, hipStream_t s);
 }  // namespace salsa
 
 using namespace salsa;
@@ -70,6 +71,8 @@ struct salsa_ctx {
   int32_t *cls_count_h = nullptr;     // pinned host [3]
   // salsa_gradient_device: per-cell J and dJ/dE, then the two sums
   double *gbuf = nullptr;
+  // salsa_sedim_device: precipitation number / mass changes and removal
+  double *sed_work = nullptr;
   // staging for salsa_step_host
   double *stage = nullptr;
   size_t stage_bytes = 0;
@@ -113,6 +116,7 @@ static void release(salsa_ctx *c) {
   (void)hipFree(c->lists); (void)hipFree(c->cls_count);
   (void)hipFree(c->stage);
   (void)hipFree(c->gbuf);
+  (void)hipFree(c->sed_work);
   if (c->cls_count_h) (void)hipHostFree(c->cls_count_h);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -637,5 +641,56 @@ extern "C" int salsa_les_update_device(salsa_ctx *c, double dtlt, const double *
   a.diag = diag;
   const int rc = launch_les_update(a, c->stream);
   if (rc) { std::snprintf(g_err, sizeof(g_err), "LES update launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  return SALSA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sedimentation of the liquid categories (level 4)
+// ---------------------------------------------------------------------------
+extern "C" int salsa_sedim_device(salsa_ctx *c, double dtlt, const double *dzt, const double *tk, const double *th,
+                                  const double *adn, const double *ustar, const double *const n_p[4],
+                                  const double *const m_p[4], double *const n_t[4], double *const m_t[4],
+                                  double *tlt, double *rrate, double *sfcrrate, double *vt_prc, int32_t sed_aero,
+                                  int32_t sed_cloud, int32_t sed_precp) {
+  if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  const salsa_params &P = c->params;
+  const salsa_bin_desc &d = c->desc;
+  if (P.level != 4) { set_error("sedimentation is carried at level 4 only (no ice)"); return SALSA_ERR_ARG; }
+  if (P.hz != 1 || P.hxy != 2) { set_error("sedimentation needs the LES halo widths (hz 1, hxy 2)"); return SALSA_ERR_ARG; }
+  if (c->nzp < 4 || c->nzp > 256 || c->nxp < 5 || c->nyp < 5) {
+    set_error("grid out of range for sedimentation (4 <= nzp <= 256, nxp and nyp >= 5)");
+    return SALSA_ERR_ARG;
+  }
+  if (!(dtlt > 0.) || !dzt || !tk || !th || !adn || !ustar || !n_p || !m_p || !n_t || !m_t || !tlt || !rrate ||
+      !sfcrrate || !vt_prc) {
+    set_error("null argument or non-positive dtlt");
+    return SALSA_ERR_ARG;
+  }
+  for (int k = 0; k < 3; ++k)
+    if (!n_p[k] || !m_p[k] || !n_t[k] || !m_t[k]) { set_error("null field pointer"); return SALSA_ERR_ARG; }
+  SedArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nzp = c->nzp; a.nxp = c->nxp; a.nyp = c->nyp; a.n3 = c->n3; a.dtlt = dtlt;
+  a.nb[0] = d.nbins; a.nb[1] = d.ncld; a.nb[2] = d.nprc;
+  a.ns = d.iwa + 1; a.iwa = d.iwa;
+  for (int s = 0; s < SALSA_MAXSPEC; ++s) a.rholiq[s] = d.rholiq[s];
+  for (int k = 0; k < c->nzp; ++k) {
+    if (!(dzt[k] > 0.)) { set_error("dzt must be positive"); return SALSA_ERR_ARG; }
+    a.dzt[k] = dzt[k];
+  }
+  const int64_t ncol = (int64_t)c->nxp * c->nyp;
+  const size_t need = (size_t)((int64_t)d.nprc * c->n3 * (1 + a.ns) + ncol * d.nprc * a.ns) * sizeof(double);
+  if (!c->sed_work && hipMalloc(&c->sed_work, need) != hipSuccess) {
+    set_error("sedimentation work allocation failed");
+    return SALSA_ERR_NOMEM;
+  }
+  a.prn = c->sed_work;
+  a.prv = a.prn + (int64_t)d.nprc * c->n3;
+  a.rem = a.prv + (int64_t)d.nprc * a.ns * c->n3;
+  a.tk = tk; a.th = th; a.adn = adn; a.ustar = ustar;
+  for (int k = 0; k < 3; ++k) { a.n_p[k] = n_p[k]; a.m_p[k] = m_p[k]; a.n_t[k] = n_t[k]; a.m_t[k] = m_t[k]; }
+  a.tlt = tlt; a.rrate = rrate; a.sfcrrate = sfcrrate; a.vt_prc = vt_prc;
+  const int rc = launch_sedim(a, sed_aero, sed_cloud, sed_precp, c->stream);
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "sedimentation launch failed (%d)", rc); return SALSA_ERR_HIP; }
   return SALSA_OK;
 }

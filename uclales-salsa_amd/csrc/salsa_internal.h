@@ -198,4 +198,21 @@ struct LesArgs {
   double *diag;                       // [6][n3]: rc, srp, snrp, rtot, ri, riri
 };
 
+// Arguments of the level-4 sedimentation passes (salsa_sedim_device):
+// sedim_SALSA with DepositionSlow (aerosol, cloud) and DepositionFast
+// (precipitation) on the LES arrays.
+struct SedArgs {
+  int32_t nzp, nxp, nyp;
+  int64_t n3;
+  double dtlt;
+  int32_t nb[3], ns, iwa;
+  double rholiq[kMaxSpec];
+  double dzt[256];                    // inverse level thicknesses (nzp <= 256)
+  const double *tk, *th, *adn, *ustar;
+  const double *n_p[3], *m_p[3];
+  double *n_t[3], *m_t[3];
+  double *tlt, *rrate, *sfcrrate, *vt_prc;
+  double *prn, *prv, *rem;            // work: [nprc][n3], [nprc*ns][n3], [nxp*nyp][nprc*ns]
+};
+
 }  // namespace salsa
