@@ -221,6 +221,62 @@ def les_leg(capi, desc, f, fd, n_p, m_p, calls=20):
             "achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS, "bound": "hbm"}
 
 
+def sedim_leg(capi, desc, f, fd, n_p, m_p, calls=20):
+    """Level-4 sedimentation on device (salsa_sedim_device) on the same grid,
+    with 10 drops/kg per precipitation bin at its mid diameter in every
+    cell (the workload carries no precipitation), timed with HIP events on
+    the context stream.  Algorithmic bytes per call: aerosol and cloud
+    elements read with their tendency read and written (24 B), precipitation
+    elements read, their tendency read and written and their change staged
+    and read back (40 B), fall speeds written (8 B per precipitation bin and
+    cell), per cell tk, th, adn read, tlt read and written, rrate written
+    (48 B)."""
+    import numpy as np
+    import torch
+    g = f.grid
+    n3 = g.nzp * g.nxp * g.nyp
+    dev = fd["temp"].device
+    np_ = [x.clone() for x in n_p]
+    mp_ = [x.clone() for x in m_p]
+    nb = desc.nprc
+    dmid = np.array(desc.dmid[desc.nbins + desc.ncld:desc.nbins + desc.ncld + nb])
+    np_[2][:] = 10.
+    mp_[2][:] = 0.
+    for b in range(nb):
+        mp_[2][desc.iwa * nb + b] = 10. * 1000. * 0.5235988 * dmid[b] ** 3
+        mp_[2][b] = 1e-3 * mp_[2][desc.iwa * nb + b]
+    nt = [torch.zeros_like(x) for x in np_]
+    mt = [torch.zeros_like(x) for x in mp_]
+    rng = np.random.default_rng(0)
+    sf = dict(tk=fd["temp"], th=fd["temp"] * 1.02, adn=fd["dn"],
+              ustar=torch.from_numpy(0.1 + 0.4 * rng.random(g.nxp * g.nyp)).to(dev))
+    tlt = torch.zeros(n3, dtype=torch.float64, device=dev)
+    rr = torch.zeros(n3, dtype=torch.float64, device=dev)
+    sr = torch.zeros(g.nxp * g.nyp, dtype=torch.float64, device=dev)
+    vt = torch.zeros((nb, n3), dtype=torch.float64, device=dev)
+    dzt = 1. / np.full(g.nzp, 20.)
+    ctx = capi.Context(desc, capi.params(level=4), g.nzp, g.nxp, g.nyp)
+    ctx.sedim_device(1.0, dzt, sf, np_, mp_, nt, mt, tlt, rr, sr, vt)
+    ctx.sync()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(calls):
+        ctx.sedim_device(1.0, dzt, sf, np_, mp_, nt, mt, tlt, rr, sr, vt)
+    e1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / calls
+    ctx.close()
+    slow = sum(x.numel() for x in np_[:2] + mp_[:2])
+    prec = np_[2].numel() + mp_[2].numel()
+    nbytes = 24 * slow + 40 * prec + 8 * nb * n3 + 48 * n3
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"ms_per_call": ms, "cells": n3, "bytes_per_call": nbytes, "achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS,
+            "frac": gbs / HBM_PEAK_GBS, "bound": "hbm",
+            "max_fall_speed_m_s": float(vt.max().item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,6 +402,7 @@ def main():
                                    for k, v in e.items()}
     if not args.no_les:
         out["les_update"] = les_leg(capi, desc, f, fd, n_p, m_p)
+        out["sedimentation"] = sedim_leg(capi, desc, f, fd, n_p, m_p)
     if not args.no_gradient:
         out["gradient"] = gradient_leg(capi, W, desc)
     out["setup_s"] = gen_s
