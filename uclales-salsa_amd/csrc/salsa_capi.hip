@@ -679,7 +679,9 @@ extern "C" int salsa_sedim_device(salsa_ctx *c, double dtlt, const double *dzt, 
     a.dzt[k] = dzt[k];
   }
   const int64_t ncol = (int64_t)c->nxp * c->nyp;
-  const size_t need = (size_t)((int64_t)d.nprc * c->n3 * (1 + a.ns) + ncol * d.nprc * a.ns) * sizeof(double);
+  const int nbslow = d.nbins > d.ncld ? d.nbins : d.ncld;
+  const size_t need = (size_t)((int64_t)d.nprc * c->n3 * (1 + a.ns) + ncol * d.nprc * a.ns +
+                               (int64_t)nbslow * c->n3) * sizeof(double);
   if (!c->sed_work && hipMalloc(&c->sed_work, need) != hipSuccess) {
     set_error("sedimentation work allocation failed");
     return SALSA_ERR_NOMEM;
@@ -687,6 +689,7 @@ extern "C" int salsa_sedim_device(salsa_ctx *c, double dtlt, const double *dzt, 
   a.prn = c->sed_work;
   a.prv = a.prn + (int64_t)d.nprc * c->n3;
   a.rem = a.prv + (int64_t)d.nprc * a.ns * c->n3;
+  a.vslow = a.rem + ncol * d.nprc * a.ns;
   a.tk = tk; a.th = th; a.adn = adn; a.ustar = ustar;
   for (int k = 0; k < 3; ++k) { a.n_p[k] = n_p[k]; a.m_p[k] = m_p[k]; a.n_t[k] = n_t[k]; a.m_t[k] = m_t[k]; }
   a.tlt = tlt; a.rrate = rrate; a.sfcrrate = sfcrrate; a.vt_prc = vt_prc;

@@ -213,6 +213,7 @@ struct SedArgs {
   double *n_t[3], *m_t[3];
   double *tlt, *rrate, *sfcrrate, *vt_prc;
   double *prn, *prv, *rem;            // work: [nprc][n3], [nprc*ns][n3], [nxp*nyp][nprc*ns]
+  double *vslow;                      // work: fall speeds of one slow category [nb][n3]
 };
 
 }  // namespace salsa

@@ -3,9 +3,12 @@
 //
 //   k_sed_zero   rrate, sfcrrate, VtPrc and the work arrays to zero
 //                (sedim_SALSA :795-801, DepositionFast :1127-1134)
-//   k_sed_slow   DepositionSlow (:910-1041) for one category, one thread per
-//                interior cell: the bin fluxes at the cell's level and the
-//                level above, their divergence into the tendencies, the
+//   k_sed_vslow  DepositionSlow's (:910-1041) fall speeds of one category per
+//                (cell, bin), with the dry deposition velocity at the lowest
+//                level, into a work array
+//   k_sed_slow   one thread per interior cell: the bin fluxes at the cell's
+//                level and the level above, their divergence into the
+//                tendencies, the
 //                water divergence into tlt (:808-813, :834-839), the rain
 //                rate and, at the lowest level, the dry deposition velocity
 //                and the surface rate.  One launch per category, so the
@@ -13,8 +16,9 @@
 //   k_sed_vt     DepositionFast's fall speed per (cell, precipitation bin)
 //                into VtPrc.
 //   k_sed_fall   the fall-distance redistribution (:1187-1260) per (column,
-//                bin), levels from the top down as in the reference, into
-//                the work arrays; deposited mass into the removal array.
+//                bin, moved quantity), levels from the top down as in the
+//                reference, into the work arrays; deposited mass into the
+//                removal array.
 //   k_sed_apply  per cell: rain rate (bins in order), tendencies += change /
 //                dtlt, tlt (:856-864); the surface rate from the removal
 //                array at the lowest level.
@@ -100,7 +104,40 @@ __global__ void __launch_bounds__(256) k_sed_zero(SedArgs a) {
   }
 }
 
-// DepositionSlow for category cat (0 aerosol, 1 cloud) at one interior cell
+// DepositionSlow fall speeds of category cat per (interior cell, bin), with
+// the dry deposition velocity at the lowest level; 0 below the number limit
+__global__ void __launch_bounds__(128) k_sed_vslow(SedArgs a, int cat) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (x >= a.n3) return;
+  const int k = (int)(x % a.nzp);
+  const int64_t col = x / a.nzp;
+  const int i = (int)(col % a.nxp), j = (int)(col / a.nxp);
+  if (k < 1 || k > a.nzp - 2 || !sd_interior(a, i, j)) return;
+  const int64_t n3 = a.n3;
+  const double zpn = a.n_p[cat][(int64_t)b * n3 + x];
+  double vc = 0.;
+  if (!(zpn < kNlim)) {
+    const double ad = a.adn[x];
+    const Air ar = sd_air(a.tk[x], ad);
+    double dwet, GG;
+    vc = sd_vc(a, zpn, a.m_p[cat] + (int64_t)b * n3 + x, (int64_t)a.nb[cat] * n3, ad, ar, &dwet, &GG);
+    vc = fmin(vc, fmin(0.5 * (1. / a.dzt[k]) / a.dtlt, 2.0));
+    if (k == 1) {                                 // the level just above the surface
+      const double ust = a.ustar[col];
+      const double mdiff = (kKb * a.tk[x] * GG) / (3.0 * kPiFull * ar.avis * dwet);
+      const double Sc = (ar.avis / ad) / mdiff;
+      double St = vc * ::pow(ust, 2.0) / kG * (ar.avis / ad);
+      if (St < 0.01) St = 0.01;
+      const double rt = 1.0 / fmax(kEpsR8, (ust * (::pow(Sc, -2.0 / 3.0) + ::pow(10., -3.0 / St))));
+      vc = (1. / rt) + vc;
+    }
+  }
+  a.vslow[(int64_t)b * n3 + x] = vc;
+}
+
+// DepositionSlow for category cat at one interior cell: flux divergence from
+// the fall speeds of the cell's level and the level above
 __global__ void __launch_bounds__(128) k_sed_slow(SedArgs a, int cat) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= a.n3) return;
@@ -110,60 +147,37 @@ __global__ void __launch_bounds__(128) k_sed_slow(SedArgs a, int cat) {
   if (k < 1 || k > a.nzp - 2 || !sd_interior(a, i, j)) return;
   const int nb = a.nb[cat], ns = a.ns, iw = a.iwa;
   const int64_t n3 = a.n3;
-  const double *N = a.n_p[cat], *M = a.m_p[cat];
-  const int64_t ms = (int64_t)nb * n3;
-  const double dtlt = a.dtlt, dz = a.dzt[k];
+  const double *__restrict__ N = a.n_p[cat];
+  const double *__restrict__ M = a.m_p[cat];
+  const double *__restrict__ V = a.vslow;
+  const double dz = a.dzt[k];
   const int64_t x1 = x + 1;
   const bool top = k == a.nzp - 2;              // rflm(nzp) = 0
-  const Air air = sd_air(a.tk[x], a.adn[x]);
-  const Air air1 = top ? air : sd_air(a.tk[x1], a.adn[x1]);
   const double adn = a.adn[x], adn1 = a.adn[x1];
-  const double kvis = air.avis / adn;
-  const double ust = a.ustar[col];
-  // level fluxes of one bin: fall speed at level kk (0 below clim)
-  auto speed = [&](int b, int64_t xx, const Air &ar, double ad, int kk) -> double {
-    const double zpn = N[(int64_t)b * n3 + xx];
-    if (zpn < kNlim) return 0.;
-    double dwet, GG;
-    double vc = sd_vc(a, zpn, M + (int64_t)b * n3 + xx, ms, ad, ar, &dwet, &GG);
-    vc = fmin(vc, fmin(0.5 * (1. / a.dzt[kk]) / dtlt, 2.0));
-    if (kk == 1) {
-      const double mdiff = (kKb * a.tk[xx] * GG) / (3.0 * kPiFull * ar.avis * dwet);
-      const double Sc = (ar.avis / ad) / mdiff;
-      double St = vc * ::pow(ust, 2.0) / kG * (ar.avis / ad);
-      if (St < 0.01) St = 0.01;
-      const double rt = 1.0 / fmax(kEpsR8, (ust * (::pow(Sc, -2.0 / 3.0) + ::pow(10., -3.0 / St))));
-      vc = (1. / rt) + vc;
-    }
-    return vc;
-  };
-  (void)kvis;
-  double sw = 0., sdw = 0.;
-  // the rain-rate sum runs over flux entries (ns-1)*nb .. ns*nb (1-based), as written
-  const int q0 = ns * nb - nb - 1;
+  double sdw = 0.;
   for (int b = 0; b < nb; ++b) {
-    const double vc = speed(b, x, air, adn, k);
-    const double vc1 = top ? 0. : speed(b, x1, air1, adn1, k + 1);
     const double zpn = N[(int64_t)b * n3 + x], zpn1 = top ? 0. : N[(int64_t)b * n3 + x1];
-    const double rfn = zpn < kNlim ? 0. : -zpn * vc;
-    const double rfn1 = (top || zpn1 < kNlim) ? 0. : -zpn1 * vc1;
+    const bool on = !(zpn < kNlim), on1 = !top && !(zpn1 < kNlim);
+    const double vc = V[(int64_t)b * n3 + x], vc1 = on1 ? V[(int64_t)b * n3 + x1] : 0.;
+    const double rfn = on ? -zpn * vc : 0.;
+    const double rfn1 = on1 ? -zpn1 * vc1 : 0.;
     const double dn = (rfn1 * adn1 - rfn * adn) * dz / adn;
     a.n_t[cat][(int64_t)b * n3 + x] = a.n_t[cat][(int64_t)b * n3 + x] - dn;
     for (int s = 0; s < ns; ++s) {
-      const double m = M[((int64_t)s * nb + b) * n3 + x], m1 = top ? 0. : M[((int64_t)s * nb + b) * n3 + x1];
-      const double rfm = zpn < kNlim ? 0. : -m * vc;
-      const double rfm1 = (top || zpn1 < kNlim) ? 0. : -m1 * vc1;
+      const double m = M[((int64_t)s * nb + b) * n3 + x], m1 = on1 ? M[((int64_t)s * nb + b) * n3 + x1] : 0.;
+      const double rfm = on ? -m * vc : 0.;
+      const double rfm1 = on1 ? -m1 * vc1 : 0.;
       const double dm = (rfm1 * adn1 - rfm * adn) * dz / adn;
       a.m_t[cat][((int64_t)s * nb + b) * n3 + x] = a.m_t[cat][((int64_t)s * nb + b) * n3 + x] - dm;
       if (s == iw) sdw = sdw + dm;
     }
   }
-  // rain rate over the flux entries q0 .. ns*nb-1 in order
-  for (int q = q0; q < ns * nb; ++q) {
+  // rain rate over the flux entries (ns-1)*nb .. ns*nb (1-based), as written
+  double sw = 0.;
+  for (int q = ns * nb - nb - 1; q < ns * nb; ++q) {
     const int s = q / nb, b = q % nb;
-    const double zpn = N[(int64_t)b * n3 + x];
-    if (zpn < kNlim) continue;
-    sw = sw + (-M[((int64_t)s * nb + b) * n3 + x] * speed(b, x, air, adn, k));
+    if (N[(int64_t)b * n3 + x] < kNlim) continue;
+    sw = sw + (-M[((int64_t)s * nb + b) * n3 + x] * V[(int64_t)b * n3 + x]);
   }
   a.rrate[x] = a.rrate[x] - sw * adn * kAlvl;
   a.tlt[x] = a.tlt[x] + sdw * (kAlvl / kCp) * a.th[x] / a.tk[x];
@@ -187,61 +201,85 @@ __global__ void __launch_bounds__(128) k_sed_vt(SedArgs a) {
       sd_vc(a, zpn, a.m_p[2] + (int64_t)b * n3 + x, (int64_t)a.nb[2] * n3, a.adn[x], air, nullptr, nullptr);
 }
 
-// fall-distance redistribution per (interior column, bin), top level down
+// fall-distance redistribution per (interior column, bin), top level down;
+// blockIdx.y selects the quantity moved: 0 the number, 1 + s mass species s
+// (each thread repeats the bin's fall-distance decisions)
 __global__ void __launch_bounds__(64) k_sed_fall(SedArgs a) {
   const int nx = a.nxp - 4, ny = a.nyp - 4, nb = a.nb[2], ns = a.ns;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int qy = blockIdx.y;                     // 0: number, 1 + s: species s
   if (t >= nx * ny * nb) return;
   const int b = t % nb, c = t / nb;
   const int i = 2 + c % nx, j = 2 + c / nx;
   const int64_t col = (int64_t)i + (int64_t)a.nxp * j, base = col * a.nzp, n3 = a.n3;
-  const double *N = a.n_p[2], *M = a.m_p[2];
-  double *prn = a.prn + (int64_t)b * n3 + base;
-  double *rem = a.rem + col * nb * ns;
+  const double *__restrict__ Nb = a.n_p[2] + (int64_t)b * n3 + base;
+  const double *__restrict__ Q = qy == 0 ? Nb : a.m_p[2] + ((int64_t)(qy - 1) * nb + b) * n3 + base;
+  const double *__restrict__ vt = a.vt_prc + (int64_t)b * n3 + base;
+  const double *__restrict__ adn = a.adn + base;
+  double *__restrict__ out = qy == 0 ? a.prn + (int64_t)b * n3 + base : a.prv + ((int64_t)(qy - 1) * nb + b) * n3 + base;
+  double *rem = qy > 0 ? a.rem + col * nb * ns + (qy - 1) * nb + b : nullptr;   // deposited mass (species only)
   const double dtlt = a.dtlt;
-  for (int k = a.nzp - 2; k >= 1; --k) {
-    const double zpn = N[(int64_t)b * n3 + base + k];
-    if (zpn < kPrlim) continue;
-    const double vc = a.vt_prc[(int64_t)b * n3 + base + k];
-    const double fdmax = dtlt * vc;
-    double fd = 0.;
-    int fi = 0;
-    bool dep = false;
-    while (fd < fdmax) {                      // bounded: ends at the lowest level
-      fd = fd + (1. / a.dzt[k - fi]);
-      fi = fi + 1;
-      if (k - fi <= 0) { dep = true; break; }
-    }
-    fi = fi - 1;
-    const int kf = k - fi;
-    fd = fd - (1. / a.dzt[kf]);
-    const double fdos = fmin(fmax(fdmax - fd, 0.), 1. / a.dzt[kf]);
-    prn[k] = prn[k] - zpn;
-    for (int s = 0; s < ns; ++s) {
-      double *prv = a.prv + ((int64_t)s * nb + b) * n3 + base;
-      prv[k] = prv[k] - M[((int64_t)s * nb + b) * n3 + base + k];
-    }
-    if (dep)
-      for (int s = 0; s < ns; ++s)
-        rem[s * nb + b] = rem[s * nb + b] + M[((int64_t)s * nb + b) * n3 + base + k] * a.adn[base + k] * vc;
-    const bool whole = fdos * a.dzt[kf] > 0.5;
-    if (whole) {
-      prn[kf - 1] = prn[kf - 1] + zpn * a.dzt[kf - 1] / a.dzt[k];
-    } else {
-      prn[kf - 1] = prn[kf - 1] + (fdos * a.dzt[kf]) * zpn * a.dzt[kf - 1] / a.dzt[kf];
-      prn[kf] = prn[kf] + (1. - fdos * a.dzt[kf]) * zpn;
-    }
-    for (int s = 0; s < ns; ++s) {
-      double *prv = a.prv + ((int64_t)s * nb + b) * n3 + base;
-      const double m = M[((int64_t)s * nb + b) * n3 + base + k];
-      if (whole) {
-        prv[kf - 1] = prv[kf - 1] + m * a.dzt[kf - 1] / a.dzt[k];
+  // The changes of the four levels at and below the current source level are
+  // kept in registers (w[0] = level k, ..., w[3] = level k-3): a level
+  // receives only from sources at or above it, so it is final once its own
+  // source is done and is written once; landings further down go to memory,
+  // before their level enters the window.  The additions happen in the
+  // reference's order (sources from the top down).
+  const int ktop = a.nzp - 2;
+  double w[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) w[u] = ktop - u >= 0 ? out[ktop - u] : 0.;
+  for (int k = ktop; k >= 1; --k) {
+    const double zpn = Nb[k];
+    if (!(zpn < kPrlim)) {
+      const double q = Q[k];
+      const double vc = vt[k];
+      const double fdmax = dtlt * vc;
+      double fd = 0.;
+      int fi = 0;
+      bool dep = false;
+      while (fd < fdmax) {                    // bounded: ends at the lowest level
+        fd = fd + (1. / a.dzt[k - fi]);
+        fi = fi + 1;
+        if (k - fi <= 0) { dep = true; break; }
+      }
+      fi = fi - 1;
+      const int kf = k - fi;
+      fd = fd - (1. / a.dzt[kf]);
+      const double fdos = fmin(fmax(fdmax - fd, 0.), 1. / a.dzt[kf]);
+      w[0] = w[0] - q;
+      if (dep && qy > 0) *rem = *rem + q * adn[k] * vc;
+      const int d = k - kf;                   // kf = k - d, landing at kf - 1 and kf
+      double c1, c0;                          // to kf - 1, to kf
+      bool two;
+      if (fdos * a.dzt[kf] > 0.5) {
+        c1 = q * a.dzt[kf - 1] / a.dzt[k];
+        c0 = 0.;
+        two = false;
       } else {
-        prv[kf - 1] = prv[kf - 1] + (fdos * a.dzt[kf]) * m * a.dzt[kf - 1] / a.dzt[kf];
-        prv[kf] = prv[kf] + (1. - fdos * a.dzt[kf]) * m;
+        c1 = (fdos * a.dzt[kf]) * q * a.dzt[kf - 1] / a.dzt[kf];
+        c0 = (1. - fdos * a.dzt[kf]) * q;
+        two = true;
+      }
+      // level kf - 1 first, then kf, as the reference adds them
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (u == d + 1) w[u] = w[u] + c1;
+      if (d + 1 > 3) out[kf - 1] = out[kf - 1] + c1;
+      if (two) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (u == d) w[u] = w[u] + c0;
+        if (d > 3) out[kf] = out[kf] + c0;
       }
     }
+    out[k] = w[0];                            // level k is final
+    w[0] = w[1];
+    w[1] = w[2];
+    w[2] = w[3];
+    w[3] = k - 4 >= 0 ? out[k - 4] : 0.;
   }
+  out[0] = w[0];                              // the level below the surface
 }
 
 // per cell: rain rate, tendencies and tlt of the precipitation; surface rate
@@ -284,12 +322,15 @@ __global__ void __launch_bounds__(256) k_sed_apply(SedArgs a) {
 int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, hipStream_t s) {
   const unsigned g256 = (unsigned)((a.n3 + 255) / 256), g128 = (unsigned)((a.n3 + 127) / 128);
   hipLaunchKernelGGL(k_sed_zero, dim3(g256 < 65536 ? g256 : 65536), dim3(256), 0, s, a);
-  if (sed_aero) hipLaunchKernelGGL(k_sed_slow, dim3(g128), dim3(128), 0, s, a, 0);
-  if (sed_cloud) hipLaunchKernelGGL(k_sed_slow, dim3(g128), dim3(128), 0, s, a, 1);
+  for (int cat = 0; cat < 2; ++cat) {
+    if (!(cat == 0 ? sed_aero : sed_cloud)) continue;
+    hipLaunchKernelGGL(k_sed_vslow, dim3(g128, a.nb[cat]), dim3(128), 0, s, a, cat);
+    hipLaunchKernelGGL(k_sed_slow, dim3(g128), dim3(128), 0, s, a, cat);
+  }
   if (sed_precp) {
     hipLaunchKernelGGL(k_sed_vt, dim3(g128, a.nb[2]), dim3(128), 0, s, a);
     const int nthr = (a.nxp - 4) * (a.nyp - 4) * a.nb[2];
-    hipLaunchKernelGGL(k_sed_fall, dim3((nthr + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_sed_fall, dim3((nthr + 63) / 64, 1 + a.ns), dim3(64), 0, s, a);
     hipLaunchKernelGGL(k_sed_apply, dim3(g256), dim3(256), 0, s, a);
   }
   return (int)hipGetLastError();
