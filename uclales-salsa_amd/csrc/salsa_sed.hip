@@ -229,11 +229,19 @@ __global__ void __launch_bounds__(64) k_sed_fall(SedArgs a) {
   double w[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) w[u] = ktop - u >= 0 ? out[ktop - u] : 0.;
+  // the next level's inputs are loaded one iteration ahead
+  double zpn_n = Nb[ktop], q_n = Q[ktop], vc_n = vt[ktop], adn_n = adn[ktop];
   for (int k = ktop; k >= 1; --k) {
-    const double zpn = Nb[k];
+    const double zpn = zpn_n, q_k = q_n, vc_k = vc_n, adn_k = adn_n;
+    if (k > 1) {
+      zpn_n = Nb[k - 1];
+      q_n = Q[k - 1];
+      vc_n = vt[k - 1];
+      adn_n = adn[k - 1];
+    }
     if (!(zpn < kPrlim)) {
-      const double q = Q[k];
-      const double vc = vt[k];
+      const double q = q_k;
+      const double vc = vc_k;
       const double fdmax = dtlt * vc;
       double fd = 0.;
       int fi = 0;
@@ -248,7 +256,7 @@ __global__ void __launch_bounds__(64) k_sed_fall(SedArgs a) {
       fd = fd - (1. / a.dzt[kf]);
       const double fdos = fmin(fmax(fdmax - fd, 0.), 1. / a.dzt[kf]);
       w[0] = w[0] - q;
-      if (dep && qy > 0) *rem = *rem + q * adn[k] * vc;
+      if (dep && qy > 0) *rem = *rem + q * adn_k * vc;
       const int d = k - kf;                   // kf = k - d, landing at kf - 1 and kf
       double c1, c0;                          // to kf - 1, to kf
       bool two;
