@@ -124,6 +124,33 @@ MODULE mo_salsa_amd_cabi
        REAL(C_DOUBLE), INTENT(out) :: totals(2)
      END FUNCTION salsa_gradient_device
 
+     ! The LES side after a step on device: tend_constrain2, update_sclrs,
+     ! tend0 and SALSA_diagnostics for the SALSA tracers (dzt on the host)
+     INTEGER(C_INT) FUNCTION salsa_les_update_device(ctx, dtlt, dzt, temp, rh, rhi, rp, n_p, n_t, m_p, m_t, &
+                                                     charge_time, diag) BIND(C, name="salsa_les_update_device")
+       IMPORT :: C_INT, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE         :: ctx
+       REAL(C_DOUBLE), VALUE      :: dtlt
+       REAL(C_DOUBLE), INTENT(in) :: dzt(*)
+       TYPE(C_PTR), VALUE         :: temp, rh, rhi, rp
+       TYPE(C_PTR), INTENT(in)    :: n_p(4), n_t(4), m_p(4), m_t(4)
+       TYPE(C_PTR), VALUE         :: charge_time, diag
+     END FUNCTION salsa_les_update_device
+
+     ! Sedimentation of the liquid categories at level 4 (sedim_SALSA)
+     INTEGER(C_INT) FUNCTION salsa_sedim_device(ctx, dtlt, dzt, tk, th, adn, ustar, n_p, m_p, n_t, m_t, tlt, &
+                                                rrate, sfcrrate, vt_prc, sed_aero, sed_cloud, sed_precp) &
+                                                BIND(C, name="salsa_sedim_device")
+       IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE         :: ctx
+       REAL(C_DOUBLE), VALUE      :: dtlt
+       REAL(C_DOUBLE), INTENT(in) :: dzt(*)
+       TYPE(C_PTR), VALUE         :: tk, th, adn, ustar
+       TYPE(C_PTR), INTENT(in)    :: n_p(4), m_p(4), n_t(4), m_t(4)
+       TYPE(C_PTR), VALUE         :: tlt, rrate, sfcrrate, vt_prc
+       INTEGER(C_INT32_T), VALUE  :: sed_aero, sed_cloud, sed_precp
+     END FUNCTION salsa_sedim_device
+
      INTEGER(C_INT) FUNCTION salsa_ctx_sync(ctx) BIND(C, name="salsa_ctx_sync")
        IMPORT :: C_INT, C_PTR
        TYPE(C_PTR), VALUE :: ctx
