@@ -274,7 +274,7 @@ int salsa_gradient_device(salsa_ctx *ctx, const double *press, const double *tem
  * (constrain_SALSA.f90:18-36), update_sclrs (step.f90:450-489: a += a_t*dtlt
  * on the interior, sclrset 'mixd' util.f90:71-107, periodic lateral halos
  * of a process that owns the whole grid), tend0 (step.f90:379-396) and
- * SALSA_diagnostics(.FALSE., lcharge) (constrain_SALSA.f90:50-404).
+ * SALSA_diagnostics(.FALSE., lcharge) (constrain_SALSA.f90:50-402).
  *   dtlt       : time step [s]
  *   dzt        : HOST array of nzp inverse level thicknesses (mo_aux_state
  *                dzt); the top boundary extrapolates with dzt(nzp-2)/dzt(nzp-1)
@@ -297,9 +297,9 @@ int salsa_les_update_device(salsa_ctx *ctx, double dtlt, const double *dzt, cons
 
 /*
  * Sedimentation of the liquid categories at level 4 on device (sedim_SALSA,
- * mcrp.f90:745-905): DepositionSlow (:910-1041) for aerosols (sed_aero) and
+ * mcrp.f90:745-921): DepositionSlow (:926-1069) for aerosols (sed_aero) and
  * cloud droplets (sed_cloud), with the dry deposition velocity at the lowest
- * level; DepositionFast (:1045-1283) for precipitation (sed_precp).
+ * level; DepositionFast (:1073-1275) for precipitation (sed_precp).
  *   dzt        : HOST array of nzp inverse level thicknesses
  *   tk, th, adn: temperature, potential temperature, air density (device)
  *   ustar      : friction velocity, nxp*nyp (device)

@@ -1882,7 +1882,7 @@ int so_step_batch(const so_layout *L, const so_params *P, long n, const double *
 /*   update_sclrs      step.f90:450-489 (update; sclrset 'mixd',           */
 /*                     util.f90:71-107, with periodic lateral halos)       */
 /*   tend0             step.f90:379-396                                    */
-/*   SALSA_diagnostics constrain_SALSA.f90:50-404, findDry4Wet :406-469,   */
+/*   SALSA_diagnostics constrain_SALSA.f90:50-402, findDry4Wet :408-470,   */
 /*                     calc_correlation util.f90:1005-1024                 */
 /* in the order of cfll (step.f90:330-333).  Arrays are the LES ones:      */
 /* n(c)[b][n3], m(c)[s*nb+b][n3], n3 index k + nzp*(i + nxp*j).            */
@@ -1980,7 +1980,7 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
         const long x = IDX(k, i, j);
 #define NP(c, b) n_p[c][(long)(b) * n3 + x]
 #define MP(c, s, b) m_p[c][((long)(s) * nb[c] + (b)) * n3 + x]
-        /* particles with number but no mass (:94-146) */
+        /* particles with number but no mass (:103-154) */
         for (int b = 0; b < nb[0]; ++b) {
           double sm = 0.;
           for (int s = 0; s < nwet; ++s) sm = sm + MP(0, s, b);
@@ -2016,7 +2016,7 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
           }
         /* aerosol numbers and dry masses of the cell for findDry4Wet */
         double na[SO_MAXBIN], ma[SO_MAXSPEC][SO_MAXBIN];
-        /* evaporating cloud droplets (:152-203) */
+        /* evaporating cloud droplets (:156-207) */
         for (int bc = L->ica.cur; bc <= L->fcb.cur; ++bc) {
           if (!(NP(1, bc) > nlim_c && rh[x] < 0.999 && MP(1, iwa, bc) < 1.e-5)) continue;
           double ns = 0.;
@@ -2040,7 +2040,7 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
             if (charge_time) charge_time[(long)(nb[0] + bc) * n3 + x] = 0.;
           }
         }
-        /* evaporating precipitation (:205-257) */
+        /* evaporating precipitation (:209-258) */
         for (int bc = 0; bc < nb[2]; ++bc) {
           if (!(NP(2, bc) > prlim_c && rh[x] < 0.999 && MP(2, iwa, bc) < 1.e-6)) continue;
           double ns = 0.;
@@ -2070,7 +2070,7 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
             if (charge_time) charge_time[(long)(nb[0] + nb[1] + bc) * n3 + x] = 0.;
           }
         }
-        /* sublimating ice (:259-309) */
+        /* sublimating ice (:260-315) */
         if (level == 5)
           for (int bc = 0; bc < nb[3]; ++bc) {
             const double mt = MP(3, iwa, bc) + MP(3, irim, bc);
@@ -2101,7 +2101,7 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
               MP(3, irim, bc) = 0.;
             }
           }
-        /* aerosols below 0.1 nm dry diameter (:312-353); the gas-phase
+        /* aerosols below 0.1 nm dry diameter (:317-366); the gas-phase
            transfer of lscndgas is not carried */
         for (int b = 0; b < nb[0]; ++b) {
           if (!(NP(0, b) > nlim_c)) continue;
@@ -2116,7 +2116,7 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
 #undef NP
 #undef MP
       }
-  /* diagnostic tracers over the whole grid (:357-385):
+  /* diagnostic tracers over the whole grid (:370-400):
      diag[0] rc, [1] srp, [2] snrp, [3] rtot, [4] ri, [5] riri */
   for (long x = 0; x < n3; ++x) {
     double sa = 0., sc = 0., sp = 0., sn = 0.;
@@ -2145,9 +2145,9 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
 
 /* ====================================================================== */
 /* Sedimentation of the liquid categories at level 4 (sedim_SALSA,        */
-/* mcrp.f90:745-905): DepositionSlow for aerosols and cloud droplets      */
-/* (:910-1041, with the dry deposition velocity at the lowest level) and  */
-/* DepositionFast for precipitation (:1045-1283, fall distance per bin    */
+/* mcrp.f90:745-921): DepositionSlow for aerosols and cloud droplets      */
+/* (:926-1069, with the dry deposition velocity at the lowest level) and  */
+/* DepositionFast for precipitation (:1073-1275, fall distance per bin    */
 /* and time step, with the reference's numerical-diffusion switch);       */
 /* calcDiamLES and terminal_vel (mo_particle_external_properties.f90:     */
 /* 43-100, 169-201) for liquid particles.  Constants of defs.f90: pi with */
@@ -2272,7 +2272,7 @@ int so_sedim(const so_layout *L, int nzp, int nxp, int nyp, double dtlt, const d
               m_t[cat][((long)s * nb + b) * n3 + x] = m_t[cat][((long)s * nb + b) * n3 + x] - dm;
             }
           }
-          /* SUM over the water rows in bin order (mcrp.f90:808-813) */
+          /* SUM over the water rows in bin order (mcrp.f90:810-816) */
           {
             double sdw = 0.;
             for (int b = 0; b < nb; ++b)

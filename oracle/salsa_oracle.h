@@ -172,7 +172,7 @@ void so_unpack_tend(const so_layout *L, int level, double dn, double tstep,
 
 /* LES coupling after run_SALSA for one process owning the whole grid:
  * tend_constrain2, update_sclrs (update, sclrset 'mixd', periodic halos),
- * tend0 and SALSA_diagnostics (constrain_SALSA.f90:18-404, step.f90:330-333,
+ * tend0 and SALSA_diagnostics (constrain_SALSA.f90:18-402, step.f90:330-333,
  * 379-489, util.f90:71-107, 1005-1024).  Updates n_p/m_p in place, zeroes
  * n_t/m_t, and writes diag[6][n3] = rc, srp, snrp, rtot, ri, riri (the last
  * two at level 5 only).  charge_time: NULL or [nbins+ncld+nprc][n3].
@@ -182,8 +182,8 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
                   double *const n_p[4], double *const n_t[4], double *const m_p[4], double *const m_t[4],
                   double *charge_time, double *diag);
 double so_les_corr(const double *x, const double *y, int n);
-/* Sedimentation at level 4 (sedim_SALSA, mcrp.f90:745-905, DepositionSlow
- * :910-1041 for aerosols and cloud, DepositionFast :1045-1283 for
+/* Sedimentation at level 4 (sedim_SALSA, mcrp.f90:745-921, DepositionSlow
+ * :926-1069 for aerosols and cloud, DepositionFast :1073-1275 for
  * precipitation): adds to the tendencies n_t/m_t and tlt, writes rrate
  * [n3], sfcrrate [nxp*nyp] and vt_prc [nprc][n3] (zeroed first).  dzt, tk,
  * th, adn: LES fields; ustar [nxp*nyp].  Returns 0, or -1 for a grid too

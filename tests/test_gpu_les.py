@@ -1,5 +1,5 @@
 """salsa_les_update_device (tend_constrain2, update_sclrs, tend0 and
-SALSA_diagnostics on device, constrain_SALSA.f90:18-404, step.f90:330-333)
+SALSA_diagnostics on device, constrain_SALSA.f90:18-402, step.f90:330-333)
 against the oracle's so_les_update on the same fields: bit-exact for the
 updated numbers and masses, the zeroed tendencies, the zeroed charge times
 and the diagnostic sums (every operation is a comparison, a move or a sum

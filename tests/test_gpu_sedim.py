@@ -1,5 +1,5 @@
 """salsa_sedim_device (level-4 sedimentation: DepositionSlow for aerosols and
-cloud droplets, DepositionFast for precipitation; mcrp.f90:745-1283) against
+cloud droplets, DepositionFast for precipitation; mcrp.f90:745-1275) against
 the oracle's so_sedim on the same fields.  The fall speeds go through pow,
 exp and log, whose device and host results may differ in the last bit, and
 the tendencies are differences of level fluxes: the bar is 1e-10 of each

@@ -1,5 +1,5 @@
 """The oracle's LES coupling after a step (so_les_update: tend_constrain2,
-update_sclrs, tend0, SALSA_diagnostics; constrain_SALSA.f90:18-404,
+update_sclrs, tend0, SALSA_diagnostics; constrain_SALSA.f90:18-402,
 step.f90:330-333, 379-489, util.f90:71-107, 1005-1024), pinned by what the
 physics and the reference's own lines fix:
 

@@ -1,5 +1,5 @@
 """The oracle's level-4 sedimentation (so_sedim: sedim_SALSA,
-DepositionSlow, DepositionFast, terminal_vel; mcrp.f90:745-1283,
+DepositionSlow, DepositionFast, terminal_vel; mcrp.f90:745-1275,
 mo_particle_external_properties.f90:43-100), pinned by:
 
 * fall speeds of water drops against the Gunn & Kinzer (1949) measurements
@@ -66,7 +66,7 @@ def test_precipitation_conserved_per_column(oracle_mod):
     desc = capi.bin_desc()
     shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields(desc, (20, 4, 4), 5)
     # a uniform grid: where the drops land between two levels, the reference
-    # moves the lower share without the dzt(kf)/dzt(k) factor (mcrp.f90:1255),
+    # moves the lower share without the dzt(kf)/dzt(k) factor (mcrp.f90:1246),
     # which conserves sum_k N/dzt only when the levels are equally thick
     dzt = np.full_like(dzt, 1. / 25.)
     n_t = [np.zeros_like(a) for a in n_p]

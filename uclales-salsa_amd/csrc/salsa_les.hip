@@ -9,10 +9,10 @@
 //           the top row extrapolates with dzf2 = dzt(nzp-2)/dzt(nzp-1); the
 //           lateral halos are the periodic images (cyclics/cyclicc of a
 //           process that owns the whole grid).  One thread per (row, column).
-//   pass D  SALSA_diagnostics (constrain_SALSA.f90:50-404): clamps on every
+//   pass D  SALSA_diagnostics (constrain_SALSA.f90:50-402): clamps on every
 //           cell; on interior columns the removal of particles with number
 //           but no mass, cloud / precipitation / ice bins that evaporated
-//           back to the aerosol bins (findDry4Wet :406-469 for
+//           back to the aerosol bins (findDry4Wet :408-470 for
 //           precipitation and ice), aerosols below 0.1 nm dry diameter; then
 //           the diagnostic sums rc, srp, snrp, rtot, ri, riri on every cell.
 //           One thread per cell; per cell the bins are visited in the
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_les_diag(LesArgs a) {
   const int nwet = a.nwet, ndry = a.ndry, iwa = a.iwa, irim = a.irim;
   const int64_t n3 = a.n3;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n3; x += (int64_t)gridDim.x * blockDim.x) {
-    // remove negative values (:86-97); only values that change are written.
+    // remove negative values (:88-100); only values that change are written.
     // Rows are loaded 8 at a time before any store so that the loads are in
     // flight together.
     for (int c = 0; c < ncat; ++c)
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) k_les_diag(LesArgs a) {
       auto ctime = [&](int row) {
         if (a.charge_time) a.charge_time[(int64_t)row * n3 + x] = 0.;
       };
-      // particles with number but no mass (:99-146)
+      // particles with number but no mass (:103-154)
       for (int c = 0; c < 2; ++c)
         for (int b = 0; b < a.nb[c]; ++b) {
           double sm = 0.;
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(256) k_les_diag(LesArgs a) {
           }
         }
       const double T = a.temp[x], rh = a.rh[x];
-      // cloud droplets that evaporated (:152-203)
+      // cloud droplets that evaporated (:156-207)
       for (int bc = a.ica_cur; bc <= a.fcb_cur; ++bc) {
         if (!(N(1, bc) > kNlim && rh < 0.999 && M(1, iwa, bc) < 1.e-5)) continue;
         double ns = 0.;
@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(256) k_les_diag(LesArgs a) {
           ctime(a.nb[0] + bc);
         }
       }
-      // precipitation that evaporated (:205-257)
+      // precipitation that evaporated (:209-258)
       for (int bc = 0; bc < a.nb[2]; ++bc) {
         if (!(N(2, bc) > kPrlim && rh < 0.999 && M(2, iwa, bc) < 1.e-6)) continue;
         double ns = 0.;
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) k_les_diag(LesArgs a) {
           ctime(a.nb[0] + a.nb[1] + bc);
         }
       }
-      // ice that sublimated (:259-309)
+      // ice that sublimated (:260-315)
       if (a.level == 5) {
         const double rhi = a.rhi[x];
         for (int bc = 0; bc < a.nb[3]; ++bc) {
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(256) k_les_diag(LesArgs a) {
           }
         }
       }
-      // aerosols below 0.1 nm dry diameter (:312-353); no gas-phase transfer
+      // aerosols below 0.1 nm dry diameter (:317-366); no gas-phase transfer
       for (int b = 0; b < a.nb[0]; ++b) {
         if (!(N(0, b) > kNlim)) continue;
         double zvol = 0.;
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(256) k_les_diag(LesArgs a) {
         }
       }
     }
-    // diagnostic tracers (:357-385)
+    // diagnostic tracers (:370-400)
     const double *ma = a.m_p[0] + (int64_t)iwa * a.nb[0] * n3 + x;
     const double *mc = a.m_p[1] + (int64_t)iwa * a.nb[1] * n3 + x;
     const double *mp = a.m_p[2] + (int64_t)iwa * a.nb[2] * n3 + x;

@@ -1,26 +1,26 @@
 // salsa_sed.hip -- sedimentation of the liquid SALSA categories at level 4
-// (sedim_SALSA, mcrp.f90:745-905) on device.
+// (sedim_SALSA, mcrp.f90:745-921) on device.
 //
 //   k_sed_zero   rrate, sfcrrate, VtPrc and the work arrays to zero
-//                (sedim_SALSA :795-801, DepositionFast :1127-1134)
-//   k_sed_vslow  DepositionSlow's (:910-1041) fall speeds of one category per
+//                (sedim_SALSA :792, DepositionFast :1129-1132)
+//   k_sed_vslow  DepositionSlow's (:926-1069) fall speeds of one category per
 //                (cell, bin), with the dry deposition velocity at the lowest
 //                level, into a work array
 //   k_sed_slow   one thread per interior cell: the bin fluxes at the cell's
 //                level and the level above, their divergence into the
 //                tendencies, the
-//                water divergence into tlt (:808-813, :834-839), the rain
+//                water divergence into tlt (:810-816, :833-839), the rain
 //                rate and, at the lowest level, the dry deposition velocity
 //                and the surface rate.  One launch per category, so the
 //                accumulation into rrate / tlt keeps the reference's order.
 //   k_sed_vt     DepositionFast's fall speed per (cell, precipitation bin)
 //                into VtPrc.
-//   k_sed_fall   the fall-distance redistribution (:1187-1260) per (column,
+//   k_sed_fall   the fall-distance redistribution (:1188-1253) per (column,
 //                bin, moved quantity), levels from the top down as in the
 //                reference, into the work arrays; deposited mass into the
 //                removal array.
 //   k_sed_apply  per cell: rain rate (bins in order), tendencies += change /
-//                dtlt, tlt (:856-864); the surface rate from the removal
+//                dtlt, tlt (:862-868); the surface rate from the removal
 //                array at the lowest level.
 // Terminal velocities: terminal_vel (mo_particle_external_properties.f90:
 // 43-100), liquid branches.  Contraction is off so that the products and
