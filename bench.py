@@ -63,10 +63,24 @@ def _cpu_worker(args):
     return hi - lo, dt
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(desc, f, max_cores=16):
     """CPU leg on the host cores: one process per core over contiguous slices
     of the same 491,520 cells, one step each.  Throughput = cells / the
-    slowest process's compute time (the processes run concurrently)."""
+    slowest process's compute time (the processes run concurrently).  The
+    process count is the CPU affinity of this process, capped at max_cores:
+    a one-GPU share of the benchmark host is 16 cores, whatever the machine
+    itself has (both counts are recorded)."""
     import multiprocessing as mp
     from multiprocessing import shared_memory
 
@@ -94,9 +108,10 @@ def cpu_baseline(desc, f, max_cores=16):
         off += a.nbytes
     del arrays, nc, vc
     try:
-        cores = min(max_cores, len(os.sched_getaffinity(0)))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = min(max_cores, os.cpu_count() or 1)
+        affinity = os.cpu_count() or 1
+    cores = min(max_cores, affinity)
     bounds = np.linspace(0, n, cores + 1).astype(int)
     ctx = mp.get_context("spawn")
     try:
@@ -111,7 +126,8 @@ def cpu_baseline(desc, f, max_cores=16):
             "src/src_salsa built with amdflang -O2; its own clock, file I/O excluded)"
             if kind == "reference" else "the FP64 C restatement (oracle/salsa_oracle.c, gcc -O2)")
     return {"value": n / slowest, "unit": "cell-steps/s", "cores": cores, "kind": kind, "per_core": per_core,
-            "sample": "all %d cells of the 64x64x120 workload, 1 step, %s, one process per core" % (n, what)}
+            "sample": "all %d cells of the 64x64x120 workload, 1 step, %s, one process per core" % (n, what),
+            "host_cpu_count": os.cpu_count(), "host_affinity": affinity, "cpu_model": _cpu_model()}
 
 
 def counter_run(kind, lanes, steps):
@@ -337,6 +353,7 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         kms, ksteps = ctx.kernel_time()
+        phases = ctx.phase_time()
         ctx.timing(False)
         shapes = [ctx.shape_of(k) for k in range(3)]
         # one in-place step for the accuracy comparison
@@ -349,16 +366,43 @@ def main():
         ctx.sync()
         ctx.close()
         st = (rv.cpu().numpy(), [x.cpu().numpy() for x in np_], [x.cpu().numpy() for x in mp_])
-        return wall / args.steps, kms / max(ksteps, 1), shapes, st
+        return wall / args.steps, kms / max(ksteps, 1), shapes, st, phases
 
-    step_mixed, kern_mixed, shapes_mixed, st_mixed = run(capi.SALSA_FP32_MIXED)
+    step_mixed, kern_mixed, shapes_mixed, st_mixed, ph_mixed = run(capi.SALSA_FP32_MIXED)
     res64 = None if args.no_fp64 else run(capi.SALSA_FP64)
 
     ncell = g.ncells
     value = ncell / step_mixed
     flops = opcount.flops_per_cell_step(desc, 4) * ncell
     bytes_ = opcount.bytes_per_cell_step(desc, 4, tendency=True) * ncell
-    achieved_tf = flops / (kern_mixed * 1e-3) / 1e12
+    fpc = opcount.flops_per_cell_step(desc, 4)
+    step_tf = flops / (kern_mixed * 1e-3) / 1e12
+
+    def kernels(ph):
+        """Per launch: the class kernels' average duration and cells, and
+        the dominant one (largest GPU time per step)."""
+        ms, cells, n = ph
+        n = max(n, 1)
+        per = {"gather_classify_ms": ms[0] / n, "scatter_ms": ms[4] / n}
+        cls = []
+        for k in range(3):
+            c = cells[k] / n
+            t = ms[1 + k] / n
+            per["class%d" % k] = {"ms": t, "cells": c,
+                                  "TFLOPs": fpc * c / (t * 1e-3) / 1e12 if t > 0 and c > 0 else 0.}
+            cls.append((t, k))
+        dom = max(cls)[1]
+        return per, dom
+
+    per_mixed, dom = kernels(ph_mixed)
+    d = per_mixed["class%d" % dom]
+    achieved_tf = d["TFLOPs"]
+    prof_traffic = None
+    prof = os.path.join(ROOT, "profiles", "traffic_k_salsa_step_64x64x120.json")
+    if os.path.exists(prof):
+        with open(prof) as fh:
+            tr = json.load(fh)
+        prof_traffic = tr.get("dominant_kernel_bytes_per_launch")
     out = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": step_mixed * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -367,28 +411,32 @@ def main():
                                "(nbin=(3,4)), SO4+H2O, dt=1 s, tendency mode, cloudy slab z in [60,90)" % ncell,
                    "cells": ncell, "layout": "11/8/7", "precision": "FP32-mixed",
                    "launch_lanes_per_cell": shapes_mixed},
-        "roofline": {"bound": "mfma", "engine": "VALU (no MFMA: not a dense contraction)",
+        "roofline": {"bound": "valu", "engine": "VALU (no MFMA: not a dense contraction)",
                      "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tf / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms_per_step": kern_mixed,
-                     "flop_per_cell_step": opcount.flops_per_cell_step(desc, 4),
+                     "frac": achieved_tf / FP32_PEAK_TFLOPS, "traffic": prof_traffic,
+                     "kernel": "k_salsa_step, cost class %d (%s), %d lanes per cell"
+                               % (dom, ("clear, RH <= 0.98", "clear, RH > 0.98", "cloudy")[dom],
+                                  shapes_mixed[dom]),
+                     "kernel_ms_per_launch": d["ms"], "cells_per_launch": d["cells"],
+                     "flop_per_cell_step": fpc,
+                     "step_achieved": step_tf, "step_frac": step_tf / FP32_PEAK_TFLOPS,
+                     "gpu_ms_per_step": kern_mixed, "launches": per_mixed,
                      "hbm_GBs_algorithmic": bytes_ / (kern_mixed * 1e-3) / 1e9,
                      "hbm_frac": bytes_ / (kern_mixed * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample",
+                                                                      "host_cpu_count", "host_affinity",
+                                                                      "cpu_model")},
     }
-    prof = os.path.join(ROOT, "profiles", "traffic_k_salsa_step_64x64x120.json")
-    if os.path.exists(prof):
-        with open(prof) as fh:
-            tr = json.load(fh)
-        out["roofline"]["traffic"] = tr.get("bytes_per_step")
     if cpu is not None:
         out["cpu_baseline"]["per_core"] = cpu["per_core"]
         out["speedup_vs_cpu"] = value / cpu["value"]
     if res64 is not None:
-        step64, kern64, shapes64, st64 = res64
-        out["fp64"] = {"value": ncell / step64, "ms_per_step": step64 * 1e3, "kernel_ms_per_step": kern64,
-                       "launch_lanes_per_cell": shapes64,
-                       "roofline_frac": flops / (kern64 * 1e-3) / 1e12 / 78.6}
+        step64, kern64, shapes64, st64, ph64 = res64
+        per64, dom64 = kernels(ph64)
+        out["fp64"] = {"value": ncell / step64, "ms_per_step": step64 * 1e3, "gpu_ms_per_step": kern64,
+                       "launch_lanes_per_cell": shapes64, "launches": per64,
+                       "dominant_kernel_frac": per64["class%d" % dom64]["TFLOPs"] / 78.6,
+                       "step_frac": flops / (kern64 * 1e-3) / 1e12 / 78.6, "peak_TFLOPs": 78.6}
         out["speedup_vs_fp64"] = step64 / step_mixed
         # accuracy of the mixed path against the FP64 path, every bin of every cell
         fo = lambda st: W.to_salsa_units(desc, W.Fields(grid=g, level=4, press=f.press, temp=f.temp, rsl=f.rsl,

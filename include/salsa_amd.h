@@ -101,7 +101,11 @@ enum {
 enum {
   SALSA_STEP_TENDENCY = 0,     /* run_SALSA semantics: p-fields read-only, t-fields accumulated     */
   SALSA_STEP_INPLACE = 1,      /* box-model / device-resident: write the new state into the p-fields */
-  SALSA_STEP_NO_COMPACT = 2    /* do not reorder cells by cost class (debug / comparison)          */
+  SALSA_STEP_NO_COMPACT = 2,   /* do not reorder cells by cost class (debug / comparison)          */
+  SALSA_STEP_INIT = 4          /* initialization call, run_SALSA(initialize=.TRUE.) from init.f90:
+                                  101-107: equilibration(init_rh = 0.3, init = .TRUE.) of every
+                                  cell's aerosol bins before the step (mo_salsa_driver.f90:282-283),
+                                  starting from the cell's own wet diameters                        */
 };
 
 /*
@@ -331,6 +335,12 @@ int salsa_autotune(salsa_ctx *ctx, const double *press, const double *temp,
  * cpu_time per LES step, step.f90:134-141.) */
 int salsa_ctx_timing(salsa_ctx *ctx, int32_t enable);
 int salsa_ctx_kernel_time(salsa_ctx *ctx, double *ms, int64_t *nsteps);
+/* The same events split by launch, summed over the timed steps: ms[0] gather
+ * and cost classification, ms[1..3] the step kernel of cost class 0, 1, 2,
+ * ms[4] scatter; cells[0..2] the cells each class kernel processed.  Each
+ * figure spans only its own launches (the class-count read-back between the
+ * classification and the class kernels is outside every span). */
+int salsa_ctx_phase_time(salsa_ctx *ctx, double ms[5], int64_t cells[3], int64_t *nsteps);
 
 /* Number of interior cells and of active bins in the context's layout. */
 int64_t salsa_ctx_ncells(const salsa_ctx *ctx);

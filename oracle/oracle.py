@@ -58,7 +58,7 @@ class Params(C.Structure):
         ("lcharge", C.c_int), ("charge_coll_enh", C.c_double),
         ("kpert", C.c_double), ("kseed", C.c_ulonglong),
         ("lsdistupdate", C.c_int), ("level", C.c_int), ("lsactiv", C.c_int),
-        ("diag_growth", C.c_int),
+        ("diag_growth", C.c_int), ("init", C.c_int),
     ]
 
 
@@ -140,8 +140,11 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def step_cells(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, rates=False, kstore=None, kmodes=None):
-    """Run `nsteps` box-model steps for each cell (SALSA units).
+def step_cells(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, rates=False, kstore=None, kmodes=None,
+               init=False):
+    """Run `nsteps` box-model steps for each cell (SALSA units).  With init the
+    first step is run_SALSA's initialization call (equilibration at RH 0.3
+    before the step, mo_salsa_driver.f90:282-283).
 
     numc: [ncell, ntotal]; volc: [ncell, ntotal, 9].  Returns
     (rv, numc, volc, status[, rates[ncell, 20, 10]]) as new arrays.
@@ -156,20 +159,26 @@ def step_cells(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, rates=Fals
     status = np.zeros(n, dtype=np.int32)
     rr = np.zeros((n, NRATE, MAXSPEC + 1)) if rates else None
     rbuf = (Rate * NRATE)()
+    Pi = None
+    if init:
+        Pi = Params()
+        C.memmove(C.byref(Pi), C.byref(P), C.sizeof(Params))
+        Pi.init = 1
     for i in range(n):
         x = C.c_double(rv[i])
         nc = numc[i]
         vc = volc[i]
         st = 0
         for k in range(nsteps):
+            Pk = Pi if (init and k == 0) else P
             if kstore is None:
-                st |= lb.so_step_cell(C.byref(L), C.byref(P), float(T[i]), float(p[i]), float(rs[i]),
+                st |= lb.so_step_cell(C.byref(L), C.byref(Pk), float(T[i]), float(p[i]), float(rs[i]),
                                       float(rsi[i]), C.byref(x), dt, _dp(nc), _dp(vc),
                                       rbuf if rates else None)
             else:
                 ks = kstore[i]
                 assert ks.flags["C_CONTIGUOUS"] and ks.shape == (L.ntotal, L.ntotal)
-                st |= lb.so_step_cell_ks(C.byref(L), C.byref(P), float(T[i]), float(p[i]), float(rs[i]),
+                st |= lb.so_step_cell_ks(C.byref(L), C.byref(Pk), float(T[i]), float(p[i]), float(rs[i]),
                                          float(rsi[i]), C.byref(x), dt, _dp(nc), _dp(vc),
                                          rbuf if rates else None, _dp(ks), int(kmodes[k]))
         rv[i] = x.value

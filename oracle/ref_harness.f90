@@ -17,7 +17,14 @@
 ! With sw(15) = 1 each step ends with distr_update (mo_salsa.f90:148-149).
 ! With lsauto%mode = 2 autoconv2 follows condensation (mo_salsa.f90:110-112);
 ! with dh(12) = 1 interstitial cloud activation (lsactiv%mode 1) follows it
-! (mo_salsa.f90:115-119).
+! (mo_salsa.f90:115-119).  With dh(13) = 1 the first step is an initialization
+! call: run_SALSA(initialize=.TRUE.) runs equilibration(init_rh = 0.3,
+! init = .TRUE.) on the packed cell before salsa() (mo_salsa_driver.f90:
+! 282-283, init_rh :39).  Its initial wet-diameter guess there is whatever
+! Section%dwet holds (run_SALSA leaves updateDiameter commented out at :270, so
+! the value comes from the cell processed before); this harness gives it the
+! cell's own updateDiameter(limit, "all") value instead, which is defined per
+! cell.
 !
 ! Usage: ref_harness <input.bin> <output.bin>
 PROGRAM ref_harness
@@ -32,6 +39,7 @@ PROGRAM ref_harness
   USE mo_salsa_dynamics, ONLY : coagulation, condensation
   USE mo_salsa_update, ONLY : distr_update
   USE mo_salsa_cloud, ONLY : autoconv2, cloud_activation
+  USE mo_salsa_properties, ONLY : equilibration
   USE classSection, ONLY : Section
   USE mo_salsa_coagulation_kernels, ONLY : update_coagulation_kernels
   USE classBinLayout, ONLY : BinLayout
@@ -46,7 +54,8 @@ PROGRAM ref_harness
   INTEGER :: ic, ist, nb, k, i, j, ntot_in
   REAL, ALLOCATABLE :: cellin(:), cellout(:), kmat(:,:)
   REAL :: T(1,1), P(1,1), RS(1,1), RSI(1,1), RV(1,1), dt
-  REAL :: g1(1,1), g2(1,1), g3(1,1), g4(1,1), g5(1,1), W(1,1)
+  REAL :: g1(1,1), g2(1,1), g3(1,1), g4(1,1), g5(1,1), W(1,1), RH0(1,1)
+  LOGICAL :: linit
   TYPE(Section), ALLOCATABLE :: pactd(:,:,:)
   INTEGER :: pbl(1)
   INTEGER(8) :: c0, c1, crate
@@ -86,6 +95,7 @@ PROGRAM ref_harness
   ncell = hdr(15)
   ntot_in = hdr(16)
   mode = sw(16)
+  linit = (dh(13) > 0.5)
   dt = dh(9)
   eddy_dis = dh(11)
   lscgaa = sw(1) /= 0; lscgcc = sw(2) /= 0; lscgpp = sw(3) /= 0; lscgii = sw(4) /= 0
@@ -141,6 +151,13 @@ PROGRAM ref_harness
            IF (sw(14) == 1) THEN
               lscoag%mode = 2
               lcgupdt = (ist == 1)
+           END IF
+           IF (linit .AND. ist == 1) THEN                ! mo_salsa_driver.f90:282-283
+              DO nb = 1, nt
+                 CALL allSALSA(1,1,nb)%updateDiameter(limit=.TRUE., type="all")
+              END DO
+              RH0 = 0.3
+              CALL equilibration(1, 1, 1, RH0, T, .TRUE.)
            END IF
            IF (lscoag%state) CALL coagulation(1, 1, 1, dt, T, P, lcharge)
            g1 = 0.; g2 = 0.; g3 = 0.; g4 = 0.; g5 = 0.; pbl = 1

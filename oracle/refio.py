@@ -18,7 +18,7 @@ def available():
 
 
 def run(L, P, level, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, mode=1, lcharge=0,
-        charge_enh=0.0, kstore=False, distr=False, activ=False):
+        charge_enh=0.0, kstore=False, distr=False, activ=False, init=False):
     """L, P: oracle Layout/Params (ctypes); numc [n, nt], volc [n, nt, 9] in
     SALSA units with nt = L.ntotal.  Returns dict with rv, numc, volc, rates
     (mode 1) or kmat [n, nt, nt] (mode 2), plus the reference bin table."""
@@ -33,6 +33,7 @@ def run(L, P, level, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, mode=1, lc
     dh[:11] = [L.reglim[0], L.reglim[1], L.reglim[2], L.reglim[3], L.prc_dlo, L.prc_vratio,
                L.ice_dlo, L.ice_vratio, dt, charge_enh, P.eddy_dis]
     dh[11] = 1.0 if activ else 0.0
+    dh[12] = 1.0 if init else 0.0          # first step is run_SALSA(initialize=.TRUE.)
     sw = np.zeros(16, np.int32)
     sw[:13] = [P.lscgaa, P.lscgcc, P.lscgpp, P.lscgii, P.lscgca, P.lscgpa, P.lscgia, P.lscgpc,
                P.lscgic, P.lscgip, P.lscndh2oae, P.lscndh2ocl, P.lscndh2oic]

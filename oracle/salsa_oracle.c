@@ -1768,6 +1768,14 @@ int so_step_cell_ks(const so_layout *L, const so_params *P,
     update_rhomean(L, &c, b);   /* mo_salsa_driver.f90:272 */
   }
   int status = 0;
+  if (P->init) {
+    /* run_SALSA initialization call, mo_salsa_driver.f90:282-283 with
+     * init_rh = 0.3 (:39).  The reference's initial guess is the Section's
+     * dwet as left by the cell processed before (updateDiameter is commented
+     * out at :270); the cell's own updateDiameter value stands in for it. */
+    for (int b = 0; b < L->ntotal; ++b) update_diameter(L, &c, b);
+    status |= equilibration(L, &c, 0.3, ptemp, 1);
+  }
   if (P->lscoag) status |= coagulation(L, P, &c, ptstep, ptemp, ppres, kstore, kstore ? kmode : 0);
   else for (int b = 0; b < L->ntotal; ++b) update_diameter(L, &c, b);
   if (P->lscnd && (P->lscndh2ocl || P->lscndh2oae || P->lscndh2oic))

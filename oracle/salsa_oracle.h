@@ -88,6 +88,11 @@ typedef struct {
    * not bind (the product over the substeps of the largest per-bin
    * multiplier).  Default 0. */
   int diag_growth;
+  int init;                    /* initialization call of run_SALSA (initialize =
+                                  .TRUE.): equilibration(init_rh = 0.3, init =
+                                  .TRUE.) before the step, mo_salsa_driver.f90:
+                                  282-283; initial wet diameters from
+                                  updateDiameter of the cell itself           */
 } so_params;
 
 /* One Rate (classProcessRates.f90:40-45): volc(9) and numc */

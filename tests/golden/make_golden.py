@@ -45,15 +45,16 @@ def pick(f, n, seed):
     return np.sort(np.concatenate(out))[:n]
 
 
-def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0, kstore=False, distr=False, activ=False):
+def save_step(name, L, level, P, cells, nsteps=1, lcharge=0, enh=0.0, kstore=False, distr=False, activ=False,
+              init=False):
     T, p, rs, rsi, rv, nc, vc = cells
     R = refio.run(L, P, level, T, p, rs, rsi, rv, nc, vc, nsteps=nsteps, lcharge=lcharge, charge_enh=enh,
-                  kstore=kstore, distr=distr, activ=activ)
+                  kstore=kstore, distr=distr, activ=activ, init=init)
     ns = L.nspec
     np.savez_compressed(
         os.path.join(HERE, name),
         level=level, nsteps=nsteps, lcharge=lcharge, charge_enh=enh, kstore=int(kstore), distr=int(distr),
-        lsactiv=int(activ),
+        lsactiv=int(activ), init=int(init),
         lsauto=P.lsauto_state, lsauto_mode=P.lsauto_mode, nbin=np.array(list(L.nbin)), nprc=L.nprc, nice=L.nice,
         species=np.array(list(L.dry_species)[:L.nspec_dry]),
         T=T, p=p, rs=rs, rsi=rsi, rv=rv, numc=nc, volc=vc[:, :, :ns],
@@ -148,7 +149,28 @@ def activation_cases():
               activ=True)
 
 
+def init_cases():
+    """run_SALSA's initialization call (initialize = .TRUE., mo_salsa_driver.f90:
+    282-283): equilibration at RH 0.3 before the step, then a second, normal
+    step.  Config-1 cells (water drawn far from the RH-0.3 equilibrium, so the
+    initialization moves it) and mixed-phase cells over one step (written
+    separately: `--only init`)."""
+    L = O.make_layout()
+    f1 = W.to_salsa_units(L, W.config_fields(L, 1), level=4)
+    nc, vc = pad(L, f1[5], f1[6])
+    cells = tuple(f1[:5]) + (nc, vc)
+    sel = pick(cells, 120, 41)
+    save_step("step_cfg1_init.npz", L, 4, O.make_params(level=4), tuple(x[sel] for x in cells), nsteps=2,
+              init=True)
+    L5 = O.make_layout(species=(0, 3))
+    f5 = W.to_salsa_units(L5, W.config_fields(L5, 5, grid_override=(20, 8, 8)), level=5)
+    sel = np.sort(np.random.default_rng(45).choice(len(f5[0]), 80, replace=False))
+    save_step("step_cfg5_init.npz", L5, 5, O.make_params(level=5), tuple(x[sel] for x in f5), init=True)
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "init":
+        return init_cases()
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "kstore":
         return kstore_case()
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "distr":

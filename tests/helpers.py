@@ -120,7 +120,7 @@ class FlatCells:
 
 
 def oracle_sensitivity(oracle_mod, L, P, T, p, rs, rsi, rv, numc, volc, eps=1e-13, seed=0, kernel_eps=0.0,
-                       nsteps=1):
+                       nsteps=1, init=False):
     """Spread of the oracle's own outputs under a relative input perturbation
     of size eps: the part of the reference's answer that its own rounding
     does not determine.  With kernel_eps > 0 the spread under a random
@@ -128,16 +128,16 @@ def oracle_sensitivity(oracle_mod, L, P, T, p, rs, rsi, rv, numc, volc, eps=1e-1
     in as well: what FP32-rounded collision kernels cannot determine.
     Returns (d_rv, d_numc, d_volc) relative spreads."""
     rng = np.random.default_rng(seed)
-    base = oracle_mod.step_cells(L, P, T, p, rs, rsi, rv, numc, volc, nsteps=nsteps)
+    base = oracle_mod.step_cells(L, P, T, p, rs, rsi, rv, numc, volc, nsteps=nsteps, init=init)
     runs = [oracle_mod.step_cells(L, P, T, p, rs, rsi, rv * (1 + eps * rng.standard_normal(len(rv))),
                                   numc * (1 + eps * rng.standard_normal(numc.shape)),
-                                  volc * (1 + eps * rng.standard_normal(volc.shape)), nsteps=nsteps)]
+                                  volc * (1 + eps * rng.standard_normal(volc.shape)), nsteps=nsteps, init=init)]
     if kernel_eps > 0:
         for ks in (1, 2):
             Pk = type(P).from_buffer_copy(P)
             Pk.kpert = kernel_eps
             Pk.kseed = ks
-            runs.append(oracle_mod.step_cells(L, Pk, T, p, rs, rsi, rv, numc, volc, nsteps=nsteps))
+            runs.append(oracle_mod.step_cells(L, Pk, T, p, rs, rsi, rv, numc, volc, nsteps=nsteps, init=init))
     out = []
     for i, fl in ((0, FLOOR_RV), (1, FLOOR_N), (2, FLOOR_V)):
         out.append(np.max([rel(r[i], base[i], fl) for r in runs], axis=0))

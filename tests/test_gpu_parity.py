@@ -28,8 +28,9 @@ def errors(rv, nc, vc, orv, onc, ovc, ns):
             max(rel(vc[:, :, s], ovc[:, :, s], FLOOR_V).max() for s in range(ns)))
 
 
-def run_flat(capi, desc, level, prec, cells, flags=1, nsteps=1, params_kw=None, rates=None):
-    """Run the GPU step on SALSA-unit cells (in place); returns SALSA-unit state."""
+def run_flat(capi, desc, level, prec, cells, flags=1, nsteps=1, params_kw=None, rates=None, init=False):
+    """Run the GPU step on SALSA-unit cells (in place); returns SALSA-unit state.
+    With init the first step is the initialization call (SALSA_STEP_INIT)."""
     T, p, rs, rsi, rv, numc, volc = cells
     fc = FlatCells(desc, level, T, p, rs, rsi, rv, numc, volc)
     kw = dict(hz=0, hxy=0)
@@ -41,8 +42,8 @@ def run_flat(capi, desc, level, prec, cells, flags=1, nsteps=1, params_kw=None, 
     f = dict(fc.f)
     f["rv"] = f["rv"].copy()
     inputs = fc.unpack(fc.n_p, fc.m_p, fc.f["rv"])
-    for _ in range(nsteps):
-        ctx.step_host(f, n_p, m_p, flags=flags, rates=rates)
+    for k in range(nsteps):
+        ctx.step_host(f, n_p, m_p, flags=flags | (capi.STEP_INIT if init and k == 0 else 0), rates=rates)
     out = fc.unpack(n_p, m_p, f["rv"])
     flags_out = ctx.cell_flags()
     ctx.close()
@@ -64,6 +65,8 @@ GOLDEN_CASES = [
     ("step_cfg1_activ.npz", {"lsactiv": 1}),
     ("step_cfg4_activ_auto2_distr.npz", {"lsactiv": 1, "lsauto_mode": 2, "lsdistupdate": 1}),
     ("step_cfg5_activ.npz", {"lsactiv": 1}),
+    ("step_cfg1_init.npz", {}),
+    ("step_cfg5_init.npz", {}),
 ]
 
 
@@ -82,8 +85,9 @@ def test_gpu_matches_reference_golden(capi, oracle_mod, name, kw, prec):
     volc = full_volc(g["volc"])[:, :nt]
     cells = (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], numc, volc)
     nsteps = int(g["nsteps"])
+    init = bool(int(g["init"])) if "init" in g else False     # first step is run_SALSA(initialize=.TRUE.)
     print("case %s prec %d, %d steps" % (name, prec, nsteps), flush=True)
-    inp, out, fl = run_flat(capi, desc, level, prec, cells, params_kw=kw, nsteps=nsteps)
+    inp, out, fl = run_flat(capi, desc, level, prec, cells, params_kw=kw, nsteps=nsteps, init=init)
     assert not np.any(fl & capi.CELL_NONFINITE)
     # against the compiled reference (inputs differ from the packed ones by <= 1 ulp)
     e_ref = errors(out[0], out[1], out[2], g["out_rv"], g["out_numc"][:, :nt], g["out_volc"][:, :nt], ns)
@@ -95,11 +99,12 @@ def test_gpu_matches_reference_golden(capi, oracle_mod, name, kw, prec):
     vco = np.zeros((len(irv), L.ntotal, 9))
     nco[:, :nt] = inc
     vco[:, :nt] = ivc
-    orv, onc, ovc, _ = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco, nsteps=nsteps)
+    orv, onc, ovc, _ = oracle_mod.step_cells(L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco, nsteps=nsteps,
+                                             init=init)
     e_orc = errors(out[0], out[1], out[2], orv, onc[:, :nt], ovc[:, :nt], ns)
     print("  vs reference rv %.2e N %.2e V %.2e | vs oracle rv %.2e N %.2e V %.2e" % (e_ref + e_orc))
     srv, snc, svc = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco,
-                                       kernel_eps=KEPS[prec], nsteps=nsteps)
+                                       kernel_eps=KEPS[prec], nsteps=nsteps, init=init)
     sens = (srv, snc[:, :nt], svc[:, :nt])
     nbs = [desc.nbins, desc.ncld, desc.nprc] + ([desc.nice] if level == 5 else [])
     cats = list(zip(np.cumsum([0] + nbs)[:-1], np.cumsum(nbs)))
@@ -115,7 +120,7 @@ def test_gpu_matches_reference_golden(capi, oracle_mod, name, kw, prec):
     # fraction (vhi_i - vhi)/(vhi_i - vlo_i) of its particles that a few
     # ulps change relatively by 1e-9 and more (mo_salsa_update.f90:96-107).
     s15 = oracle_sensitivity(oracle_mod, L, P, g["T"], g["p"], g["rs"], g["rsi"], irv, nco, vco, eps=1e-15,
-                             nsteps=nsteps)
+                             nsteps=nsteps, init=init)
     n_ill = int(np.sum(10 * s15[0] > TOL[prec]) + np.sum(10 * s15[1][:, :nt] > TOL[prec]) +
                 np.sum(10 * s15[2][:, :nt, :ns] > TOL[prec]))
     lim = max(2, nel // 200, n_ill)

@@ -174,3 +174,34 @@ def test_oracle_activation_matches_reference(oracle_mod, name):
                                        (sens[0], sens[1], sens[2][:, :, :ns]), 1e-12, ns, name)
     print(name, "max rel", out, "elements needing the conditioning allowance:", nflag)
     assert nflag <= 0.002 * nc.size
+
+
+INIT_CASES = ["step_cfg1_init.npz", "step_cfg5_init.npz"]
+
+
+@pytest.mark.parametrize("name", INIT_CASES)
+def test_oracle_init_call_matches_reference(oracle_mod, name):
+    """run_SALSA's initialization call (initialize = .TRUE.): equilibration at
+    RH 0.3 with init = .TRUE. before the step (mo_salsa_driver.f90:282-283,
+    mo_salsa_properties.f90:73-380), then the step itself, against the
+    compiled reference run the same way; each element within max(1e-12,
+    10 x the oracle's own spread under 1e-13 input noise)."""
+    g = golden(name)
+    assert int(g["init"]) == 1
+    L = oracle_layout_for(g)
+    P = oracle_params_for(g)
+    nsteps = int(g["nsteps"])
+    args = (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"], full_volc(g["volc"]))
+    rv, nc, vc, st = oracle_mod.step_cells(L, P, *args, nsteps=nsteps, init=True)
+    assert not np.any(st & 2)
+    # the initialization acted: the same steps without it end elsewhere
+    rv0, _, vc0, _ = oracle_mod.step_cells(L, P, *args, nsteps=nsteps)
+    iwa = L.iwa
+    moved = rel(vc0[:, :L.nbins, iwa], g["out_volc"][:, :L.nbins, iwa], FLOOR_V) > 1e-6
+    assert np.count_nonzero(moved) > 0.05 * moved.size
+    sens = oracle_sensitivity(oracle_mod, L, P, *args, nsteps=nsteps, init=True)
+    ns = g["volc"].shape[2]
+    out, nflag = check_condition_aware((rv, nc, vc[:, :, :ns]), (g["out_rv"], g["out_numc"], g["out_volc"]),
+                                       (sens[0], sens[1], sens[2][:, :, :ns]), 1e-12, ns, name)
+    print(name, "max rel", out, "elements needing the conditioning allowance:", nflag)
+    assert nflag <= 0.002 * nc.size

@@ -76,10 +76,13 @@ struct salsa_ctx {
   // staging for salsa_step_host
   double *stage = nullptr;
   size_t stage_bytes = 0;
-  // kernel timing
+  // kernel timing: per timed step kEvPerStep events around the launches
+  // (gather, the three class kernels, scatter) and the class cell counts
   bool timing = false;
-  std::vector<hipEvent_t> ev;         // start/stop pairs of the timed steps
+  std::vector<hipEvent_t> ev;
+  std::vector<int64_t> ev_cells;      // [3] per timed step
 };
+constexpr int kEvPerStep = 10;
 
 // ---------------------------------------------------------------------------
 // small kernels: fault reset, cost classes
@@ -255,6 +258,7 @@ static StepArgs make_args(salsa_ctx *c, const double *press, const double *temp,
   a.dt = tstep;
   a.nsub = substeps(tstep);
   a.inplace = (flags & SALSA_STEP_INPLACE) ? 1 : 0;
+  a.init = (flags & SALSA_STEP_INIT) ? 1 : 0;
   a.lsfreeRH = c->params.lsfreeRH;
   a.rhlim = c->params.rhlim;
   a.rates = rates;
@@ -297,6 +301,7 @@ static int timing_mark(salsa_ctx *c) {
   c->ev.push_back(e);
   return SALSA_OK;
 }
+#define MARK() do { if (timing_mark(c)) return SALSA_ERR_HIP; } while (0)
 
 static int launch_class(salsa_ctx *c, int k, const StepArgs &a, hipStream_t s) {
   const bool hyd = k == 2;
@@ -316,6 +321,7 @@ static int enqueue_gather(salsa_ctx *c, const StepArgs &a, bool compact) {
   int rc = launch_gather(c->hostL, c->devL, a, c->rec_in, compact ? c->cls : nullptr, c->tcount, c->toff, c->lists,
                          c->cls_count, s);
   if (rc) { std::snprintf(g_err, sizeof(g_err), "gather launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  MARK();
   if (compact) {
     HIPCHK(hipMemcpyAsync(c->cls_count_h, c->cls_count, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -333,26 +339,33 @@ static StepArgs class_args(salsa_ctx *c, const StepArgs &a, int k) {
 static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
   hipStream_t s = c->stream;
   const bool compact = !(flags & SALSA_STEP_NO_COMPACT);
-  if (timing_mark(c)) return SALSA_ERR_HIP;
+  MARK();
   int rc = enqueue_gather(c, a, compact);
   if (rc) return rc;
-  if (!compact) {
+  int64_t cells[3] = {0, 0, 0};
+  for (int k = 0; k < 3; ++k) {
+    MARK();
     StepArgs b = a;
-    b.cell_list = nullptr;
-    rc = launch_class(c, 2, b, s);
-    if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
-  } else {
-    for (int k = 0; k < 3; ++k) {
-      const StepArgs b = class_args(c, a, k);
-      if (b.ncells == 0) continue;
+    if (!compact) {
+      b.cell_list = nullptr;          // every cell through the cloudy variant
+      if (k != 2) b.ncells = 0;
+    } else {
+      b = class_args(c, a, k);
+    }
+    if (b.ncells > 0) {
       rc = launch_class(c, k, b, s);
       if (rc) { std::snprintf(g_err, sizeof(g_err), "kernel launch failed (%d)", rc); return SALSA_ERR_HIP; }
     }
+    cells[k] = b.ncells;
+    MARK();
   }
   // cell records -> tendencies or new state
+  MARK();
   rc = launch_scatter(c->hostL, c->devL, a, s);
   if (rc) { std::snprintf(g_err, sizeof(g_err), "scatter launch failed (%d)", rc); return SALSA_ERR_HIP; }
-  return timing_mark(c);
+  MARK();
+  if (c->timing) c->ev_cells.insert(c->ev_cells.end(), cells, cells + 3);
+  return SALSA_OK;
 }
 
 extern "C" int salsa_ctx_timing(salsa_ctx *c, int32_t enable) {
@@ -360,7 +373,31 @@ extern "C" int salsa_ctx_timing(salsa_ctx *c, int32_t enable) {
   HIPCHK(hipStreamSynchronize(c->stream));
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   c->ev.clear();
+  c->ev_cells.clear();
   c->timing = enable != 0;
+  return SALSA_OK;
+}
+
+extern "C" int salsa_ctx_phase_time(salsa_ctx *c, double ms[5], int64_t cells[3], int64_t *nsteps) {
+  if (!c || !ms || !cells || !nsteps) { set_error("null argument"); return SALSA_ERR_ARG; }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 5; ++k) ms[k] = 0.;
+  for (int k = 0; k < 3; ++k) cells[k] = 0;
+  const size_t n = c->ev.size() / kEvPerStep;
+  auto el = [&](size_t st, int i, int j, double &acc) -> int {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev[st * kEvPerStep + i], c->ev[st * kEvPerStep + j]));
+    acc += t;
+    return SALSA_OK;
+  };
+  for (size_t st = 0; st < n; ++st) {
+    if (el(st, 0, 1, ms[0])) return SALSA_ERR_HIP;                 // gather + classification
+    for (int k = 0; k < 3; ++k)
+      if (el(st, 2 + 2 * k, 3 + 2 * k, ms[1 + k])) return SALSA_ERR_HIP;   // class k step kernel
+    if (el(st, 8, 9, ms[4])) return SALSA_ERR_HIP;                 // scatter
+    for (int k = 0; k < 3; ++k) cells[k] += c->ev_cells[3 * st + k];
+  }
+  *nsteps = (int64_t)n;
   return SALSA_OK;
 }
 
@@ -368,13 +405,14 @@ extern "C" int salsa_ctx_kernel_time(salsa_ctx *c, double *ms, int64_t *nsteps) 
   if (!c || !ms || !nsteps) { set_error("null argument"); return SALSA_ERR_ARG; }
   HIPCHK(hipStreamSynchronize(c->stream));
   double tot = 0.;
-  for (size_t k = 0; k + 1 < c->ev.size(); k += 2) {
+  const size_t n = c->ev.size() / kEvPerStep;
+  for (size_t st = 0; st < n; ++st) {
     float t = 0.f;
-    HIPCHK(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
+    HIPCHK(hipEventElapsedTime(&t, c->ev[st * kEvPerStep], c->ev[st * kEvPerStep + kEvPerStep - 1]));
     tot += t;
   }
   *ms = tot;
-  *nsteps = (int64_t)(c->ev.size() / 2);
+  *nsteps = (int64_t)n;
   return SALSA_OK;
 }
 

@@ -162,6 +162,7 @@ struct StepArgs {
   double dt;
   int32_t nsub;                       // condensation substeps for dt
   int32_t inplace;
+  int32_t init;                       // initialization call: equilibration at RH 0.3 first
   int32_t lsfreeRH;
   double rhlim;
   uint8_t *cell_flags;                // one byte per interior cell, or null

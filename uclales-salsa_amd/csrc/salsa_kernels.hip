@@ -355,6 +355,112 @@ __device__ __forceinline__ int64_t les_index(const StepArgs &a, int64_t cell) {
   return (int64_t)(k + a.hz) + (int64_t)a.nzp * ((int64_t)(i + a.hxy) + (int64_t)a.nxp * (j + a.hxy));
 }
 
+// Equilibration (mo_salsa_properties.f90:73-380) of one cell's aerosol bins,
+// both regimes in one loop.  Regime 1 (1a bins, :165-269) is one sequential
+// chain -- the Kelvin factor carries over from bin to bin -- and runs on lane
+// 0 of the cell; the bins of regime 2 (2a/2b, the init branch :275-378, run
+// when reg2) are independent and go to the cell's lanes 1, 2, ... (lane 0
+// takes what is left after its chain).  Every lane iterates its current
+// bin's Kelvin fixed point; the wavefront loops until all lanes are done, so
+// the two regimes overlap.  zrh is the relative humidity before the regime
+// clamps ([0.05, 0.98] and [0.37, 0.98]); the initial wet diameters come from
+// sDW.  The aerosol water volumes are written to sV.
+template <int L>
+__device__ __forceinline__ void equilibrate(const DevLayout &D, double *sN, double *sV, int nsV, const double *sDW,
+                                            double T, double zrh, bool reg2, bool live, int sub, uint32_t &flags) {
+#define VV(b, s) sV[(b) * nsV + (s)]
+  const int iwa = D.iwa, ndry = D.ndry;
+  const int A0 = D.off[0];
+  int isu = -1, ioc = -1;
+  for (int s = 0; s < ndry; ++s) {
+    if (D.dry_id[s] == 0) isu = s;
+    if (D.dry_id[s] == 1) ioc = s;
+  }
+  const int n1 = D.fn1a - D.in1a + 1;
+  const int n2 = reg2 ? D.fn2b - D.in2a + 1 : 0;
+  // this lane's queue: regime-1 bins (lane 0), then regime-2 bins j with
+  // (j + 1) % L == sub
+  int q1 = sub == 0 ? 0 : n1;           // next regime-1 position
+  int q2 = (sub + L - 1) % L;           // next regime-2 index
+  double zke1 = 1.001;                  // regime-1 Kelvin factor, carried over
+  int b = -1, reg = 0, count = 0;
+  double n = 1., zcore = 0., c1 = 0., c2 = 0., c5 = 0., zr = 0., tol = 0.;
+  double zdwet = 0., zdold = 1., zlwc = 0., zke = 1.;
+  auto next_task = [&]() {
+    reg = 0;
+    while (q1 < n1) {                   // regime 1: skip bins with numc <= nlim
+      const int bb = A0 + D.in1a + q1++;
+      if (sN[bb] > kNlim) {
+        b = bb; reg = 1; n = sN[bb];
+        const double v1 = isu >= 0 ? VV(bb, isu) : 0., v2 = ioc >= 0 ? VV(bb, ioc) : 0.;
+        zcore = v1 / n + v2 / n;
+        c1 = v1 * (kRhosu / kMsu);
+        c2 = kEpsoc * v2 * (kRhooc / kMoc);
+        c5 = 0.;
+        zr = fmin(fmax(zrh, 0.05), 0.98);
+        tol = 1.e-2;
+        zke = zke1;
+        zdwet = sDW[bb]; zdold = 1.; zlwc = 0.; count = 0;
+        return;
+      }
+    }
+    while (q2 < n2) {                   // regime 2
+      const int bb = A0 + D.in2a + q2;
+      q2 += L;
+      if (sN[bb] > kNlim) {
+        b = bb; reg = 2; n = sN[bb];
+        double zv[5] = {0., 0., 0., 0., 0.};
+        for (int s = 0; s < ndry; ++s)
+          if (D.dry_id[s] < 5) zv[D.dry_id[s]] = VV(bb, s);
+        zcore = zv[0] / n + zv[1] / n + zv[2] / n + zv[3] / n + zv[4] / n;
+        c1 = zv[0] * (kRhosu / kMsu);
+        c2 = kEpsoc * (zv[1] * (kRhooc / kMoc));
+        c5 = zv[4] * (kRhoss / kMss);
+        zr = fmin(fmax(zrh, 0.37), 0.98);
+        tol = 1.e-12;
+        zke = 1.02;
+        zdwet = sDW[bb]; zdold = 1.; zlwc = 0.; count = 0;
+        return;
+      }
+    }
+  };
+  if (live) next_task();
+  // a task is live while the reference's loop condition holds
+  auto finish = [&]() {
+    VV(b, iwa) = zlwc / kRhowa;
+    if (reg == 1) zke1 = zke;
+    next_task();
+  };
+  while (reg && !(fabs(zdwet / zdold - 1.) > tol)) finish();   // condition false on entry
+#pragma unroll 1
+  while (__any(reg != 0)) {
+    if (reg) {
+      zdold = fmax(zdwet, 1.e-20);
+      const double aw = div_nr(zr, zke);
+      const double aw2 = aw * aw;
+      const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * aw2
+                         - 3.1543839e+2 * (aw2 * aw) + 6.770824e+1 * (aw2 * aw2);
+      const double bm2 = rcp_nr(aw * kMwa) - 1. / kMwa;
+      double l = div_nr(c1, bm1) + div_nr(c2, bm2);
+      if (reg == 2) {
+        const double bm5 = 5.875248e+1 - 1.8781997e+2 * aw + 2.7211377e+2 * aw2
+                           - 1.8458287e+2 * (aw2 * aw) + 4.153689e+1 * (aw2 * aw2);
+        l = l + div_nr(c5, bm5);
+      }
+      zlwc = l;
+      zdwet = cbrt_fast(div_nr(div_nr(div_nr(zlwc, n), kRhowa), kPi6) + div_nr(zcore, kPi6));
+      zke = ::exp(div_nr(2. * kSurfw0 * kMvwa, kBoltz * T * zdwet));
+      bool done = !(fabs(zdwet / zdold - 1.) > tol);
+      if (++count > 100) { flags |= SALSA_CELL_NOCONV; done = true; }
+      if (done) {
+        finish();
+        while (reg && !(fabs(zdwet / zdold - 1.) > tol)) finish();
+      }
+    }
+  }
+#undef VV
+}
+
 // Per-cell rate accumulators of the process-rate diagnostics
 // (classProcessRates.f90:60-98), as the volumes the LES fields are made from.
 enum RateAcc {
@@ -487,6 +593,31 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   }
   for (int e = lane; e < D.nent_total; e += 64) tent[e] = D.ent[e];
   __syncthreads();
+
+  // ---- 0. initialization call (run_SALSA with initialize = .TRUE.,
+  //      mo_salsa_driver.f90:282-283): equilibration(init_rh = 0.3,
+  //      init = .TRUE.) of the packed cell before the step.  The initial wet
+  //      diameters are the cell's own updateDiameter(limit, "all") values
+  //      (classSection.f90:91-159); the reference uses whatever the shared
+  //      Section array was left holding by the cell processed before. ----
+  if (a.init) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < D.nb[0]) {
+        const double n = sN[b];
+        double vs = 0.;
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          if (s < nsV) vs += VV(b, s);
+        const double dw = n > D.nlim[b] ? cbrt_fast(vs / n / kPi6) : D.dmid[b];
+        sDW[b] = fmin(dw, D.dlim[b]);
+      }
+    }
+    __syncthreads();
+    equilibrate<L>(D, sN, sV, nsV, sDW, T, 0.3, true, live, sub, flags);
+    __syncthreads();
+  }
 
   // ---- 2./3./4. coagulation (mo_salsa_dynamics.f90:92-202) ----
   if (D.lscoag) {
@@ -869,104 +1000,9 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
     }
     w1 = cell_sum<L>(w1) * kRhowa;
 
-    // Equilibration (mo_salsa_properties.f90:73-380), both regimes in one
-    // loop.  Regime 1 (1a bins, :165-269) is one sequential chain -- the
-    // Kelvin factor carries over from bin to bin -- and runs on lane 0 of the
-    // cell; the bins of regime 2 (2a/2b, init call, :275-378; only when
-    // zrh < 0.98 or without aerosol condensation) are independent and go to
-    // the cell's lanes 1, 2, ... (lane 0 takes what is left after its chain).
-    // Every lane iterates its current bin's Kelvin fixed point; the
-    // wavefront loops until all lanes are done, so the two regimes overlap.
-    int isu = -1, ioc = -1;
-    for (int s = 0; s < ndry; ++s) {
-      if (D.dry_id[s] == 0) isu = s;
-      if (D.dry_id[s] == 1) ioc = s;
-    }
-    {
-      const bool reg2 = zrh < 0.98 || !D.lscndh2oae;
-      const int n1 = D.fn1a - D.in1a + 1;
-      const int n2 = reg2 ? D.fn2b - D.in2a + 1 : 0;
-      // this lane's queue: regime-1 bins (lane 0), then regime-2 bins j with
-      // (j + 1) % L == sub
-      int q1 = sub == 0 ? 0 : n1;           // next regime-1 position
-      int q2 = (sub + L - 1) % L;           // next regime-2 index
-      double zke1 = 1.001;                  // regime-1 Kelvin factor, carried over
-      int b = -1, reg = 0, count = 0;
-      double n = 1., zcore = 0., c1 = 0., c2 = 0., c5 = 0., zr = 0., tol = 0.;
-      double zdwet = 0., zdold = 1., zlwc = 0., zke = 1.;
-      auto next_task = [&]() {
-        reg = 0;
-        while (q1 < n1) {                   // regime 1: skip bins with numc <= nlim
-          const int bb = A0 + D.in1a + q1++;
-          if (sN[bb] > kNlim) {
-            b = bb; reg = 1; n = sN[bb];
-            const double v1 = isu >= 0 ? VV(bb, isu) : 0., v2 = ioc >= 0 ? VV(bb, ioc) : 0.;
-            zcore = v1 / n + v2 / n;
-            c1 = v1 * (kRhosu / kMsu);
-            c2 = kEpsoc * v2 * (kRhooc / kMoc);
-            c5 = 0.;
-            zr = fmin(fmax(zrh, 0.05), 0.98);
-            tol = 1.e-2;
-            zke = zke1;
-            zdwet = sDW[bb]; zdold = 1.; zlwc = 0.; count = 0;
-            return;
-          }
-        }
-        while (q2 < n2) {                   // regime 2
-          const int bb = A0 + D.in2a + q2;
-          q2 += L;
-          if (sN[bb] > kNlim) {
-            b = bb; reg = 2; n = sN[bb];
-            double zv[5] = {0., 0., 0., 0., 0.};
-            for (int s = 0; s < ndry; ++s)
-              if (D.dry_id[s] < 5) zv[D.dry_id[s]] = VV(bb, s);
-            zcore = zv[0] / n + zv[1] / n + zv[2] / n + zv[3] / n + zv[4] / n;
-            c1 = zv[0] * (kRhosu / kMsu);
-            c2 = kEpsoc * (zv[1] * (kRhooc / kMoc));
-            c5 = zv[4] * (kRhoss / kMss);
-            zr = fmin(fmax(zrh, 0.37), 0.98);
-            tol = 1.e-12;
-            zke = 1.02;
-            zdwet = sDW[bb]; zdold = 1.; zlwc = 0.; count = 0;
-            return;
-          }
-        }
-      };
-      if (live) next_task();
-      // a task is live while the reference's loop condition holds
-      auto finish = [&]() {
-        VV(b, iwa) = zlwc / kRhowa;
-        if (reg == 1) zke1 = zke;
-        next_task();
-      };
-      while (reg && !(fabs(zdwet / zdold - 1.) > tol)) finish();   // condition false on entry
-#pragma unroll 1
-      while (__any(reg != 0)) {
-        if (reg) {
-          zdold = fmax(zdwet, 1.e-20);
-          const double aw = div_nr(zr, zke);
-          const double aw2 = aw * aw;
-          const double bm1 = 1.1065495e+2 - 3.6759197e+2 * aw + 5.0462934e+2 * aw2
-                             - 3.1543839e+2 * (aw2 * aw) + 6.770824e+1 * (aw2 * aw2);
-          const double bm2 = rcp_nr(aw * kMwa) - 1. / kMwa;
-          double l = div_nr(c1, bm1) + div_nr(c2, bm2);
-          if (reg == 2) {
-            const double bm5 = 5.875248e+1 - 1.8781997e+2 * aw + 2.7211377e+2 * aw2
-                               - 1.8458287e+2 * (aw2 * aw) + 4.153689e+1 * (aw2 * aw2);
-            l = l + div_nr(c5, bm5);
-          }
-          zlwc = l;
-          zdwet = cbrt_fast(div_nr(div_nr(div_nr(zlwc, n), kRhowa), kPi6) + div_nr(zcore, kPi6));
-          zke = ::exp(div_nr(2. * kSurfw0 * kMvwa, kBoltz * T * zdwet));
-          bool done = !(fabs(zdwet / zdold - 1.) > tol);
-          if (++count > 100) { flags |= SALSA_CELL_NOCONV; done = true; }
-          if (done) {
-            finish();
-            while (reg && !(fabs(zdwet / zdold - 1.) > tol)) finish();
-          }
-        }
-      }
-    }
+    // Equilibration (mo_salsa_properties.f90:73-380): regime 1 always, regime
+    // 2 only when zrh < 0.98 or without aerosol condensation (:769-779)
+    equilibrate<L>(D, sN, sV, nsV, sDW, T, zrh, zrh < 0.98 || !D.lscndh2oae, live, sub, flags);
     __syncthreads();
     double w2 = 0.;
 #pragma unroll

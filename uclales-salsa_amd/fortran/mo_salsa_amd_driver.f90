@@ -68,6 +68,7 @@ CONTAINS
     TYPE(C_PTR) :: n_p(4), n_t(4), m_p(4), m_t(4), crsi
     INTEGER :: nrf, f
     INTEGER(C_INT) :: rc
+    INTEGER(C_INT32_T) :: sflags
 
     CALL check_switches()
     IF (.NOT. C_ASSOCIATED(ctx)) CALL create_context(level, lcharge, nzp, nxp, nyp)
@@ -110,10 +111,14 @@ CONTAINS
        rates(:,:,:,f) = fld%d(:,:,:)
     END DO
 
+    ! the initialization call (init.f90:101-107) equilibrates the aerosol water
+    ! at RH 0.3 before the step (mo_salsa_driver.f90:282-283)
+    sflags = SALSA_STEP_TENDENCY
+    IF (initialize) sflags = IOR(sflags, SALSA_STEP_INIT)
     rc = salsa_step_host(ctx, C_LOC(ppress%d(1,1,1)), C_LOC(ptk%d(1,1,1)), C_LOC(prs%d(1,1,1)), crsi,   &
                          C_LOC(pdn%d(1,1,1)), C_LOC(prv%d(1,1,1)), C_LOC(prt%d(1,1,1)),                  &
                          n_p, n_t, m_p, m_t, C_LOC(rates(1,1,1,1)), REAL(tstep, C_DOUBLE),               &
-                         REAL(time, C_DOUBLE), INT(istp, C_INT32_T), SALSA_STEP_TENDENCY)
+                         REAL(time, C_DOUBLE), INT(istp, C_INT32_T), sflags)
     IF (rc /= SALSA_OK) THEN
        WRITE(*,*) 'run_SALSA (libsalsa_amd): ', TRIM(salsa_error_message())
        STOP
@@ -127,7 +132,7 @@ CONTAINS
     ! newly activated droplets: activation is not part of this step
     pa_nactd = 0.
     pa_vactd = 0.
-    IF (.FALSE.) WRITE(*,*) wp(1,1,1), initialize
+    IF (.FALSE.) WRITE(*,*) wp(1,1,1)
   END SUBROUTINE run_SALSA
 
   SUBROUTINE salsa_amd_finalize()

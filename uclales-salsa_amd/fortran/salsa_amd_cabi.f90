@@ -16,7 +16,7 @@ MODULE mo_salsa_amd_cabi
   ! precision modes and step flags
   INTEGER(C_INT32_T), PARAMETER :: SALSA_FP64 = 0, SALSA_FP32_MIXED = 1
   INTEGER(C_INT32_T), PARAMETER :: SALSA_STEP_TENDENCY = 0, SALSA_STEP_INPLACE = 1, &
-                                   SALSA_STEP_NO_COMPACT = 2
+                                   SALSA_STEP_NO_COMPACT = 2, SALSA_STEP_INIT = 4
   ! process-rate fields
   INTEGER, PARAMETER :: SALSA_NRATE_L4 = 12, SALSA_NRATE_L5 = 17
 

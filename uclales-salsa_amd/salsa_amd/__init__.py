@@ -2,7 +2,7 @@
 
 salsa_amd.capi     -- ctypes binding of the C ABI (include/salsa_amd.h)
 salsa_amd.workload -- seeded synthetic LES fields
-salsa_amd.driver   -- run_SALSA-shaped Python front end over device tensors
+salsa_amd.opcount  -- algorithmic work per cell-step (flops, HBM bytes)
 """
 from . import capi, workload  # noqa: F401
 

@@ -28,6 +28,7 @@ SALSA_FP32_MIXED = 1
 STEP_TENDENCY = 0
 STEP_INPLACE = 1
 STEP_NO_COMPACT = 2
+STEP_INIT = 4
 
 CELL_NEG_NUMC = 1
 CELL_NOCONV = 2
@@ -43,7 +44,7 @@ EXPORTS = [
     "salsa_step_host", "salsa_step_device", "salsa_ctx_sync", "salsa_get_cell_flags",
     "salsa_set_launch_shape", "salsa_get_launch_shape", "salsa_autotune", "salsa_ctx_ncells",
     "salsa_ctx_ntotal", "salsa_ctx_stream", "salsa_last_error", "salsa_ctx_timing",
-    "salsa_ctx_kernel_time",
+    "salsa_ctx_kernel_time", "salsa_ctx_phase_time",
 ]
 
 
@@ -161,6 +162,9 @@ def lib():
     L.salsa_ctx_timing.restype = C.c_int
     L.salsa_ctx_kernel_time.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.salsa_ctx_kernel_time.restype = C.c_int
+    L.salsa_ctx_phase_time.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64)]
+    L.salsa_ctx_phase_time.restype = C.c_int
     L.salsa_gradient_device.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [_P4, _P4, C.c_double, C.c_int32,
                                                                           _P4, C.c_void_p, C.c_void_p, _dp]
     L.salsa_gradient_device.restype = C.c_int
@@ -353,6 +357,16 @@ class Context:
         n = C.c_int64()
         _check(lib().salsa_ctx_kernel_time(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def phase_time(self):
+        """(ms[5], cells[3], steps): GPU time of gather + classification, of
+        the class 0/1/2 step kernels and of the scatter, and the cells of each
+        class, summed over the steps timed since timing(True)."""
+        ms = (C.c_double * 5)()
+        cells = (C.c_int64 * 3)()
+        n = C.c_int64()
+        _check(lib().salsa_ctx_phase_time(self._h, ms, cells, C.byref(n)))
+        return list(ms), list(cells), n.value
 
     def set_shape(self, cls, lanes):
         _check(lib().salsa_set_launch_shape(self._h, cls, lanes))
