@@ -137,6 +137,7 @@ struct SmemGeom {
   int32_t it0, it1;    // sweep entries visited
   uint64_t jmask;      // collector bins that can be active
   int32_t orr;         // byte offset of the rate accumulators, -1 without rates
+  int32_t ovd;         // byte offset of the fall-speed residuals [2][ntab] (mixed path), -1 in FP64
 };
 
 // Per-launch arguments (passed by value).

@@ -212,37 +212,47 @@ __device__ PTerms particle_terms(const CellC &cc, double diam, double mass, doub
   return r;
 }
 
-// The same terms in FP32 for a liquid particle (mixed path: they only feed the
-// FP32 kernel coefficients).  The terminal-velocity regime is chosen on the
-// FP64 diameter so that both precision paths take the same branch.
-struct PTermsF { float D, c, w, v; };
+// The same terms for a liquid particle in the mixed path: diffusion
+// coefficient, thermal speed and omega in FP32 (they only feed the FP32
+// kernel coefficients); the Cunningham slip correction and the terminal
+// velocity in FP64 (vd; the kernel keeps it as its FP32 value v plus an FP32
+// residual), since pair_coef takes the difference of two particles' fall
+// speeds, which cancels (twin pairs, neighbouring drop bins), and then the
+// gravitational efficiency takes stok - 1.214, which cancels again -- FP32
+// speeds with an FP32 slip factor left some kernels 7e-6 off.  Only the small
+// exponential term of the slip correction is FP32.  The regime is chosen on
+// the FP64 diameter, as in the FP64 path.
+struct PTermsF { float D, c, w, v; double vd; };
 __device__ PTermsF particle_terms_liquid_f(const CellC &cc, double diam_d, double mass_d, double rhop_d) {
   using M = Mth<float>;
-  const float diam = (float)diam_d, mass = (float)mass_d, rhop = (float)rhop_d;
-  const float T = (float)cc.T, mfp = (float)cc.mfp, visc = (float)cc.visc, rhoa = (float)cc.zrhoa;
-  const float pi = (float)kPi, g = (float)kGrav, kb = (float)kBoltz;
+  const float diam = (float)diam_d, mass = (float)mass_d;
+  const float T = (float)cc.T, visc = (float)cc.visc;
+  const float pi = (float)kPi, kb = (float)kBoltz;
   PTermsF r;
-  const float knud = M::div_(2.f * mfp, diam);
-  const float beta = 1.f + knud * (1.142f + 0.558f * M::exp_(M::div_(-0.999f, knud)));
+  const double knud_d = div_nr(2. * cc.mfp, diam_d);
+  const float knud = (float)knud_d;
+  const double beta_d = 1. + knud_d * (1.142 + 0.558 * (double)M::exp_(M::div_(-0.999f, knud)));
+  const float beta = (float)beta_d;
   r.D = M::div_(beta * kb * T, 3.f * pi * visc * diam);
   r.c = M::sqrt_(M::div_(8.f * kb * T, pi * mass));
   r.w = M::div_(8.f * r.D, pi * r.c);
-  // terminal_vel, Beard (1976), mo_particle_external_properties.f90:61-90: the
-  // Stokes regime in FP32; the two polynomial regimes (drops above 19 um) in
-  // FP64, since their degree-6 polynomials in log(Davies/Bond number) cancel
-  // too much for FP32 at the 1e-5 output tolerance
+  // terminal_vel, Beard (1976), mo_particle_external_properties.f90:61-90
   if (diam_d <= 19.e-6)
-    r.v = M::div_((diam * diam) * (rhop - rhoa) * g * beta, 18.f * visc);
+    r.vd = div_nr((diam_d * diam_d) * (rhop_d - cc.zrhoa) * kGrav * beta_d, 18. * cc.visc);
   else
-    r.v = (float)terminal_vel_liquid(diam_d, rhop_d, cc.zrhoa, cc.visc, (double)beta);
+    r.vd = terminal_vel_liquid(diam_d, rhop_d, cc.zrhoa, cc.visc, beta_d);
+  r.v = (float)r.vd;
   return r;
 }
 
 // Collision kernel of one pair from the two particles' terms (coagc :288-407).
 // a_lrg: particle A is the larger (diam(1) >= diam(2), evaluated in FP64).
+// vAl, vBl: in the mixed path the residuals of the FP32 fall speeds
+// (vX + vXl is the FP64 speed to about 2^-48 relative); unused in FP64.
 template <class KT>
 __device__ __forceinline__ KT pair_coef(KT dA, KT DA, KT cA, KT wA, KT vA, KT dB, KT DB, KT cB, KT wB,
-                                        KT vB, bool a_lrg, KT vkin, KT cshear, KT cinert) {
+                                        KT vB, bool a_lrg, KT vkin, KT cshear, KT cinert, KT vAl = KT(0.),
+                                        KT vBl = KT(0.)) {
   using M = Mth<KT>;
   const KT pi = KT(kPi);
   KT md = KT(0.5) * (dA + dB);
@@ -266,8 +276,33 @@ __device__ __forceinline__ KT pair_coef(KT dA, KT DA, KT cA, KT wA, KT vA, KT dB
   KT d_l = a_lrg ? dA : dB, v_l = a_lrg ? vA : vB, v_s = a_lrg ? vB : vA, D_s = a_lrg ? DB : DA;
   KT reyn = M::div_(d_l * v_l, vkin);
   KT schm = M::div_(vkin, D_s);
-  KT dv = vA > vB ? vA - vB : vB - vA;
-  KT stok = M::div_(KT(2.) * v_s * dv, d_l * KT(kGrav));
+  KT dv, stok, stm;
+  bool big;
+  if constexpr (sizeof(KT) == 8) {
+    dv = vA > vB ? vA - vB : vB - vA;
+    stok = M::div_(KT(2.) * v_s * dv, d_l * KT(kGrav));
+    big = stok > KT(1.214);
+    stm = stok - KT(1.214);
+  } else {
+    // The fall-speed difference cancels (twin pairs, neighbouring drop bins):
+    // vA - vB is exact in FP32 when the speeds are within a factor two of
+    // each other (Sterbenz), and the residuals restore what rounding the
+    // speeds to FP32 lost.  Where the collision efficiency takes stok - 1.214
+    // close to the threshold (a second cancellation: below stok = 3 it loses
+    // more than a factor 1.7) the Stokes number is formed in FP64 from the
+    // FP64 speeds.
+    dv = fabsf((vA - vB) + (vAl - vBl));
+    stok = M::div_(KT(2.) * v_s * dv, d_l * KT(kGrav));
+    big = stok > KT(1.214);
+    stm = stok - KT(1.214);
+    if (stok > KT(1.2) && stok < KT(3.)) {
+      const double vAx = (double)vA + (double)vAl, vBx = (double)vB + (double)vBl;
+      const double stokd = div_nr(2. * (a_lrg ? vBx : vAx) * fabs(vAx - vBx), (double)d_l * kGrav);
+      stok = (KT)stokd;
+      big = stokd > 1.214;
+      stm = (KT)(stokd - 1.214);
+    }
+  }
   KT zbrconv = KT(0.);
   if (reyn <= KT(1.)) zbrconv = KT(0.45) * zbrown * M::cbrt_(reyn * schm);
   else if (reyn > KT(1.)) zbrconv = KT(0.45) * zbrown * M::sqrt_(reyn) * M::cbrt_(schm);
@@ -276,8 +311,8 @@ __device__ __forceinline__ KT pair_coef(KT dA, KT DA, KT cA, KT wA, KT vA, KT dB
   KT s1 = stok + KT(0.5);
   KT zea = M::div_(stok * stok, s1 * s1);
   KT zev = KT(0.);
-  if (stok > KT(1.214)) {
-    zev = KT(0.75) * M::div_(M::log_(KT(2.) * stok), stok - KT(1.214));
+  if (big) {
+    zev = KT(0.75) * M::div_(M::log_(KT(2.) * stok), stm);
     zev = M::div_(KT(1.), (KT(1.) + zev) * (KT(1.) + zev));
   }
   KT zecoll = M::div_(KT(60.) * zev + zea * reyn, KT(60.) + reyn);
@@ -329,7 +364,8 @@ __device__ __forceinline__ double act_h2o(double znw, double zns) {
 // (mo_ice_shape.f90:145-191).
 template <class KT>
 __device__ __forceinline__ void ice_bin_terms(const DevLayout &D, const CellC &cc, double n, double nlim, const double *v,
-                                           double dw, PT4<KT> *dst, PT4<KT> *dst_twin) {
+                                           double dw, PT4<KT> *dst, PT4<KT> *dst_twin, KT *vres = nullptr,
+                                           KT *vres_twin = nullptr) {
   double rho;
   if (n > nlim) {
     const double mp = kRhoic * v[D.iwa], mr = kRhori * v[D.irim];
@@ -345,6 +381,7 @@ __device__ __forceinline__ void ice_bin_terms(const DevLayout &D, const CellC &c
   const PTerms p1 = particle_terms(cc, 1.1 * dw, 1.33 * mass, rho, 4, dw, sh);
   *dst = PT4<KT>{(KT)p0.D, (KT)p0.c, (KT)p0.w, (KT)p0.v};
   *dst_twin = PT4<KT>{(KT)p1.D, (KT)p1.c, (KT)p1.w, (KT)p1.v};
+  if (vres) { *vres = (KT)(p0.v - (double)(KT)p0.v); *vres_twin = (KT)(p1.v - (double)(KT)p1.v); }
 }
 
 // LES array index of interior cell `cell` (k fastest)
@@ -543,6 +580,8 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
   double *sDW = reinterpret_cast<double *>(cb + G.odw);
   PT4<KT> *sPR = reinterpret_cast<PT4<KT> *>(cb + G.opr);   // [2][ntab]: bin, self-offset twin
   KT *sK = reinterpret_cast<KT *>(cb + G.oK);
+  // mixed path: residuals of the FP32 fall speeds of every bin and its twin [2][ntab]
+  KT *sVL = sizeof(KT) == 4 ? reinterpret_cast<KT *>(cb + G.ovd) : nullptr;
   // precipitation formation per cloud-cloud pair: bits 0-4 target precip bin + 1
   // (0: none), bit 5 D_new > 50 um, bit 6 D_new > 80 um
   uint8_t *sPF = reinterpret_cast<uint8_t *>(cb + G.opf);
@@ -640,7 +679,11 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
           if (!(n < D.nlim[b])) abits |= 1ull << b;
           const int ph = D.phase[b];
           if (ph == 4) {
-            ice_bin_terms<KT>(D, cc, n, D.nlim[b], sV + b * nsV, dw, sPR + b, sPR + ntab + b);
+            if (sizeof(KT) == 4)
+              ice_bin_terms<KT>(D, cc, n, D.nlim[b], sV + b * nsV, dw, sPR + b, sPR + ntab + b, sVL + b,
+                                sVL + ntab + b);
+            else
+              ice_bin_terms<KT>(D, cc, n, D.nlim[b], sV + b * nsV, dw, sPR + b, sPR + ntab + b);
           } else {
             const double rho = kRhowa;                       // updateRhomean, classSection.f90:169-195
             const double mass = rho * kPi6 * cubed(dw);
@@ -650,6 +693,8 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
               const PTermsF p1 = particle_terms_liquid_f(cc, 1.1 * dw, 1.33 * mass, rho);
               sPR[b] = PT4<KT>{(KT)p0.D, (KT)p0.c, (KT)p0.w, (KT)p0.v};
               sPR[ntab + b] = PT4<KT>{(KT)p1.D, (KT)p1.c, (KT)p1.w, (KT)p1.v};
+              sVL[b] = (KT)(p0.vd - (double)p0.v);
+              sVL[ntab + b] = (KT)(p1.vd - (double)p1.v);
             } else {
               const Shape sh{0., 0., 0., 0.};
               const PTerms p0 = particle_terms(cc, dw, mass, rho, ph, dw, sh);
@@ -692,9 +737,10 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
           // particle 1 is the twin of particle 2 when the bins' dmid agree within 1%
           const double dB = sDW[b2];
           const double dA = tw ? 1.1 * dB : sDW[b1];
-          const PT4<KT> A = sPR[tw ? ntab + b2 : b1], B = sPR[b2];
+          const int ia = tw ? ntab + b2 : b1;
+          const PT4<KT> A = sPR[ia], B = sPR[b2];
           k = pair_coef<KT>((KT)dA, A.D, A.c, A.w, A.v, (KT)dB, B.D, B.c, B.w, B.v, dA >= dB, vkin, cshear,
-                            cinert);
+                            cinert, sVL ? sVL[ia] : KT(0.), sVL ? sVL[b2] : KT(0.));
           k = k * cfac;                                 // charge enhancement (:411-416)
           if (HYD && p >= pf0 && p < pf1) {
             // precip bin reached by coalescence (accumulatePrecipFormation, processes.f90:1461-1467)
@@ -1650,6 +1696,8 @@ int make_geom(const DevLayout &Lay, bool hyd, int kt_bytes, int lanes, bool rate
   G.odw = o; o += 8 * Lay.nt;
   o = align_up(o, 16);
   G.opr = o; o += 2 * G.ntab * 4 * kt_bytes;
+  G.ovd = -1;
+  if (kt_bytes == 4) { G.ovd = o; o += align_up(2 * G.ntab * 4, 16); }   // fall-speed residuals (mixed path)
   G.oK = o; o += G.npairs * kt_bytes;
   // the condensation set-up reuses the particle-term and kernel regions
   if (o - G.opr < 5 * 8 * Lay.nt) o = G.opr + 5 * 8 * Lay.nt;
