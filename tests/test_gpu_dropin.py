@@ -134,7 +134,7 @@ def test_fortran_dropin_run_salsa(capi, oracle_mod, tmp_path):
                                          (sens[0], sens[1][:, :nt], sens[2][:, :nt]), 1e-9, 2,
                                          "init" if init else "step")
         print("  %s call: max rel rv %.2e N %.2e V %.2e %.2e; inside the allowance %d"
-              % (("init" if init else "step"),) + tuple(e) + (nflag,))
+              % ((("init" if init else "step"),) + tuple(e) + (nflag,)))
         assert nflag <= 10
         outs.append(gvc)
     # the initialization call changed the aerosol water
