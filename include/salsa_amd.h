@@ -268,6 +268,28 @@ int salsa_gradient_device(salsa_ctx *ctx, const double *press, const double *tem
                           double *const djdn[4], double *j_cell, double *djde_cell,
                           double totals[2]);
 
+/*
+ * salsa_gradient_device without any device memory allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory during the call (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Arguments and results as
+ * salsa_gradient_device, which allocates its per-cell work buffer on first
+ * use; this variant keeps the per-cell J and dJ/dE and the two totals in a
+ * static device array of the library instead (16 MiB on every GPU the
+ * library runs on, held until the process ends; its address is resolved at
+ * salsa_ctx_create).  The context must have at most SALSA_NOALLOC_MAX_CELLS
+ * interior cells (else SALSA_ERR_ARG) and its device must be current.  All
+ * calls on one GPU share the array: calls that overlap on one GPU (other
+ * threads or streams) race, and each call overwrites what the last left.
+ */
+#define SALSA_NOALLOC_MAX_CELLS (1 << 20)
+int salsa_gradient_device_noalloc(salsa_ctx *ctx, const double *press, const double *temp,
+                                  const double *rsl, const double *rsi, const double *dn,
+                                  const double *rv, const double *const n_p[4],
+                                  const double *const m_p[4], double tstep, int32_t nsteps,
+                                  double *const djdn[4], double *j_cell, double *djde_cell,
+                                  double totals[2]);
+
 /* Launch-shape selection.  lanes_per_cell in {8,16,32,64} for each cost class
  * (0 = clear and dry, 1 = clear and humid, 2 = cloudy); 0 keeps the current
  * value.  salsa_autotune times the candidates on the given device-resident

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -81,8 +82,39 @@ struct salsa_ctx {
   bool timing = false;
   std::vector<hipEvent_t> ev;
   std::vector<int64_t> ev_cells;      // [3] per timed step
+  // this device's copy of the static work space of the allocation-free entry
+  double *scratch = nullptr;
+  double *tot_h = nullptr;            // pinned host [2]: gradient totals
+  int32_t *fault_h = nullptr;         // pinned host [1]: the OR of the cell flags
 };
 constexpr int kEvPerStep = 10;
+
+// ---------------------------------------------------------------------------
+// Work space of salsa_gradient_device_noalloc: per-cell J, per-cell dJ/dE and
+// the two totals for up to SALSA_NOALLOC_MAX_CELLS cells, 16 MiB per GPU,
+// held until the process ends.  Host code never uses the array's name as an
+// address: its device address is resolved once per device (hipGetSymbolAddress
+// at context creation, with that device current) and kept in g_scratch_dev.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[2 * (int64_t)SALSA_NOALLOC_MAX_CELLS + 2];
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[64];
+
+static int resolve_scratch(int device, double **out) {
+  if (device < 0 || device >= 64) { set_error("device index out of range"); return SALSA_ERR_ARG; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      std::snprintf(g_err, sizeof(g_err), "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return SALSA_ERR_HIP;
+    }
+    g_scratch_dev[device] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_dev[device];
+  return SALSA_OK;
+}
 
 // ---------------------------------------------------------------------------
 // small kernels: fault reset, cost classes
@@ -121,6 +153,8 @@ static void release(salsa_ctx *c) {
   (void)hipFree(c->gbuf);
   (void)hipFree(c->sed_work);
   if (c->cls_count_h) (void)hipHostFree(c->cls_count_h);
+  if (c->tot_h) (void)hipHostFree(c->tot_h);
+  if (c->fault_h) (void)hipHostFree(c->fault_h);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -180,9 +214,15 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
       hipMalloc(&c->cls_count, 3 * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->rec_in, nc * c->hostL.nslot * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->rec_out, nc * c->hostL.nout * sizeof(double)) != hipSuccess ||
-      hipHostMalloc(&c->cls_count_h, 3 * sizeof(int32_t)) != hipSuccess) {
+      hipHostMalloc(&c->cls_count_h, 3 * sizeof(int32_t)) != hipSuccess ||
+      hipHostMalloc(&c->tot_h, 2 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&c->fault_h, sizeof(int32_t)) != hipSuccess) {
     set_error("device allocation failed");
     return fail(SALSA_ERR_NOMEM);
+  }
+  {
+    const int rs = resolve_scratch(c->device, &c->scratch);
+    if (rs) return fail(rs);
   }
   if (c->params.lscoag_mode == 2 && c->params.lscoag) {
     const size_t kt = c->params.precision == SALSA_FP64 ? 8 : 4;
@@ -430,9 +470,9 @@ extern "C" int salsa_step_device(salsa_ctx *c, const double *press, const double
 
 extern "C" int salsa_ctx_sync(salsa_ctx *c) {
   if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  HIPCHK(hipMemcpyAsync(c->fault_h, c->fault, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  int32_t f = 0;
-  HIPCHK(hipMemcpy(&f, c->fault, sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int32_t f = *c->fault_h;
   if (f & SALSA_CELL_NONFINITE) { set_error("non-finite values produced in some cell"); return SALSA_ERR_NONFINITE; }
   if (f & SALSA_CELL_NOCONV) { set_error("equilibration did not converge in some cell"); return SALSA_ERR_NOCONV; }
   if (f & SALSA_CELL_DISTR) { set_error("size-distribution update did not settle in some cell"); return SALSA_ERR_DISTR; }
@@ -517,11 +557,15 @@ extern "C" int salsa_step_host(salsa_ctx *c, const double *press, const double *
 // Forward-mode gradient of the precipitation volume after nsteps in-place
 // steps: gather the cell records, run the derivative kernel (one lane per
 // direction: the cell's bins, then E) and sum the per-cell results.
-extern "C" int salsa_gradient_device(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
-                                     const double *rsi, const double *dn, const double *rv,
-                                     const double *const n_p[4], const double *const m_p[4], double tstep,
-                                     int32_t nsteps, double *const djdn[4], double *j_cell, double *djde_cell,
-                                     double totals[2]) {
+static int gradient_run(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                        const double *rsi, const double *dn, const double *rv, const double *const n_p[4],
+                        const double *const m_p[4], double tstep, int32_t nsteps, double *const djdn[4],
+                        double *j_cell, double *djde_cell, double totals[2], double *buf);
+
+static int gradient_check(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                          const double *rsi, const double *dn, const double *rv, const double *const n_p[4],
+                          const double *const m_p[4], double tstep, int32_t nsteps, double *const djdn[4],
+                          double totals[2]) {
   int rc = check_ptrs(c, press, temp, rsl, rsi, dn, rv, nullptr, n_p, nullptr, m_p, nullptr, nullptr, tstep,
                       SALSA_STEP_INPLACE);
   if (rc) return rc;
@@ -534,11 +578,60 @@ extern "C" int salsa_gradient_device(salsa_ctx *c, const double *press, const do
     set_error("the gradient needs lscoag_mode 1 (kernels evaluated every step)");
     return SALSA_ERR_ARG;
   }
+  return SALSA_OK;
+}
+
+extern "C" int salsa_gradient_device(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                                     const double *rsi, const double *dn, const double *rv,
+                                     const double *const n_p[4], const double *const m_p[4], double tstep,
+                                     int32_t nsteps, double *const djdn[4], double *j_cell, double *djde_cell,
+                                     double totals[2]) {
+  int rc = gradient_check(c, press, temp, rsl, rsi, dn, rv, n_p, m_p, tstep, nsteps, djdn, totals);
+  if (rc) return rc;
   const int64_t n = c->ncells;
   if (!c->gbuf && hipMalloc(&c->gbuf, (size_t)(2 * n + 2) * sizeof(double)) != hipSuccess) {
     set_error("gradient buffer allocation failed");
     return SALSA_ERR_NOMEM;
   }
+  return gradient_run(c, press, temp, rsl, rsi, dn, rv, n_p, m_p, tstep, nsteps, djdn, j_cell, djde_cell, totals,
+                      c->gbuf);
+}
+
+// The same with the per-cell results and totals in the library's static
+// work space: no device memory is allocated.
+extern "C" int salsa_gradient_device_noalloc(salsa_ctx *c, const double *press, const double *temp,
+                                             const double *rsl, const double *rsi, const double *dn,
+                                             const double *rv, const double *const n_p[4],
+                                             const double *const m_p[4], double tstep, int32_t nsteps,
+                                             double *const djdn[4], double *j_cell, double *djde_cell,
+                                             double totals[2]) {
+  int rc = gradient_check(c, press, temp, rsl, rsi, dn, rv, n_p, m_p, tstep, nsteps, djdn, totals);
+  if (rc) return rc;
+  if (c->ncells > SALSA_NOALLOC_MAX_CELLS) {
+    std::snprintf(g_err, sizeof(g_err), "%lld interior cells exceed SALSA_NOALLOC_MAX_CELLS (%d)",
+                  (long long)c->ncells, (int)SALSA_NOALLOC_MAX_CELLS);
+    return SALSA_ERR_ARG;
+  }
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != c->device || !c->scratch) {
+    set_error("the context's device is not the current device");
+    return SALSA_ERR_ARG;
+  }
+  return gradient_run(c, press, temp, rsl, rsi, dn, rv, n_p, m_p, tstep, nsteps, djdn, j_cell, djde_cell, totals,
+                      c->scratch);
+}
+
+// Gather the cell records, run the derivative kernel (one lane per
+// direction: the cell's bins, then E) and sum the per-cell results; buf holds
+// [J per cell | dJ/dE per cell | the two totals].
+static int gradient_run(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                        const double *rsi, const double *dn, const double *rv, const double *const n_p[4],
+                        const double *const m_p[4], double tstep, int32_t nsteps, double *const djdn[4],
+                        double *j_cell, double *djde_cell, double totals[2], double *buf) {
+  const DevLayout &L = c->hostL;
+  const int64_t n = c->ncells;
+  int rc;
   hipStream_t s = c->stream;
   StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv, nullptr, n_p, nullptr, m_p, nullptr, nullptr, tstep,
                          SALSA_STEP_INPLACE);
@@ -553,17 +646,16 @@ extern "C" int salsa_gradient_device(salsa_ctx *c, const double *press, const do
   g.nsteps = nsteps;
   g.lanes = L.nt + 1 <= 32 ? 32 : 64;
   for (int k = 0; k < 4; ++k) g.djdn[k] = k < L.ncat ? djdn[k] : nullptr;
-  g.j_cell = c->gbuf;
-  g.djde_cell = c->gbuf + n;
-  rc = launch_gradient(c->cap, c->params.precision, c->devL, g, c->gbuf + 2 * n, s);
+  g.j_cell = buf;
+  g.djde_cell = buf + n;
+  rc = launch_gradient(c->cap, c->params.precision, c->devL, g, buf + 2 * n, s);
   if (rc) { std::snprintf(g_err, sizeof(g_err), "gradient launch failed (%d)", rc); return SALSA_ERR_HIP; }
-  if (j_cell) HIPCHK(hipMemcpyAsync(j_cell, c->gbuf, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-  if (djde_cell) HIPCHK(hipMemcpyAsync(djde_cell, c->gbuf + n, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-  double tot[2] = {0., 0.};
-  HIPCHK(hipMemcpyAsync(tot, c->gbuf + 2 * n, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (j_cell) HIPCHK(hipMemcpyAsync(j_cell, buf, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  if (djde_cell) HIPCHK(hipMemcpyAsync(djde_cell, buf + n, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->tot_h, buf + 2 * n, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   rc = salsa_ctx_sync(c);
-  totals[0] = tot[0];
-  totals[1] = tot[1];
+  totals[0] = c->tot_h[0];
+  totals[1] = c->tot_h[1];
   return rc;
 }
 

@@ -44,8 +44,10 @@ EXPORTS = [
     "salsa_step_host", "salsa_step_device", "salsa_ctx_sync", "salsa_get_cell_flags",
     "salsa_set_launch_shape", "salsa_get_launch_shape", "salsa_autotune", "salsa_ctx_ncells",
     "salsa_ctx_ntotal", "salsa_ctx_stream", "salsa_last_error", "salsa_ctx_timing",
-    "salsa_ctx_kernel_time", "salsa_ctx_phase_time",
+    "salsa_ctx_kernel_time", "salsa_ctx_phase_time", "salsa_gradient_device_noalloc",
 ]
+
+NOALLOC_MAX_CELLS = 1 << 20
 
 
 # process-rate fields (include/salsa_amd.h SALSA_RATE_*)
@@ -168,6 +170,8 @@ def lib():
     L.salsa_gradient_device.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [_P4, _P4, C.c_double, C.c_int32,
                                                                           _P4, C.c_void_p, C.c_void_p, _dp]
     L.salsa_gradient_device.restype = C.c_int
+    L.salsa_gradient_device_noalloc.argtypes = L.salsa_gradient_device.argtypes
+    L.salsa_gradient_device_noalloc.restype = C.c_int
     L.salsa_les_update_device.argtypes = [C.c_void_p, C.c_double, _dp] + [C.c_void_p] * 4 + [_P4] * 4 + \
         [C.c_void_p, C.c_void_p]
     L.salsa_les_update_device.restype = C.c_int
@@ -298,16 +302,19 @@ class Context:
     def sync(self):
         return _check(lib().salsa_ctx_sync(self._h))
 
-    def gradient_device(self, f, n_p, m_p, djdn, tstep=1.0, nsteps=1, j_cell=None, djde_cell=None):
+    def gradient_device(self, f, n_p, m_p, djdn, tstep=1.0, nsteps=1, j_cell=None, djde_cell=None,
+                        noalloc=False):
         """Forward-mode gradient of J = precipitation volume concentration
         summed over interior cells after `nsteps` in-place steps (inputs are
         not modified).  djdn: per category a float64 tensor shaped like
         n_p[c] receiving dJ/dN0 (interior cells); j_cell / djde_cell:
         optional per-cell outputs.  Returns (J, dJ/dE) after waiting for the
-        context stream."""
+        context stream.  noalloc: the entry point that allocates no device
+        memory (salsa_gradient_device_noalloc)."""
         P = lambda xs: _P4(*[_ptr(x) for x in (list(xs) + [None] * (4 - len(xs)))])
         tot = (C.c_double * 2)()
-        rc = self._ordered(lambda: lib().salsa_gradient_device(
+        fn = lib().salsa_gradient_device_noalloc if noalloc else lib().salsa_gradient_device
+        rc = self._ordered(lambda: fn(
             self._h, _ptr(f["press"]), _ptr(f["temp"]), _ptr(f["rsl"]), _ptr(f.get("rsi")), _ptr(f["dn"]),
             _ptr(f["rv"]), P(n_p), P(m_p), tstep, nsteps, P(djdn), _ptr(j_cell), _ptr(djde_cell),
             C.cast(tot, _dp)))
