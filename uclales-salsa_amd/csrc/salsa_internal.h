@@ -180,6 +180,28 @@ struct GradArgs {
   double *j_cell, *djde_cell;         // per interior cell: J and dJ/dE
 };
 
+// Arguments of the adjoint kernel (salsa_adjoint_device).  The state of a
+// cell is kept in SALSA units as x[cell][nvar] = rv, N[nt], V[nt][nsV].
+// mode 0: one step x_in -> x_out (the forward pass and the recomputation
+// between checkpoints); mode 1: the step's forward internals are recomputed
+// from x_in, then its reverse turns lam (adjoint of the state after the
+// step) into the adjoint of the state before it, in place, and adds the
+// step's share of dJ/dE to ebar.
+struct AdjArgs {
+  int64_t ncells;
+  const double *rec;                  // cell records (T, p, rs, rsi, dn at slots 0..4)
+  int32_t nslot, nvar, nsub, mode;
+  const double *x_in;
+  double *x_out;
+  double *lam;
+  double *ebar;
+  double *tape;                       // per block: condensation substeps [nsub][nt + 1]
+  double dt, rhlim;
+  int32_t lsfreeRH;
+  uint8_t *cell_flags;                // OR-ed with the step's fault bits
+  int32_t *fault_any;
+};
+
 // Arguments of the LES coupling passes after a step (salsa_les_update_device):
 // tend_constrain2, update_sclrs, tend0 and SALSA_diagnostics on the LES
 // arrays n(c)[b][n3], m(c)[s*nb+b][n3] of a grid with halo 1 in z and 2
