@@ -282,6 +282,33 @@ int salsa_gradient_device(salsa_ctx *ctx, const double *press, const double *tem
  * calls on one GPU share the array: calls that overlap on one GPU (other
  * threads or streams) race, and each call overwrites what the last left.
  */
+/*
+ * Reverse-mode (adjoint) gradient of the objective of salsa_gradient_device
+ * (no counterpart in the reference).  Arguments and results as
+ * salsa_gradient_device, plus
+ *   checkpoint_every : the forward pass keeps the cell states every this many
+ *                      steps (>= 1); the reverse pass recomputes the states
+ *                      between two checkpoints from the earlier one, then runs
+ *                      the steps' adjoints backwards.
+ * One reverse sweep gives dJ/dN0 of every bin and dJ/dE at once.  Device
+ * memory, kept by the context and grown on demand: ceil(nsteps /
+ * checkpoint_every) + min(checkpoint_every, nsteps) + 2 cell states of
+ * 8 (1 + nt (1 + nwet)) bytes per interior cell, and a tape of about 36 KiB
+ * per launched wavefront.  Level 4 with lscoag_mode 1 and without autoconv2,
+ * activation and the size-distribution update; SALSA_ERR_ARG otherwise, or
+ * for a bin layout beyond the kernel's capacity (32 bins, 2 wet species).
+ * SALSA_CELL_GRAD_UNSTABLE marks cells whose condensation substeps amplify a
+ * water perturbation more than 1e4-fold in one step, or whose adjoint is not
+ * finite; dJ/dE totals exclude them.  Inputs are device pointers and are not
+ * modified; the call waits for the context stream.
+ */
+int salsa_adjoint_device(salsa_ctx *ctx, const double *press, const double *temp,
+                         const double *rsl, const double *rsi, const double *dn,
+                         const double *rv, const double *const n_p[4],
+                         const double *const m_p[4], double tstep, int32_t nsteps,
+                         int32_t checkpoint_every, double *const djdn[4], double *j_cell,
+                         double *djde_cell, double totals[2]);
+
 #define SALSA_NOALLOC_MAX_CELLS (1 << 20)
 int salsa_gradient_device_noalloc(salsa_ctx *ctx, const double *press, const double *temp,
                                   const double *rsl, const double *rsi, const double *dn,

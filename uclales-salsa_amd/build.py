@@ -15,8 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SALSA_ARCH", "gfx950")
-SRCS = ["csrc/salsa_layout.cpp", "csrc/salsa_kernels.hip", "csrc/salsa_gradient.hip", "csrc/salsa_les.hip",
-        "csrc/salsa_sed.hip", "csrc/salsa_capi.hip"]
+SRCS = ["csrc/salsa_layout.cpp", "csrc/salsa_kernels.hip", "csrc/salsa_gradient.hip", "csrc/salsa_adjoint.hip",
+        "csrc/salsa_les.hip", "csrc/salsa_sed.hip", "csrc/salsa_capi.hip"]
 OUT = os.path.join(HERE, "salsa_amd", "libsalsa_amd.so")
 FLAGS = ["-O3", "--offload-arch=" + ARCH, "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

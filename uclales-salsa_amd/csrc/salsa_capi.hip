@@ -30,6 +30,13 @@ int cap_bins(int cap);
 int cap_spec(int cap);
 int launch_gradient(int cap, int precision, const DevLayout *d, const GradArgs &g, double *sum_out,
                     hipStream_t s);
+int launch_adjoint_step(int precision, const DevLayout *d, const AdjArgs &a, unsigned nblk, hipStream_t s);
+int launch_adjoint_pack(const DevLayout *d, const double *rec, int nslot, int64_t ncells, int nvar, double *x,
+                        uint8_t *flags, hipStream_t s);
+int launch_adjoint_seed(const DevLayout *d, const double *x, int64_t ncells, int nvar, double *lam, double *ebar,
+                        double *jc, hipStream_t s);
+int launch_adjoint_out(const DevLayout *d, const AdjOut &o, const double *jc, const double *ebar,
+                       const uint8_t *flags, double *sum_out, hipStream_t s);
 int launch_les_update(const LesArgs &a, hipStream_t s);
 int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, hipStream_t s);
 }  // namespace salsa
@@ -72,6 +79,10 @@ struct salsa_ctx {
   int32_t *cls_count_h = nullptr;     // pinned host [3]
   // salsa_gradient_device: per-cell J and dJ/dE, then the two sums
   double *gbuf = nullptr;
+  // salsa_adjoint_device: checkpoints, recomputed states, work states, the
+  // adjoint, per-cell J and dJ/dE, totals and the per-block tape
+  double *adj_buf = nullptr;
+  size_t adj_bytes = 0;
   // salsa_sedim_device: precipitation number / mass changes and removal
   double *sed_work = nullptr;
   // staging for salsa_step_host
@@ -151,6 +162,7 @@ static void release(salsa_ctx *c) {
   (void)hipFree(c->lists); (void)hipFree(c->cls_count);
   (void)hipFree(c->stage);
   (void)hipFree(c->gbuf);
+  (void)hipFree(c->adj_buf);
   (void)hipFree(c->sed_work);
   if (c->cls_count_h) (void)hipHostFree(c->cls_count_h);
   if (c->tot_h) (void)hipHostFree(c->tot_h);
@@ -620,6 +632,136 @@ extern "C" int salsa_gradient_device_noalloc(salsa_ctx *c, const double *press, 
   }
   return gradient_run(c, press, temp, rsl, rsi, dn, rv, n_p, m_p, tstep, nsteps, djdn, j_cell, djde_cell, totals,
                       c->scratch);
+}
+
+// Reverse-mode gradient: gather and pack the initial states, run the steps
+// forward keeping a checkpoint every `every` steps, seed the adjoint with
+// dJ/dx_n, then for each stretch between checkpoints (last first) recompute
+// its states and run the steps' adjoints backwards; scatter dJ/dN0 and sum.
+extern "C" int salsa_adjoint_device(salsa_ctx *c, const double *press, const double *temp, const double *rsl,
+                                    const double *rsi, const double *dn, const double *rv,
+                                    const double *const n_p[4], const double *const m_p[4], double tstep,
+                                    int32_t nsteps, int32_t every, double *const djdn[4], double *j_cell,
+                                    double *djde_cell, double totals[2]) {
+  int rc = gradient_check(c, press, temp, rsl, rsi, dn, rv, n_p, m_p, tstep, nsteps, djdn, totals);
+  if (rc) return rc;
+  const DevLayout &L = c->hostL;
+  if (every < 1) { set_error("checkpoint_every must be at least 1"); return SALSA_ERR_ARG; }
+  if (L.level != 4 || L.autoconv2 || L.lsactiv || L.lsdistupdate) {
+    set_error("the adjoint covers level-4 coagulation and condensation (no autoconv2, activation or "
+              "size-distribution update)");
+    return SALSA_ERR_ARG;
+  }
+  bool fits = L.nt <= 32 && L.nwet <= 3 && L.npair <= 528 && L.nb[2] <= 8 && L.ncc <= 528 &&
+              L.fn1a - L.in1a + 1 <= 4 && L.fn2b - L.in2a + 1 <= 63;
+  for (int it = 0; it < L.ntarget; ++it) fits = fits && L.tgt[it].nent <= 64;
+  if (!fits) { set_error("bin layout beyond the adjoint kernel's capacity"); return SALSA_ERR_ARG; }
+  const int64_t n = c->ncells;
+  const int nvar = 1 + L.nt * (1 + L.nwet);
+  const int nsub = substeps(tstep);
+  const int nck = (nsteps + every - 1) / every;
+  const int nseg = (every < nsteps ? every : nsteps) - 1;
+  const int64_t sz = n * nvar;                           // doubles per set of cell states
+  const unsigned nblk = (unsigned)(n < 4096 ? n : 4096);
+  int64_t stride = (int64_t)nsub * (L.nt + 1);
+  const int64_t t_p = stride;
+  stride += 2 * (int64_t)L.npair;
+  const int64_t t_pf = stride;
+  stride += (int64_t)L.ntarget * L.nb[2] * (1 + L.nwet);
+  const int64_t t_eq = stride;
+  stride += 4 * 104;
+  stride = (stride + 31) / 32 * 32;
+  const size_t need = (size_t)(((int64_t)nck + nseg + 3) * sz + 2 * n + 32 + (int64_t)nblk * stride) * 8;
+  if (need > c->adj_bytes) {
+    (void)hipFree(c->adj_buf);
+    c->adj_buf = nullptr;
+    c->adj_bytes = 0;
+    if (hipMalloc(&c->adj_buf, need) != hipSuccess) {
+      set_error("adjoint buffer allocation failed");
+      return SALSA_ERR_NOMEM;
+    }
+    c->adj_bytes = need;
+  }
+  double *ck = c->adj_buf, *seg = ck + nck * sz, *wa = seg + nseg * sz, *wb = wa + sz, *lam = wb + sz;
+  double *ebar = lam + sz, *jc = ebar + n, *tot = jc + n, *tape = tot + 32;
+  hipStream_t s = c->stream;
+  StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv, nullptr, n_p, nullptr, m_p, nullptr, nullptr, tstep,
+                         SALSA_STEP_INPLACE);
+  a.kmode = 0;
+  a.kstore = nullptr;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, c->fault, 1);
+  rc = launch_gather(L, c->devL, a, c->rec_in, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+  if (!rc) rc = launch_adjoint_pack(c->devL, c->rec_in, L.nslot, n, nvar, ck, c->flags, s);
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "adjoint set-up launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  AdjArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.ncells = n;
+  g.rec = c->rec_in;
+  g.nslot = L.nslot;
+  g.nvar = nvar;
+  g.nsub = nsub;
+  g.lam = lam;
+  g.ebar = ebar;
+  g.tape = tape;
+  g.tape_stride = stride;
+  g.t_p = t_p;
+  g.t_pf = t_pf;
+  g.t_eq = t_eq;
+  g.dt = tstep;
+  g.rhlim = c->params.rhlim;
+  g.lsfreeRH = c->params.lsfreeRH;
+  g.cell_flags = c->flags;
+  g.fault_any = c->fault;
+  const int prec = c->params.precision;
+  // forward pass
+  const double *in = ck;
+  for (int k = 0; k < nsteps && !rc; ++k) {
+    double *out = ((k + 1) % every == 0 && k + 1 < nsteps) ? ck + (int64_t)((k + 1) / every) * sz
+                                                           : (in == wa ? wb : wa);
+    g.mode = 0;
+    g.x_in = in;
+    g.x_out = out;
+    rc = launch_adjoint_step(prec, c->devL, g, nblk, s);
+    in = out;
+  }
+  if (!rc) rc = launch_adjoint_seed(c->devL, in, n, nvar, lam, ebar, jc, s);
+  // reverse pass, one stretch between checkpoints at a time
+  for (int q = nck - 1; q >= 0 && !rc; --q) {
+    const int s0 = q * every, s1 = (s0 + every < nsteps) ? s0 + every : nsteps;
+    const double *x0 = ck + (int64_t)q * sz;
+    const double *prev = x0;
+    for (int k = s0 + 1; k < s1 && !rc; ++k) {           // states x_{s0+1} .. x_{s1-1}
+      g.mode = 0;
+      g.x_in = prev;
+      g.x_out = seg + (int64_t)(k - s0 - 1) * sz;
+      rc = launch_adjoint_step(prec, c->devL, g, nblk, s);
+      prev = g.x_out;
+    }
+    for (int k = s1 - 1; k >= s0 && !rc; --k) {
+      g.mode = 1;
+      g.x_in = k == s0 ? x0 : seg + (int64_t)(k - s0 - 1) * sz;
+      g.x_out = nullptr;
+      rc = launch_adjoint_step(prec, c->devL, g, nblk, s);
+    }
+  }
+  if (!rc) {
+    AdjOut o;
+    std::memset(&o, 0, sizeof(o));
+    o.ncells = n; o.n3 = c->n3;
+    o.nz = c->nz; o.nx = c->nx; o.nzp = c->nzp; o.nxp = c->nxp; o.hz = c->params.hz; o.hxy = c->params.hxy;
+    o.nslot = L.nslot; o.nvar = nvar;
+    o.rec = c->rec_in; o.lam = lam;
+    for (int k = 0; k < 4; ++k) o.djdn[k] = k < L.ncat ? djdn[k] : nullptr;
+    rc = launch_adjoint_out(c->devL, o, jc, ebar, c->flags, tot, s);
+  }
+  if (rc) { std::snprintf(g_err, sizeof(g_err), "adjoint launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  if (j_cell) HIPCHK(hipMemcpyAsync(j_cell, jc, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  if (djde_cell) HIPCHK(hipMemcpyAsync(djde_cell, ebar, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->tot_h, tot, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  rc = salsa_ctx_sync(c);
+  totals[0] = c->tot_h[0];
+  totals[1] = c->tot_h[1];
+  return rc;
 }
 
 // Gather the cell records, run the derivative kernel (one lane per

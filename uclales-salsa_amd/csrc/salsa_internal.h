@@ -195,11 +195,24 @@ struct AdjArgs {
   double *x_out;
   double *lam;
   double *ebar;
-  double *tape;                       // per block: condensation substeps [nsub][nt + 1]
+  // per-block tape of mode 1, tape_stride doubles per block: the condensation
+  // substeps [nsub][nt + 1] at 0, pair derivatives [2][npair] at t_p, the
+  // precipitation bins before each transfer [ntarget][nprc][1 + nwet] at
+  // t_pf, regime-1 equilibration iterates [4][104] at t_eq
+  double *tape;
+  int64_t tape_stride, t_p, t_pf, t_eq;
   double dt, rhlim;
   int32_t lsfreeRH;
   uint8_t *cell_flags;                // OR-ed with the step's fault bits
   int32_t *fault_any;
+};
+
+// The adjoint's outputs: dJ/dN0 scattered to the LES number fields.
+struct AdjOut {
+  int64_t ncells, n3;
+  int32_t nz, nx, nzp, nxp, hz, hxy, nslot, nvar;
+  const double *rec, *lam;
+  double *djdn[4];
 };
 
 // Arguments of the LES coupling passes after a step (salsa_les_update_device):

@@ -44,7 +44,7 @@ EXPORTS = [
     "salsa_step_host", "salsa_step_device", "salsa_ctx_sync", "salsa_get_cell_flags",
     "salsa_set_launch_shape", "salsa_get_launch_shape", "salsa_autotune", "salsa_ctx_ncells",
     "salsa_ctx_ntotal", "salsa_ctx_stream", "salsa_last_error", "salsa_ctx_timing",
-    "salsa_ctx_kernel_time", "salsa_ctx_phase_time", "salsa_gradient_device_noalloc",
+    "salsa_ctx_kernel_time", "salsa_ctx_phase_time", "salsa_gradient_device_noalloc", "salsa_adjoint_device",
 ]
 
 NOALLOC_MAX_CELLS = 1 << 20
@@ -172,6 +172,9 @@ def lib():
     L.salsa_gradient_device.restype = C.c_int
     L.salsa_gradient_device_noalloc.argtypes = L.salsa_gradient_device.argtypes
     L.salsa_gradient_device_noalloc.restype = C.c_int
+    L.salsa_adjoint_device.argtypes = [C.c_void_p] + [C.c_void_p] * 6 + [_P4, _P4, C.c_double, C.c_int32,
+                                                                         C.c_int32, _P4, C.c_void_p, C.c_void_p, _dp]
+    L.salsa_adjoint_device.restype = C.c_int
     L.salsa_les_update_device.argtypes = [C.c_void_p, C.c_double, _dp] + [C.c_void_p] * 4 + [_P4] * 4 + \
         [C.c_void_p, C.c_void_p]
     L.salsa_les_update_device.restype = C.c_int
@@ -318,6 +321,21 @@ class Context:
             self._h, _ptr(f["press"]), _ptr(f["temp"]), _ptr(f["rsl"]), _ptr(f.get("rsi")), _ptr(f["dn"]),
             _ptr(f["rv"]), P(n_p), P(m_p), tstep, nsteps, P(djdn), _ptr(j_cell), _ptr(djde_cell),
             C.cast(tot, _dp)))
+        _check(rc)
+        return tot[0], tot[1]
+
+    def adjoint_device(self, f, n_p, m_p, djdn, tstep=1.0, nsteps=1, checkpoint_every=5, j_cell=None,
+                       djde_cell=None):
+        """Reverse-mode gradient of the objective of gradient_device
+        (salsa_adjoint_device): one adjoint sweep gives dJ/dN0 of every bin
+        and dJ/dE; the forward pass keeps the state every `checkpoint_every`
+        steps.  Arguments and return value as gradient_device."""
+        P = lambda xs: _P4(*[_ptr(x) for x in (list(xs) + [None] * (4 - len(xs)))])
+        tot = (C.c_double * 2)()
+        rc = self._ordered(lambda: lib().salsa_adjoint_device(
+            self._h, _ptr(f["press"]), _ptr(f["temp"]), _ptr(f["rsl"]), _ptr(f.get("rsi")), _ptr(f["dn"]),
+            _ptr(f["rv"]), P(n_p), P(m_p), tstep, nsteps, checkpoint_every, P(djdn), _ptr(j_cell),
+            _ptr(djde_cell), C.cast(tot, _dp)))
         _check(rc)
         return tot[0], tot[1]
 
