@@ -51,7 +51,8 @@ enum {
   SALSA_CELL_NOCONV = 2,       /* equilibration did not converge in 100 iterations                 */
   SALSA_CELL_SUBSTEPS = 4,     /* > 100 condensation substeps (mo_salsa_dynamics.f90:1051)         */
   SALSA_CELL_NONFINITE = 8,    /* NaN/Inf produced                                                  */
-  SALSA_CELL_GRAD_UNSTABLE = 16, /* salsa_gradient_device: the linearised trajectory of the cell is
+  SALSA_CELL_GRAD_UNSTABLE = 16, /* salsa_gradient_device / salsa_adjoint_device (whose bound is on J
+                                   alone): the linearised trajectory of the cell is
                                    unstable -- a relative input change is amplified more than 1e8
                                    times in J, the vapour or an active bin, the explicit water
                                    update of one step amplifies a perturbation more than 1e4 times
@@ -298,8 +299,9 @@ int salsa_gradient_device(salsa_ctx *ctx, const double *press, const double *tem
  * activation and the size-distribution update; SALSA_ERR_ARG otherwise, or
  * for a bin layout beyond the kernel's capacity (32 bins, 2 wet species).
  * SALSA_CELL_GRAD_UNSTABLE marks cells whose condensation substeps amplify a
- * water perturbation more than 1e4-fold in one step, or whose adjoint is not
- * finite; dJ/dE totals exclude them.  Inputs are device pointers and are not
+ * water perturbation more than 1e4-fold in one step, whose J amplifies a
+ * relative change of the state before some step, or of E, more than 1e8
+ * times, or whose adjoint is not finite; dJ/dE totals exclude them.  Inputs are device pointers and are not
  * modified; the call waits for the context stream.
  */
 int salsa_adjoint_device(salsa_ctx *ctx, const double *press, const double *temp,

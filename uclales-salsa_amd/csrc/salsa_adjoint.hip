@@ -63,9 +63,23 @@ constexpr int kAL = 64;        // lanes per cell
 constexpr int kEqIt = 104;     // stored regime-1 iterates per bin (at most 101 are used)
 
 __device__ __forceinline__ double wsum(double x) { return cell_sum<kAL>(x); }
+// largest value over the wavefront (every lane gets it): DPP within rows,
+// then the row and half exchanges
 __device__ __forceinline__ double wmax(double x) {
-#pragma unroll
-  for (int o = 1; o < kAL; o <<= 1) x = fmax(x, __shfl_xor(x, o));
+  x = fmax(x, dpp_d<0xB1>(x));
+  x = fmax(x, dpp_d<0x4E>(x));
+  x = fmax(x, dpp_d<0x141>(x));
+  x = fmax(x, dpp_d<0x140>(x));
+  {
+    auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(x), false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(x), false, false);
+    x = fmax(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
+  }
+  {
+    auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(x), false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(x), false, false);
+    x = fmax(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
+  }
   return x;
 }
 
@@ -208,7 +222,8 @@ __device__ __forceinline__ void act_h2o_d(double x, double s, double &dx, double
 // the step / adjoint kernel: one cell per wavefront, cells strided over the grid
 // ---------------------------------------------------------------------------
 template <class KT, int MB, int NS, int NP, bool REV>
-__global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restrict__ Dp, AdjArgs a) {
+__global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__restrict__ Dp, AdjArgs a) {
+  static_assert(NS <= 3, "a target's tape row holds three species sums");
   const DevLayout &D = *Dp;
   const int lane = threadIdx.x;
   const int nt = D.nt, nsV = D.nwet, np = D.npair, iwa = D.iwa, ndry = D.ndry;
@@ -235,6 +250,7 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
   double *tP = tape + a.t_p;           // [2][np] dK/d(dw_b1), dK/d(dw_b2)
   double *tPF = tape + a.t_pf;         // [ntarget][nb2][1 + nsV] precipitation bins before a transfer
   double *tEQ = tape + a.t_eq;         // [4][kEqIt] regime-1 iterates
+  double *tTG = tape + a.t_tg;         // [ntarget][8] each target's sums: sink, src[3], so, sx, transfer mask
 
   for (int64_t cell = blockIdx.x; cell < a.ncells; cell += gridDim.x) {
     const double *rec = a.rec + cell * a.nslot;
@@ -437,6 +453,16 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
         fix_own = fmax(1. - dt * so, 0.1);
         fix_oth = fmax(1. - dt * sx, 0.1);
       }
+      if constexpr (rev) {
+        if (lane == 0) {
+          double *row = tTG + it * 8;
+          row[0] = sink;
+#pragma unroll
+          for (int s = 0; s < NS; ++s) row[1 + s] = s < nupd ? src[s] : 0.;
+          row[4] = so;
+          row[5] = sx;
+        }
+      }
       double sink_self = 0., kself = 0., fixs = 1., volsink[NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) volsink[s] = 0.;
@@ -488,6 +514,9 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
       if (fl & (TF_PF | TF_PF_SELF)) {
         uint32_t pm = cell_or<kAL>(pmask);
         if (pself >= 0) pm |= 1u << pself;
+        if constexpr (rev) {
+          if (lane == 0) tTG[it * 8 + 6] = (double)pm;
+        }
         while (pm) {
           const int pb = __builtin_ctz(pm);
           pm &= pm - 1;
@@ -672,9 +701,12 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
       double grow = 1.;
       for (int st = 0; st < nsub; ++st) {               // :986-1053
         if constexpr (rev) {
-          double *row = tape + (int64_t)st * (nt + 1);
+          double *row = tape + (int64_t)st * (nt + 2);
           if (own) row[lane] = zcw;
-          if (lane == 0) row[nt] = zcwc;
+          if (lane == 0) {
+            row[nt] = zcwc;
+            row[nt + 1] = s1;
+          }
         }
         const double zhlp1 = zcwc + adt * s1;
         const double zhlp2 = 1. + adt * s2;
@@ -761,12 +793,12 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
         double azcwtot = 0., asfix = 0., as2 = 0.;
         const double zhlp2 = 1. + adt * s2;
         for (int st = nsub - 1; st >= 0; --st) {
-          const double *row = tape + (int64_t)st * (nt + 1);
+          const double *row = tape + (int64_t)st * (nt + 2);
           const double zcw = own ? row[lane] : 0.;
           const double zcwc = row[nt];
           const double act = upd && st > 0 ? act_h2o<true>(zcw, zns) : 0.;
           const double zws = upd ? (st == 0 ? zws0 : act * zkl) : 0.;
-          const double s1 = wsum(upd ? zmt * zws * csul : 0.);
+          const double s1 = row[nt + 1];
           const double zhlp1 = zcwc + adt * s1;
           const double A = div_nr(zhlp1, zhlp2);
           const bool tookA = !(zcwtot < A);
@@ -891,11 +923,13 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
         if (eq2 > 0) {
           // fixed point z = H(z): zlwc = L(z) with dz = (dH/dp) dp / (1 - dH/dz)
           const EqTask q = eq_task(D, sN, sV, nsV, bb, 2, zrh, isu, ioc);
-          EqBar t0{0., 0., 0., 0., 0., 0.}, pb{0., 0., 0., 0., 0., 0.};
-          const double Lz = eq_iter_rev(q, T, eqz, 1., 0., t0);
-          const double Hz = eq_iter_rev(q, T, eqz, 0., 1., t0);
+          EqBar pl{0., 0., 0., 0., 0., 0.}, ph{0., 0., 0., 0., 0., 0.};
+          const double Lz = eq_iter_rev(q, T, eqz, 1., 0., pl);
+          const double Hz = eq_iter_rev(q, T, eqz, 0., 1., ph);
           const double wz = al * Lz / (1. - Hz);
-          (void)eq_iter_rev(q, T, eqz, al, wz, pb);
+          // the parameters' adjoints are linear in (al, wz)
+          const EqBar pb{al * pl.zr + wz * ph.zr, al * pl.c1 + wz * ph.c1, al * pl.c2 + wz * ph.c2,
+                         al * pl.c5 + wz * ph.c5, al * pl.n + wz * ph.n, al * pl.zcore + wz * ph.zcore};
           azrh += eq_task_rev(D, q, pb, nsV, bb, zrh, aN, aV);
         }
       }
@@ -970,11 +1004,16 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
           aN[t] = 0.;
           for (int s = 0; s < NS; ++s) if (s < nsV) aV[t * nsV + s] = 0.;
         }
-        // the target's sums at the restored state
-        double sink = 0., so = 0., sx = 0., src[NS], Vj[NS];
+        // the entries at the restored state; the target's sums as the forward formed them
+        const double *trow = tTG + it * 8;
+        const double sink = trow[0], so = trow[4], sx = trow[5];
+        double src[NS], Vj[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) { src[s] = 0.; Vj[s] = 0.; }
-        uint32_t pmask = 0, role = 0;
+        for (int s = 0; s < NS; ++s) {
+          src[s] = trow[1 + s];
+          Vj[s] = 0.;
+        }
+        uint32_t role = 0;
         int j = 0, pid = 0, epb = -1;
         double kj = 0., Nj = 0.;
         if (lane < ne) {
@@ -986,34 +1025,10 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
           Nj = sN[j];
 #pragma unroll
           for (int s = 0; s < NS; ++s) Vj[s] = s < nsV ? sV[j * nsV + s] : 0.;
-          if (role & R_SINK) sink += kj * Nj;
-          if (role & R_SRC) {
-#pragma unroll
-            for (int s = 0; s < NS; ++s)
-              if (s < nupd) src[s] += kj * Vj[s];
-          }
-          if (role & (R_PF_OWN | R_PF_OTH)) {
-            epb = (int)(sPF[pid - pf0] & 31u) - 1;
-            if (epb >= 0) {
-              sink += kj * Nj;
-              pmask |= 1u << epb;
-            } else {
-#pragma unroll
-              for (int s = 0; s < NS; ++s)
-                if (s < nupd) src[s] += kj * Vj[s];
-            }
-          }
-          if (role & R_FIX_OWN) so += kj * Nj;
-          if (role & R_FIX_OTH) sx += kj * Nj;
+          if (role & (R_PF_OWN | R_PF_OTH)) epb = (int)(sPF[pid - pf0] & 31u) - 1;
         }
-        sink = wsum(sink);
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-          if (s < nupd) src[s] = wsum(src[s]);
         double fix_own = 1., fix_oth = 1.;
         if (fl & TF_PF) {
-          so = wsum(so);
-          sx = wsum(sx);
           fix_own = fmax(1. - dt * so, 0.1);
           fix_oth = fmax(1. - dt * sx, 0.1);
         }
@@ -1039,8 +1054,7 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
         for (int s = 0; s < NS; ++s) { aVt[s] = 0.; avol[s] = 0.; aVj[s] = 0.; aVt_l[s] = 0.; }
         // transfer into the precipitation bins: N_r += dt npp, V_r += dt vp
         if (pft) {
-          uint32_t pm = cell_or<kAL>(pmask);
-          if (pself >= 0) pm |= 1u << pself;
+          uint32_t pm = (uint32_t)trow[6];
           while (pm) {
             const int pb = __builtin_ctz(pm);
             pm &= pm - 1;
@@ -1210,9 +1224,22 @@ __global__ void __launch_bounds__(64) k_salsa_adjoint(const DevLayout *__restric
       }
     }
     if (!isfinite(arv_l) || !isfinite(aE)) flags |= SALSA_CELL_GRAD_UNSTABLE;
+    // Stability of the linearisation: a relative change of the state before
+    // this step, or of E, amplified more than 1e8 times in the cell's J
+    // (no perturbation resolvable in FP64 follows the derivative then)
+    double amp = 0.;
+    if (own) {
+      amp = fabs(aN[lane] * x[1 + lane]);
+      for (int s = 0; s < NS; ++s)
+        if (s < nsV) amp += fabs(aV[lane * nsV + s] * x[1 + nt + lane * nsV + s]);
+    }
+    amp = wsum(amp) + fabs(arv_l * x[0]);
     if (lane == 0) {
       lam[0] = arv_l;
-      a.ebar[cell] += aE;
+      const double eb = a.ebar[cell] + aE;
+      a.ebar[cell] = eb;
+      const double Jc = a.jcell[cell];
+      if (amp > 1.e8 * Jc || fabs(eb) * D.charge_factor > 1.e8 * Jc) flags |= SALSA_CELL_GRAD_UNSTABLE;
     }
     flags = cell_or<kAL>(flags);
     if (lane == 0 && flags) {

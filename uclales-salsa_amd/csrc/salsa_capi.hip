@@ -663,13 +663,15 @@ extern "C" int salsa_adjoint_device(salsa_ctx *c, const double *press, const dou
   const int nseg = (every < nsteps ? every : nsteps) - 1;
   const int64_t sz = n * nvar;                           // doubles per set of cell states
   const unsigned nblk = (unsigned)(n < 4096 ? n : 4096);
-  int64_t stride = (int64_t)nsub * (L.nt + 1);
+  int64_t stride = (int64_t)nsub * (L.nt + 2);
   const int64_t t_p = stride;
   stride += 2 * (int64_t)L.npair;
   const int64_t t_pf = stride;
   stride += (int64_t)L.ntarget * L.nb[2] * (1 + L.nwet);
   const int64_t t_eq = stride;
   stride += 4 * 104;
+  const int64_t t_tg = stride;
+  stride += 8 * (int64_t)L.ntarget;
   stride = (stride + 31) / 32 * 32;
   const size_t need = (size_t)(((int64_t)nck + nseg + 3) * sz + 2 * n + 32 + (int64_t)nblk * stride) * 8;
   if (need > c->adj_bytes) {
@@ -702,11 +704,13 @@ extern "C" int salsa_adjoint_device(salsa_ctx *c, const double *press, const dou
   g.nsub = nsub;
   g.lam = lam;
   g.ebar = ebar;
+  g.jcell = jc;
   g.tape = tape;
   g.tape_stride = stride;
   g.t_p = t_p;
   g.t_pf = t_pf;
   g.t_eq = t_eq;
+  g.t_tg = t_tg;
   g.dt = tstep;
   g.rhlim = c->params.rhlim;
   g.lsfreeRH = c->params.lsfreeRH;

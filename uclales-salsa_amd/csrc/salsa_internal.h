@@ -11,8 +11,8 @@ namespace salsa {
 
 constexpr int kMaxBin = 64;           // master bins (aerosol|cloud|precip|ice)
 constexpr int kMaxSpec = 9;           // Section%volc(9)
-constexpr int kThermo = 6;
-constexpr int kRateLanes = 16;        // lanes per cell of the step variant that writes rate diagnostics            // T, p, rs, rsi, dn, rv at the head of a cell record
+constexpr int kThermo = 6;            // T, p, rs, rsi, dn, rv at the head of a cell record
+constexpr int kRateLanes = 16;        // lanes per cell of the step variant that writes rate diagnostics
 constexpr int kMaxSlot = kThermo + kMaxBin * (1 + kMaxSpec);
 
 // Physical constants exactly as written in src/src_shared/mo_submctl.f90:253-338.
@@ -195,12 +195,14 @@ struct AdjArgs {
   double *x_out;
   double *lam;
   double *ebar;
+  const double *jcell;                // J per cell (mode 1: the amplification bound of the flag)
   // per-block tape of mode 1, tape_stride doubles per block: the condensation
-  // substeps [nsub][nt + 1] at 0, pair derivatives [2][npair] at t_p, the
+  // substeps [nsub][nt + 2] at 0, pair derivatives [2][npair] at t_p, the
   // precipitation bins before each transfer [ntarget][nprc][1 + nwet] at
-  // t_pf, regime-1 equilibration iterates [4][104] at t_eq
+  // t_pf, regime-1 equilibration iterates [4][104] at t_eq, each sweep
+  // target's sums [ntarget][8] at t_tg
   double *tape;
-  int64_t tape_stride, t_p, t_pf, t_eq;
+  int64_t tape_stride, t_p, t_pf, t_eq, t_tg;
   double dt, rhlim;
   int32_t lsfreeRH;
   uint8_t *cell_flags;                // OR-ed with the step's fault bits
