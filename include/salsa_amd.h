@@ -294,8 +294,8 @@ int salsa_gradient_device(salsa_ctx *ctx, const double *press, const double *tem
  * One reverse sweep gives dJ/dN0 of every bin and dJ/dE at once.  Device
  * memory, kept by the context and grown on demand: ceil(nsteps /
  * checkpoint_every) + min(checkpoint_every, nsteps) + 2 cell states of
- * 8 (1 + nt (1 + nwet)) bytes per interior cell, and a tape of about 36 KiB
- * per launched wavefront.  Level 4 with lscoag_mode 1 and without autoconv2,
+ * 8 (1 + nt (1 + nwet)) bytes per interior cell, and a tape of about 30 KiB
+ * per cell slot (two per wavefront, at most 8192 slots).  Level 4 with lscoag_mode 1 and without autoconv2,
  * activation and the size-distribution update; SALSA_ERR_ARG otherwise, or
  * for a bin layout beyond the kernel's capacity (32 bins, 2 wet species).
  * SALSA_CELL_GRAD_UNSTABLE marks cells whose condensation substeps amplify a

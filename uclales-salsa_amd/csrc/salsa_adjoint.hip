@@ -59,28 +59,37 @@ namespace salsa {
 
 namespace {
 
-constexpr int kAL = 64;        // lanes per cell
 constexpr int kEqIt = 104;     // stored regime-1 iterates per bin (at most 101 are used)
 
-__device__ __forceinline__ double wsum(double x) { return cell_sum<kAL>(x); }
-// largest value over the wavefront (every lane gets it): DPP within rows,
-// then the row and half exchanges
-__device__ __forceinline__ double wmax(double x) {
+// largest value over the L lanes of a cell (every lane of the cell gets it):
+// DPP within rows, then the row and half exchanges
+template <int L>
+__device__ __forceinline__ double cell_max(double x) {
   x = fmax(x, dpp_d<0xB1>(x));
   x = fmax(x, dpp_d<0x4E>(x));
   x = fmax(x, dpp_d<0x141>(x));
   x = fmax(x, dpp_d<0x140>(x));
-  {
+  if constexpr (L >= 32) {
     auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(x), false, false);
     auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(x), false, false);
     x = fmax(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
   }
-  {
+  if constexpr (L >= 64) {
     auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(x), false, false);
     auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(x), false, false);
     x = fmax(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
   }
   return x;
+}
+
+// Ordering of the cell's LDS traffic between its lanes.  A block is one
+// wavefront, so no barrier between waves is needed; the fence keeps the
+// compiler from moving LDS accesses across it and waits for the
+// outstanding ones.  Unlike a block barrier it is valid where the two cells
+// of the wavefront take different branches.
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
 }
 
 // a polynomial of degree 4 in aw and its derivative
@@ -221,11 +230,15 @@ __device__ __forceinline__ void act_h2o_d(double x, double s, double &dx, double
 // ---------------------------------------------------------------------------
 // the step / adjoint kernel: one cell per wavefront, cells strided over the grid
 // ---------------------------------------------------------------------------
-template <class KT, int MB, int NS, int NP, bool REV>
+template <class KT, int MB, int NS, int NP, bool REV, int LPC>
 __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__restrict__ Dp, AdjArgs a) {
   static_assert(NS <= 3, "a target's tape row holds three species sums");
+  static_assert(MB <= LPC, "one lane per bin");
+  constexpr int CPW = 64 / LPC;        // cells per wavefront
   const DevLayout &D = *Dp;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x % LPC;  // the cell's lane (its bin)
+  const int cs = threadIdx.x / LPC;    // the cell's slot in the wavefront
+  const int lb = cs * LPC;             // its first lane in the wavefront
   const int nt = D.nt, nsV = D.nwet, np = D.npair, iwa = D.iwa, ndry = D.ndry;
   const int R = D.off[2], A0 = D.off[0], pf0 = D.pf0, pf1 = D.pf0 + D.ncc, nb2 = D.nb[2];
   const double dt = a.dt;
@@ -233,26 +246,36 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
   const bool coag = D.lscoag, cond = D.lscnd && (D.lscndh2oae || D.lscndh2ocl);
   const int n1 = D.fn1a - D.in1a + 1;
 
-  __shared__ double sN[MB], sV[MB * NS], sDW[MB], sK[NP];
-  __shared__ uint8_t sPF[NP];
-  __shared__ double aN[MB], aV[MB * NS], aDW[MB];
-  __shared__ double gT[2 * MB][8];     // FP64 particle terms (D c w v) and their d/d(dw): bins, then twins
-  __shared__ PT4<KT> sPR[2 * MB];
-  __shared__ KT sVL[2 * MB];
-  __shared__ int eqK[4];               // regime-1 iterations per 1a bin (-1: bin skipped)
+  // LDS of the wavefront's cells
+  __shared__ double sN_[CPW][MB], sV_[CPW][MB * NS], sDW_[CPW][MB];
+  __shared__ KT sK_[CPW][NP];
+  __shared__ uint8_t sPF_[CPW][NP];
+  __shared__ double aN_[CPW][MB], aV_[CPW][MB * NS], aDW_[CPW][MB];
+  __shared__ PT4<KT> sPR_[CPW][2 * MB];
+  __shared__ KT sVL_[CPW][2 * MB];
+  __shared__ int eqK_[CPW][4];         // regime-1 iterations per 1a bin (-1: bin skipped)
+  double *sN = sN_[cs], *sV = sV_[cs], *sDW = sDW_[cs], *aN = aN_[cs], *aV = aV_[cs], *aDW = aDW_[cs];
+  KT *sK = sK_[cs], *sVL = sVL_[cs];
+  uint8_t *sPF = sPF_[cs];
+  PT4<KT> *sPR = sPR_[cs];
+  int *eqK = eqK_[cs];
 
   int isu = -1, ioc = -1;
   for (int s = 0; s < ndry; ++s) {
     if (D.dry_id[s] == 0) isu = s;
     if (D.dry_id[s] == 1) ioc = s;
   }
-  double *tape = a.tape + (int64_t)blockIdx.x * a.tape_stride;
+  double *tape = a.tape + ((int64_t)blockIdx.x * CPW + cs) * a.tape_stride;
   double *tP = tape + a.t_p;           // [2][np] dK/d(dw_b1), dK/d(dw_b2)
   double *tPF = tape + a.t_pf;         // [ntarget][nb2][1 + nsV] precipitation bins before a transfer
   double *tEQ = tape + a.t_eq;         // [4][kEqIt] regime-1 iterates
   double *tTG = tape + a.t_tg;         // [ntarget][8] each target's sums: sink, src[3], so, sx, transfer mask
+  double (*gT)[8] = reinterpret_cast<double (*)[8]>(tape + a.t_gt);   // [2 MB][8] FP64 particle terms
+                                       // (D c w v) and their d/d(dw): bins, then twins
 
-  for (int64_t cell = blockIdx.x; cell < a.ncells; cell += gridDim.x) {
+  for (int64_t grp = blockIdx.x; grp * CPW < a.ncells; grp += gridDim.x) {
+    const int64_t cell = grp * CPW + cs;
+    if (cell >= a.ncells) continue;                      // the wavefront's other cell goes on alone
     const double *rec = a.rec + cell * a.nslot;
     const double T = rec[0], P = rec[1], rs = rec[2], dn = rec[4];
     const double *x = a.x_in + cell * a.nvar;
@@ -268,7 +291,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
       sN[lane] = x[1 + lane];
       for (int s = 0; s < NS; ++s) if (s < nsV) sV[lane * nsV + s] = x[1 + nt + lane * nsV + s];
     }
-    __syncthreads();
+    wsync();
 
     // ---- bin diameters (updateDiameter, classSection.f90:91-159) and particle terms ----
     const CellC cc = cell_consts(D, T, P);
@@ -319,8 +342,9 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
         }
       }
     }
-    const uint64_t abits = __ballot(actb);    // bins with numc >= nlim take part in kernels
-    __syncthreads();
+    // bins with numc >= nlim take part in kernels
+    const uint64_t abits = (__ballot(actb) >> lb) & (LPC == 64 ? ~0ull : (1ull << (LPC & 63)) - 1);
+    wsync();
 
     // ---- pair kernels (buildKernel*, kernels.f90:94-180) and their diameter derivatives ----
     if (coag) {
@@ -328,7 +352,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
       const KT cfac = (KT)D.charge_factor;
       GCellC gc{};
       if (rev) gc = g_cell_consts(D, T, P);
-      for (int p = lane; p < np; p += kAL) {
+      for (int p = lane; p < np; p += LPC) {
         const uint32_t pw = D.pw[p];
         const int b1 = pw & 255u, b2 = (pw >> 8) & 255u, tw = (pw >> 16) & 1u;
         KT k = KT(0.);
@@ -366,7 +390,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
             }
           }
         }
-        sK[p] = (double)k;
+        sK[p] = k;
         if (p >= pf0 && p < pf1) sPF[p - pf0] = (uint8_t)pfb;
         if constexpr (rev) {
           tP[p] = p1;
@@ -374,7 +398,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
         }
       }
     }
-    __syncthreads();
+    wsync();
 
     // ---- the sweep: coag_precp, coag_aero, coag_cloud (mo_salsa_dynamics.f90:166-183) ----
     // Lane t keeps target t's values before its update (and bin t+1's, for
@@ -442,14 +466,14 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
         if (role & R_FIX_OWN) so += kj * Nj;
         if (role & R_FIX_OTH) sx += kj * Nj;
       }
-      sink = wsum(sink);
+      sink = cell_sum<LPC>(sink);
 #pragma unroll
       for (int s = 0; s < NS; ++s)
-        if (s < nupd) src[s] = wsum(src[s]);
+        if (s < nupd) src[s] = cell_sum<LPC>(src[s]);
       double fix_own = 1., fix_oth = 1.;
       if (fl & TF_PF) {                                  // fix_coag (:1440-1456)
-        so = wsum(so);
-        sx = wsum(sx);
+        so = cell_sum<LPC>(so);
+        sx = cell_sum<LPC>(sx);
         fix_own = fmax(1. - dt * so, 0.1);
         fix_oth = fmax(1. - dt * sx, 0.1);
       }
@@ -488,7 +512,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
 #pragma unroll
       for (int s = 0; s < NS; ++s) Vn[s] = s < nupd ? (Vt[s] + dt * src[s] * Nt) * rden : Vt[s];
       double Nn = Nt * rcp_nr(1. + dt * (sink + sink_self));
-      __syncthreads();                                   // every lane has read the state
+      wsync();                                   // every lane has read the state
       if (fl & TF_PRECIP_SELFCOLL) {                     // precipSelfCoag (:1588-1611)
         const double nslf = 0.5 * kself * Nt * Nt;
 #pragma unroll
@@ -512,7 +536,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
       // coalesced cloud drops into precipitation bins (applyCoagPrecipFormation :649-703);
       // the entries read cloud bins other than t, the writes go to precipitation bins
       if (fl & (TF_PF | TF_PF_SELF)) {
-        uint32_t pm = cell_or<kAL>(pmask);
+        uint32_t pm = cell_or<LPC>(pmask);
         if (pself >= 0) pm |= 1u << pself;
         if constexpr (rev) {
           if (lane == 0) tTG[it * 8 + 6] = (double)pm;
@@ -537,10 +561,10 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
               npp += kj * Nj * Nt * fx;
             }
           }
-          npp = wsum(npp);
+          npp = cell_sum<LPC>(npp);
 #pragma unroll
           for (int s = 0; s < NS; ++s)
-            if (s < nupd) vp[s] = wsum(vp[s]);
+            if (s < nupd) vp[s] = cell_sum<LPC>(vp[s]);
           if (pb == pself) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) vp[s] += volsink[s];
@@ -555,7 +579,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           }
         }
       }
-      __syncthreads();
+      wsync();
     }
     // clamp (mo_salsa_dynamics.f90:189-198); lane b keeps its bin before it
     double cNr = 0., cVr[NS];
@@ -573,7 +597,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           sV[b * nsV + s] = fmax(0., sV[b * nsV + s]);
         }
     }
-    __syncthreads();
+    wsync();
 
     // ---- condensation (mo_salsa_dynamics.f90:276-342, 691-1125) ----
     const double zrh = rv / rs;
@@ -590,7 +614,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
     double eqz = 0.;
     int eq2 = -1;                       // -1: no task; else the iteration count
     if (cond) {
-      double w1 = wsum(lane < D.nb[0] ? sV[lane * nsV + iwa] : 0.) * kRhowa;
+      double w1 = cell_sum<LPC>(lane < D.nb[0] ? sV[lane * nsV + iwa] : 0.) * kRhowa;
       // Equilibration (mo_salsa_properties.f90:73-380): regime 1 on lane 0 (a
       // chain: the Kelvin factor carries over), regime 2 bin j on lane j + 1
       const bool reg2 = zrh < 0.98 || !D.lscndh2oae;
@@ -631,8 +655,8 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           eq2 = K;
         }
       }
-      __syncthreads();
-      const double w2 = wsum(lane < D.nb[0] ? sV[lane * nsV + iwa] : 0.) * kRhowa;
+      wsync();
+      const double w2 = cell_sum<LPC>(lane < D.nb[0] ? sV[lane * nsV + iwa] : 0.) * kRhowa;
       rv = rv - (w2 - w1) / Qv;
 
       // cell-skip predicate (:784-788; .AND. binds before .OR.)
@@ -644,7 +668,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
         if (c == 1 && n > kNlim) any |= 1u;
         if (c == 2 && n > kPrlim) any |= 2u;
       }
-      any = cell_or<kAL>(any);
+      any = cell_or<LPC>(any);
       pass = ((any & 1u) || ((any & 2u) && D.lscndh2ocl)) || ((any & 8u) && zrh > 0.98 && D.lscndh2oae);
     }
     if (pass) {
@@ -688,10 +712,10 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
       }
       double zcw = zcw0, zws = upd ? zws0 : 0.;
       double zcwc = rv * rhoair / kMwa;
-      const double stot = wsum(own ? zcw0 : 0.);
-      const double sfix = wsum((own && !upd) ? zcw0 : 0.);
-      double s1 = wsum(upd ? zmt * zws * csul : 0.);
-      s2 = wsum(upd ? zmt : 0.);
+      const double stot = cell_sum<LPC>(own ? zcw0 : 0.);
+      const double sfix = cell_sum<LPC>((own && !upd) ? zcw0 : 0.);
+      double s1 = cell_sum<LPC>(upd ? zmt * zws * csul : 0.);
+      s2 = cell_sum<LPC>(upd ? zmt : 0.);
       zcwtot = zcwc + stot;
       double zcwint = 0.;
       // Stability of the linearisation: where the +-10 % limiter does not
@@ -726,17 +750,17 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           zws = act_h2o<true>(nw, zns) * zkl;
           s1n = zmt * zws * csul;
         }
-        const double tsum = wsum(nw);
-        s1 = wsum(s1n);
+        const double tsum = cell_sum<LPC>(nw);
+        s1 = cell_sum<LPC>(s1n);
         zcwint = zcwtot - (sfix + tsum);
         zcwc = zcwint;
-        if (rev) grow = fmin(grow * wmax(m), 1.e300);
+        if (rev) grow = fmin(grow * cell_max<LPC>(m), 1.e300);
       }
       if (grow > 1.e4) flags |= SALSA_CELL_GRAD_UNSTABLE;
       if (nsub > 100) flags |= SALSA_CELL_SUBSTEPS;
       rv = zcwint * kMwa / rhoair;
       if (own) sV[lane * nsV + iwa] = fmax(0., zcw * kMwa / kRhowa);   // write-back (:1058-1117)
-      __syncthreads();
+      wsync();
     }
 
     if constexpr (!rev) {
@@ -756,12 +780,12 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
       }
       if (!isfinite(rv)) flags |= SALSA_CELL_NONFINITE;
       if (lane == 0) xo[0] = rv;
-      flags = cell_or<kAL>(flags);
+      flags = cell_or<LPC>(flags);
       if (lane == 0 && flags) {
         if (a.cell_flags) a.cell_flags[cell] |= (uint8_t)flags;
         if (a.fault_any) atomicOr(a.fault_any, (int)flags);
       }
-      __syncthreads();
+      wsync();
       continue;
     }
 
@@ -776,7 +800,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
     }
     const double arv = lam[0];
     double aE = 0.;                                      // this lane's share of dJ/dE
-    __syncthreads();
+    wsync();
 
     double arv_l = arv;                                  // adjoint of rv after the RH limit
     if (cond) {
@@ -825,7 +849,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           } else if (own) {
             azcw = acw;                                  // a bin outside the substeps keeps its water
           }
-          azcwi = wsum(azcwi);
+          azcwi = cell_sum<LPC>(azcwi);
           double aA = 0.;
           if (tookA) aA = azcwi;
           else azcwtot += azcwi;
@@ -889,12 +913,12 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           }
           aN[b] += an;
         }
-        __syncthreads();
+        wsync();
       }
       // ---- vapour budget of the equilibration: rv_e = rv_l - (w2 - w1) / Qv ----
       const double aw = arv_e / Qv * kRhowa;
       if (lane < D.nb[0]) aV[lane * nsV + iwa] -= aw;    // w2: the water after the equilibration
-      __syncthreads();
+      wsync();
       double azrh = 0.;
       if (lane == 0) {
         // regime 1 in reverse bin order, the Kelvin factor carried back
@@ -933,10 +957,10 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           azrh += eq_task_rev(D, q, pb, nsV, bb, zrh, aN, aV);
         }
       }
-      __syncthreads();
+      wsync();
       if (lane < D.nb[0]) aV[lane * nsV + iwa] += aw;    // w1: the water before it
-      arv_l = arv_e + wsum(azrh) / rs;                   // zrh = rv_l / rs
-      __syncthreads();
+      arv_l = arv_e + cell_sum<LPC>(azrh) / rs;                   // zrh = rv_l / rs
+      wsync();
     }
 
     if (coag) {
@@ -949,7 +973,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           sV[lane * nsV + s] = cVr[s];
         }
       }
-      __syncthreads();
+      wsync();
 
       // ---- the sweep in reverse ----
       const double cf = D.charge_factor;
@@ -957,25 +981,25 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
         const DevTarget &TG = D.tgt[it];
         const int t = TG.t;
         const uint32_t fl = TG.flags;
-        if (!__shfl((int)tAct, t)) continue;            // the target did not change anything
+        if (!__shfl((int)tAct, lb + t)) continue;       // the target did not change anything
         const bool pft = fl & (TF_PF | TF_PF_SELF);
         const bool scoll = fl & TF_PRECIP_SELFCOLL;
         const int nupd = TG.nupd, ne = TG.nent;
         const uint32_t *ent = D.ent + TG.ent0;
-        const double Nt = __shfl(tNr, t);
+        const double Nt = __shfl(tNr, lb + t);
         double Vt[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) Vt[s] = __shfl(tVr[s], t);
+        for (int s = 0; s < NS; ++s) Vt[s] = __shfl(tVr[s], lb + t);
         // restore the bins the target changed to their values before its update
         if (lane == 0) {
           sN[t] = Nt;
           for (int s = 0; s < NS; ++s) if (s < nsV) sV[t * nsV + s] = Vt[s];
         }
         if (scoll) {
-          const double N1 = __shfl(t1Nr, t);
+          const double N1 = __shfl(t1Nr, lb + t);
           double V1[NS];
 #pragma unroll
-          for (int s = 0; s < NS; ++s) V1[s] = __shfl(t1Vr[s], t);
+          for (int s = 0; s < NS; ++s) V1[s] = __shfl(t1Vr[s], lb + t);
           if (lane == 0) {
             sN[t + 1] = N1;
             for (int s = 0; s < NS; ++s) if (s < nsV) sV[(t + 1) * nsV + s] = V1[s];
@@ -999,7 +1023,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
 #pragma unroll
           for (int s = 0; s < NS; ++s) aV1[s] = s < nsV ? aV[(t + 1) * nsV + s] : 0.;
         }
-        __syncthreads();
+        wsync();
         if (lane == 0) {                                 // bin t is rewritten: its adjoint restarts
           aN[t] = 0.;
           for (int s = 0; s < NS; ++s) if (s < nsV) aV[t * nsV + s] = 0.;
@@ -1150,7 +1174,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
         }
         double aso = 0., asx = 0.;
         if (fl & TF_PF) {
-          const double afo = wsum(afo_l), afx = wsum(afx_l);
+          const double afo = cell_sum<LPC>(afo_l), afx = cell_sum<LPC>(afx_l);
           if (1. - dt * so > 0.1) aso = -dt * afo;
           if (1. - dt * sx > 0.1) asx = -dt * afx;
         }
@@ -1180,19 +1204,19 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           adwt = aK * (b1 == t ? tP[pid] : tP[np + pid]);
           aE += aK * kj / cf;
         }
-        __syncthreads();                                 // bin t's adjoint is cleared
+        wsync();                                 // bin t's adjoint is cleared
         if (lane < ne) {
           const int b1 = D.pw[pid] & 255u;
           aDW[j] += aK * (b1 == t ? tP[np + pid] : tP[pid]);   // one lane per bin j
           aN[j] += aNj;
           for (int s = 0; s < NS; ++s) if (s < nsV) aV[j * nsV + s] += aVj[s];
         }
-        adwt = wsum(adwt);
-        aNt_l = wsum(aNt_l);
+        adwt = cell_sum<LPC>(adwt);
+        aNt_l = cell_sum<LPC>(aNt_l);
 #pragma unroll
         for (int s = 0; s < NS; ++s)
-          if (s < nupd) aVt_l[s] = wsum(aVt_l[s]);
-        __syncthreads();
+          if (s < nupd) aVt_l[s] = cell_sum<LPC>(aVt_l[s]);
+        wsync();
         if (lane == 0) {
           if (pid_self >= 0) {                           // the self pair's kernel
             adwt += akself * (tP[pid_self] + tP[np + pid_self]);
@@ -1202,19 +1226,19 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
           for (int s = 0; s < NS; ++s) if (s < nsV) aV[t * nsV + s] += aVt[s] + aVt_l[s];
           aDW[t] += adwt;
         }
-        __syncthreads();
+        wsync();
       }
       // ---- wet diameters (of the pair kernels) onto N and V ----
       if (own) {
         aN[lane] += aDW[lane] * dwn;
         for (int s = 0; s < NS; ++s) if (s < nsV) aV[lane * nsV + s] += aDW[lane] * dwv;
       }
-      __syncthreads();
+      wsync();
     }
 
     // ---- the relative-humidity limit, and out ----
     if (rvlim) arv_l = 0.;
-    aE = wsum(aE);
+    aE = cell_sum<LPC>(aE);
     if (own) {
       lam[1 + lane] = aN[lane];
       if (!isfinite(aN[lane])) flags |= SALSA_CELL_GRAD_UNSTABLE;
@@ -1233,7 +1257,7 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
       for (int s = 0; s < NS; ++s)
         if (s < nsV) amp += fabs(aV[lane * nsV + s] * x[1 + nt + lane * nsV + s]);
     }
-    amp = wsum(amp) + fabs(arv_l * x[0]);
+    amp = cell_sum<LPC>(amp) + fabs(arv_l * x[0]);
     if (lane == 0) {
       lam[0] = arv_l;
       const double eb = a.ebar[cell] + aE;
@@ -1241,12 +1265,12 @@ __global__ void __launch_bounds__(64, 2) k_salsa_adjoint(const DevLayout *__rest
       const double Jc = a.jcell[cell];
       if (amp > 1.e8 * Jc || fabs(eb) * D.charge_factor > 1.e8 * Jc) flags |= SALSA_CELL_GRAD_UNSTABLE;
     }
-    flags = cell_or<kAL>(flags);
+    flags = cell_or<LPC>(flags);
     if (lane == 0 && flags) {
       if (a.cell_flags) a.cell_flags[cell] |= (uint8_t)flags;
       if (a.fault_any) atomicOr(a.fault_any, (int)flags);
     }
-    __syncthreads();
+    wsync();
   }
 }
 
@@ -1350,11 +1374,11 @@ int launch_adjoint_step(int precision, const DevLayout *d, const AdjArgs &a, uns
   if (nblk == 0) return 0;
   const bool fp64 = precision == SALSA_FP64;
   if (a.mode == 1) {
-    if (fp64) hipLaunchKernelGGL((k_salsa_adjoint<double, 32, 3, 528, true>), dim3(nblk), dim3(64), 0, s, d, a);
-    else hipLaunchKernelGGL((k_salsa_adjoint<float, 32, 3, 528, true>), dim3(nblk), dim3(64), 0, s, d, a);
+    if (fp64) hipLaunchKernelGGL((k_salsa_adjoint<double, 32, 3, 528, true, 32>), dim3(nblk), dim3(64), 0, s, d, a);
+    else hipLaunchKernelGGL((k_salsa_adjoint<float, 32, 3, 528, true, 32>), dim3(nblk), dim3(64), 0, s, d, a);
   } else {
-    if (fp64) hipLaunchKernelGGL((k_salsa_adjoint<double, 32, 3, 528, false>), dim3(nblk), dim3(64), 0, s, d, a);
-    else hipLaunchKernelGGL((k_salsa_adjoint<float, 32, 3, 528, false>), dim3(nblk), dim3(64), 0, s, d, a);
+    if (fp64) hipLaunchKernelGGL((k_salsa_adjoint<double, 32, 3, 528, false, 32>), dim3(nblk), dim3(64), 0, s, d, a);
+    else hipLaunchKernelGGL((k_salsa_adjoint<float, 32, 3, 528, false, 32>), dim3(nblk), dim3(64), 0, s, d, a);
   }
   return (int)hipGetLastError();
 }

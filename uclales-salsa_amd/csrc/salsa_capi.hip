@@ -662,7 +662,8 @@ extern "C" int salsa_adjoint_device(salsa_ctx *c, const double *press, const dou
   const int nck = (nsteps + every - 1) / every;
   const int nseg = (every < nsteps ? every : nsteps) - 1;
   const int64_t sz = n * nvar;                           // doubles per set of cell states
-  const unsigned nblk = (unsigned)(n < 4096 ? n : 4096);
+  const int64_t ngrp = (n + 1) / 2;                      // two cells per wavefront
+  const unsigned nblk = (unsigned)(ngrp < 4096 ? ngrp : 4096);
   int64_t stride = (int64_t)nsub * (L.nt + 2);
   const int64_t t_p = stride;
   stride += 2 * (int64_t)L.npair;
@@ -672,8 +673,10 @@ extern "C" int salsa_adjoint_device(salsa_ctx *c, const double *press, const dou
   stride += 4 * 104;
   const int64_t t_tg = stride;
   stride += 8 * (int64_t)L.ntarget;
+  const int64_t t_gt = stride;
+  stride += 2 * 32 * 8;
   stride = (stride + 31) / 32 * 32;
-  const size_t need = (size_t)(((int64_t)nck + nseg + 3) * sz + 2 * n + 32 + (int64_t)nblk * stride) * 8;
+  const size_t need = (size_t)(((int64_t)nck + nseg + 3) * sz + 2 * n + 32 + 2 * (int64_t)nblk * stride) * 8;
   if (need > c->adj_bytes) {
     (void)hipFree(c->adj_buf);
     c->adj_buf = nullptr;
@@ -711,6 +714,7 @@ extern "C" int salsa_adjoint_device(salsa_ctx *c, const double *press, const dou
   g.t_pf = t_pf;
   g.t_eq = t_eq;
   g.t_tg = t_tg;
+  g.t_gt = t_gt;
   g.dt = tstep;
   g.rhlim = c->params.rhlim;
   g.lsfreeRH = c->params.lsfreeRH;

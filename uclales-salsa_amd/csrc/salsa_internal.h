@@ -200,9 +200,11 @@ struct AdjArgs {
   // substeps [nsub][nt + 2] at 0, pair derivatives [2][npair] at t_p, the
   // precipitation bins before each transfer [ntarget][nprc][1 + nwet] at
   // t_pf, regime-1 equilibration iterates [4][104] at t_eq, each sweep
-  // target's sums [ntarget][8] at t_tg
+  // target's sums [ntarget][8] at t_tg, the bins' particle terms with their
+  // diameter derivatives [64][8] at t_gt; one tape per cell slot (two cells
+  // per block)
   double *tape;
-  int64_t tape_stride, t_p, t_pf, t_eq, t_tg;
+  int64_t tape_stride, t_p, t_pf, t_eq, t_tg, t_gt;
   double dt, rhlim;
   int32_t lsfreeRH;
   uint8_t *cell_flags;                // OR-ed with the step's fault bits
