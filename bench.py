@@ -196,6 +196,107 @@ def gradient_leg(capi, W, desc, nsteps=20):
     return res
 
 
+def adjoint_leg(capi, W, desc, nsteps=20, every=5):
+    """Reverse-mode gradient (salsa_adjoint_device) of the same objective on
+    the same workload: checkpointed forward pass (every 5 steps), the stretch
+    between checkpoints recomputed, one adjoint sweep; timed per precision
+    path around the whole call (inputs resident in HBM, result totals read
+    back)."""
+    import numpy as np
+    import torch
+    f = W.config_fields(desc, 3)
+    g = f.grid
+    dev = torch.device("cuda")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fd = dict(press=T(f.press), temp=T(f.temp), rsl=T(f.rsl), rsi=T(f.rsi), dn=T(f.dn), rv=T(f.rv))
+    n_p = [T(x) for x in f.n[:3]]
+    m_p = [T(x) for x in f.m[:3]]
+    djdn = [torch.zeros_like(x) for x in n_p]
+    res = {"workload": "32x32x80 = %d cells, bins 11/8/7, %d steps, checkpoints every %d steps, "
+                       "J = precipitation volume, dJ/dN0 of every bin and dJ/dE" % (g.ncells, nsteps, every),
+           "unit": "cell-steps/s"}
+    for prec, name in ((capi.SALSA_FP64, "fp64"), (capi.SALSA_FP32_MIXED, "mixed")):
+        ctx = capi.Context(desc, capi.params(level=4, precision=prec), g.nzp, g.nxp, g.nyp)
+        ctx.adjoint_device(fd, n_p, m_p, djdn, nsteps=nsteps, checkpoint_every=every)    # warm-up, buffers
+        best = None
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            J, dJdE = ctx.adjoint_device(fd, n_p, m_p, djdn, nsteps=nsteps, checkpoint_every=every)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        ctx.close()
+        res[name] = {"value": g.ncells * nsteps / best, "seconds": best, "J": J, "dJdE": dJdE}
+    return res
+
+
+def config_leg(capi, W, cfg, nsteps=10, gradient=False):
+    """The two 128x128x100 configurations (1,638,400 cells), FP32-mixed,
+    `nsteps` in-place steps after the autotuner and one warm-up step: config 4
+    (level 4, 15 % cloudy cells with every cloud and precipitation bin
+    active, the rest aerosol-only: the cost-class compaction at work) and
+    config 5 (level 5, SO4 + dust, ice bins with rime, T 240-275 K).  Timed
+    around the steps (wall clock, inputs resident in HBM) and on the GPU (HIP
+    events around each step's launches).  With `gradient`, config 5 also runs
+    the forward-mode derivative over the same 10 steps (level 5 has no
+    reverse sweep)."""
+    import numpy as np
+    import torch
+    level = 5 if cfg == 5 else 4
+    desc = capi.bin_desc(species=("SO4", "DU") if cfg == 5 else ("SO4",))
+    t0 = time.perf_counter()
+    f = W.config_fields(desc, cfg)
+    gen = time.perf_counter() - t0
+    g = f.grid
+    ncat = 4 if level == 5 else 3
+    dev = torch.device("cuda")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fd = dict(press=T(f.press), temp=T(f.temp), rsl=T(f.rsl), rsi=T(f.rsi), dn=T(f.dn), rv=T(f.rv))
+    n_p = [T(x) for x in f.n[:ncat]]
+    m_p = [T(x) for x in f.m[:ncat]]
+    ctx = capi.Context(desc, capi.params(level=level, precision=capi.SALSA_FP32_MIXED), g.nzp, g.nxp, g.nyp)
+    ctx.autotune(fd, n_p, m_p)
+    st = dict(fd)
+    st["rv"] = fd["rv"].clone()
+    nn = [x.clone() for x in n_p]
+    mm = [x.clone() for x in m_p]
+    ctx.step_device(st, nn, mm, flags=capi.STEP_INPLACE)       # warm-up on a copy
+    st["rv"].copy_(fd["rv"])
+    for a, b in zip(nn + mm, n_p + m_p):
+        a.copy_(b)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        ctx.step_device(st, nn, mm, flags=capi.STEP_INPLACE)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms, ksteps = ctx.kernel_time()
+    ms, cells, n = ctx.phase_time()
+    ctx.timing(False)
+    shapes = [ctx.shape_of(k) for k in range(3)]
+    res = {"workload": "config %d: 128x128x100 = %d cells, level %d, bins %d/%d/%d%s, FP32-mixed, %d in-place steps"
+                       % (cfg, g.ncells, level, desc.nbins, desc.ncld, desc.nprc,
+                          "/%d ice, SO4+DU" % desc.nice if level == 5 else ", SO4", nsteps),
+           "value": g.ncells * nsteps / wall, "unit": "cell-steps/s", "ms_per_step": wall / nsteps * 1e3,
+           "gpu_ms_per_step": kms / max(ksteps, 1), "cells_per_class": [c / max(n, 1) for c in cells],
+           "launch_lanes_per_cell": shapes, "field_generation_s": gen}
+    if gradient:
+        djdn = [torch.zeros_like(x) for x in n_p]
+        ctx.gradient_device(fd, n_p, m_p, djdn, nsteps=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J, dJdE = ctx.gradient_device(fd, n_p, m_p, djdn, nsteps=nsteps)
+        dt = time.perf_counter() - t0
+        fl = ctx.cell_flags()
+        res["gradient"] = {"value": g.ncells * nsteps / dt, "seconds": dt, "J": J, "dJdE": dJdE,
+                           "unstable_cells": int(np.count_nonzero(fl & capi.CELL_GRAD_UNSTABLE))}
+    ctx.close()
+    return res
+
+
 def les_leg(capi, desc, f, fd, n_p, m_p, calls=20):
     """The LES side after a step on device (salsa_les_update_device:
     tend_constrain2, update_sclrs, tend0, SALSA_diagnostics) on the same
@@ -301,6 +402,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 128x128x100 legs (configs 4 and 5)")
+    ap.add_argument("--config5-gradient", action="store_true",
+                    help="also the forward-mode derivative at config 5 full size (about a minute)")
     ap.add_argument("--no-les", action="store_true")
     ap.add_argument("--counter-run", choices=("mixed", "fp64", "copy"))
     ap.add_argument("--lanes", default="16,16,16")
@@ -453,6 +558,13 @@ def main():
         out["sedimentation"] = sedim_leg(capi, desc, f, fd, n_p, m_p)
     if not args.no_gradient:
         out["gradient"] = gradient_leg(capi, W, desc)
+        out["adjoint"] = adjoint_leg(capi, W, desc)
+        for name in ("fp64", "mixed"):
+            out["adjoint"][name]["vs_forward_tangents"] = (out["adjoint"][name]["value"] /
+                                                           out["gradient"][name]["value"])
+    if not args.no_configs:
+        out["config4"] = config_leg(capi, W, 4)
+        out["config5"] = config_leg(capi, W, 5, gradient=args.config5_gradient)
     out["setup_s"] = gen_s
     print(json.dumps(out))
 

@@ -124,6 +124,21 @@ MODULE mo_salsa_amd_cabi
        REAL(C_DOUBLE), INTENT(out) :: totals(2)
      END FUNCTION salsa_gradient_device
 
+     ! The same gradient by the reverse (adjoint) sweep, level 4, with the
+     ! states kept every checkpoint_every steps and the stretches between
+     ! checkpoints recomputed
+     INTEGER(C_INT) FUNCTION salsa_adjoint_device(ctx, press, temp, rsl, rsi, dn, rv, n_p, m_p, tstep, &
+                                                  nsteps, checkpoint_every, djdn, j_cell, djde_cell, totals) &
+                                                  BIND(C, name="salsa_adjoint_device")
+       IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE        :: ctx, press, temp, rsl, rsi, dn, rv
+       TYPE(C_PTR), INTENT(in)   :: n_p(4), m_p(4), djdn(4)
+       REAL(C_DOUBLE), VALUE     :: tstep
+       INTEGER(C_INT32_T), VALUE :: nsteps, checkpoint_every
+       TYPE(C_PTR), VALUE        :: j_cell, djde_cell
+       REAL(C_DOUBLE), INTENT(out) :: totals(2)
+     END FUNCTION salsa_adjoint_device
+
      ! The LES side after a step on device: tend_constrain2, update_sclrs,
      ! tend0 and SALSA_diagnostics for the SALSA tracers (dzt on the host)
      INTEGER(C_INT) FUNCTION salsa_les_update_device(ctx, dtlt, dzt, temp, rh, rhi, rp, n_p, n_t, m_p, m_t, &
