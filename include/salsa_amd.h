@@ -229,8 +229,9 @@ int salsa_step_device(salsa_ctx *ctx, const double *press, const double *temp,
                       const double *const m_p[4], double *const m_t[4],
                       double *rates, double tstep, double time, int32_t istp, int32_t flags);
 
-/* Wait for the context stream; returns SALSA_ERR_NOCONV / SALSA_ERR_NONFINITE
- * if any cell of the last step raised the corresponding fault. */
+/* Wait for the context stream; returns SALSA_ERR_NONFINITE / SALSA_ERR_NOCONV /
+ * SALSA_ERR_DISTR if any cell of the last step raised the corresponding fault
+ * (SALSA_OK on a context that has not stepped yet: its flags start at zero). */
 int salsa_ctx_sync(salsa_ctx *ctx);
 
 /* Copy the per-cell fault bits of the last step (one byte per interior cell,

@@ -23,27 +23,23 @@ def capi():
     return c
 
 
-@pytest.mark.parametrize("seed,grid,sed", [(1, (20, 8, 8), (1, 1, 1)), (2, (40, 12, 10), (1, 1, 1)),
-                                           (3, (20, 8, 8), (0, 0, 1)), (4, (20, 8, 8), (1, 0, 0))])
-def test_sedim_matches_oracle(capi, oracle_mod, seed, grid, sed):
+def _compare(capi, oracle_mod, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, seed, sed, dtlt=1.0):
+    """One call of salsa_sedim_device against so_sedim on the same inputs."""
     import torch
-    desc = capi.bin_desc()
-    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields(desc, grid, seed)
     nzp, nxp, nyp = shape
     n3 = nzp * nxp * nyp
-    print("case sedimentation, seed %d, grid %s, switches %s" % (seed, shape, sed), flush=True)
     tlt = np.random.default_rng(seed).normal(size=n3) * 1e-4
     ont, omt, otlt = [a.copy() for a in n_t], [a.copy() for a in m_t], tlt.copy()
-    rr, sr, vt = run(oracle_mod, desc, shape, dzt, fld, n_p, m_p, ont, omt, otlt, sed=sed)
+    rr, sr, vt = run(oracle_mod, desc, shape, dzt, fld, n_p, m_p, ont, omt, otlt, sed=sed, dtlt=dtlt)
     dev = torch.device("cuda")
     D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     dn, dm, dnt, dmt = ([D(a) for a in x] for x in (n_p, m_p, n_t, m_t))
     df = {k: D(v) for k, v in fld.items()}
-    dtlt, drr, dsr, dvt = D(tlt), torch.zeros(n3, dtype=torch.float64, device=dev), \
+    dtl, drr, dsr, dvt = D(tlt), torch.zeros(n3, dtype=torch.float64, device=dev), \
         torch.zeros(nxp * nyp, dtype=torch.float64, device=dev), torch.zeros((desc.nprc, n3), dtype=torch.float64,
                                                                              device=dev)
     ctx = capi.Context(desc, capi.params(level=4), nzp, nxp, nyp)
-    ctx.sedim_device(1.0, dzt, df, dn, dm, dnt, dmt, dtlt, drr, dsr, dvt, sed=sed)
+    ctx.sedim_device(dtlt, dzt, df, dn, dm, dnt, dmt, dtl, drr, dsr, dvt, sed=sed)
     ctx.sync()
     ctx.close()
 
@@ -56,14 +52,52 @@ def test_sedim_matches_oracle(capi, oracle_mod, seed, grid, sed):
     colmax = lambda a: np.broadcast_to(np.abs(a).reshape(-1, nyp * nxp, nzp).max(axis=2, keepdims=True),
                                        (a.reshape(-1, nyp * nxp, nzp).shape)).reshape(a.shape)
     for c in range(3):
-        close("n_t[%d]" % c, dnt[c].cpu().numpy(), ont[c], colmax(n_p[c]) + np.abs(n_t[c]))
-        close("m_t[%d]" % c, dmt[c].cpu().numpy(), omt[c], colmax(m_p[c]) + np.abs(m_t[c]))
-    close("tlt", dtlt.cpu().numpy(), otlt, np.abs(otlt) + np.abs(otlt - tlt).max())
+        close("n_t[%d]" % c, dnt[c].cpu().numpy(), ont[c], colmax(n_p[c]) / dtlt + np.abs(n_t[c]))
+        close("m_t[%d]" % c, dmt[c].cpu().numpy(), omt[c], colmax(m_p[c]) / dtlt + np.abs(m_t[c]))
+    close("tlt", dtl.cpu().numpy(), otlt, np.abs(otlt) + np.abs(otlt - tlt).max())
     close("rrate", drr.cpu().numpy(), rr, np.abs(rr).max() + 1e-300)
     close("sfcrrate", dsr.cpu().numpy(), sr, np.abs(sr).max() + 1e-300)
     close("VtPrc", dvt.cpu().numpy(), vt, np.abs(vt) + 1e-300)
+    return ont, vt
+
+
+@pytest.mark.parametrize("seed,grid,sed", [(1, (20, 8, 8), (1, 1, 1)), (2, (40, 12, 10), (1, 1, 1)),
+                                           (3, (20, 8, 8), (0, 0, 1)), (4, (20, 8, 8), (1, 0, 0))])
+def test_sedim_matches_oracle(capi, oracle_mod, seed, grid, sed):
+    desc = capi.bin_desc()
+    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields(desc, grid, seed)
+    print("case sedimentation, seed %d, grid %s, switches %s" % (seed, shape, sed), flush=True)
+    ont, vt = _compare(capi, oracle_mod, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, seed, sed)
     if sed[2]:
         assert vt.max() > 0.5 and np.abs(ont[2] - n_t[2]).max() > 0
+
+
+@pytest.mark.parametrize("dz,dtlt,nzp", [(20., 3.0, 30), (20., 7.0, 30), (20., 11.0, 30), (2., 5.0, 40)])
+def test_sedim_multi_level_falls(capi, oracle_mod, dz, dtlt, nzp):
+    """Precipitation falling several levels in one call: a uniform grid of
+    20 m levels with dtlt 3, 7 and 11 s, and 2 m levels with dtlt 5 s, where
+    the drops cross four levels and more (the landing window of the fall
+    kernel and its spill path beyond it).  All three categories sediment."""
+    desc = capi.bin_desc()
+    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields(desc, (nzp, 8, 8), 11)
+    dzt = np.full_like(dzt, 1. / dz)
+    print("case sedimentation multi-level, dz %.0f m, dtlt %.0f s, grid %s" % (dz, dtlt, shape), flush=True)
+    ont, vt = _compare(capi, oracle_mod, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, 11, (1, 1, 1), dtlt)
+    levels = vt.max() * dtlt / dz
+    print("  largest fall %.1f levels" % levels, flush=True)
+    assert levels > (4 if dz < 10 else 1)
+
+
+def test_fresh_context_reports_no_faults(capi):
+    """A context that has not stepped has no cell flags: sync after a
+    sedimentation-only call reports success and the flags read zero, whatever
+    memory the context's flag words were carved from."""
+    desc = capi.bin_desc()
+    for _ in range(3):
+        ctx = capi.Context(desc, capi.params(level=4), 12, 8, 8)
+        ctx.sync()
+        assert not ctx.cell_flags().any()
+        ctx.close()
 
 
 def test_sedim_refuses_level5(capi):

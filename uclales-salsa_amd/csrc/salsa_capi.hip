@@ -232,6 +232,11 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
     set_error("device allocation failed");
     return fail(SALSA_ERR_NOMEM);
   }
+  // no step has run yet: no cell flags, nothing for salsa_ctx_sync to report
+  if (hipMemset(c->flags, 0, nc) != hipSuccess || hipMemset(c->fault, 0, sizeof(int32_t)) != hipSuccess) {
+    set_error("flag initialisation failed");
+    return fail(SALSA_ERR_HIP);
+  }
   {
     const int rs = resolve_scratch(c->device, &c->scratch);
     if (rs) return fail(rs);
