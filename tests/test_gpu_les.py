@@ -21,19 +21,24 @@ def capi():
     return c
 
 
-@pytest.mark.parametrize("level,seed,grid", [(4, 1, (12, 10, 9)), (4, 2, (30, 16, 16)), (5, 3, (12, 10, 9)),
-                                             (5, 4, (30, 16, 16))])
-def test_les_update_matches_oracle(capi, oracle_mod, level, seed, grid):
+# Level 4 with SO4 (78 tracer rows) and level 5 with SO4 (106 rows) run the
+# diagnostics pass with the rows staged in LDS; level 5 with SO4 and DU (139
+# rows) runs it on the LES arrays directly.
+@pytest.mark.parametrize("level,seed,grid,dry", [(4, 1, (12, 10, 9), 1), (4, 2, (30, 16, 16), 1),
+                                                 (5, 3, (12, 10, 9), 2), (5, 4, (30, 16, 16), 2),
+                                                 (5, 5, (30, 16, 16), 1)])
+def test_les_update_matches_oracle(capi, oracle_mod, level, seed, grid, dry):
     import torch
-    desc = capi.bin_desc(species=("SO4", "DU") if level == 5 else ("SO4",))
+    species = ("SO4", "DU")[:dry]
+    desc = capi.bin_desc(species=species)
     shape, dzt, f, n_p, n_t, m_p, m_t = les_case(desc, level, grid, seed)
     nzp, nxp, nyp = shape
     n3 = nzp * nxp * nyp
-    print("case LES update, level %d, seed %d, grid %s" % (level, seed, shape), flush=True)
+    print("case LES update, level %d, seed %d, grid %s, species %s" % (level, seed, shape, species), flush=True)
     nrow_ct = desc.nbins + desc.ncld + desc.nprc
     ct = np.ones((nrow_ct, n3))
     # oracle
-    L = oracle_mod.make_layout(species=(0, 3) if level == 5 else (0,))
+    L = oracle_mod.make_layout(species=(0, 3)[:dry])
     on, ont, om, omt = ([a.copy() for a in x] for x in (n_p, n_t, m_p, m_t))
     oct_ = ct.copy()
     four = lambda x: (list(x) + [None] * 4)[:4]

@@ -83,8 +83,10 @@ struct salsa_ctx {
   // adjoint, per-cell J and dJ/dE, totals and the per-block tape
   double *adj_buf = nullptr;
   size_t adj_bytes = 0;
-  // salsa_sedim_device: precipitation number / mass changes and removal
+  // salsa_sedim_device: fall speeds of one slow category (aerosol or cloud)
   double *sed_work = nullptr;
+  // salsa_les_update_device: the cells the removals visit ([n3]) and their number
+  int32_t *les_list = nullptr;
   // staging for salsa_step_host
   double *stage = nullptr;
   size_t stage_bytes = 0;
@@ -164,6 +166,7 @@ static void release(salsa_ctx *c) {
   (void)hipFree(c->gbuf);
   (void)hipFree(c->adj_buf);
   (void)hipFree(c->sed_work);
+  (void)hipFree(c->les_list);
   if (c->cls_count_h) (void)hipHostFree(c->cls_count_h);
   if (c->tot_h) (void)hipHostFree(c->tot_h);
   if (c->fault_h) (void)hipHostFree(c->fault_h);
@@ -924,6 +927,13 @@ extern "C" int salsa_les_update_device(salsa_ctx *c, double dtlt, const double *
   }
   a.charge_time = charge_time;
   a.diag = diag;
+  if (c->n3 >= INT32_MAX) { set_error("grid too large for the LES update"); return SALSA_ERR_ARG; }
+  if (!c->les_list && hipMalloc(&c->les_list, ((size_t)c->n3 + 1) * sizeof(int32_t)) != hipSuccess) {
+    set_error("LES update work allocation failed");
+    return SALSA_ERR_NOMEM;
+  }
+  a.list = c->les_list;
+  a.count = c->les_list + c->n3;
   const int rc = launch_les_update(a, c->stream);
   if (rc) { std::snprintf(g_err, sizeof(g_err), "LES update launch failed (%d)", rc); return SALSA_ERR_HIP; }
   return SALSA_OK;
@@ -963,18 +973,13 @@ extern "C" int salsa_sedim_device(salsa_ctx *c, double dtlt, const double *dzt, 
     if (!(dzt[k] > 0.)) { set_error("dzt must be positive"); return SALSA_ERR_ARG; }
     a.dzt[k] = dzt[k];
   }
-  const int64_t ncol = (int64_t)c->nxp * c->nyp;
   const int nbslow = d.nbins > d.ncld ? d.nbins : d.ncld;
-  const size_t need = (size_t)((int64_t)d.nprc * c->n3 * (1 + a.ns) + ncol * d.nprc * a.ns +
-                               (int64_t)nbslow * c->n3) * sizeof(double);
+  const size_t need = (size_t)((int64_t)nbslow * c->n3) * sizeof(double);
   if (!c->sed_work && hipMalloc(&c->sed_work, need) != hipSuccess) {
     set_error("sedimentation work allocation failed");
     return SALSA_ERR_NOMEM;
   }
-  a.prn = c->sed_work;
-  a.prv = a.prn + (int64_t)d.nprc * c->n3;
-  a.rem = a.prv + (int64_t)d.nprc * a.ns * c->n3;
-  a.vslow = a.rem + ncol * d.nprc * a.ns;
+  a.vslow = c->sed_work;
   a.tk = tk; a.th = th; a.adn = adn; a.ustar = ustar;
   for (int k = 0; k < 3; ++k) { a.n_p[k] = n_p[k]; a.m_p[k] = m_p[k]; a.n_t[k] = n_t[k]; a.m_t[k] = m_t[k]; }
   a.tlt = tlt; a.rrate = rrate; a.sfcrrate = sfcrrate; a.vt_prc = vt_prc;

@@ -237,6 +237,7 @@ struct LesArgs {
   double *n_p[4], *n_t[4], *m_p[4], *m_t[4];
   double *charge_time;                // [nbins+ncld+nprc][n3] or null
   double *diag;                       // [6][n3]: rc, srp, snrp, rtot, ri, riri
+  int32_t *list, *count;              // work: [n3] cells the removals visit, [1] their number
 };
 
 // Arguments of the level-4 sedimentation passes (salsa_sedim_device):
@@ -253,7 +254,6 @@ struct SedArgs {
   const double *n_p[3], *m_p[3];
   double *n_t[3], *m_t[3];
   double *tlt, *rrate, *sfcrrate, *vt_prc;
-  double *prn, *prv, *rem;            // work: [nprc][n3], [nprc*ns][n3], [nxp*nyp][nprc*ns]
   double *vslow;                      // work: fall speeds of one slow category [nb][n3]
 };
 

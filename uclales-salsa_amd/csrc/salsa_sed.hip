@@ -13,15 +13,13 @@
 //                rate and, at the lowest level, the dry deposition velocity
 //                and the surface rate.  One launch per category, so the
 //                accumulation into rrate / tlt keeps the reference's order.
-//   k_sed_vt     DepositionFast's fall speed per (cell, precipitation bin)
-//                into VtPrc.
-//   k_sed_fall   the fall-distance redistribution (:1188-1253) per (column,
-//                bin, moved quantity), levels from the top down as in the
-//                reference, into the work arrays; deposited mass into the
-//                removal array.
-//   k_sed_apply  per cell: rain rate (bins in order), tendencies += change /
-//                dtlt, tlt (:862-868); the surface rate from the removal
-//                array at the lowest level.
+//   k_sed_prc    DepositionFast (:1073-1275) and its part of sedim_SALSA
+//                (:862-868), one workgroup per interior column and one lane
+//                per level: fall speeds into VtPrc, the fall-distance
+//                redistribution of every bin and moved quantity with each
+//                level's changes summed in the reference's top-down order,
+//                tendencies += change / dtlt, rain rate (bins in order), tlt
+//                and the surface rate from the deposited mass.
 // Terminal velocities: terminal_vel (mo_particle_external_properties.f90:
 // 43-100), liquid branches.  Contraction is off so that the products and
 // sums round as the oracle's.
@@ -37,6 +35,7 @@ namespace salsa {
 
 namespace {
 
+constexpr int kSedMaxZ = 256;                 // levels per column (SedArgs::dzt)
 constexpr double kPiFull = 3.14159265358979323846, kKb = 1.3807e-23, kG = 9.81, kCp = 1005., kAlvl = 2.5e6;
 
 __device__ double sd_terminal_vel(double diam, double rhop, double rhoa, double visc, double beta) {
@@ -70,11 +69,9 @@ __device__ __forceinline__ Air sd_air(double tk, double adn) {
 }
 
 // the bin's wet diameter, density, slip correction and fall speed
-__device__ double sd_vc(const SedArgs &a, double zpn, const double *mb, int64_t mstride, double adn, const Air &air,
-                        double *dwet_out, double *gg_out) {
-  double v = 0., ms = 0.;
-  for (int s = 0; s < a.ns; ++s) v = v + mb[s * mstride] / a.rholiq[s];
-  for (int s = 0; s < a.ns; ++s) ms = ms + mb[s * mstride];
+// from the bin's volume v = sum m_s / rho_s and mass ms = sum m_s
+__device__ __forceinline__ double sd_vc_sums(double zpn, double v, double ms, double adn, const Air &air,
+                                             double *dwet_out, double *gg_out) {
   const double dwet = zpn < 1.e-15 ? 0. : ::pow(v / zpn / kPi6, 1. / 3.);
   const double zdn = ms / zpn / (kPi6 * (dwet * dwet * dwet));
   const double Kn = 2. * air.lambda / dwet;
@@ -82,6 +79,14 @@ __device__ double sd_vc(const SedArgs &a, double zpn, const double *mb, int64_t 
   if (dwet_out) *dwet_out = dwet;
   if (gg_out) *gg_out = GG;
   return sd_terminal_vel(dwet, zdn, adn, air.avis, GG);
+}
+
+__device__ double sd_vc(const SedArgs &a, double zpn, const double *mb, int64_t mstride, double adn, const Air &air,
+                        double *dwet_out, double *gg_out) {
+  double v = 0., ms = 0.;
+  for (int s = 0; s < a.ns; ++s) v = v + mb[s * mstride] / a.rholiq[s];
+  for (int s = 0; s < a.ns; ++s) ms = ms + mb[s * mstride];
+  return sd_vc_sums(zpn, v, ms, adn, air, dwet_out, gg_out);
 }
 
 __device__ __forceinline__ bool sd_interior(const SedArgs &a, int i, int j) {
@@ -92,15 +97,8 @@ __global__ void __launch_bounds__(256) k_sed_zero(SedArgs a) {
   const int64_t n3 = a.n3, ncol = (int64_t)a.nxp * a.nyp, nb = a.nb[2];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n3; e += (int64_t)gridDim.x * blockDim.x) {
     a.rrate[e] = 0.;
-    for (int b = 0; b < nb; ++b) {
-      a.vt_prc[b * n3 + e] = 0.;
-      a.prn[b * n3 + e] = 0.;
-    }
-    for (int q = 0; q < nb * a.ns; ++q) a.prv[q * n3 + e] = 0.;
-    if (e < ncol) {
-      a.sfcrrate[e] = 0.;
-      for (int q = 0; q < nb * a.ns; ++q) a.rem[e * nb * a.ns + q] = 0.;
-    }
+    for (int b = 0; b < nb; ++b) a.vt_prc[b * n3 + e] = 0.;
+    if (e < ncol) a.sfcrrate[e] = 0.;
   }
 }
 
@@ -184,143 +182,170 @@ __global__ void __launch_bounds__(128) k_sed_slow(SedArgs a, int cat) {
   if (k == 1) a.sfcrrate[col] = a.sfcrrate[col] - sw * adn * kAlvl;
 }
 
-// precipitation fall speed per (cell, bin)
-__global__ void __launch_bounds__(128) k_sed_vt(SedArgs a) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (x >= a.n3) return;
-  const int k = (int)(x % a.nzp);
-  const int64_t col = x / a.nzp;
-  const int i = (int)(col % a.nxp), j = (int)(col / a.nxp);
-  if (k < 1 || k > a.nzp - 2 || !sd_interior(a, i, j)) return;
-  const int64_t n3 = a.n3;
-  const double zpn = a.n_p[2][(int64_t)b * n3 + x];
-  if (zpn < kPrlim) return;
-  const Air air = sd_air(a.tk[x], a.adn[x]);
-  a.vt_prc[(int64_t)b * n3 + x] =
-      sd_vc(a, zpn, a.m_p[2] + (int64_t)b * n3 + x, (int64_t)a.nb[2] * n3, a.adn[x], air, nullptr, nullptr);
-}
+// DepositionFast for one interior column: one workgroup per column, one lane
+// per level (blockDim = nzp rounded up to 64).  Per precipitation bin, in
+// order:
+//   1. every source level k (1 <= k <= nzp-2 with N >= prlim) evaluates its
+//      fall speed (VtPrc) and its fall-distance decision (:1188-1225): the
+//      landing level kf, one or two receiving levels and their shares;
+//   2. per moved quantity (the number, then the mass species), every level L
+//      sums what lands on it from the sources above it, in decreasing source
+//      level as the reference's top-down walk adds them, then its own
+//      departure (-q) and, when the drops stay within the level, its own
+//      share; the change goes straight into the tendency (/ dtlt).  Only
+//      sources within the column's largest fall distance of L are scanned.
+// Deposited mass (sources whose walk reaches the surface) is summed per
+// species and bin by lane 0 over those sources, top down.  Rain rate (bins in
+// order), tlt and the surface rate as in sedim_SALSA (:862-868).
+struct SedPrcLds {
+  double dz[kSedMaxZ], rdz[kSedMaxZ], adn[kSedMaxZ];
+  double vc[2][kSedMaxZ];                 // source fall speed, by bin parity
+  int dec[2][kSedMaxZ];                   // bit 0 source, 1 two levels, 2 deposits; (d + 1) << 8
+  double c1[2][kSedMaxZ], c0[2][kSedMaxZ], q[2][kSedMaxZ];   // by quantity parity
+  int wmax[2][kSedMaxZ / 64];             // largest fall distance per wavefront, by bin parity
+  double rem[kMaxSpec * kMaxBin];         // deposited mass per (species, bin)
+};
 
-// fall-distance redistribution per (interior column, bin), top level down;
-// blockIdx.y selects the quantity moved: 0 the number, 1 + s mass species s
-// (each thread repeats the bin's fall-distance decisions)
-__global__ void __launch_bounds__(64) k_sed_fall(SedArgs a) {
-  const int nx = a.nxp - 4, ny = a.nyp - 4, nb = a.nb[2], ns = a.ns;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int qy = blockIdx.y;                     // 0: number, 1 + s: species s
-  if (t >= nx * ny * nb) return;
-  const int b = t % nb, c = t / nb;
-  const int i = 2 + c % nx, j = 2 + c / nx;
-  const int64_t col = (int64_t)i + (int64_t)a.nxp * j, base = col * a.nzp, n3 = a.n3;
-  const double *__restrict__ Nb = a.n_p[2] + (int64_t)b * n3 + base;
-  const double *__restrict__ Q = qy == 0 ? Nb : a.m_p[2] + ((int64_t)(qy - 1) * nb + b) * n3 + base;
-  const double *__restrict__ vt = a.vt_prc + (int64_t)b * n3 + base;
-  const double *__restrict__ adn = a.adn + base;
-  double *__restrict__ out = qy == 0 ? a.prn + (int64_t)b * n3 + base : a.prv + ((int64_t)(qy - 1) * nb + b) * n3 + base;
-  double *rem = qy > 0 ? a.rem + col * nb * ns + (qy - 1) * nb + b : nullptr;   // deposited mass (species only)
+template <int MS>
+__global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
+  __shared__ SedPrcLds S;
+  const int nx = a.nxp - 4;
+  const int i = 2 + (int)blockIdx.x % nx, j = 2 + (int)blockIdx.x / nx;
+  const int64_t col = (int64_t)i + (int64_t)a.nxp * j, n3 = a.n3;
+  const int k = threadIdx.x, nzp = a.nzp, ktop = nzp - 2, nw = blockDim.x / 64;
+  const int nb = a.nb[2], ns = a.ns, iw = a.iwa;
+  const bool lev = k < nzp, src = k >= 1 && k <= ktop;
+  const int64_t x = col * nzp + k;
   const double dtlt = a.dtlt;
-  // The changes of the four levels at and below the current source level are
-  // kept in registers (w[0] = level k, ..., w[3] = level k-3): a level
-  // receives only from sources at or above it, so it is final once its own
-  // source is done and is written once; landings further down go to memory,
-  // before their level enters the window.  The additions happen in the
-  // reference's order (sources from the top down).
-  const int ktop = a.nzp - 2;
-  double w[4];
+  // the bin's rows at this level, quantity u: 0 the number, 1 + s species s
+  // (the next bin's are loaded while the current one is worked on)
+  double qv[MS + 1], tv[MS + 1], qn[MS + 1], tn[MS + 1];
+  auto load = [&](int b, double *q, double *t) {
 #pragma unroll
-  for (int u = 0; u < 4; ++u) w[u] = ktop - u >= 0 ? out[ktop - u] : 0.;
-  // the next level's inputs are loaded one iteration ahead
-  double zpn_n = Nb[ktop], q_n = Q[ktop], vc_n = vt[ktop], adn_n = adn[ktop];
-  for (int k = ktop; k >= 1; --k) {
-    const double zpn = zpn_n, q_k = q_n, vc_k = vc_n, adn_k = adn_n;
-    if (k > 1) {
-      zpn_n = Nb[k - 1];
-      q_n = Q[k - 1];
-      vc_n = vt[k - 1];
-      adn_n = adn[k - 1];
+    for (int u = 0; u <= MS; ++u) {
+      const bool use = lev && b < nb && u <= ns;
+      const int64_t r = u == 0 ? (int64_t)b * n3 : ((int64_t)(u - 1) * nb + b) * n3;
+      q[u] = use ? (u == 0 ? a.n_p[2] : a.m_p[2])[r + x] : 0.;
+      t[u] = use ? (u == 0 ? a.n_t[2] : a.m_t[2])[r + x] : 0.;
     }
-    if (!(zpn < kPrlim)) {
-      const double q = q_k;
-      const double vc = vc_k;
+  };
+  load(0, qv, tv);
+  Air air = {0., 0.};
+  double rr = 0., sw = 0.;
+  if (lev) {
+    S.dz[k] = a.dzt[k];
+    S.rdz[k] = 1. / a.dzt[k];
+    S.adn[k] = a.adn[x];
+  }
+  if (src) {
+    air = sd_air(a.tk[x], a.adn[x]);
+    rr = a.rrate[x];
+  }
+  __syncthreads();
+  for (int b = 0; b < nb; ++b) {
+    const int pb = b & 1;
+    load(b + 1, qn, tn);
+    const double zpn = qv[0];
+    const bool on = src && !(zpn < kPrlim);
+    double vc = 0., pre = 0., dA = 0., dB = 0.;
+    int d = -1;
+    bool two = false, dep = false;
+    if (on) {
+      double v = 0., ms = 0.;
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (s < ns) v = v + qv[1 + s] / a.rholiq[s];
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (s < ns) ms = ms + qv[1 + s];
+      vc = sd_vc_sums(zpn, v, ms, S.adn[k], air, nullptr, nullptr);
+      a.vt_prc[(int64_t)b * n3 + x] = vc;
       const double fdmax = dtlt * vc;
       double fd = 0.;
       int fi = 0;
-      bool dep = false;
-      while (fd < fdmax) {                    // bounded: ends at the lowest level
-        fd = fd + (1. / a.dzt[k - fi]);
+      while (fd < fdmax) {                      // bounded: ends at the lowest level
+        fd = fd + S.rdz[k - fi];
         fi = fi + 1;
         if (k - fi <= 0) { dep = true; break; }
       }
       fi = fi - 1;
-      const int kf = k - fi;
-      fd = fd - (1. / a.dzt[kf]);
-      const double fdos = fmin(fmax(fdmax - fd, 0.), 1. / a.dzt[kf]);
-      w[0] = w[0] - q;
-      if (dep && qy > 0) *rem = *rem + q * adn_k * vc;
-      const int d = k - kf;                   // kf = k - d, landing at kf - 1 and kf
-      double c1, c0;                          // to kf - 1, to kf
-      bool two;
-      if (fdos * a.dzt[kf] > 0.5) {
-        c1 = q * a.dzt[kf - 1] / a.dzt[k];
-        c0 = 0.;
-        two = false;
-      } else {
-        c1 = (fdos * a.dzt[kf]) * q * a.dzt[kf - 1] / a.dzt[kf];
-        c0 = (1. - fdos * a.dzt[kf]) * q;
-        two = true;
+      const int kf = k - fi;                    // kf - 1 >= 0; kf <= k + 1 <= nzp - 1
+      fd = fd - S.rdz[kf];
+      const double fdos = fmin(fmax(fdmax - fd, 0.), S.rdz[kf]);
+      pre = fdos * S.dz[kf];
+      two = !(pre > 0.5);
+      dA = S.dz[kf - 1];
+      dB = two ? S.dz[kf] : S.dz[k];
+      d = k - kf;                               // >= -1
+      S.vc[pb][k] = vc;
+    }
+    S.dec[pb][k] = on ? (1 | (two ? 2 : 0) | (dep ? 4 : 0) | ((d + 1) << 8)) : 0;
+    int wm = d;
+    for (int o = 32; o > 0; o >>= 1) wm = max(wm, __shfl_xor(wm, o));
+    if ((k & 63) == 0) S.wmax[pb][k >> 6] = wm;
+    int dmax = -1;
+#pragma unroll
+    for (int qy = 0; qy <= MS; ++qy) {
+      if (qy > ns) break;
+      const int p = (b * (ns + 1) + qy) & 1;    // parity of the c1 / c0 / q buffers
+      const double q = qv[qy];
+      double c1 = 0., c0 = 0.;
+      if (on) {
+        // the reference's expressions, left to right (:1229-1246)
+        if (two) {
+          c1 = pre * q * dA / dB;
+          c0 = (1. - pre) * q;
+        } else {
+          c1 = q * dA / dB;
+        }
+        S.c1[p][k] = c1;
+        S.c0[p][k] = c0;
+        if (dep) S.q[p][k] = q;
       }
-      // level kf - 1 first, then kf, as the reference adds them
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (u == d + 1) w[u] = w[u] + c1;
-      if (d + 1 > 3) out[kf - 1] = out[kf - 1] + c1;
-      if (two) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (u == d) w[u] = w[u] + c0;
-        if (d > 3) out[kf] = out[kf] + c0;
+      __syncthreads();
+      if (qy == 0)
+        for (int w = 0; w < nw; ++w) dmax = max(dmax, S.wmax[pb][w]);
+      if (lev) {
+        double out = 0.;
+        if (k <= ktop) {
+          for (int s = min(ktop, k + dmax + 1); s > k; --s) {
+            const int ds = S.dec[pb][s];
+            if (!(ds & 1)) continue;
+            const int kf = s - ((ds >> 8) - 1);
+            if (kf - 1 == k) out = out + S.c1[p][s];
+            if ((ds & 2) && kf == k) out = out + S.c0[p][s];
+          }
+          if (on) {
+            out = out - q;
+            if (d == -1) out = out + c1;        // drops that do not leave the level
+            if (two && d == 0) out = out + c0;
+          }
+        }
+        const int64_t r = qy == 0 ? (int64_t)b * n3 : ((int64_t)(qy - 1) * nb + b) * n3;
+        (qy == 0 ? a.n_t[2] : a.m_t[2])[r + x] = tv[qy] + out / dtlt;
+        if (qy - 1 == iw) {
+          sw = sw + out;
+          if (on) rr = rr + q * S.adn[k] * vc * kAlvl;
+        }
+      }
+      if (qy > 0 && k == 0) {                   // lane 0 (level 0 is never a source)
+        double r = 0.;
+        for (int s = min(ktop, dmax + 1); s >= 1; --s)
+          if ((S.dec[pb][s] & 5) == 5) r = r + S.q[p][s] * S.adn[s] * S.vc[pb][s];
+        S.rem[(qy - 1) * nb + b] = r;
       }
     }
-    out[k] = w[0];                            // level k is final
-    w[0] = w[1];
-    w[1] = w[2];
-    w[2] = w[3];
-    w[3] = k - 4 >= 0 ? out[k - 4] : 0.;
-  }
-  out[0] = w[0];                              // the level below the surface
-}
-
-// per cell: rain rate, tendencies and tlt of the precipitation; surface rate
-__global__ void __launch_bounds__(256) k_sed_apply(SedArgs a) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n3) return;
-  const int k = (int)(x % a.nzp);
-  const int64_t col = x / a.nzp;
-  const int i = (int)(col % a.nxp), j = (int)(col / a.nxp);
-  if (!sd_interior(a, i, j)) return;
-  const int nb = a.nb[2], ns = a.ns, iw = a.iwa;
-  const int64_t n3 = a.n3;
-  if (k >= 1 && k <= a.nzp - 2) {
-    double r = a.rrate[x];
-    for (int b = 0; b < nb; ++b) {
-      if (a.n_p[2][(int64_t)b * n3 + x] < kPrlim) continue;
-      r = r + a.m_p[2][((int64_t)iw * nb + b) * n3 + x] * a.adn[x] * a.vt_prc[(int64_t)b * n3 + x] * kAlvl;
+#pragma unroll
+    for (int u = 0; u <= MS; ++u) {
+      qv[u] = qn[u];
+      tv[u] = tn[u];
     }
-    a.rrate[x] = r;
   }
-  for (int b = 0; b < nb; ++b)
-    a.n_t[2][(int64_t)b * n3 + x] = a.n_t[2][(int64_t)b * n3 + x] + a.prn[(int64_t)b * n3 + x] / a.dtlt;
-  for (int q = 0; q < nb * ns; ++q)
-    a.m_t[2][(int64_t)q * n3 + x] = a.m_t[2][(int64_t)q * n3 + x] + a.prv[(int64_t)q * n3 + x] / a.dtlt;
-  if (k <= a.nzp - 2) {
-    double sw = 0.;
-    for (int b = 0; b < nb; ++b) sw = sw + a.prv[((int64_t)iw * nb + b) * n3 + x];
-    a.tlt[x] = a.tlt[x] - (sw / a.dtlt) * (kAlvl / kCp) * a.th[x] / a.tk[x];
-  }
+  if (src) a.rrate[x] = rr;
+  if (lev && k <= ktop) a.tlt[x] = a.tlt[x] - (sw / dtlt) * (kAlvl / kCp) * a.th[x] / a.tk[x];
   if (k == 0) {
     double s2 = 0.;
-    const double *rem = a.rem + col * nb * ns;
-    for (int q = ns * nb - nb - 1; q < ns * nb; ++q) s2 = s2 + rem[q];
+    for (int q = ns * nb - nb - 1; q < ns * nb; ++q) s2 = s2 + S.rem[q];
     a.sfcrrate[col] = a.sfcrrate[col] + s2 * kAlvl;
   }
 }
@@ -336,10 +361,15 @@ int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, h
     hipLaunchKernelGGL(k_sed_slow, dim3(g128), dim3(128), 0, s, a, cat);
   }
   if (sed_precp) {
-    hipLaunchKernelGGL(k_sed_vt, dim3(g128, a.nb[2]), dim3(128), 0, s, a);
-    const int nthr = (a.nxp - 4) * (a.nyp - 4) * a.nb[2];
-    hipLaunchKernelGGL(k_sed_fall, dim3((nthr + 63) / 64, 1 + a.ns), dim3(64), 0, s, a);
-    hipLaunchKernelGGL(k_sed_apply, dim3(g256), dim3(256), 0, s, a);
+    // one workgroup per interior column, one lane per level
+    const unsigned ncol = (unsigned)((a.nxp - 4) * (a.nyp - 4)), nthr = (unsigned)((a.nzp + 63) / 64 * 64);
+    if (a.nzp > kSedMaxZ || a.nb[2] * a.ns > kMaxSpec * kMaxBin) return (int)hipErrorInvalidValue;
+    if (a.ns <= 3)
+      hipLaunchKernelGGL(k_sed_prc<3>, dim3(ncol), dim3(nthr), 0, s, a);
+    else if (a.ns <= 5)
+      hipLaunchKernelGGL(k_sed_prc<5>, dim3(ncol), dim3(nthr), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_sed_prc<kMaxSpec>, dim3(ncol), dim3(nthr), 0, s, a);
   }
   return (int)hipGetLastError();
 }
