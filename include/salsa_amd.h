@@ -352,25 +352,32 @@ int salsa_les_update_device(salsa_ctx *ctx, double dtlt, const double *dzt, cons
                             double *const m_t[4], double *charge_time, double *diag);
 
 /*
- * Sedimentation of the liquid categories at level 4 on device (sedim_SALSA,
- * mcrp.f90:745-921): DepositionSlow (:926-1069) for aerosols (sed_aero) and
- * cloud droplets (sed_cloud), with the dry deposition velocity at the lowest
- * level; DepositionFast (:1073-1275) for precipitation (sed_precp).
+ * Sedimentation on device (sedim_SALSA, mcrp.f90:745-921): DepositionSlow
+ * (:926-1069) for aerosols (sed_aero) and cloud droplets (sed_cloud), with the
+ * dry deposition velocity at the lowest level; DepositionFast (:1073-1275) for
+ * precipitation (sed_precp) and, at level 5, for ice (sed_ice: flag 4, the
+ * fall speed of non-spherical ice from getDiameter, getShapeCoefficients and
+ * terminal_vel's ice branch, mo_ice_shape.f90, mo_particle_external_
+ * properties.f90:91-158; water and rime in the rate, latent heat alvi).
  *   dzt        : HOST array of nzp inverse level thicknesses
  *   tk, th, adn: temperature, potential temperature, air density (device)
  *   ustar      : friction velocity, nxp*nyp (device)
- *   n_p, m_p   : fields (read); n_t, m_t, tlt : tendencies (added to)
+ *   n_p, m_p   : fields (read; ice at index 3 at level 5); n_t, m_t, tlt :
+ *                tendencies (added to)
  *   rrate      : rain-rate diagnostic [n3], W m-2 (overwritten)
  *   sfcrrate   : surface rain rate [nxp*nyp] (overwritten)
  *   vt_prc     : precipitation fall speed per bin [nprc][n3] (overwritten)
- * Needs hz = 1, hxy = 2, 4 <= nzp <= 256.  Ice sedimentation (level 5) is
- * not carried.  Work is enqueued on the context stream.
+ *   irate, sfcirate, vt_ice : the ice rate [n3], surface ice rate [nxp*nyp]
+ *                and ice fall speed [nice][n3] (level 5, overwritten; NULL
+ *                at level 4)
+ * Needs hz = 1, hxy = 2, 4 <= nzp <= 256, nice <= 16.  Work is enqueued on the
+ * context stream.
  */
 int salsa_sedim_device(salsa_ctx *ctx, double dtlt, const double *dzt, const double *tk, const double *th,
                        const double *adn, const double *ustar, const double *const n_p[4],
                        const double *const m_p[4], double *const n_t[4], double *const m_t[4], double *tlt,
-                       double *rrate, double *sfcrrate, double *vt_prc, int32_t sed_aero, int32_t sed_cloud,
-                       int32_t sed_precp);
+                       double *rrate, double *sfcrrate, double *vt_prc, double *irate, double *sfcirate,
+                       double *vt_ice, int32_t sed_aero, int32_t sed_cloud, int32_t sed_precp, int32_t sed_ice);
 
 int salsa_set_launch_shape(salsa_ctx *ctx, int32_t cls, int32_t lanes_per_cell);
 int salsa_get_launch_shape(salsa_ctx *ctx, int32_t cls);

@@ -111,6 +111,13 @@ def lib():
         L.so_sedim.restype = C.c_int
         L.so_terminal_vel.argtypes = [C.c_double] * 5
         L.so_terminal_vel.restype = C.c_double
+        L.so_sedim5.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.c_double,
+                                dp, dp, dp, dp, dp, P4, P4, P4, P4, dp, dp, dp, dp, dp, dp, dp,
+                                C.c_int, C.c_int, C.c_int, C.c_int]
+        L.so_sedim5.restype = C.c_int
+        L.so_sed_ice_vc.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_double, C.c_double, C.c_double, dp,
+                                    dp]
+        L.so_sed_ice_vc.restype = None
         _lib = L
     return _lib
 
@@ -265,6 +272,42 @@ def sedim(L, shape, dtlt, dzt, tk, th, adn, ustar, n_p, m_p, n_t, m_t, tlt, sed=
     if rc != 0:
         raise ValueError("so_sedim failed: %d" % rc)
     return rr, sr, vt
+
+
+def sedim5(L, P, shape, dtlt, dzt, tk, th, adn, ustar, n_p, m_p, n_t, m_t, tlt, sed=(1, 1, 1, 1)):
+    """so_sedim5 on LES arrays (level 5: the three liquid categories, then
+    ice).  n_t, m_t (four categories), tlt are added to in place.  Returns
+    (rrate[n3], sfcrrate[nxp*nyp], irate[n3], sfcirate[nxp*nyp],
+    vt_prc[nprc, n3], vt_ice[nice, n3])."""
+    nzp, nxp, nyp = shape
+    n3 = nzp * nxp * nyp
+    P4 = C.c_void_p * 4
+
+    def ptrs(lst):
+        for a in lst:
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        return P4(*[a.ctypes.data for a in lst])
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (dzt, tk, th, adn, ustar)]
+    assert tlt.dtype == np.float64 and tlt.flags["C_CONTIGUOUS"]
+    rr, sr, ir, si = np.zeros(n3), np.zeros(nxp * nyp), np.zeros(n3), np.zeros(nxp * nyp)
+    vt, vi = np.zeros((L.nprc, n3)), np.zeros((L.nice, n3))
+    rc = lib().so_sedim5(C.byref(L), C.byref(P), nzp, nxp, nyp, dtlt, *(_dp(a) for a in arrs), ptrs(n_p[:4]),
+                         ptrs(m_p[:4]), ptrs(n_t[:4]), ptrs(m_t[:4]), _dp(tlt), _dp(rr), _dp(sr), _dp(ir), _dp(si),
+                         _dp(vt), _dp(vi), *sed)
+    if rc != 0:
+        raise ValueError("so_sedim5 failed: %d" % rc)
+    return rr, sr, ir, si, vt, vi
+
+
+def sed_ice_vc(L, P, tk, adn, numc, masses):
+    """DepositionFast's fall-speed chain of one ice bin (flag 4): returns
+    (dwet, dnsp, alpha, beta, gamma, sigma, zdn, GG, vc).  masses: the bin's
+    LES mass mixing ratios of every species, rime last."""
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    assert m.size == L.nspec
+    out = np.zeros(9)
+    lib().so_sed_ice_vc(C.byref(L), C.byref(P), tk, adn, numc, _dp(m), _dp(out))
+    return out
 
 
 def terminal_vel(diam, rhop, rhoa, visc, beta):

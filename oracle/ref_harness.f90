@@ -9,7 +9,13 @@
 ! condensation (the enabled branch of salsa(), mo_salsa.f90:92-107).
 !
 ! Modes: 1 = full step(s); 2 = dump the coagulation kernel matrices built
-! by update_coagulation_kernels for the input state.  With sw(14) = 1 the
+! by update_coagulation_kernels for the input state; 3 = the per-bin ice
+! fall-speed chain of DepositionFast (flag 4, mcrp.f90:1146-1176): for every
+! ice bin of the input (T in slot 1, air density in slot 2, numbers and
+! LES mass mixing ratios in the bin arrays) the reference's calcDiamLES
+! (spherical and maximum dimension), getShapeCoefficients and terminal_vel,
+! with the air viscosity, mean free path and Cunningham factor formed as
+! DepositionFast forms them.  With sw(14) = 1 the
 ! steps of mode 1 use lscoag%mode = 2: the first step of each cell is an
 ! update step (lcgupdt = .TRUE.), the later ones reuse the kernels left in
 ! the zcc* arrays by it, which is what fetch_coag provides for a single cell
@@ -44,6 +50,9 @@ PROGRAM ref_harness
   USE mo_salsa_coagulation_kernels, ONLY : update_coagulation_kernels
   USE classBinLayout, ONLY : BinLayout
   USE classProcessRates, ONLY : Rates
+  USE mo_particle_external_properties, ONLY : calcDiamLES, terminal_vel
+  USE mo_ice_shape, ONLY : t_shape_coeffs, getShapeCoefficients
+  USE mo_submctl, ONLY : spec, pi6
   IMPLICIT NONE
 
   INTEGER, PARAMETER :: MAGIC = 370010
@@ -62,6 +71,11 @@ PROGRAM ref_harness
   REAL :: tcomp
   CHARACTER(len=3), PARAMETER :: names(7) = ['SO4','OC ','BC ','DU ','SS ','NO ','NH ']
   LOGICAL :: lcharge
+  ! mode 3 (DepositionFast's constants: defs.f90:45-46, mcrp.f90:1095-1099)
+  REAL, PARAMETER :: pi_defs = 3.14159265358979323846264338327, kb_defs = 1.3807e-23
+  REAL :: tk1, adn1, avis, va, lambda, zpn1, pm(9), dwet1, dnsp1, zdn1, kn1, gg1, vc1
+  TYPE(t_shape_coeffs) :: shp
+  INTEGER :: nsi
 
   CALL get_command_argument(1, fin)
   CALL get_command_argument(2, fout)
@@ -118,6 +132,8 @@ PROGRAM ref_harness
   W = 0.
   IF (mode == 1) THEN
      ALLOCATE(cellout(1 + nt + 9*nt + 20*10))
+  ELSE IF (mode == 3) THEN
+     ALLOCATE(cellout(10*nice))
   ELSE
      ALLOCATE(cellout(nt*nt), kmat(nt,nt))
   END IF
@@ -179,6 +195,27 @@ PROGRAM ref_harness
         DO k = 1, 20
            cellout(1 + nt + 9*nt + (k-1)*10 + 1:1 + nt + 9*nt + (k-1)*10 + 9) = Rates(k)%volc(1:9)
            cellout(1 + nt + 9*nt + (k-1)*10 + 10) = Rates(k)%numc
+        END DO
+     ELSE IF (mode == 3) THEN
+        tk1 = cellin(1); adn1 = cellin(2)
+        avis = 1.8325e-5*(416.16/(tk1+120.0))*(tk1/296.16)**1.5
+        va = sqrt(8.*kb_defs*tk1/(pi_defs*4.8096e-26))
+        lambda = 2.*avis/(adn1*va)
+        nsi = spec%getNSpec(type='total')
+        cellout = 0.
+        DO i = 1, nice
+           nb = iice + i - 1
+           zpn1 = cellin(5 + nb)
+           pm(1:nsi) = cellin(5 + nt + (nb-1)*9 + 1:5 + nt + (nb-1)*9 + nsi)
+           dwet1 = calcDiamLES(nsi, zpn1, pm(1:nsi), 4, sph=.TRUE.)
+           dnsp1 = calcDiamLES(nsi, zpn1, pm(1:nsi), 4, sph=.FALSE.)
+           CALL getShapeCoefficients(shp, SUM(pm(1:nsi-1)), pm(nsi), zpn1)
+           zdn1 = SUM(pm(1:nsi))/zpn1/(pi6*dwet1**3)
+           kn1 = 2.*lambda/dwet1
+           gg1 = 1. + kn1*(1.249 + 0.42*exp(-0.87/kn1))
+           vc1 = terminal_vel(dwet1, zdn1, adn1, avis, gg1, 4, shp, dnsp1)
+           cellout((i-1)*10 + 1:(i-1)*10 + 10) = [dwet1, dnsp1, shp%alpha, shp%beta, shp%gamma, shp%sigma, &
+                                                   zdn1, gg1, vc1, avis]
         END DO
      ELSE
         DO nb = 1, nt

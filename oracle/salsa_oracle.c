@@ -289,6 +289,30 @@ static so_shape shape_coeffs(const so_params *P, double mpri, double mrim, doubl
   return s;
 }
 
+/* getDiameter, mo_ice_shape.f90:47-86, with getBulkRho, D_nonsphericalIce,
+   D_nonsphericalRimed and D_spherical (:105-143): the maximum dimension of a
+   non-spherical ice particle of pristine mass mpri and rime mass mrim */
+static double ice_diameter(const so_params *P, double mpri, double mrim, double numc) {
+  if (numc < 1.e-6) return 1.e-9;
+  const double Mtot = (mpri + mrim) / numc;
+  const double Fr = fmax(fmin(mrim / (Mtot * numc), 0.9999), 0.0001);
+  const double rho_b = Fr * rhori + (1. - Fr) * rhoic;
+  const double Dth = pow(pi6 * rhoic / P->ice_alpha, 1. / (P->ice_beta - 3.));
+  const double Dgr = pow(P->ice_alpha / (pi6 * rho_b), 1. / (3. - P->ice_beta));
+  const double Dcr = pow((1. / (1. - Fr)) * P->ice_alpha / (pi6 * rho_b), 1. / (3. - P->ice_beta));
+  const double Mth = rhoic * pi6 * cube(Dth);
+  const double Mgr = rho_b * pi6 * cube(Dgr);
+  const double Mcr = rho_b * pi6 * cube(Dcr);
+  if (Mtot < Mth) return pow(Mtot / (pi6 * rhoic), 1. / 3.);                  /* small spherical */
+  if (Mtot >= Mth && Mtot < Mgr) return pow(Mtot / P->ice_alpha, 1. / P->ice_beta);   /* dense non-spherical */
+  if (Mtot >= Mgr && Mtot < Mcr) return pow(Mtot / (pi6 * rho_b), 1. / 3.);   /* graupel */
+  if (Mtot >= Mcr) {                                                         /* partially rimed */
+    const double hlp = 1. / (1. - Fr);
+    return pow(Mtot / (hlp * P->ice_alpha), 1. / P->ice_beta);
+  }
+  return 1.e-9;                                                              /* NaN masses */
+}
+
 /* terminal_vel, liquid branch (flag 1..3), mo_particle_external_properties.f90:61-90 */
 double so_terminal_vel_liquid(double diam, double rhop, double rhoa, double visc, double beta) {
   double v;
@@ -2212,6 +2236,134 @@ static double sd_vc(const so_layout *L, int ns, double numc, const double *m, lo
   return sd_terminal_vel(dwet, zdn, adn, avis, GG);
 }
 
+/* DepositionFast's fall speed of one ice bin (flag 4, mcrp.f90:1146-1176):
+   calcDiamLES with sph = .TRUE. (volume over the ice densities) and
+   .FALSE. (getDiameter of the pristine and rime masses), getShapeCoefficients,
+   the bulk density over the spherical diameter, and terminal_vel's ice
+   branch (Khvorostyanov and Curry 2002; the Cunningham factor is formed but
+   not used there).  The bin's ns masses (rime last) at m[s*mstride].  out,
+   if given: dwet, dnsp, alpha, beta, gamma, sigma, zdn, GG. */
+static double sd_vc_ice(const so_layout *L, const so_params *P, int ns, double numc, const double *m, long mstride,
+                        double adn, double avis, double lambda, double *out) {
+  double v = 0., mpri = 0., ms = 0.;
+  for (int s = 0; s < ns; ++s) v = v + m[s * mstride] / L->rhoice[s];
+  for (int s = 0; s < ns - 1; ++s) mpri = mpri + m[s * mstride];
+  const double mrim = m[(long)(ns - 1) * mstride];
+  const double dwet = numc < 1.e-15 ? 0. : pow(v / numc / pi6, 1. / 3.);
+  const double dnsp = numc < 1.e-15 ? 0. : ice_diameter(P, mpri, mrim, numc);
+  const so_shape sh = shape_coeffs(P, mpri, mrim, numc);
+  for (int s = 0; s < ns; ++s) ms = ms + m[s * mstride];
+  const double zdn = ms / numc / (pi6 * (dwet * dwet * dwet));
+  const double Kn = 2. * lambda / dwet;
+  const double GG = 1. + Kn * (1.249 + 0.42 * exp(-0.87 / Kn));
+  if (out) {
+    out[0] = dwet; out[1] = dnsp; out[2] = sh.alpha; out[3] = sh.beta; out[4] = sh.gamma; out[5] = sh.sigma;
+    out[6] = zdn; out[7] = GG;
+  }
+  return terminal_vel_ice(dwet, zdn, adn, avis, sh, dnsp);
+}
+
+/* the ice fall-speed chain of one bin for a level of temperature tk and
+   air density adn (masses: ns values, rime last); out[9]: the eight values
+   of sd_vc_ice, then the fall speed */
+void so_sed_ice_vc(const so_layout *L, const so_params *P, double tk, double adn, double numc, const double *m,
+                   double *out) {
+  double avis, lambda;
+  sd_air(tk, adn, &avis, &lambda);
+  out[8] = sd_vc_ice(L, P, L->nspec, numc, m, 1, adn, avis, lambda, out);
+}
+
+/* DepositionFast (mcrp.f90:1073-1275) for one category on every interior
+   column and its part of sedim_SALSA: tendencies += change / dtlt (:862-863,
+   :884-885), tlt (:871-878, :897-904).  ice = 0: precipitation (flag 3,
+   water the last of ns species, alvl); ice = 1: ice (flag 4, ns species
+   with water and rime last, their sum in the rate, alvi). */
+static int sd_fast(const so_layout *L, const so_params *P, int ice, int nzp, int nxp, int nyp, double dtlt,
+                   const double *dzt, const double *tk, const double *th, const double *adn, int nb, int ns,
+                   const double *N, const double *M, double *n_t, double *m_t, double *tlt, double *rate,
+                   double *srate, double *vt) {
+  const long n3 = (long)nzp * nxp * nyp;
+  const double alv = ice ? 2.834e6 : sd_alvl;
+  double prn[SO_MAXBIN][256], prv[SO_MAXBIN * SO_MAXSPEC][256];
+  double rem[SO_MAXBIN * SO_MAXSPEC];
+  if (nzp > 256 || nb * ns > SO_MAXBIN * SO_MAXSPEC) return -1;
+#define IX(k, i, j) ((long)(k) + (long)nzp * ((long)(i) + (long)nxp * (long)(j)))
+  for (int j = 2; j <= nyp - 3; ++j)
+    for (int i = 2; i <= nxp - 3; ++i) {
+      for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < nzp; ++k) prn[b][k] = 0.;
+      for (int q = 0; q < nb * ns; ++q) {
+        for (int k = 0; k < nzp; ++k) prv[q][k] = 0.;
+        rem[q] = 0.;
+      }
+      for (int k = nzp - 2; k >= 1; --k) {
+        const long x = IX(k, i, j);
+        double avis, lambda;
+        sd_air(tk[x], adn[x], &avis, &lambda);
+        for (int b = 0; b < nb; ++b) {
+          const double zpn = N[(long)b * n3 + x];
+          if (zpn < prlim_c) continue;
+          const double *mb = M + (long)b * n3 + x;
+          const double vc = ice ? sd_vc_ice(L, P, ns, zpn, mb, (long)nb * n3, adn[x], avis, lambda, NULL)
+                                : sd_vc(L, ns, zpn, mb, (long)nb * n3, adn[x], avis, lambda);
+          vt[(long)b * n3 + x] = vc;
+          const double fdmax = dtlt * vc;
+          double fd = 0.;
+          int fi = 0, dep = 0;
+          while (fd < fdmax) {
+            fd = fd + (1. / dzt[k - fi]);
+            fi = fi + 1;
+            if (k - fi <= 0) { dep = 1; break; }     /* 1-based k-fi <= 1 */
+          }
+          fi = fi - 1;
+          const int kf = k - fi;
+          fd = fd - (1. / dzt[kf]);
+          const double fdos = fmin(fmax(fdmax - fd, 0.), 1. / dzt[kf]);
+          prn[b][k] = prn[b][k] - zpn;
+          for (int s = 0; s < ns; ++s) prv[s * nb + b][k] = prv[s * nb + b][k] - mb[(long)s * nb * n3];
+          if (dep)
+            for (int s = 0; s < ns; ++s) rem[s * nb + b] = rem[s * nb + b] + mb[(long)s * nb * n3] * adn[x] * vc;
+          if (ice)                                     /* pristine and rimed ice (:1208-1210) */
+            rate[x] = rate[x] + (mb[(long)(ns - 2) * nb * n3] + mb[(long)(ns - 1) * nb * n3]) * adn[x] * vc * alv;
+          else
+            rate[x] = rate[x] + mb[(long)(ns - 1) * nb * n3] * adn[x] * vc * alv;
+          if (fdos * dzt[kf] > 0.5) {
+            prn[b][kf - 1] = prn[b][kf - 1] + zpn * dzt[kf - 1] / dzt[k];
+            for (int s = 0; s < ns; ++s)
+              prv[s * nb + b][kf - 1] = prv[s * nb + b][kf - 1] + mb[(long)s * nb * n3] * dzt[kf - 1] / dzt[k];
+          } else {
+            prn[b][kf - 1] = prn[b][kf - 1] + (fdos * dzt[kf]) * zpn * dzt[kf - 1] / dzt[kf];
+            prn[b][kf] = prn[b][kf] + (1. - fdos * dzt[kf]) * zpn;
+            for (int s = 0; s < ns; ++s) {
+              prv[s * nb + b][kf - 1] =
+                  prv[s * nb + b][kf - 1] + (fdos * dzt[kf]) * mb[(long)s * nb * n3] * dzt[kf - 1] / dzt[kf];
+              prv[s * nb + b][kf] = prv[s * nb + b][kf] + (1. - fdos * dzt[kf]) * mb[(long)s * nb * n3];
+            }
+          }
+        }
+      }
+      /* surface rate: SUM(remprc(i,j,(ns-1)*nb:ns*nb)) for precipitation,
+         (ns-2)*nb:ns*nb for ice (1-based, one entry before the water rows, as
+         written; :1256-1260) */
+      double s2 = 0.;
+      for (int q = (ice ? ns - 2 : ns - 1) * nb - 1; q < ns * nb; ++q) s2 = s2 + rem[q];
+      srate[(long)i + (long)nxp * j] = srate[(long)i + (long)nxp * j] + s2 * alv;
+      for (int k = 0; k < nzp; ++k) {
+        const long x = IX(k, i, j);
+        for (int b = 0; b < nb; ++b) n_t[(long)b * n3 + x] = n_t[(long)b * n3 + x] + prn[b][k] / dtlt;
+        for (int q = 0; q < nb * ns; ++q) m_t[(long)q * n3 + x] = m_t[(long)q * n3 + x] + prv[q][k] / dtlt;
+        if (k <= nzp - 2) {                            /* 1-based k = 1 .. nzp-1 */
+          /* water rows in bin order; for ice the rime rows follow */
+          double sw = 0.;
+          for (int q = (ice ? ns - 2 : ns - 1) * nb; q < ns * nb; ++q) sw = sw + prv[q][k];
+          tlt[x] = tlt[x] - (sw / dtlt) * (alv / sd_cp) * th[x] / tk[x];
+        }
+      }
+    }
+#undef IX
+  return 0;
+}
+
 int so_sedim(const so_layout *L, int nzp, int nxp, int nyp, double dtlt, const double *dzt, const double *tk,
              const double *th, const double *adn, const double *ustar, double *const n_p[3], double *const m_p[3],
              double *const n_t[3], double *const m_t[3], double *tlt, double *rrate, double *sfcrrate, double *vt_prc,
@@ -2298,76 +2450,31 @@ int so_sedim(const so_layout *L, int nzp, int nxp, int nyp, double dtlt, const d
       }
   }
   /* fast deposition: precipitation (flag 3) */
-  if (sed_precp) {
-    const int nb = nbs[2];
-    double *N = n_p[2], *M = m_p[2];
-    double prn[SO_MAXBIN][256], prv[SO_MAXBIN * SO_MAXSPEC][256], rem[SO_MAXBIN * SO_MAXSPEC];
-    if (nzp > 256) return -1;
-    for (int j = 2; j <= nyp - 3; ++j)
-      for (int i = 2; i <= nxp - 3; ++i) {
-        for (int b = 0; b < nb; ++b)
-          for (int k = 0; k < nzp; ++k) prn[b][k] = 0.;
-        for (int q = 0; q < nb * ns; ++q) {
-          for (int k = 0; k < nzp; ++k) prv[q][k] = 0.;
-          rem[q] = 0.;
-        }
-        for (int k = nzp - 2; k >= 1; --k) {
-          const long x = IX(k, i, j);
-          double avis, lambda;
-          sd_air(tk[x], adn[x], &avis, &lambda);
-          for (int b = 0; b < nb; ++b) {
-            const double zpn = N[(long)b * n3 + x];
-            if (zpn < prlim_c) continue;
-            const double *mb = M + (long)b * n3 + x;
-            const double vc = sd_vc(L, ns, zpn, mb, (long)nb * n3, adn[x], avis, lambda);
-            vt_prc[(long)b * n3 + x] = vc;
-            const double fdmax = dtlt * vc;
-            double fd = 0.;
-            int fi = 0, dep = 0;
-            while (fd < fdmax) {
-              fd = fd + (1. / dzt[k - fi]);
-              fi = fi + 1;
-              if (k - fi <= 0) { dep = 1; break; }     /* 1-based k-fi <= 1 */
-            }
-            fi = fi - 1;
-            const int kf = k - fi;
-            fd = fd - (1. / dzt[kf]);
-            const double fdos = fmin(fmax(fdmax - fd, 0.), 1. / dzt[kf]);
-            prn[b][k] = prn[b][k] - zpn;
-            for (int s = 0; s < ns; ++s) prv[s * nb + b][k] = prv[s * nb + b][k] - mb[(long)s * nb * n3];
-            if (dep)
-              for (int s = 0; s < ns; ++s) rem[s * nb + b] = rem[s * nb + b] + mb[(long)s * nb * n3] * adn[x] * vc;
-            rrate[x] = rrate[x] + mb[(long)iw * nb * n3] * adn[x] * vc * sd_alvl;
-            if (fdos * dzt[kf] > 0.5) {
-              prn[b][kf - 1] = prn[b][kf - 1] + zpn * dzt[kf - 1] / dzt[k];
-              for (int s = 0; s < ns; ++s)
-                prv[s * nb + b][kf - 1] = prv[s * nb + b][kf - 1] + mb[(long)s * nb * n3] * dzt[kf - 1] / dzt[k];
-            } else {
-              prn[b][kf - 1] = prn[b][kf - 1] + (fdos * dzt[kf]) * zpn * dzt[kf - 1] / dzt[kf];
-              prn[b][kf] = prn[b][kf] + (1. - fdos * dzt[kf]) * zpn;
-              for (int s = 0; s < ns; ++s) {
-                prv[s * nb + b][kf - 1] =
-                    prv[s * nb + b][kf - 1] + (fdos * dzt[kf]) * mb[(long)s * nb * n3] * dzt[kf - 1] / dzt[kf];
-                prv[s * nb + b][kf] = prv[s * nb + b][kf] + (1. - fdos * dzt[kf]) * mb[(long)s * nb * n3];
-              }
-            }
-          }
-        }
-        double s2 = 0.;
-        for (int q = ns * nb - nb - 1; q < ns * nb; ++q) s2 = s2 + rem[q];
-        sfcrrate[(long)i + (long)nxp * j] = sfcrrate[(long)i + (long)nxp * j] + s2 * sd_alvl;
-        for (int k = 0; k < nzp; ++k) {
-          const long x = IX(k, i, j);
-          for (int b = 0; b < nb; ++b) n_t[2][(long)b * n3 + x] = n_t[2][(long)b * n3 + x] + prn[b][k] / dtlt;
-          for (int q = 0; q < nb * ns; ++q) m_t[2][(long)q * n3 + x] = m_t[2][(long)q * n3 + x] + prv[q][k] / dtlt;
-          if (k <= nzp - 2) {                          /* 1-based k = 1 .. nzp-1 */
-            double sw = 0.;
-            for (int b = 0; b < nb; ++b) sw = sw + prv[iw * nb + b][k];
-            tlt[x] = tlt[x] - (sw / dtlt) * (sd_alvl / sd_cp) * th[x] / tk[x];
-          }
-        }
-      }
-  }
+  if (sed_precp && sd_fast(L, NULL, 0, nzp, nxp, nyp, dtlt, dzt, tk, th, adn, L->nprc, ns, n_p[2], m_p[2], n_t[2],
+                           m_t[2], tlt, rrate, sfcrrate, vt_prc))
+    return -1;
 #undef IX
   return 0;
+}
+
+/* sedim_SALSA at level 5 (mcrp.f90:745-921): the liquid categories as at
+   level 4 (their ns = the wet species), then ice with DepositionFast flag 4
+   (ns = the wet species and rime) into irate, sfcirate and VtIce, which are
+   zeroed first (:790-795, :1127). */
+int so_sedim5(const so_layout *L, const so_params *P, int nzp, int nxp, int nyp, double dtlt, const double *dzt,
+              const double *tk, const double *th, const double *adn, const double *ustar, double *const n_p[4],
+              double *const m_p[4], double *const n_t[4], double *const m_t[4], double *tlt, double *rrate,
+              double *sfcrrate, double *irate, double *sfcirate, double *vt_prc, double *vt_ice, int sed_aero,
+              int sed_cloud, int sed_precp, int sed_ice) {
+  const long n3 = (long)nzp * nxp * nyp;
+  int rc = so_sedim(L, nzp, nxp, nyp, dtlt, dzt, tk, th, adn, ustar, n_p, m_p, n_t, m_t, tlt, rrate, sfcrrate,
+                    vt_prc, sed_aero, sed_cloud, sed_precp);
+  if (rc) return rc;
+  for (long x = 0; x < n3; ++x) irate[x] = 0.;
+  for (long c = 0; c < (long)nxp * nyp; ++c) sfcirate[c] = 0.;
+  for (long x = 0; x < (long)L->nice * n3; ++x) vt_ice[x] = 0.;
+  if (sed_ice)
+    rc = sd_fast(L, P, 1, nzp, nxp, nyp, dtlt, dzt, tk, th, adn, L->nice, L->nspec, n_p[3], m_p[3], n_t[3], m_t[3],
+                 tlt, irate, sfcirate, vt_ice);
+  return rc;
 }

@@ -198,6 +198,13 @@ int so_sedim(const so_layout *L, int nzp, int nxp, int nyp, double dtlt, const d
              double *const n_t[3], double *const m_t[3], double *tlt, double *rrate, double *sfcrrate, double *vt_prc,
              int sed_aero, int sed_cloud, int sed_precp);
 double so_terminal_vel(double diam, double rhop, double rhoa, double visc, double beta);
+int so_sedim5(const so_layout *L, const so_params *P, int nzp, int nxp, int nyp, double dtlt, const double *dzt,
+              const double *tk, const double *th, const double *adn, const double *ustar, double *const n_p[4],
+              double *const m_p[4], double *const n_t[4], double *const m_t[4], double *tlt, double *rrate,
+              double *sfcrrate, double *irate, double *sfcirate, double *vt_prc, double *vt_ice, int sed_aero,
+              int sed_cloud, int sed_precp, int sed_ice);
+void so_sed_ice_vc(const so_layout *L, const so_params *P, double tk, double adn, double numc, const double *m,
+                   double *out);
 
 #ifdef __cplusplus
 }

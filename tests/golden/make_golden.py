@@ -168,7 +168,43 @@ def init_cases():
     save_step("step_cfg5_init.npz", L5, 5, O.make_params(level=5), tuple(x[sel] for x in f5), init=True)
 
 
+def ice_fall_cases():
+    """DepositionFast's per-bin fall-speed chain for ice (flag 4, mcrp.f90:
+    1146-1176) through the reference's calcDiamLES, getDiameter,
+    getShapeCoefficients and terminal_vel (harness mode 3), on SO4 + dust
+    ice bins whose particle masses (1e-14 to 1e-7 kg) and rime fractions
+    (0 to 0.95) cover the four mass-diameter regimes of mo_ice_shape.f90
+    (written separately: `--only icefall`)."""
+    L5 = O.make_layout(species=(0, 3))
+    P5 = O.make_params(level=5)
+    rng = np.random.default_rng(51)
+    n, nt, ni, i0 = 60, L5.ntotal, L5.nice, L5.ntotal - L5.nice
+    T = rng.uniform(240., 275., n)
+    adn = rng.uniform(0.8, 1.3, n)
+    numc = np.zeros((n, nt))
+    volc = np.zeros((n, nt, 9))
+    num = 10 ** rng.uniform(-5, 5, (n, ni))
+    mp = 10 ** rng.uniform(-14, -7, (n, ni))
+    fr = rng.uniform(0., 0.95, (n, ni))
+    fr[rng.random((n, ni)) < 0.2] = 0.
+    tot = mp * num
+    dry = tot * 10 ** rng.uniform(-4, -2, (n, ni))
+    numc[:, i0:] = num
+    volc[:, i0:, 0] = 0.3 * dry                     # SO4
+    volc[:, i0:, 1] = 0.7 * dry                     # dust
+    volc[:, i0:, 2] = tot * (1. - fr) - dry         # pristine ice (water)
+    volc[:, i0:, 3] = tot * fr                      # rime
+    z = np.zeros(n)
+    R = refio.run(L5, P5, 5, T, adn, z, z, z, numc, volc, mode=3)
+    np.savez_compressed(os.path.join(HERE, "ice_fall_cfg5.npz"), T=T, adn=adn, numc=num,
+                        mass=volc[:, i0:, :L5.nspec],
+                        **{k: R[k] for k in ("dwet", "dnsp", "alpha", "beta", "gamma", "sigma", "zdn", "gg", "vc")})
+    print("wrote ice_fall_cfg5.npz", n * ni, "bins")
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "icefall":
+        return ice_fall_cases()
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "init":
         return init_cases()
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "kstore":

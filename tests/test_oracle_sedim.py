@@ -13,10 +13,12 @@ mo_particle_external_properties.f90:43-100), pinned by:
   surface water flux that the surface rain rate reports;
 * the fall distance on a uniform grid worked by hand.
 Runs the FP64 oracle only."""
+import os
+
 import numpy as np
 import pytest
 
-from helpers import les_case
+from helpers import GOLDEN, les_case
 
 ALVL = 2.5e6
 
@@ -150,3 +152,149 @@ def test_fall_distance_uniform_grid(oracle_mod):
             expect[k - fi - 1] += f * N
             expect[k - fi] += (1 - f) * N
         np.testing.assert_allclose(got, expect, rtol=1e-12, atol=1e-9 * N)
+
+
+# ---- level 5: ice (DepositionFast flag 4, mcrp.f90:880-905, 1073-1275) ----
+
+ALVI = 2.834e6
+
+
+def sed_fields5(desc, grid, seed):
+    """Level-5 LES fields (SO4 + dust, ice with rime) in the state
+    SALSA_diagnostics leaves, with the fields sedimentation reads."""
+    shape, dzt, f, n_p, n_t, m_p, m_t = les_case(desc, 5, grid, seed)
+    rng = np.random.default_rng(seed + 7)
+    n3 = int(np.prod(shape))
+    for a in n_p + m_p:
+        np.abs(a, out=a)
+    for c in range(4):
+        nb = n_p[c].shape[0]
+        nsw = desc.iwa + (2 if c == 3 else 1)
+        tot = sum(m_p[c][s * nb:(s + 1) * nb] for s in range(nsw))
+        n_p[c][tot <= 1e-25 * np.maximum(n_p[c], 1.)] = 0.
+    tk = f["temp"]
+    fields = dict(tk=tk, th=tk * 1.02, adn=np.full(n3, 1.1) + 0.1 * rng.random(n3),
+                  ustar=0.1 + 0.4 * rng.random(shape[1] * shape[2]))
+    n_t = [rng.normal(size=a.shape) * 1e-3 * a for a in n_p]
+    m_t = [rng.normal(size=a.shape) * 1e-3 * a for a in m_p]
+    return shape, dzt, fields, n_p, m_p, n_t, m_t
+
+
+def run5(O, shape, dzt, fld, n_p, m_p, n_t, m_t, tlt, sed=(1, 1, 1, 1), dtlt=1.0):
+    L = O.make_layout(species=(0, 3))
+    P = O.make_params(level=5)
+    return O.sedim5(L, P, shape, dtlt, dzt, fld["tk"], fld["th"], fld["adn"], fld["ustar"], n_p, m_p, n_t, m_t, tlt,
+                    sed)
+
+
+def test_ice_fall_speed_chain_against_reference(oracle_mod):
+    """The per-bin fall speed of ice against the compiled reference (calcDiamLES
+    both ways, getDiameter, getShapeCoefficients, terminal_vel flag 4;
+    tests/golden/ice_fall_cfg5.npz): the diameters, shape coefficients,
+    density and Cunningham factor to the last bit, the fall speed to 1e-12
+    (pow and exp of the two runtimes).  The cases span the four mass-diameter
+    regimes of mo_ice_shape.f90."""
+    O = oracle_mod
+    d = np.load(os.path.join(GOLDEN, "ice_fall_cfg5.npz"))
+    L = O.make_layout(species=(0, 3))
+    P = O.make_params(level=5)
+    keys = ("dwet", "dnsp", "alpha", "beta", "gamma", "sigma", "zdn", "gg", "vc")
+    n, ni = d["numc"].shape
+    got = np.array([[O.sed_ice_vc(L, P, d["T"][c], d["adn"][c], d["numc"][c, b], d["mass"][c, b]) for b in range(ni)]
+                    for c in range(n)])
+    for q, k in enumerate(keys):
+        rtol = 1e-12 if k == "vc" else 0.
+        np.testing.assert_allclose(got[:, :, q], d[k], rtol=rtol, atol=0, err_msg=k)
+    # regimes: spherical (small or graupel), dense non-spherical, partially rimed
+    sph = np.isclose(d["alpha"], 0.5235988 * 917.)
+    dense = d["alpha"] == P.ice_alpha
+    rimed = ~sph & ~dense
+    mtot = d["mass"].sum(axis=-1) / d["numc"]
+    assert sph.sum() > 20 and dense.sum() > 5 and rimed.sum() > 20
+    assert mtot[sph].min() < 1e-12 < mtot[sph].max()          # below Mth and in the graupel band
+
+
+def test_ice_conserved_per_column(oracle_mod):
+    """Ice moves only between levels (the level below the surface included):
+    on a uniform grid sum_k N/dzt and sum_k M_s/dzt of every species, rime
+    included, are conserved per column."""
+    from salsa_amd import capi
+    desc = capi.bin_desc(species=("SO4", "DU"))
+    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields5(desc, (20, 4, 4), 5)
+    dzt = np.full_like(dzt, 1. / 25.)
+    n_t = [np.zeros_like(a) for a in n_p]
+    m_t = [np.zeros_like(a) for a in m_p]
+    tlt = np.zeros(int(np.prod(shape)))
+    rr, sr, ir, si, vt, vi = run5(oracle_mod, shape, dzt, fld, n_p, m_p, n_t, m_t, tlt, sed=(0, 0, 0, 1),
+                                  dtlt=20.)
+    nzp, nxp, nyp = shape
+    assert vi.max() > 0.3, "no fast ice: the case does not exercise the fall distance"
+    assert not any(a.any() for a in n_t[:3] + m_t[:3]) and not rr.any() and not vt.any()
+    for a, src in ((n_t[3], n_p[3]), (m_t[3], m_p[3])):
+        s = a.reshape(a.shape[0], nyp, nxp, nzp)[:, 2:nyp - 2, 2:nxp - 2, :]
+        s0 = src.reshape(s.shape[0], nyp, nxp, nzp)[:, 2:nyp - 2, 2:nxp - 2, :]
+        col = (s / dzt[None, None, None, :]).sum(axis=-1)
+        scale = (np.abs(s0) / dzt[None, None, None, :]).sum(axis=-1) + 1e-300
+        assert np.all(np.abs(col) <= 1e-14 * scale)
+        assert (np.abs(s) > 0).sum() > 0
+
+
+def test_ice_surface_rate_from_lowest_level(oracle_mod):
+    """Ice only at the level above the surface: every bin there deposits, so
+    the surface rate (the deposited water and rime, times alvi) equals that
+    level's ice rate (water plus rime of every bin, times alvi), and the rime
+    alone already gives a positive rate; the latent heat is alvi, so tlt
+    changes by -(deposited water + rime)/dtlt alvi/cp th/tk summed with the
+    landing level below the surface."""
+    from salsa_amd import capi
+    desc = capi.bin_desc(species=("SO4", "DU"))
+    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields5(desc, (12, 4, 4), 8)
+    nzp, nxp, nyp = shape
+    n3 = nzp * nxp * nyp
+    nb = desc.nice
+    k = np.arange(n3) % nzp
+    n_p[3][:, k != 1] = 0.
+    m_p[3][:, k != 1] = 0.
+    m_p[3][(desc.iwa - 1) * nb + nb - 1] = 0.     # the rate sum starts one entry early (as written): keep it empty
+    for rime_only in (False, True):
+        mp = [a.copy() for a in m_p]
+        if rime_only:
+            mp[3][desc.iwa * nb:(desc.iwa + 1) * nb] = 0.
+        n_t = [np.zeros_like(a) for a in n_p]
+        m_t = [np.zeros_like(a) for a in mp]
+        tlt = np.zeros(n3)
+        rr, sr, ir, si, vt, vi = run5(oracle_mod, shape, dzt, fld, n_p, mp, n_t, m_t, tlt, sed=(0, 0, 0, 1))
+        irk = ir.reshape(nyp, nxp, nzp)[2:nyp - 2, 2:nxp - 2, 1]
+        sik = si.reshape(nyp, nxp)[2:nyp - 2, 2:nxp - 2]
+        assert np.all(irk > 0)
+        np.testing.assert_allclose(sik, irk, rtol=1e-13)
+        # tlt: levels 0 and 1 of the interior columns (the ice leaves level 1)
+        w = sum(m_t[3][q] for q in range(desc.iwa * nb, (desc.irim + 1) * nb))
+        exp_tlt = -w * (ALVI / 1005.) * fld["th"] / fld["tk"]
+        inner = np.zeros((nyp, nxp, nzp), bool)
+        inner[2:nyp - 2, 2:nxp - 2, :nzp - 1] = True
+        np.testing.assert_allclose(tlt[inner.ravel()], exp_tlt[inner.ravel()], rtol=1e-12, atol=1e-300)
+        assert np.abs(tlt).max() > 0
+
+
+def test_level5_liquid_categories_as_level4(oracle_mod):
+    """At level 5 the aerosol, cloud and precipitation passes are those of
+    level 4 on the same arrays (the wet species; mcrp.f90:797-878)."""
+    from salsa_amd import capi
+    desc = capi.bin_desc(species=("SO4", "DU"))
+    shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields5(desc, (12, 6, 5), 9)
+    L = oracle_mod.make_layout(species=(0, 3))
+    tl4, tl5 = np.zeros(int(np.prod(shape))), np.zeros(int(np.prod(shape)))
+    nt4, mt4 = [a.copy() for a in n_t[:3]], [a.copy() for a in m_t[:3]]
+    r4 = oracle_mod.sedim(L, shape, 1.0, dzt, fld["tk"], fld["th"], fld["adn"], fld["ustar"], n_p[:3], m_p[:3],
+                          nt4, mt4, tl4)
+    nt5, mt5 = [a.copy() for a in n_t], [a.copy() for a in m_t]
+    r5 = run5(oracle_mod, shape, dzt, fld, n_p, m_p, nt5, mt5, tl5, sed=(1, 1, 1, 0))
+    for a, b in zip(r4, (r5[0], r5[1], r5[4])):
+        np.testing.assert_array_equal(a, b)
+    for c in range(3):
+        np.testing.assert_array_equal(nt4[c], nt5[c])
+        np.testing.assert_array_equal(mt4[c], mt5[c])
+    np.testing.assert_array_equal(tl4, tl5)
+    np.testing.assert_array_equal(nt5[3], n_t[3])
+    assert not r5[2].any() and not r5[3].any() and not r5[5].any()

@@ -38,7 +38,7 @@ int launch_adjoint_seed(const DevLayout *d, const double *x, int64_t ncells, int
 int launch_adjoint_out(const DevLayout *d, const AdjOut &o, const double *jc, const double *ebar,
                        const uint8_t *flags, double *sum_out, hipStream_t s);
 int launch_les_update(const LesArgs &a, hipStream_t s);
-int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, hipStream_t s);
+int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, int sed_ice, hipStream_t s);
 }  // namespace salsa
 
 using namespace salsa;
@@ -945,30 +945,34 @@ extern "C" int salsa_les_update_device(salsa_ctx *c, double dtlt, const double *
 extern "C" int salsa_sedim_device(salsa_ctx *c, double dtlt, const double *dzt, const double *tk, const double *th,
                                   const double *adn, const double *ustar, const double *const n_p[4],
                                   const double *const m_p[4], double *const n_t[4], double *const m_t[4],
-                                  double *tlt, double *rrate, double *sfcrrate, double *vt_prc, int32_t sed_aero,
-                                  int32_t sed_cloud, int32_t sed_precp) {
+                                  double *tlt, double *rrate, double *sfcrrate, double *vt_prc, double *irate,
+                                  double *sfcirate, double *vt_ice, int32_t sed_aero, int32_t sed_cloud,
+                                  int32_t sed_precp, int32_t sed_ice) {
   if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
   const salsa_params &P = c->params;
   const salsa_bin_desc &d = c->desc;
-  if (P.level != 4) { set_error("sedimentation is carried at level 4 only (no ice)"); return SALSA_ERR_ARG; }
+  const int l5 = P.level == 5;
+  if (P.level != 4 && !l5) { set_error("sedimentation is carried at levels 4 and 5"); return SALSA_ERR_ARG; }
   if (P.hz != 1 || P.hxy != 2) { set_error("sedimentation needs the LES halo widths (hz 1, hxy 2)"); return SALSA_ERR_ARG; }
   if (c->nzp < 4 || c->nzp > 256 || c->nxp < 5 || c->nyp < 5) {
     set_error("grid out of range for sedimentation (4 <= nzp <= 256, nxp and nyp >= 5)");
     return SALSA_ERR_ARG;
   }
   if (!(dtlt > 0.) || !dzt || !tk || !th || !adn || !ustar || !n_p || !m_p || !n_t || !m_t || !tlt || !rrate ||
-      !sfcrrate || !vt_prc) {
+      !sfcrrate || !vt_prc || (l5 && (!irate || !sfcirate || !vt_ice))) {
     set_error("null argument or non-positive dtlt");
     return SALSA_ERR_ARG;
   }
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 3 + l5; ++k)
     if (!n_p[k] || !m_p[k] || !n_t[k] || !m_t[k]) { set_error("null field pointer"); return SALSA_ERR_ARG; }
+  if (l5 && d.nice > 16) { set_error("ice sedimentation carries at most 16 ice bins"); return SALSA_ERR_ARG; }
   SedArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.nzp = c->nzp; a.nxp = c->nxp; a.nyp = c->nyp; a.n3 = c->n3; a.dtlt = dtlt;
-  a.nb[0] = d.nbins; a.nb[1] = d.ncld; a.nb[2] = d.nprc;
-  a.ns = d.iwa + 1; a.iwa = d.iwa;
-  for (int s = 0; s < SALSA_MAXSPEC; ++s) a.rholiq[s] = d.rholiq[s];
+  a.nzp = c->nzp; a.nxp = c->nxp; a.nyp = c->nyp; a.level = P.level; a.n3 = c->n3; a.dtlt = dtlt;
+  a.nb[0] = d.nbins; a.nb[1] = d.ncld; a.nb[2] = d.nprc; a.nb[3] = l5 ? d.nice : 0;
+  a.ns = d.iwa + 1; a.ns_ice = d.iwa + 2; a.iwa = d.iwa; a.irim = d.irim;
+  for (int s = 0; s < SALSA_MAXSPEC; ++s) { a.rholiq[s] = d.rholiq[s]; a.rhoice[s] = d.rhoice[s]; }
+  a.ice_alpha = P.ice_alpha; a.ice_beta = P.ice_beta; a.ice_gamma = P.ice_gamma; a.ice_sigma = P.ice_sigma;
   for (int k = 0; k < c->nzp; ++k) {
     if (!(dzt[k] > 0.)) { set_error("dzt must be positive"); return SALSA_ERR_ARG; }
     a.dzt[k] = dzt[k];
@@ -981,9 +985,10 @@ extern "C" int salsa_sedim_device(salsa_ctx *c, double dtlt, const double *dzt, 
   }
   a.vslow = c->sed_work;
   a.tk = tk; a.th = th; a.adn = adn; a.ustar = ustar;
-  for (int k = 0; k < 3; ++k) { a.n_p[k] = n_p[k]; a.m_p[k] = m_p[k]; a.n_t[k] = n_t[k]; a.m_t[k] = m_t[k]; }
+  for (int k = 0; k < 3 + l5; ++k) { a.n_p[k] = n_p[k]; a.m_p[k] = m_p[k]; a.n_t[k] = n_t[k]; a.m_t[k] = m_t[k]; }
   a.tlt = tlt; a.rrate = rrate; a.sfcrrate = sfcrrate; a.vt_prc = vt_prc;
-  const int rc = launch_sedim(a, sed_aero, sed_cloud, sed_precp, c->stream);
+  a.irate = irate; a.sfcirate = sfcirate; a.vt_ice = vt_ice;
+  const int rc = launch_sedim(a, sed_aero, sed_cloud, sed_precp, sed_ice, c->stream);
   if (rc) { std::snprintf(g_err, sizeof(g_err), "sedimentation launch failed (%d)", rc); return SALSA_ERR_HIP; }
   return SALSA_OK;
 }

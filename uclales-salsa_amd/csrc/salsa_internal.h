@@ -244,16 +244,19 @@ struct LesArgs {
 // sedim_SALSA with DepositionSlow (aerosol, cloud) and DepositionFast
 // (precipitation) on the LES arrays.
 struct SedArgs {
-  int32_t nzp, nxp, nyp;
+  int32_t nzp, nxp, nyp, level;
   int64_t n3;
   double dtlt;
-  int32_t nb[3], ns, iwa;
-  double rholiq[kMaxSpec];
+  int32_t nb[4];                      // aerosol, cloud, precipitation, ice bins
+  int32_t ns, ns_ice, iwa, irim;      // mass rows per bin (liquid: wet species; ice: + rime); water, rime
+  double rholiq[kMaxSpec], rhoice[kMaxSpec];
+  double ice_alpha, ice_beta, ice_gamma, ice_sigma;   // mo_ice_shape.f90:25-30
   double dzt[256];                    // inverse level thicknesses (nzp <= 256)
   const double *tk, *th, *adn, *ustar;
-  const double *n_p[3], *m_p[3];
-  double *n_t[3], *m_t[3];
+  const double *n_p[4], *m_p[4];
+  double *n_t[4], *m_t[4];
   double *tlt, *rrate, *sfcrrate, *vt_prc;
+  double *irate, *sfcirate, *vt_ice;  // level 5
   double *vslow;                      // work: fall speeds of one slow category [nb][n3]
 };
 

@@ -1,8 +1,9 @@
-// salsa_sed.hip -- sedimentation of the liquid SALSA categories at level 4
-// (sedim_SALSA, mcrp.f90:745-921) on device.
+// salsa_sed.hip -- sedimentation of the SALSA categories (sedim_SALSA,
+// mcrp.f90:745-921) on device: the liquid categories at levels 4 and 5, ice
+// at level 5.
 //
-//   k_sed_zero   rrate, sfcrrate, VtPrc and the work arrays to zero
-//                (sedim_SALSA :792, DepositionFast :1129-1132)
+//   k_sed_zero   rrate, sfcrrate, VtPrc (level 5: irate, sfcirate, VtIce) to
+//                zero (sedim_SALSA :792-795, DepositionFast :1127)
 //   k_sed_vslow  DepositionSlow's (:926-1069) fall speeds of one category per
 //                (cell, bin), with the dry deposition velocity at the lowest
 //                level, into a work array
@@ -14,22 +15,25 @@
 //                and the surface rate.  One launch per category, so the
 //                accumulation into rrate / tlt keeps the reference's order.
 //   k_sed_prc    DepositionFast (:1073-1275) and its part of sedim_SALSA
-//                (:862-868), one workgroup per interior column and one lane
-//                per level: fall speeds into VtPrc, the fall-distance
+//                (:862-878, :884-904) for precipitation (flag 3) or ice
+//                (flag 4), one workgroup per interior column and one lane
+//                per level: fall speeds into VtPrc / VtIce, the fall-distance
 //                redistribution of every bin and moved quantity with each
 //                level's changes summed in the reference's top-down order,
-//                tendencies += change / dtlt, rain rate (bins in order), tlt
-//                and the surface rate from the deposited mass.
+//                tendencies += change / dtlt, rate (bins in order), tlt and
+//                the surface rate from the deposited mass.
 // Terminal velocities: terminal_vel (mo_particle_external_properties.f90:
-// 43-100), liquid branches.  Contraction is off so that the products and
+// 43-100), liquid and ice branches.  Contraction is off so that the products and
 // sums round as the oracle's.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
 
-#include "salsa_internal.h"
-
+// (before the shared headers, so that their inline physics rounds the same way here)
 #pragma clang fp contract(off)
+
+#include "salsa_internal.h"
+#include "salsa_physics.h"
 
 namespace salsa {
 
@@ -99,6 +103,11 @@ __global__ void __launch_bounds__(256) k_sed_zero(SedArgs a) {
     a.rrate[e] = 0.;
     for (int b = 0; b < nb; ++b) a.vt_prc[b * n3 + e] = 0.;
     if (e < ncol) a.sfcrrate[e] = 0.;
+    if (a.level == 5) {                         // irate, sfcirate, VtIce (:793-795, :1127)
+      a.irate[e] = 0.;
+      for (int b = 0; b < a.nb[3]; ++b) a.vt_ice[b * n3 + e] = 0.;
+      if (e < ncol) a.sfcirate[e] = 0.;
+    }
   }
 }
 
@@ -182,21 +191,27 @@ __global__ void __launch_bounds__(128) k_sed_slow(SedArgs a, int cat) {
   if (k == 1) a.sfcrrate[col] = a.sfcrrate[col] - sw * adn * kAlvl;
 }
 
-// DepositionFast for one interior column: one workgroup per column, one lane
-// per level (blockDim = nzp rounded up to 64).  Per precipitation bin, in
+// DepositionFast for one category on one interior column: one workgroup per
+// column, one lane per level (blockDim = nzp rounded up to 64).  Per bin, in
 // order:
 //   1. every source level k (1 <= k <= nzp-2 with N >= prlim) evaluates its
-//      fall speed (VtPrc) and its fall-distance decision (:1188-1225): the
-//      landing level kf, one or two receiving levels and their shares;
+//      fall speed (VtPrc / VtIce) and its fall-distance decision
+//      (:1188-1225): the landing level kf, one or two receiving levels and
+//      their shares;
 //   2. per moved quantity (the number, then the mass species), every level L
 //      sums what lands on it from the sources above it, in decreasing source
 //      level as the reference's top-down walk adds them, then its own
-//      departure (-q) and, when the drops stay within the level, its own
+//      departure (-q) and, when the particles stay within the level, its own
 //      share; the change goes straight into the tendency (/ dtlt).  Only
 //      sources within the column's largest fall distance of L are scanned.
 // Deposited mass (sources whose walk reaches the surface) is summed per
-// species and bin by lane 0 over those sources, top down.  Rain rate (bins in
-// order), tlt and the surface rate as in sedim_SALSA (:862-868).
+// species and bin by lane 0 over those sources, top down.  Rate (bins in
+// order), tlt and the surface rate as in sedim_SALSA (:862-878, :884-904).
+// ICE: flag 4 -- the fall speed of non-spherical ice (calcDiamLES both ways,
+// getShapeCoefficients, terminal_vel's ice branch), water and rime in the
+// rate, the surface rate and tlt, latent heat alvi; MS: the most mass rows
+// per bin (registers per level).
+constexpr int kSedMaxIce = 16;                // ice bins whose rime changes a workgroup keeps in LDS
 struct SedPrcLds {
   double dz[kSedMaxZ], rdz[kSedMaxZ], adn[kSedMaxZ];
   double vc[2][kSedMaxZ];                 // source fall speed, by bin parity
@@ -206,14 +221,24 @@ struct SedPrcLds {
   double rem[kMaxSpec * kMaxBin];         // deposited mass per (species, bin)
 };
 
-template <int MS>
+template <int MS, bool ICE>
 __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
   __shared__ SedPrcLds S;
+  __shared__ double rout[ICE ? kSedMaxIce : 1][kSedMaxZ];   // rime change per bin (ICE)
+  constexpr int cat = ICE ? 3 : 2;
+  constexpr double alv = ICE ? kAls : kAlvl;
   const int nx = a.nxp - 4;
   const int i = 2 + (int)blockIdx.x % nx, j = 2 + (int)blockIdx.x / nx;
   const int64_t col = (int64_t)i + (int64_t)a.nxp * j, n3 = a.n3;
   const int k = threadIdx.x, nzp = a.nzp, ktop = nzp - 2, nw = blockDim.x / 64;
-  const int nb = a.nb[2], ns = a.ns, iw = a.iwa;
+  const int nb = a.nb[cat], ns = ICE ? a.ns_ice : a.ns, iw = a.iwa, ir = a.irim;
+  const double *__restrict__ Np = a.n_p[cat];
+  const double *__restrict__ Mp = a.m_p[cat];
+  double *__restrict__ Nt = a.n_t[cat];
+  double *__restrict__ Mt = a.m_t[cat];
+  double *__restrict__ vt = ICE ? a.vt_ice : a.vt_prc;
+  double *__restrict__ rate = ICE ? a.irate : a.rrate;
+  double *__restrict__ srate = ICE ? a.sfcirate : a.sfcrrate;
   const bool lev = k < nzp, src = k >= 1 && k <= ktop;
   const int64_t x = col * nzp + k;
   const double dtlt = a.dtlt;
@@ -225,8 +250,8 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
     for (int u = 0; u <= MS; ++u) {
       const bool use = lev && b < nb && u <= ns;
       const int64_t r = u == 0 ? (int64_t)b * n3 : ((int64_t)(u - 1) * nb + b) * n3;
-      q[u] = use ? (u == 0 ? a.n_p[2] : a.m_p[2])[r + x] : 0.;
-      t[u] = use ? (u == 0 ? a.n_t[2] : a.m_t[2])[r + x] : 0.;
+      q[u] = use ? (u == 0 ? Np : Mp)[r + x] : 0.;
+      t[u] = use ? (u == 0 ? Nt : Mt)[r + x] : 0.;
     }
   };
   load(0, qv, tv);
@@ -239,7 +264,7 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
   }
   if (src) {
     air = sd_air(a.tk[x], a.adn[x]);
-    rr = a.rrate[x];
+    rr = rate[x];
   }
   __syncthreads();
   for (int b = 0; b < nb; ++b) {
@@ -251,15 +276,46 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
     int d = -1;
     bool two = false, dep = false;
     if (on) {
-      double v = 0., ms = 0.;
+      if (ICE) {
+        // calcDiamLES (mo_particle_external_properties.f90:169-201) both
+        // ways, getShapeCoefficients, the bulk density, terminal_vel flag 4
+        double v = 0., mpri = 0., ms = 0., mrim = 0.;
 #pragma unroll
-      for (int s = 0; s < MS; ++s)
-        if (s < ns) v = v + qv[1 + s] / a.rholiq[s];
+        for (int s = 0; s < MS; ++s)
+          if (s < ns) v = v + qv[1 + s] / a.rhoice[s];
 #pragma unroll
-      for (int s = 0; s < MS; ++s)
-        if (s < ns) ms = ms + qv[1 + s];
-      vc = sd_vc_sums(zpn, v, ms, S.adn[k], air, nullptr, nullptr);
-      a.vt_prc[(int64_t)b * n3 + x] = vc;
+        for (int s = 0; s < MS; ++s)
+          if (s < ns - 1) mpri = mpri + qv[1 + s];
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+          if (s == ns - 1) mrim = qv[1 + s];
+        const double dwet = zpn < 1.e-15 ? 0. : ::pow(v / zpn / kPi6, 1. / 3.);
+        const double dnsp = zpn < 1.e-15 ? 0. : ice_diameter(a.ice_alpha, a.ice_beta, mpri, mrim, zpn);
+        const Shape sh = shape_coeffs_p(a.ice_alpha, a.ice_beta, a.ice_gamma, a.ice_sigma, mpri, mrim, zpn);
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+          if (s < ns) ms = ms + qv[1 + s];
+        const double zdn = ms / zpn / (kPi6 * (dwet * dwet * dwet));
+        vc = terminal_vel_ice(dwet, zdn, S.adn[k], air.avis, sh, dnsp);
+      } else {
+        double v = 0., ms = 0.;
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+          if (s < ns) v = v + qv[1 + s] / a.rholiq[s];
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+          if (s < ns) ms = ms + qv[1 + s];
+        vc = sd_vc_sums(zpn, v, ms, S.adn[k], air, nullptr, nullptr);
+      }
+      vt[(int64_t)b * n3 + x] = vc;
+      // the rate: water (and, for ice, rime) falling out of the level (:1206-1212)
+      double wv = 0., rv = 0.;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        if (s == iw) wv = qv[1 + s];
+        if (s == ir) rv = qv[1 + s];
+      }
+      rr = ICE ? rr + (wv + rv) * S.adn[k] * vc * alv : rr + wv * S.adn[k] * vc * alv;
       const double fdmax = dtlt * vc;
       double fd = 0.;
       int fi = 0;
@@ -317,16 +373,14 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
           }
           if (on) {
             out = out - q;
-            if (d == -1) out = out + c1;        // drops that do not leave the level
+            if (d == -1) out = out + c1;        // particles that do not leave the level
             if (two && d == 0) out = out + c0;
           }
         }
         const int64_t r = qy == 0 ? (int64_t)b * n3 : ((int64_t)(qy - 1) * nb + b) * n3;
-        (qy == 0 ? a.n_t[2] : a.m_t[2])[r + x] = tv[qy] + out / dtlt;
-        if (qy - 1 == iw) {
-          sw = sw + out;
-          if (on) rr = rr + q * S.adn[k] * vc * kAlvl;
-        }
+        (qy == 0 ? Nt : Mt)[r + x] = tv[qy] + out / dtlt;
+        if (qy - 1 == iw) sw = sw + out;
+        if (ICE && qy - 1 == ir) rout[ICE ? b : 0][k] = out;
       }
       if (qy > 0 && k == 0) {                   // lane 0 (level 0 is never a source)
         double r = 0.;
@@ -341,18 +395,25 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
       tv[u] = tn[u];
     }
   }
-  if (src) a.rrate[x] = rr;
-  if (lev && k <= ktop) a.tlt[x] = a.tlt[x] - (sw / dtlt) * (kAlvl / kCp) * a.th[x] / a.tk[x];
+  if (src) rate[x] = rr;
+  if (lev && k <= ktop) {
+    // SUM over the water rows in bin order, for ice then the rime rows
+    if (ICE)
+      for (int b = 0; b < nb; ++b) sw = sw + rout[ICE ? b : 0][k];
+    a.tlt[x] = a.tlt[x] - (sw / dtlt) * (alv / kCp) * a.th[x] / a.tk[x];
+  }
   if (k == 0) {
+    // SUM(remprc((ns-1)*nb:ns*nb)) (ice: (ns-2)*nb:ns*nb), 1-based, one entry
+    // before the water rows as written (:1256-1260)
     double s2 = 0.;
-    for (int q = ns * nb - nb - 1; q < ns * nb; ++q) s2 = s2 + S.rem[q];
-    a.sfcrrate[col] = a.sfcrrate[col] + s2 * kAlvl;
+    for (int q = (ICE ? ns - 2 : ns - 1) * nb - 1; q < ns * nb; ++q) s2 = s2 + S.rem[q];
+    srate[col] = srate[col] + s2 * alv;
   }
 }
 
 }  // namespace
 
-int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, hipStream_t s) {
+int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, int sed_ice, hipStream_t s) {
   const unsigned g256 = (unsigned)((a.n3 + 255) / 256), g128 = (unsigned)((a.n3 + 127) / 128);
   hipLaunchKernelGGL(k_sed_zero, dim3(g256 < 65536 ? g256 : 65536), dim3(256), 0, s, a);
   for (int cat = 0; cat < 2; ++cat) {
@@ -365,11 +426,22 @@ int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, h
     const unsigned ncol = (unsigned)((a.nxp - 4) * (a.nyp - 4)), nthr = (unsigned)((a.nzp + 63) / 64 * 64);
     if (a.nzp > kSedMaxZ || a.nb[2] * a.ns > kMaxSpec * kMaxBin) return (int)hipErrorInvalidValue;
     if (a.ns <= 3)
-      hipLaunchKernelGGL(k_sed_prc<3>, dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<3, false>), dim3(ncol), dim3(nthr), 0, s, a);
     else if (a.ns <= 5)
-      hipLaunchKernelGGL(k_sed_prc<5>, dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<5, false>), dim3(ncol), dim3(nthr), 0, s, a);
     else
-      hipLaunchKernelGGL(k_sed_prc<kMaxSpec>, dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<kMaxSpec, false>), dim3(ncol), dim3(nthr), 0, s, a);
+  }
+  if (sed_ice && a.level == 5) {
+    const unsigned ncol = (unsigned)((a.nxp - 4) * (a.nyp - 4)), nthr = (unsigned)((a.nzp + 63) / 64 * 64);
+    if (a.nzp > kSedMaxZ || a.nb[3] > kSedMaxIce || a.nb[3] * a.ns_ice > kMaxSpec * kMaxBin)
+      return (int)hipErrorInvalidValue;
+    if (a.ns_ice <= 3)
+      hipLaunchKernelGGL((k_sed_prc<3, true>), dim3(ncol), dim3(nthr), 0, s, a);
+    else if (a.ns_ice <= 5)
+      hipLaunchKernelGGL((k_sed_prc<5, true>), dim3(ncol), dim3(nthr), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_sed_prc<kMaxSpec, true>), dim3(ncol), dim3(nthr), 0, s, a);
   }
   return (int)hipGetLastError();
 }

@@ -154,7 +154,8 @@ MODULE mo_salsa_amd_cabi
 
      ! Sedimentation of the liquid categories at level 4 (sedim_SALSA)
      INTEGER(C_INT) FUNCTION salsa_sedim_device(ctx, dtlt, dzt, tk, th, adn, ustar, n_p, m_p, n_t, m_t, tlt, &
-                                                rrate, sfcrrate, vt_prc, sed_aero, sed_cloud, sed_precp) &
+                                                rrate, sfcrrate, vt_prc, irate, sfcirate, vt_ice, &
+                                                sed_aero, sed_cloud, sed_precp, sed_ice) &
                                                 BIND(C, name="salsa_sedim_device")
        IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
        TYPE(C_PTR), VALUE         :: ctx
@@ -163,7 +164,8 @@ MODULE mo_salsa_amd_cabi
        TYPE(C_PTR), VALUE         :: tk, th, adn, ustar
        TYPE(C_PTR), INTENT(in)    :: n_p(4), m_p(4), n_t(4), m_t(4)
        TYPE(C_PTR), VALUE         :: tlt, rrate, sfcrrate, vt_prc
-       INTEGER(C_INT32_T), VALUE  :: sed_aero, sed_cloud, sed_precp
+       TYPE(C_PTR), VALUE         :: irate, sfcirate, vt_ice        ! level 5; C_NULL_PTR at level 4
+       INTEGER(C_INT32_T), VALUE  :: sed_aero, sed_cloud, sed_precp, sed_ice
      END FUNCTION salsa_sedim_device
 
      INTEGER(C_INT) FUNCTION salsa_ctx_sync(ctx) BIND(C, name="salsa_ctx_sync")

@@ -179,7 +179,7 @@ def lib():
         [C.c_void_p, C.c_void_p]
     L.salsa_les_update_device.restype = C.c_int
     L.salsa_sedim_device.argtypes = [C.c_void_p, C.c_double, _dp] + [C.c_void_p] * 4 + [_P4] * 4 + \
-        [C.c_void_p] * 4 + [C.c_int32] * 3
+        [C.c_void_p] * 7 + [C.c_int32] * 4
     L.salsa_sedim_device.restype = C.c_int
     L.salsa_last_error.argtypes = []
     L.salsa_last_error.restype = C.c_char_p
@@ -353,18 +353,21 @@ class Context:
             _ptr(f["rp"]), P(n_p), P(n_t), P(m_p), P(m_t), _ptr(charge_time), _ptr(diag)))
         _check(rc)
 
-    def sedim_device(self, dtlt, dzt, f, n_p, m_p, n_t, m_t, tlt, rrate, sfcrrate, vt_prc, sed=(1, 1, 1)):
-        """Level-4 sedimentation (salsa_sedim_device).  dzt: host float64[nzp];
-        f: dict of "tk", "th", "adn", "ustar" tensors; n_t, m_t, tlt added to;
-        rrate, sfcrrate, vt_prc overwritten."""
+    def sedim_device(self, dtlt, dzt, f, n_p, m_p, n_t, m_t, tlt, rrate, sfcrrate, vt_prc, sed=(1, 1, 1),
+                     irate=None, sfcirate=None, vt_ice=None):
+        """Sedimentation (salsa_sedim_device).  dzt: host float64[nzp]; f:
+        dict of "tk", "th", "adn", "ustar" tensors; n_t, m_t, tlt added to;
+        rrate, sfcrrate, vt_prc (level 5 also irate, sfcirate, vt_ice)
+        overwritten.  sed: (aerosol, cloud, precipitation[, ice])."""
         P = lambda xs: _P4(*[_ptr(x) for x in (list(xs) + [None] * (4 - len(xs)))])
         dz = np.ascontiguousarray(dzt, dtype=np.float64)
         if dz.size != self.shape[0]:
             raise ValueError("dzt must have nzp values")
+        sw = [int(x) for x in sed] + [0] * (4 - len(sed))
         rc = self._ordered(lambda: lib().salsa_sedim_device(
             self._h, dtlt, C.cast(dz.ctypes.data, _dp), _ptr(f["tk"]), _ptr(f["th"]), _ptr(f["adn"]),
-            _ptr(f["ustar"]), P(n_p[:3]), P(m_p[:3]), P(n_t[:3]), P(m_t[:3]), _ptr(tlt), _ptr(rrate),
-            _ptr(sfcrrate), _ptr(vt_prc), *[int(x) for x in sed]))
+            _ptr(f["ustar"]), P(n_p[:4]), P(m_p[:4]), P(n_t[:4]), P(m_t[:4]), _ptr(tlt), _ptr(rrate),
+            _ptr(sfcrrate), _ptr(vt_prc), _ptr(irate), _ptr(sfcirate), _ptr(vt_ice), *sw))
         _check(rc)
 
     def cell_flags(self):
