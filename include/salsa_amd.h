@@ -168,8 +168,10 @@ typedef struct {
                                   off (coagulation + condensation only)         */
   int32_t lsactiv;             /* cloud activation after condensation: 0 off, 1
                                   interstitial (lsactiv%mode 1, mo_salsa.f90:115-119,
-                                  actInterst mo_salsa_cloud.f90:859-1195); cloud-base
-                                  activation (mode 2) is not on this path        */
+                                  actInterst mo_salsa_cloud.f90:859-1195), 2 at cloud
+                                  base (mode 2, ActCloudBase :161-375; the state is
+                                  unchanged, the activated droplets go to the fields
+                                  of salsa_set_cloud_base_fields)                 */
 } salsa_params;
 
 typedef struct salsa_ctx salsa_ctx;
@@ -228,6 +230,20 @@ int salsa_step_device(salsa_ctx *ctx, const double *press, const double *temp,
                       const double *const n_p[4], double *const n_t[4],
                       const double *const m_p[4], double *const m_t[4],
                       double *rates, double tstep, double time, int32_t istp, int32_t flags);
+
+/*
+ * Cloud-base activation (params.lsactiv = 2): the LES fields that run_SALSA
+ * reads and writes for it (mo_salsa_driver.f90:360-366; cloud_activation's w
+ * and pactd, mo_salsa.f90:115-119), device arrays of the caller used by every
+ * following step of this context, salsa_step_host included:
+ *   w     : vertical velocity [n3] (m/s); cells with w <= 0 activate nothing
+ *   nactd : pa_nactd [ncld][n3], activated droplets per cloud bin (#/kg)
+ *   vactd : pa_vactd [nwet*ncld][n3] (row s*ncld + b), their mass mixing
+ *           ratios per wet species (kg/kg)
+ * Every interior cell of a step overwrites its nactd / vactd entries.  All
+ * three non-null, or all null (then a step with lsactiv = 2 is refused).
+ */
+int salsa_set_cloud_base_fields(salsa_ctx *ctx, const double *w, double *nactd, double *vactd);
 
 /* Wait for the context stream; returns SALSA_ERR_NONFINITE / SALSA_ERR_NOCONV /
  * SALSA_ERR_DISTR if any cell of the last step raised the corresponding fault

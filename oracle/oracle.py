@@ -85,6 +85,9 @@ def lib():
         L.so_step_cell.restype = C.c_int
         L.so_step_cell_ks.argtypes = L.so_step_cell.argtypes + [C.POINTER(C.c_double), C.c_int]
         L.so_step_cell_ks.restype = C.c_int
+        L.so_step_cell_act.argtypes = L.so_step_cell.argtypes + [C.c_double, C.POINTER(C.c_double),
+                                                                C.POINTER(C.c_double)]
+        L.so_step_cell_act.restype = C.c_int
         L.so_step_batch.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_long, dp, dp, dp, dp, dp,
                                     C.c_double, C.c_int, dp, dp]
         L.so_step_batch.restype = C.c_int
@@ -118,6 +121,8 @@ def lib():
         L.so_sed_ice_vc.argtypes = [C.POINTER(Layout), C.POINTER(Params), C.c_double, C.c_double, C.c_double, dp,
                                     dp]
         L.so_sed_ice_vc.restype = None
+        L.so_act_cloud_base.argtypes = [C.POINTER(Layout), C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.so_act_cloud_base.restype = None
         _lib = L
     return _lib
 
@@ -145,6 +150,27 @@ def make_params(level=4, **kw):
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def step_cells_activ(L, P, T, p, rs, rsi, rv, numc, volc, w, dt=1.0):
+    """One step per cell (SALSA units) that also returns the cloud-base
+    activation of the step (P.lsactiv = 2, updraft w per cell): (rv, numc,
+    volc, status, nact[ncell, ncld], vact[ncell, ncld, 9])."""
+    lb = lib()
+    n = len(T)
+    rv = np.array(rv, dtype=np.float64, copy=True)
+    numc = np.ascontiguousarray(numc, dtype=np.float64).copy()
+    volc = np.ascontiguousarray(volc, dtype=np.float64).copy()
+    status = np.zeros(n, dtype=np.int32)
+    nact, vact = np.zeros((n, L.ncld)), np.zeros((n, L.ncld, MAXSPEC))
+    for i in range(n):
+        x = C.c_double(rv[i])
+        na, va = nact[i], vact[i]
+        status[i] = lb.so_step_cell_act(C.byref(L), C.byref(P), float(T[i]), float(p[i]), float(rs[i]),
+                                        float(rsi[i]), C.byref(x), dt, _dp(numc[i]), _dp(volc[i]), None,
+                                        float(w[i]), _dp(na), _dp(va))
+        rv[i] = x.value
+    return rv, numc, volc, status, nact, vact
 
 
 def step_cells(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, rates=False, kstore=None, kmodes=None,
@@ -297,6 +323,17 @@ def sedim5(L, P, shape, dtlt, dzt, tk, th, adn, ustar, n_p, m_p, n_t, m_t, tlt, 
     if rc != 0:
         raise ValueError("so_sedim5 failed: %d" % rc)
     return rr, sr, ir, si, vt, vi
+
+
+def act_cloud_base(L, T, p, w, numc, volc):
+    """Cloud-base activation of one cell (ActCloudBase + activate3) on a
+    state in SALSA units (numc[ntotal], volc[ntotal, 9]): returns (nact[ncld],
+    vact[ncld, 9]) of the activated droplets."""
+    nc = np.ascontiguousarray(numc, dtype=np.float64)
+    vc = np.ascontiguousarray(volc, dtype=np.float64).reshape(-1)
+    nact, vact = np.zeros(L.ncld), np.zeros(L.ncld * 9)
+    lib().so_act_cloud_base(C.byref(L), T, p, w, _dp(nc), _dp(vc), _dp(nact), _dp(vact))
+    return nact, vact.reshape(L.ncld, 9)
 
 
 def sed_ice_vc(L, P, tk, adn, numc, masses):

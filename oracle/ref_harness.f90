@@ -15,7 +15,10 @@
 ! LES mass mixing ratios in the bin arrays) the reference's calcDiamLES
 ! (spherical and maximum dimension), getShapeCoefficients and terminal_vel,
 ! with the air viscosity, mean free path and Cunningham factor formed as
-! DepositionFast forms them.  With sw(14) = 1 the
+! DepositionFast forms them; 4 = cloud-base activation (cloud_activation
+! with lsactiv%mode 2, mo_salsa_cloud.f90:161-375, 1199-1378) of the input
+! state with T in slot 1, p in slot 2 and the updraft w in slot 4: the
+! activated droplets pactd per cloud bin (number, then the 9 volumes).  With sw(14) = 1 the
 ! steps of mode 1 use lscoag%mode = 2: the first step of each cell is an
 ! update step (lcgupdt = .TRUE.), the later ones reuse the kernels left in
 ! the zcc* arrays by it, which is what fetch_coag provides for a single cell
@@ -134,6 +137,8 @@ PROGRAM ref_harness
      ALLOCATE(cellout(1 + nt + 9*nt + 20*10))
   ELSE IF (mode == 3) THEN
      ALLOCATE(cellout(10*nice))
+  ELSE IF (mode == 4) THEN
+     ALLOCATE(cellout(10*ncld))
   ELSE
      ALLOCATE(cellout(nt*nt), kmat(nt,nt))
   END IF
@@ -195,6 +200,14 @@ PROGRAM ref_harness
         DO k = 1, 20
            cellout(1 + nt + 9*nt + (k-1)*10 + 1:1 + nt + 9*nt + (k-1)*10 + 9) = Rates(k)%volc(1:9)
            cellout(1 + nt + 9*nt + (k-1)*10 + 10) = Rates(k)%numc
+        END DO
+     ELSE IF (mode == 4) THEN
+        lsactiv%mode = 2
+        W(1,1) = cellin(4)
+        CALL cloud_activation(1, 1, 1, T, P, RV, RS, W, pactd)
+        DO i = 1, ncld
+           cellout((i-1)*10 + 1) = pactd(1,1,i)%numc
+           cellout((i-1)*10 + 2:(i-1)*10 + 10) = pactd(1,1,i)%volc(1:9)
         END DO
      ELSE IF (mode == 3) THEN
         tk1 = cellin(1); adn1 = cellin(2)

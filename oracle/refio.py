@@ -70,7 +70,11 @@ def run(L, P, level, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1, mode=1, lc
     off += 8 * (3 * ntr + nprc)
     res = {"vlolim": tab[:ntr], "vhilim": tab[ntr:2 * ntr], "dmid": tab[2 * ntr:3 * ntr],
            "precpbins": tab[3 * ntr:]}
-    if mode == 3:
+    if mode == 4:
+        out = np.frombuffer(raw, np.float64, n * 10 * L.ncld, off).reshape(n, L.ncld, 10)
+        res["nact"] = out[:, :, 0].copy()
+        res["vact"] = out[:, :, 1:].copy()
+    elif mode == 3:
         out = np.frombuffer(raw, np.float64, n * 10 * L.nice, off).reshape(n, L.nice, 10)
         for q, key in enumerate(("dwet", "dnsp", "alpha", "beta", "gamma", "sigma", "zdn", "gg", "vc", "avis")):
             res[key] = out[:, :, q].copy()

@@ -37,6 +37,7 @@ static const double allDiss[9] = {3., 1., 0., 0., 2., 1., 1., 1., 1.};
 /* named pointers, classSpecies.f90:266-294 */
 static const double rhowa = 1000., rhoic = 917., rhori = 400.;
 static const double mwa = 18.016e-3;
+static const double cpa = 1005.;                    /* mo_submctl.f90:267 */
 static const double rhosu = 1830., msu = 98.08e-3, rhooc = 1400., moc = 150.e-3;
 static const double rhoss = 2165., mss = 58.44e-3;
 
@@ -1774,13 +1775,227 @@ static void act_interst(const so_layout *L, so_cell *c, double prv, double prs, 
 }
 
 /* ====================================================================== */
+/* Cloud-base activation (cloud_activation with lsactiv%mode 2,           */
+/* mo_salsa_cloud.f90:46-106; ActCloudBase :161-375 after Abdul-Razzak and */
+/* Ghan 2002, getSolute :113-157, activate3 :1199-1378): for an updraft w */
+/* > 0 the maximum supersaturation of the rising parcel, the fraction of  */
+/* each aerosol bin above its critical supersaturation, the critical dry  */
+/* volume in the first partly activated 1a/2a bin, and per cloud bin the  */
+/* number and volume of its parallel aerosol bin above that volume from   */
+/* piecewise linear number-density profiles.  The state is not changed:   */
+/* the result is the activated droplets (pactd) that run_SALSA hands to   */
+/* the LES as pa_nactd / pa_vactd (mo_salsa_driver.f90:360-366).          */
+/* Per-bin quantities are formed where they are used (getSolute's moles,  */
+/* the critical diameters) instead of being kept in arrays.               */
+/* ====================================================================== */
+/* moles of solute per particle of aerosol bin b (getSolute :136-151) */
+static double cb_solute(const so_layout *L, const so_cell *c, int b) {
+  const int a = L->iaero + b;
+  if (!(c->numc[a] > L->nlim[a])) return 0.;
+  double ns = 0.;
+  for (int nn = 0; nn < L->ndry; ++nn) ns = ns + L->diss[nn] * c->volc[a][nn] * L->rholiq[nn] / L->mm[nn];
+  return ns / c->numc[a];
+}
+
+/* ActCloudBase's per-bin test (:327-351): whether aerosol bin b is counted
+   (number at the limit and s_max above its upper-edge critical
+   supersaturation), with its activated fraction and critical diameters */
+static int cb_bin(const so_layout *L, const so_cell *c, int b, double x, double aa, double bb, double s_max,
+                  double *frac, double *dcrit, double *dcrlo, double *dcrhi) {
+  const int a = L->iaero + b;
+  *frac = 0.; *dcrit = 0.; *dcrlo = 0.; *dcrhi = 0.;
+  if (c->numc[a] < L->nlim[a]) return 0;
+  const double ns = cb_solute(L, c, b);
+  const double dm3 = L->dmid[a] * L->dmid[a] * L->dmid[a];
+  const double nshi = ns * (L->vhilim[a] / (pi6 * dm3));     /* vratiohi, mo_salsa_init.f90:186 */
+  const double sil = exp(sqrt(x / nshi)) - 1.;
+  if (s_max < sil) return 0;
+  const double nslo = ns * (L->vlolim[a] / (pi6 * dm3));     /* vratiolo, :187 */
+  const double siu = exp(sqrt(x / nslo)) - 1.;
+  *frac = fmin(1., log(s_max / sil) / log(siu / sil));
+  *dcrlo = 2. * sqrt(3. * nslo * bb / aa);
+  *dcrhi = 2. * sqrt(3. * nshi * bb / aa);
+  *dcrit = 2. * sqrt(3. * ns * bb / aa);
+  return 1;
+}
+
+/* nact[cb] (#/m3) and vact[cb][s] (m3/m3) of cloud bin cb; zero unless w > 0 */
+static void act_cloud_base(const so_layout *L, const so_cell *c, double temp, double pres, double w, double *nact,
+                           double (*vact)[SO_MAXSPEC]) {
+  const int A = L->iaero, ndry = L->ndry, nwet = L->nwet, iwa = L->iwa;
+  for (int cb = 0; cb < L->ncld; ++cb) {
+    nact[cb] = 0.;
+    for (int s = 0; s < SO_MAXSPEC; ++s) vact[cb][s] = 0.;
+  }
+  const double bb = 6. * mwa / (pi * rhowa);
+  if (w <= 0.) return;
+  const double aa = 4. * mwa * surfw0 / (rg * rhowa * temp);
+  const double x = 4. * (aa * aa * aa) / (27. * bb);
+  double ntot = 0., sum1 = 0.;
+  for (int b = L->in1a; b <= L->fn2b; ++b) ntot = ntot + c->numc[A + b];
+  for (int b = L->in1a; b <= L->fn2b; ++b) {
+    const double scrit = exp(sqrt(x / fmax(eps_r8, cb_solute(L, c, b)))) - 1.;
+    sum1 = sum1 + c->numc[A + b] / pow(scrit, 2. / 3.);
+  }
+  if (ntot < L->nlim[A]) return;
+  const double V = w;
+  const double Lh = 2.501e6 - 2370. * (temp - 273.15);
+  const double a1 = 1. - (373.15 / temp);
+  const double a12 = a1 * a1;
+  const double ps = 101325. * exp(13.3185 * a1 - 1.976 * a12 - 0.6445 * (a12 * a1) - 0.1299 * (a12 * a12));
+  const double alpha = grav * mwa * Lh / (cpa * rg * (temp * temp)) - grav * mair / (rg * temp);
+  const double gamma = rg * temp / (ps * mwa) + mwa * (Lh * Lh) / (cpa * pres * mair * temp);
+  const double x1 = pres / 101325.;
+  const double dv1 = 1.e-4 * (0.211 / x1) * pow(temp / 273., 1.94);
+  const double ka1 = 1.e-3 * (4.39 + 0.071 * temp);
+  const double Gc = 1. / (rhowa * rg * temp / (ps * dv1 * mwa) + Lh * rhowa / (ka1 * temp) * (Lh * mwa / (temp * rg) - 1.));
+  const double r = ntot / sum1;
+  const double s_eff = sqrt(r * r * r);
+  const double avg = alpha * V / Gc;
+  const double theta = (sqrt(avg * avg * avg)) / (2. * pi * rhowa * gamma * ntot);
+  const double khi = (2. / 3.) * aa * sqrt(avg);
+  const double kt = khi / theta;
+  const double s_max = s_eff / sqrt(0.5 * sqrt(kt * kt * kt) + pow((s_eff * s_eff) / (theta + 3. * khi), 0.75));
+  const double zdcstar = 2. * aa / (3. * s_max);
+  /* the first partly activated 1a/2a bin (:355-360), and whether any bin is counted */
+  int bcrita = L->fn2a, any = 0;
+  double f, d0, d1, d2;
+  for (int b = L->in1a; b <= L->fn2b; ++b)
+    if (cb_bin(L, c, b, x, aa, bb, s_max, &f, &d0, &d1, &d2) && d0 >= eps_r8) any = 1;
+  for (int b = L->in1a; b <= L->fn2a; ++b) {
+    cb_bin(L, c, b, x, aa, bb, s_max, &f, &d0, &d1, &d2);
+    if (f < 1. && f > 0.) { bcrita = b; break; }
+  }
+  /* activate3 (:1239-1377) */
+  if (!any) return;                              /* ALL(pdcrit < epsilon) */
+  double zvcstar = 0.;
+  {
+    const int a = A + bcrita;
+    double dcrit, dcrlo, dcrhi;
+    cb_bin(L, c, bcrita, x, aa, bb, s_max, &f, &dcrit, &dcrlo, &dcrhi);
+    double Vmid;
+    if (c->numc[a] < L->nlim[a]) {
+      Vmid = pi6 * (L->dmid[a] * L->dmid[a] * L->dmid[a]);
+    } else {
+      double sd = 0.;
+      for (int s = 0; s < ndry; ++s) sd = sd + c->volc[a][s];
+      Vmid = sd / fmax(L->nlim[a], c->numc[a]);
+    }
+    const double Vhi = L->vhilim[a], Vlo = L->vlolim[a];
+    const double cs3 = zdcstar * zdcstar * zdcstar, cr3 = dcrit * dcrit * dcrit;
+    if (zdcstar >= dcrit) {
+      const double zhlp = (pi6 * cs3 - pi6 * cr3) / fmax(eps_r8, pi6 * (dcrhi * dcrhi * dcrhi) - pi6 * cr3);
+      zvcstar = Vmid + zhlp * (Vhi - Vmid);
+    } else if (zdcstar < dcrit) {
+      const double zhlp = (pi6 * cr3 - pi6 * cs3) / fmax(eps_r8, pi6 * cr3 - pi6 * (dcrlo * dcrlo * dcrlo));
+      zvcstar = Vmid - zhlp * (Vmid - Vlo);
+    }
+    zvcstar = fmax(zvcstar, L->vlolim[a]);
+    zvcstar = fmin(zvcstar, L->vhilim[a]);
+  }
+  for (int cb = L->ica.cur; cb <= L->fcb.cur; ++cb) {
+    const int ab = cb <= L->fca.cur ? L->ica.par + (cb - L->ica.cur) : L->icb.par + (cb - L->icb.cur);
+    const int a = A + ab;
+    if (c->numc[a] < L->nlim[a]) continue;
+    const double Nmid = fmax(c->numc[a], L->nlim[a]);
+    double sd = 0.;
+    for (int s = 0; s < ndry; ++s) sd = sd + c->volc[a][s];
+    const double Vmid = sd / Nmid;
+    const double Vlo = L->vlolim[a], Vhi = L->vhilim[a];
+    double Nim1, Vim1, Vlom1, Vhim1, Nip1, Vip1, Vlop1, Vhip1;
+    if (ab == L->in1a || ab == L->in2b) {
+      Nim1 = L->nlim[a]; Vim1 = Vlo / 2.; Vlom1 = 0.; Vhim1 = Vlo;
+    } else {
+      Nim1 = fmax(c->numc[a - 1], L->nlim[a - 1]);
+      if (Nim1 > L->nlim[a - 1]) {
+        double s1 = 0.;
+        for (int s = 0; s < ndry; ++s) s1 = s1 + c->volc[a - 1][s];
+        Vim1 = s1 / Nim1;
+      } else {
+        Vim1 = pi6 * (L->dmid[a - 1] * L->dmid[a - 1] * L->dmid[a - 1]);
+      }
+      Vlom1 = L->vlolim[a - 1]; Vhim1 = L->vhilim[a - 1];
+    }
+    if (ab == L->fn2a || ab == L->fn2b) {
+      Nip1 = L->nlim[a]; Vip1 = Vhi + 0.5 * (Vhi - Vlo); Vlop1 = Vhi; Vhip1 = Vhi + (Vhi - Vlo);
+    } else {
+      Nip1 = fmax(c->numc[a + 1], L->nlim[a + 1]);
+      if (Nip1 > L->nlim[a + 1]) {
+        double s1 = 0.;
+        for (int s = 0; s < ndry; ++s) s1 = s1 + c->volc[a + 1][s];
+        Vip1 = s1 / Nip1;
+      } else {
+        Vip1 = pi6 * (L->dmid[a + 1] * L->dmid[a + 1] * L->dmid[a + 1]);
+      }
+      Vlop1 = L->vlolim[a + 1]; Vhip1 = L->vhilim[a + 1];
+    }
+    Vip1 = fmax(Vlop1, fmin(Vip1, Vhip1));
+    Vim1 = fmax(Vlom1, fmin(Vim1, Vhim1));
+    const double dNim1 = Nim1 / (Vhim1 - Vlom1), dNip1 = Nip1 / (Vhip1 - Vlop1), dNmid = Nmid / (Vhi - Vlo);
+    const double zs1 = (dNmid - dNim1) / (Vmid - Vim1), zs2 = (dNip1 - dNmid) / (Vip1 - Vmid);
+    const double N01 = dNmid - zs1 * Vmid, N02 = dNip1 - zs2 * Vip1;
+    const double Nnorm = intgN(zs1, N01, Vlo, Vmid) + intgN(zs2, N02, Vmid, Vhi);
+    const double Vnorm = intgV(zs1, N01, Vlo, Vmid) + intgV(zs2, N02, Vmid, Vhi);
+    double n = 0., zactvol = 0.;
+    if (zvcstar < Vmid) {
+      const double zvcint = fmax(zvcstar, Vlo);
+      n = (Nmid / Nnorm) * (intgN(zs1, N01, zvcint, Vmid) + intgN(zs2, N02, Vmid, Vhi));
+      zactvol = (Nmid * Vmid / Vnorm) * (intgV(zs1, N01, zvcint, Vmid) + intgV(zs2, N02, Vmid, Vhi));
+    } else if (zvcstar >= Vmid) {
+      const double zvcint = fmin(zvcstar, Vhi);
+      n = (Nmid / Nnorm) * (intgN(zs2, N02, zvcint, Vhi));
+      zactvol = (Nmid * Vmid / Vnorm) * (intgV(zs2, N02, zvcint, Vhi));
+    }
+    if (zvcstar < Vmid || zvcstar >= Vmid)
+      for (int s = 0; s < nwet; ++s) vact[cb][s] = zactvol * (c->volc[a][s] / (Vmid * Nmid));
+    nact[cb] = fmax(0., n);
+    for (int s = 0; s < nwet; ++s) vact[cb][s] = fmax(0., vact[cb][s]);
+    double dcrit;
+    cb_bin(L, c, ab, x, aa, bb, s_max, &f, &dcrit, &d1, &d2);
+    const double q = 3.e-6 / fmax(eps_r8, dcrit);
+    vact[cb][iwa] = nact[cb] * pi6 * (dcrit * dcrit * dcrit) * fmin(2., q * q);
+  }
+}
+
+/* the activated droplets of one cell in SALSA units (a test entry) */
+void so_act_cloud_base(const so_layout *L, double temp, double pres, double w, const double *numc, const double *volc,
+                       double *nact, double *vact) {
+  so_cell c;
+  memset(&c, 0, sizeof(c));
+  for (int b = 0; b < L->ntotal; ++b) {
+    c.numc[b] = numc[b];
+    for (int s = 0; s < SO_MAXSPEC; ++s) c.volc[b][s] = volc[b * SO_MAXSPEC + s];
+  }
+  act_cloud_base(L, &c, temp, pres, w, nact, (double (*)[SO_MAXSPEC])vact);
+}
+
+/* ====================================================================== */
 /* One cell: salsa() with coagulation and condensation enabled,           */
 /* src/src_salsa/mo_salsa.f90:92-107                                      */
 /* ====================================================================== */
+static int step_cell(const so_layout *L, const so_params *P, double ptemp, double ppres, double prs, double prsi,
+                     double *prv, double ptstep, double *numc, double *volc, so_rate *rates, double *kstore,
+                     int kmode, double w, double *nact, double *vact);
+
 int so_step_cell_ks(const so_layout *L, const so_params *P,
                     double ptemp, double ppres, double prs, double prsi,
                     double *prv, double ptstep,
                     double *numc, double *volc, so_rate *rates, double *kstore, int kmode) {
+  return step_cell(L, P, ptemp, ppres, prs, prsi, prv, ptstep, numc, volc, rates, kstore, kmode, 0., NULL, NULL);
+}
+
+/* A step with cloud-base activation (lsactiv%mode 2): w is the cell's
+   updraft; nact[ncld] and vact[ncld][SO_MAXSPEC] receive the activated
+   droplets in SALSA units (zero with another lsactiv). */
+int so_step_cell_act(const so_layout *L, const so_params *P, double ptemp, double ppres, double prs, double prsi,
+                     double *prv, double ptstep, double *numc, double *volc, so_rate *rates, double w, double *nact,
+                     double *vact) {
+  return step_cell(L, P, ptemp, ppres, prs, prsi, prv, ptstep, numc, volc, rates, 0, 0, w, nact, vact);
+}
+
+static int step_cell(const so_layout *L, const so_params *P, double ptemp, double ppres, double prs, double prsi,
+                     double *prv, double ptstep, double *numc, double *volc, so_rate *rates, double *kstore,
+                     int kmode, double w, double *nact, double *vact) {
   so_cell c;
   memset(&c, 0, sizeof(c));
   c.rates = rates;
@@ -1806,6 +2021,14 @@ int so_step_cell_ks(const so_layout *L, const so_params *P,
     status |= gpparth2o(L, P, &c, ptemp, ppres, prs, prsi, prv, ptstep);
   if (P->lsauto_state && P->lsauto_mode == 2) autoconv2(L, &c, ptstep);   /* mo_salsa.f90:110-112 */
   if (P->lsactiv == 1) act_interst(L, &c, *prv, prs, ptemp);                /* :115-119, mode 1 */
+  if (nact && vact) {                       /* cloud_activation resets pactd; mode 2 fills it */
+    if (P->lsactiv == 2) {
+      act_cloud_base(L, &c, ptemp, ppres, w, nact, (double (*)[SO_MAXSPEC])vact);
+    } else {
+      for (int i = 0; i < L->ncld * SO_MAXSPEC; ++i) vact[i] = 0.;
+      for (int i = 0; i < L->ncld; ++i) nact[i] = 0.;
+    }
+  }
   if (P->lsdistupdate) status |= distr_update(L, &c, P->level);
   for (int b = 0; b < L->ntotal; ++b) {
     numc[b] = c.numc[b];

@@ -203,6 +203,11 @@ int so_sedim5(const so_layout *L, const so_params *P, int nzp, int nxp, int nyp,
               double *const m_p[4], double *const n_t[4], double *const m_t[4], double *tlt, double *rrate,
               double *sfcrrate, double *irate, double *sfcirate, double *vt_prc, double *vt_ice, int sed_aero,
               int sed_cloud, int sed_precp, int sed_ice);
+int so_step_cell_act(const so_layout *L, const so_params *P, double ptemp, double ppres, double prs, double prsi,
+                     double *prv, double ptstep, double *numc, double *volc, so_rate *rates, double w, double *nact,
+                     double *vact);
+void so_act_cloud_base(const so_layout *L, double temp, double pres, double w, const double *numc, const double *volc,
+                       double *nact, double *vact);
 void so_sed_ice_vc(const so_layout *L, const so_params *P, double tk, double adn, double numc, const double *m,
                    double *out);
 

@@ -202,7 +202,34 @@ def ice_fall_cases():
     print("wrote ice_fall_cfg5.npz", n * ni, "bins")
 
 
+def cloud_base_cases():
+    """Cloud-base activation (cloud_activation with lsactiv%mode 2, harness
+    mode 4) of config-1 and config-4 cells with updrafts w from -0.5 to 8
+    m/s (w <= 0 activates nothing), and of SO4 + dust cells at level 5
+    (written separately: `--only cloudbase`)."""
+    out = {}
+    for name, L, cfg, level, seed in (("l4", O.make_layout(), 1, 4, 61), ("l4b", O.make_layout(), 4, 4, 62),
+                                      ("l5", O.make_layout(species=(0, 3)), 5, 5, 63)):
+        P = O.make_params(level=level)
+        f = W.to_salsa_units(L, W.config_fields(L, cfg, grid_override=(20, 8, 8) if cfg != 1 else None),
+                             level=level)
+        nc, vc = pad(L, f[5], f[6])
+        rng = np.random.default_rng(seed)
+        sel = np.sort(rng.choice(len(f[0]), 100, replace=False))
+        w = rng.uniform(-0.5, 8., len(sel))
+        w[:8] = [-0.5, 0., 0.05, 0.2, 1., 3., 6., 8.]
+        R = refio.run(L, P, level, f[0][sel], f[1][sel], f[2][sel], w, f[4][sel], nc[sel], vc[sel], mode=4)
+        out.update({name + "_T": f[0][sel], name + "_p": f[1][sel], name + "_w": w, name + "_numc": nc[sel],
+                    name + "_volc": vc[sel][:, :, :L.nspec], name + "_nact": R["nact"],
+                    name + "_vact": R["vact"][:, :, :L.nspec]})
+        print(name, "activated cells:", int((R["nact"].sum(axis=1) > 0).sum()), "of", len(sel))
+    np.savez_compressed(os.path.join(HERE, "act_cloudbase.npz"), **out)
+    print("wrote act_cloudbase.npz")
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "cloudbase":
+        return cloud_base_cases()
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "icefall":
         return ice_fall_cases()
     if len(sys.argv) > 2 and sys.argv[1] == "--only" and sys.argv[2] == "init":

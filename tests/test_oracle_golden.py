@@ -205,3 +205,28 @@ def test_oracle_init_call_matches_reference(oracle_mod, name):
                                        (sens[0], sens[1], sens[2][:, :, :ns]), 1e-12, ns, name)
     print(name, "max rel", out, "elements needing the conditioning allowance:", nflag)
     assert nflag <= 0.002 * nc.size
+
+
+@pytest.mark.parametrize("name", ["l4", "l4b", "l5"])
+def test_oracle_cloud_base_activation_matches_reference(oracle_mod, name):
+    """Cloud-base activation (cloud_activation with lsactiv%mode 2: ActCloudBase,
+    getSolute, activate3; mo_salsa_cloud.f90:161-375, 1199-1378) against the
+    compiled reference on config-1, config-4 and level-5 cells with updrafts
+    from -0.5 to 8 m/s (tests/golden/act_cloudbase.npz): the activated number
+    and every volume per cloud bin to the last bit; cells without updraft
+    activate nothing."""
+    O = oracle_mod
+    d = golden("act_cloudbase.npz")
+    L = O.make_layout(species=(0, 3)) if name == "l5" else O.make_layout()
+    T, p, w = d[name + "_T"], d[name + "_p"], d[name + "_w"]
+    active = 0
+    for c in range(len(T)):
+        vc = np.zeros((L.ntotal, 9))
+        vc[:, :L.nspec] = d[name + "_volc"][c]
+        na, va = O.act_cloud_base(L, T[c], p[c], w[c], d[name + "_numc"][c], vc)
+        np.testing.assert_array_equal(na, d[name + "_nact"][c])
+        np.testing.assert_array_equal(va[:, :L.nspec], d[name + "_vact"][c])
+        if w[c] <= 0.:
+            assert not na.any() and not va.any()
+        active += int(na.sum() > 0)
+    assert active > 0.8 * len(T)

@@ -26,6 +26,7 @@ int launch_gather(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, d
 int64_t gather_tiles(int64_t ncells);
 int64_t class_work_ints(int64_t ncells);
 int launch_scatter(const DevLayout &Lay, const DevLayout *d, const StepArgs &a, hipStream_t s);
+int launch_act_cloud_base(const DevLayout *d, const StepArgs &a, hipStream_t s);
 int cap_bins(int cap);
 int cap_spec(int cap);
 int launch_gradient(int cap, int precision, const DevLayout *d, const GradArgs &g, double *sum_out,
@@ -83,6 +84,11 @@ struct salsa_ctx {
   // adjoint, per-cell J and dJ/dE, totals and the per-block tape
   double *adj_buf = nullptr;
   size_t adj_bytes = 0;
+  // cloud-base activation (lsactiv 2): the updraft and the activated
+  // droplets, device arrays of the caller (salsa_set_cloud_base_fields)
+  const double *act_w = nullptr;
+  double *act_n = nullptr, *act_v = nullptr;
+  double *act_state = nullptr;        // [ncells][nbins*(1+nwet)]: the aerosol state it sees
   // salsa_sedim_device: fall speeds of one slow category (aerosol or cloud)
   double *sed_work = nullptr;
   // salsa_les_update_device: the cells the removals visit ([n3]) and their number
@@ -167,6 +173,7 @@ static void release(salsa_ctx *c) {
   (void)hipFree(c->adj_buf);
   (void)hipFree(c->sed_work);
   (void)hipFree(c->les_list);
+  (void)hipFree(c->act_state);
   if (c->cls_count_h) (void)hipHostFree(c->cls_count_h);
   if (c->tot_h) (void)hipHostFree(c->tot_h);
   if (c->fault_h) (void)hipHostFree(c->fault_h);
@@ -192,8 +199,8 @@ extern "C" int salsa_ctx_create(const salsa_bin_desc *desc, const salsa_params *
     set_error("lscoag_mode must be 1 or 2");
     return SALSA_ERR_ARG;
   }
-  if (params->lsactiv != 0 && params->lsactiv != 1) {
-    set_error("lsactiv must be 0 or 1 (interstitial activation; cloud-base activation is not provided)");
+  if (params->lsactiv < 0 || params->lsactiv > 2) {
+    set_error("lsactiv must be 0, 1 (interstitial activation) or 2 (cloud-base activation)");
     return SALSA_ERR_ARG;
   }
   if (params->lscoag_mode == 2 && !(params->cgintvl > 0.)) {
@@ -332,6 +339,12 @@ static StepArgs make_args(salsa_ctx *c, const double *press, const double *temp,
   a.rec_out = c->rec_out;
   a.cell_flags = c->flags;
   a.fault_any = c->fault;
+  if (c->hostL.lsactiv == 2 && c->act_n) {
+    a.w = c->act_w;
+    a.nactd = c->act_n;
+    a.vactd = c->act_v;
+    a.act_state = c->act_state;
+  }
   return a;
 }
 
@@ -343,6 +356,10 @@ static int check_ptrs(salsa_ctx *c, const double *press, const double *temp, con
   if (!(tstep > 0.)) { set_error("tstep must be positive"); return SALSA_ERR_ARG; }
   if (!press || !temp || !rsl || !dn || !rv_p || !n_p || !m_p) { set_error("null field pointer"); return SALSA_ERR_ARG; }
   if (c->hostL.level == 5 && !rsi) { set_error("rsi required at level 5"); return SALSA_ERR_ARG; }
+  if (c->hostL.lsactiv == 2 && (!c->act_w || !c->act_n || !c->act_v)) {
+    set_error("cloud-base activation (lsactiv 2) needs w, nactd and vactd (salsa_set_cloud_base_fields)");
+    return SALSA_ERR_ARG;
+  }
   const bool inplace = flags & SALSA_STEP_INPLACE;
   if (!inplace && (!rv_t || !n_t || !m_t)) { set_error("null tendency pointer"); return SALSA_ERR_ARG; }
   for (int k = 0; k < c->hostL.ncat; ++k) {
@@ -419,6 +436,11 @@ static int enqueue_step(salsa_ctx *c, StepArgs a, int32_t flags) {
     cells[k] = b.ncells;
     MARK();
   }
+  // cloud-base activation from the aerosol state the class kernels left
+  if (a.act_state) {
+    rc = launch_act_cloud_base(c->devL, a, s);
+    if (rc) { std::snprintf(g_err, sizeof(g_err), "activation launch failed (%d)", rc); return SALSA_ERR_HIP; }
+  }
   // cell records -> tendencies or new state
   MARK();
   rc = launch_scatter(c->hostL, c->devL, a, s);
@@ -486,6 +508,24 @@ extern "C" int salsa_step_device(salsa_ctx *c, const double *press, const double
   StepArgs a = make_args(c, press, temp, rsl, rsi, dn, rv_p, rv_t, n_p, n_t, m_p, m_t, rates, tstep, flags, time,
                          istp);
   return enqueue_step(c, a, flags);
+}
+
+extern "C" int salsa_set_cloud_base_fields(salsa_ctx *c, const double *w, double *nactd, double *vactd) {
+  if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  if ((w == nullptr) != (nactd == nullptr) || (w == nullptr) != (vactd == nullptr)) {
+    set_error("w, nactd and vactd are set together (or all null)");
+    return SALSA_ERR_ARG;
+  }
+  if (nactd && !c->act_state &&
+      hipMalloc(&c->act_state, (size_t)c->ncells * c->hostL.nb[0] * (1 + c->hostL.nwet) * sizeof(double)) !=
+          hipSuccess) {
+    set_error("activation work allocation failed");
+    return SALSA_ERR_NOMEM;
+  }
+  c->act_w = w;
+  c->act_n = nactd;
+  c->act_v = vactd;
+  return SALSA_OK;
 }
 
 extern "C" int salsa_ctx_sync(salsa_ctx *c) {
@@ -655,9 +695,9 @@ extern "C" int salsa_adjoint_device(salsa_ctx *c, const double *press, const dou
   if (rc) return rc;
   const DevLayout &L = c->hostL;
   if (every < 1) { set_error("checkpoint_every must be at least 1"); return SALSA_ERR_ARG; }
-  if (L.level != 4 || L.autoconv2 || L.lsactiv || L.lsdistupdate) {
-    set_error("the adjoint covers level-4 coagulation and condensation (no autoconv2, activation or "
-              "size-distribution update)");
+  if (L.level != 4 || L.autoconv2 || L.lsactiv == 1 || L.lsdistupdate) {
+    set_error("the adjoint covers level-4 coagulation and condensation (no autoconv2, interstitial activation "
+              "or size-distribution update)");
     return SALSA_ERR_ARG;
   }
   bool fits = L.nt <= 32 && L.nwet <= 3 && L.npair <= 528 && L.nb[2] <= 8 && L.ncc <= 528 &&

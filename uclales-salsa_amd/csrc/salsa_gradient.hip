@@ -552,7 +552,7 @@ __device__ uint32_t gradient_cell(const DevLayout &D, const GradArgs &g, int64_t
       // ---- autoconversion (autoconv2), cloud activation and size-distribution
       //      update (distr_update), mo_salsa.f90:110-119, 148-149 ----
       if (D.autoconv2) autoconv2_cell<DD>(D, N, &V[0][0], NS, dt);           // mo_salsa.f90:110-112
-      if (D.lsactiv) act_interst_cell<DD>(D, N, &V[0][0], NS, rv, rs, T, tmpd);   // :115-119
+      if (D.lsactiv == 1) act_interst_cell<DD>(D, N, &V[0][0], NS, rv, rs, T, tmpd);   // :115-119
       if (D.lsdistupdate) flags |= distr_update_cell<DD>(D, N, &V[0][0], NS, tmpd);
 
       // ---- the new state through LES units, as the next run_SALSA call sees it ----

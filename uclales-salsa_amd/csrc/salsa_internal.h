@@ -115,7 +115,7 @@ struct DevLayout {
   int32_t level, lscoag, lscnd, lscndh2oae, lscndh2ocl, lscndh2oic;
   int32_t lsdistupdate;
   int32_t autoconv2;                  // lsauto%state .AND. lsauto%mode == 2 (mo_salsa.f90:110-112)
-  int32_t lsactiv;                    // 1: interstitial cloud activation (mo_salsa.f90:115-119)
+  int32_t lsactiv;                    // cloud activation (mo_salsa.f90:115-119): 1 interstitial, 2 cloud base
   int32_t any_ice_coag;               // any of lscgii/lscgia/lscgic/lscgip
   double eddy_dis, Eiagg_max, Eiagg_min;
   double ice_alpha, ice_beta, ice_gamma, ice_sigma;
@@ -168,6 +168,9 @@ struct StepArgs {
   double rhlim;
   uint8_t *cell_flags;                // one byte per interior cell, or null
   int32_t *fault_any;                 // OR of all cell flags
+  const double *w;                    // cloud-base activation: updraft [n3],
+  double *nactd, *vactd;              //   the activated droplets [ncld][n3], [nwet*ncld][n3],
+  double *act_state;                  //   the aerosol state it sees [ncells][nbins*(1+nwet)], or null
 };
 
 // Arguments of the forward-mode derivative kernel: the trajectory starts from

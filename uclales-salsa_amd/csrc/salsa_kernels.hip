@@ -969,9 +969,23 @@ k_salsa_step(const DevLayout *__restrict__ Dp, StepArgs a, SmemGeom G) {
     __syncthreads();
   }
 
-  // ---- 5a2. interstitial cloud activation (cloud_activation, lsactiv%mode 1,
-  //      mo_salsa.f90:115-119): serial over the 2a/2b bins on one lane ----
-  if (D.lsactiv) {
+  // ---- 5a2. cloud activation (cloud_activation, mo_salsa.f90:115-119):
+  //      interstitial (lsactiv%mode 1), serial over the 2a/2b bins on one
+  //      lane; cloud base (mode 2), which leaves the state as it is and writes
+  //      the activated droplets to pa_nactd / pa_vactd, on one lane ----
+  if (D.lsactiv == 2 && a.act_state && live) {
+    // the aerosol state the activation sees, to k_act_cloud_base
+    double *st = a.act_state + (int64_t)icell * D.nb[0] * (1 + D.nwet);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = sub + q * L;
+      if (b < D.nb[0]) {
+        st[b] = sN[b];
+        for (int s = 0; s < D.nwet; ++s) st[D.nb[0] + b * D.nwet + s] = VV(b, s);
+      }
+    }
+  }
+  if (D.lsactiv == 1) {
     if (sub == 0 && live) {
       const double na = act_interst_cell<double>(D, sN, sV, nsV, rv, rs, T, reinterpret_cast<double *>(cb + G.opr));
       if (R) rr[RA_ACTIV] += na;
@@ -1397,6 +1411,27 @@ int launch_step(int cap, int lanes, int precision, bool hyd, const DevLayout *d,
   SALSA_CASE(5, 40, 4)
 #undef SALSA_CASE
   return -1;
+}
+
+// Cloud-base activation (lsactiv 2) of every interior cell from the aerosol
+// state the step kernel left in act_state (the state after condensation and
+// autoconv2, which cloud_activation sees, mo_salsa.f90:110-119): one thread
+// per cell, writing pa_nactd / pa_vactd.
+__global__ void __launch_bounds__(256) k_act_cloud_base(const DevLayout *__restrict__ dl, StepArgs a) {
+  const DevLayout &D = *dl;
+  const int64_t icell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (icell >= a.ncells) return;
+  const double *st = a.act_state + icell * D.nb[0] * (1 + D.nwet);
+  const double *rec = a.rec_in + icell * D.nslot;
+  const int64_t li = les_index(a, icell);
+  act_cloud_base_cell(D, st, st + D.nb[0], D.nwet, rec[0], rec[1], a.w[li], rec[4], a.nactd + li, a.vactd + li,
+                      a.n3);
+}
+
+int launch_act_cloud_base(const DevLayout *d, const StepArgs &a, hipStream_t s) {
+  if (a.ncells <= 0) return 0;
+  hipLaunchKernelGGL(k_act_cloud_base, dim3((unsigned)((a.ncells + 255) / 256)), dim3(256), 0, s, d, a);
+  return (int)hipGetLastError();
 }
 
 int cap_bins(int cap) { return cap == 4 ? 32 : 40; }

@@ -168,6 +168,12 @@ MODULE mo_salsa_amd_cabi
        INTEGER(C_INT32_T), VALUE  :: sed_aero, sed_cloud, sed_precp, sed_ice
      END FUNCTION salsa_sedim_device
 
+     INTEGER(C_INT) FUNCTION salsa_set_cloud_base_fields(ctx, w, nactd, vactd) &
+                                                         BIND(C, name="salsa_set_cloud_base_fields")
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, w, nactd, vactd         ! device arrays: w, pa_nactd, pa_vactd
+     END FUNCTION salsa_set_cloud_base_fields
+
      INTEGER(C_INT) FUNCTION salsa_ctx_sync(ctx) BIND(C, name="salsa_ctx_sync")
        IMPORT :: C_INT, C_PTR
        TYPE(C_PTR), VALUE :: ctx

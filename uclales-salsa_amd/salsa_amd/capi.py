@@ -45,6 +45,7 @@ EXPORTS = [
     "salsa_set_launch_shape", "salsa_get_launch_shape", "salsa_autotune", "salsa_ctx_ncells",
     "salsa_ctx_ntotal", "salsa_ctx_stream", "salsa_last_error", "salsa_ctx_timing",
     "salsa_ctx_kernel_time", "salsa_ctx_phase_time", "salsa_gradient_device_noalloc", "salsa_adjoint_device",
+    "salsa_set_cloud_base_fields",
 ]
 
 NOALLOC_MAX_CELLS = 1 << 20
@@ -181,6 +182,8 @@ def lib():
     L.salsa_sedim_device.argtypes = [C.c_void_p, C.c_double, _dp] + [C.c_void_p] * 4 + [_P4] * 4 + \
         [C.c_void_p] * 7 + [C.c_int32] * 4
     L.salsa_sedim_device.restype = C.c_int
+    L.salsa_set_cloud_base_fields.argtypes = [C.c_void_p] * 4
+    L.salsa_set_cloud_base_fields.restype = C.c_int
     L.salsa_last_error.argtypes = []
     L.salsa_last_error.restype = C.c_char_p
     _lib = L
@@ -369,6 +372,14 @@ class Context:
             _ptr(f["ustar"]), P(n_p[:4]), P(m_p[:4]), P(n_t[:4]), P(m_t[:4]), _ptr(tlt), _ptr(rrate),
             _ptr(sfcrrate), _ptr(vt_prc), _ptr(irate), _ptr(sfcirate), _ptr(vt_ice), *sw))
         _check(rc)
+
+    def set_cloud_base_fields(self, w, nactd, vactd):
+        """Cloud-base activation (lsactiv = 2): the updraft w [n3] and the
+        output fields nactd [ncld, n3] and vactd [nwet*ncld, n3] (device
+        tensors) used by the following steps (salsa_set_cloud_base_fields).
+        None for all three clears them."""
+        self._keep_cb = (w, nactd, vactd)
+        _check(lib().salsa_set_cloud_base_fields(self._h, _ptr(w), _ptr(nactd), _ptr(vactd)))
 
     def cell_flags(self):
         out = np.zeros(self.ncells, dtype=np.uint8)
