@@ -342,12 +342,10 @@ def sedim_leg(capi, desc, f, fd, n_p, m_p, calls=20):
     """Level-4 sedimentation on device (salsa_sedim_device) on the same grid,
     with 10 drops/kg per precipitation bin at its mid diameter in every
     cell (the workload carries no precipitation), timed with HIP events on
-    the context stream.  Algorithmic bytes per call: aerosol and cloud
-    elements read with their tendency read and written (24 B), precipitation
-    elements read, their tendency read and written and their change staged
-    and read back (40 B), fall speeds written (8 B per precipitation bin and
-    cell), per cell tk, th, adn read, tlt read and written, rrate written
-    (48 B)."""
+    the context stream.  Algorithmic bytes per call: every aerosol, cloud and
+    precipitation element read with its tendency read and written (24 B),
+    fall speeds written (8 B per precipitation bin and cell), per cell tk,
+    th, adn read, tlt read and written, rrate written (48 B)."""
     import numpy as np
     import torch
     g = f.grid
@@ -387,7 +385,7 @@ def sedim_leg(capi, desc, f, fd, n_p, m_p, calls=20):
     ctx.close()
     slow = sum(x.numel() for x in np_[:2] + mp_[:2])
     prec = np_[2].numel() + mp_[2].numel()
-    nbytes = 24 * slow + 40 * prec + 8 * nb * n3 + 48 * n3
+    nbytes = 24 * slow + 24 * prec + 8 * nb * n3 + 48 * n3
     gbs = nbytes / (ms * 1e-3) / 1e9
     return {"ms_per_call": ms, "cells": n3, "bytes_per_call": nbytes, "achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS,
             "frac": gbs / HBM_PEAK_GBS, "bound": "hbm",
