@@ -1,9 +1,11 @@
 """salsa_sedim_device (sedimentation: DepositionSlow for aerosols and cloud
 droplets, DepositionFast for precipitation and, at level 5, ice; mcrp.f90:
 745-1275) against the oracle's so_sedim / so_sedim5 on the same fields.  The fall speeds go through pow,
-exp and log, whose device and host results may differ in the last bit, and
-the tendencies are differences of level fluxes: the bar is 1e-10 of each
-quantity's scale in its column (the fields themselves for the tendencies).
+exp, log and cube roots, whose device and host results may differ in the
+last bit (the device forms some powers as products and a Halley-refined
+cube root), and the tendencies are differences of level fluxes: the bar is
+1e-10 of each quantity's scale in its column (the fields themselves for the
+tendencies).
 Fall-distance decisions compare such speeds with level depths; none of the
 cases here sits within rounding of a decision."""
 import numpy as np

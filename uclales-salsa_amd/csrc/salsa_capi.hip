@@ -89,8 +89,6 @@ struct salsa_ctx {
   const double *act_w = nullptr;
   double *act_n = nullptr, *act_v = nullptr;
   double *act_state = nullptr;        // [ncells][nbins*(1+nwet)]: the aerosol state it sees
-  // salsa_sedim_device: fall speeds of one slow category (aerosol or cloud)
-  double *sed_work = nullptr;
   // salsa_les_update_device: the cells the removals visit ([n3]) and their number
   int32_t *les_list = nullptr;
   // staging for salsa_step_host
@@ -171,7 +169,6 @@ static void release(salsa_ctx *c) {
   (void)hipFree(c->stage);
   (void)hipFree(c->gbuf);
   (void)hipFree(c->adj_buf);
-  (void)hipFree(c->sed_work);
   (void)hipFree(c->les_list);
   (void)hipFree(c->act_state);
   if (c->cls_count_h) (void)hipHostFree(c->cls_count_h);
@@ -1017,13 +1014,10 @@ extern "C" int salsa_sedim_device(salsa_ctx *c, double dtlt, const double *dzt, 
     if (!(dzt[k] > 0.)) { set_error("dzt must be positive"); return SALSA_ERR_ARG; }
     a.dzt[k] = dzt[k];
   }
-  const int nbslow = d.nbins > d.ncld ? d.nbins : d.ncld;
-  const size_t need = (size_t)((int64_t)nbslow * c->n3) * sizeof(double);
-  if (!c->sed_work && hipMalloc(&c->sed_work, need) != hipSuccess) {
-    set_error("sedimentation work allocation failed");
-    return SALSA_ERR_NOMEM;
+  if ((size_t)(d.nbins > d.ncld ? d.nbins : d.ncld) * c->nzp * sizeof(double) > 65536) {
+    set_error("too many aerosol or cloud bins for this many levels (bins x nzp <= 8192)");
+    return SALSA_ERR_ARG;
   }
-  a.vslow = c->sed_work;
   a.tk = tk; a.th = th; a.adn = adn; a.ustar = ustar;
   for (int k = 0; k < 3 + l5; ++k) { a.n_p[k] = n_p[k]; a.m_p[k] = m_p[k]; a.n_t[k] = n_t[k]; a.m_t[k] = m_t[k]; }
   a.tlt = tlt; a.rrate = rrate; a.sfcrrate = sfcrrate; a.vt_prc = vt_prc;

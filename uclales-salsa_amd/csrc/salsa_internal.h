@@ -260,7 +260,6 @@ struct SedArgs {
   double *n_t[4], *m_t[4];
   double *tlt, *rrate, *sfcrrate, *vt_prc;
   double *irate, *sfcirate, *vt_ice;  // level 5
-  double *vslow;                      // work: fall speeds of one slow category [nb][n3]
 };
 
 }  // namespace salsa

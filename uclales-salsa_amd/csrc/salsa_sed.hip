@@ -4,16 +4,15 @@
 //
 //   k_sed_zero   rrate, sfcrrate, VtPrc (level 5: irate, sfcirate, VtIce) to
 //                zero (sedim_SALSA :792-795, DepositionFast :1127)
-//   k_sed_vslow  DepositionSlow's (:926-1069) fall speeds of one category per
-//                (cell, bin), with the dry deposition velocity at the lowest
-//                level, into a work array
-//   k_sed_slow   one thread per interior cell: the bin fluxes at the cell's
-//                level and the level above, their divergence into the
-//                tendencies, the
-//                water divergence into tlt (:810-816, :833-839), the rain
-//                rate and, at the lowest level, the dry deposition velocity
-//                and the surface rate.  One launch per category, so the
-//                accumulation into rrate / tlt keeps the reference's order.
+//   k_sed_slow   DepositionSlow (:926-1069) of one category, one workgroup
+//                per interior column and one lane per level: the bins' fall
+//                speeds (with the dry deposition velocity at the lowest
+//                level) into LDS, then the bin fluxes at each level and the
+//                level above, their divergence into the tendencies, the water
+//                divergence into tlt (:810-816, :833-839), the rain rate and
+//                at the lowest level the surface rate.  One launch per
+//                category, so the accumulation into rrate / tlt keeps the
+//                reference's order.
 //   k_sed_prc    DepositionFast (:1073-1275) and its part of sedim_SALSA
 //                (:862-878, :884-904) for precipitation (flag 3) or ice
 //                (flag 4), one workgroup per interior column and one lane
@@ -24,7 +23,9 @@
 //                the surface rate from the deposited mass.
 // Terminal velocities: terminal_vel (mo_particle_external_properties.f90:
 // 43-100), liquid and ice branches.  Contraction is off so that the products and
-// sums round as the oracle's.
+// sums round as the oracle's; the reference's x**(1./3.), x**1.5 and x**3. are
+// formed as a cube root (FP32 seed and one Halley step), x*sqrt(x) and x*x*x,
+// which differ from pow in the last bit only.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -45,7 +46,7 @@ constexpr double kPiFull = 3.14159265358979323846, kKb = 1.3807e-23, kG = 9.81, 
 __device__ double sd_terminal_vel(double diam, double rhop, double rhoa, double visc, double beta) {
   if (diam <= 19e-6) return (diam * diam) * (rhop - rhoa) * kGrav * beta / (18. * visc);
   if (diam > 19e-6 && diam < 1.07e-3) {
-    const double NDa = 4. / 3. * ::pow(diam, 3.) * kGrav * (rhop - rhoa) * rhoa / ::pow(visc, 2.);
+    const double NDa = 4. / 3. * (diam * diam * diam) * kGrav * (rhop - rhoa) * rhoa / (visc * visc);
     const double X = ::log(NDa);
     const double X2 = X * X, X3 = X2 * X, X4 = X2 * X2, X5 = X4 * X, X6 = X3 * X3;
     const double Y = -0.318657e1 + 0.992696 * X - 0.153193e-2 * X2 - 0.987059e-3 * X3 - 0.578878e-3 * X4 +
@@ -66,7 +67,8 @@ __device__ double sd_terminal_vel(double diam, double rhop, double rhoa, double 
 struct Air { double avis, lambda; };
 __device__ __forceinline__ Air sd_air(double tk, double adn) {
   Air a;
-  a.avis = 1.8325e-5 * (416.16 / (tk + 120.0)) * ::pow(tk / 296.16, 1.5);
+  const double tr = tk / 296.16;
+  a.avis = 1.8325e-5 * (416.16 / (tk + 120.0)) * (tr * ::sqrt(tr));
   const double va = ::sqrt(8 * kKb * tk / (kPiFull * 4.8096e-26));
   a.lambda = 2 * a.avis / (adn * va);
   return a;
@@ -76,7 +78,7 @@ __device__ __forceinline__ Air sd_air(double tk, double adn) {
 // from the bin's volume v = sum m_s / rho_s and mass ms = sum m_s
 __device__ __forceinline__ double sd_vc_sums(double zpn, double v, double ms, double adn, const Air &air,
                                              double *dwet_out, double *gg_out) {
-  const double dwet = zpn < 1.e-15 ? 0. : ::pow(v / zpn / kPi6, 1. / 3.);
+  const double dwet = zpn < 1.e-15 ? 0. : cbrt_fast(v / zpn / kPi6);
   const double zdn = ms / zpn / (kPi6 * (dwet * dwet * dwet));
   const double Kn = 2. * air.lambda / dwet;
   const double GG = 1. + Kn * (1.249 + 0.42 * ::exp(-0.87 / Kn));
@@ -111,71 +113,84 @@ __global__ void __launch_bounds__(256) k_sed_zero(SedArgs a) {
   }
 }
 
-// DepositionSlow fall speeds of category cat per (interior cell, bin), with
-// the dry deposition velocity at the lowest level; 0 below the number limit
-__global__ void __launch_bounds__(128) k_sed_vslow(SedArgs a, int cat) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (x >= a.n3) return;
-  const int k = (int)(x % a.nzp);
-  const int64_t col = x / a.nzp;
-  const int i = (int)(col % a.nxp), j = (int)(col / a.nxp);
-  if (k < 1 || k > a.nzp - 2 || !sd_interior(a, i, j)) return;
-  const int64_t n3 = a.n3;
-  const double zpn = a.n_p[cat][(int64_t)b * n3 + x];
-  double vc = 0.;
-  if (!(zpn < kNlim)) {
-    const double ad = a.adn[x];
-    const Air ar = sd_air(a.tk[x], ad);
-    double dwet, GG;
-    vc = sd_vc(a, zpn, a.m_p[cat] + (int64_t)b * n3 + x, (int64_t)a.nb[cat] * n3, ad, ar, &dwet, &GG);
-    vc = fmin(vc, fmin(0.5 * (1. / a.dzt[k]) / a.dtlt, 2.0));
-    if (k == 1) {                                 // the level just above the surface
-      const double ust = a.ustar[col];
-      const double mdiff = (kKb * a.tk[x] * GG) / (3.0 * kPiFull * ar.avis * dwet);
-      const double Sc = (ar.avis / ad) / mdiff;
-      double St = vc * ::pow(ust, 2.0) / kG * (ar.avis / ad);
-      if (St < 0.01) St = 0.01;
-      const double rt = 1.0 / fmax(kEpsR8, (ust * (::pow(Sc, -2.0 / 3.0) + ::pow(10., -3.0 / St))));
-      vc = (1. / rt) + vc;
-    }
-  }
-  a.vslow[(int64_t)b * n3 + x] = vc;
-}
-
-// DepositionSlow for category cat at one interior cell: flux divergence from
-// the fall speeds of the cell's level and the level above
-__global__ void __launch_bounds__(128) k_sed_slow(SedArgs a, int cat) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= a.n3) return;
-  const int k = (int)(x % a.nzp);
-  const int64_t col = x / a.nzp;
-  const int i = (int)(col % a.nxp), j = (int)(col / a.nxp);
-  if (k < 1 || k > a.nzp - 2 || !sd_interior(a, i, j)) return;
+// DepositionSlow (:926-1069) for category cat (0 aerosol, 1 cloud) on one
+// interior column: one workgroup per column, one lane per level.  First every
+// interior level evaluates the fall speed of each of its bins (0 below the
+// number limit; capped at half a level per step and 2 m/s; at the lowest
+// level plus the dry deposition velocity) into LDS, vs[b][k]; then, bin by
+// bin in order, each level forms the fluxes at its level and the level above
+// (:1036-1049; the neighbour's rows come from the same cache lines) and
+// their divergence into the tendencies, the water divergence into tlt
+// (:810-816, :833-839) and the rain rate over the flux entries
+// (ns-1)*nb .. ns*nb (1-based, one entry early, as written), at the lowest
+// level also the surface rate.  One launch per category keeps the
+// reference's category order of the rrate / tlt accumulation.
+template <int MS>
+__global__ void __launch_bounds__(kSedMaxZ) k_sed_slow(SedArgs a, int cat) {
+  extern __shared__ double vs[];                // [nb][nzp]
+  const int nx = a.nxp - 4;
+  const int i = 2 + (int)blockIdx.x % nx, j = 2 + (int)blockIdx.x / nx;
+  const int64_t col = (int64_t)i + (int64_t)a.nxp * j, n3 = a.n3;
+  const int k = threadIdx.x, nzp = a.nzp;
   const int nb = a.nb[cat], ns = a.ns, iw = a.iwa;
-  const int64_t n3 = a.n3;
+  const bool src = k >= 1 && k <= nzp - 2;
+  const int64_t x = col * nzp + k;
   const double *__restrict__ N = a.n_p[cat];
   const double *__restrict__ M = a.m_p[cat];
-  const double *__restrict__ V = a.vslow;
-  const double dz = a.dzt[k];
+  if (src) {
+    const double ad = a.adn[x];
+    const Air ar = sd_air(a.tk[x], ad);
+    const double cap = fmin(0.5 * (1. / a.dzt[k]) / a.dtlt, 2.0);
+    for (int b = 0; b < nb; ++b) {
+      const double zpn = N[(int64_t)b * n3 + x];
+      double vc = 0.;
+      if (!(zpn < kNlim)) {
+        double v = 0., ms = 0., m[MS];
+#pragma unroll
+        for (int s = 0; s < MS; ++s) m[s] = s < ns ? M[((int64_t)s * nb + b) * n3 + x] : 0.;
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+          if (s < ns) v = v + m[s] / a.rholiq[s];
+#pragma unroll
+        for (int s = 0; s < MS; ++s)
+          if (s < ns) ms = ms + m[s];
+        double dwet, GG;
+        vc = sd_vc_sums(zpn, v, ms, ad, ar, &dwet, &GG);
+        vc = fmin(vc, cap);
+        if (k == 1) {                             // the level just above the surface
+          const double ust = a.ustar[col];
+          const double mdiff = (kKb * a.tk[x] * GG) / (3.0 * kPiFull * ar.avis * dwet);
+          const double Sc = (ar.avis / ad) / mdiff;
+          double St = vc * ::pow(ust, 2.0) / kG * (ar.avis / ad);
+          if (St < 0.01) St = 0.01;
+          const double rt = 1.0 / fmax(kEpsR8, (ust * (::pow(Sc, -2.0 / 3.0) + ::pow(10., -3.0 / St))));
+          vc = (1. / rt) + vc;
+        }
+      }
+      vs[b * nzp + k] = vc;
+    }
+  }
+  __syncthreads();
+  if (!src) return;                             // no barrier below
+  const bool top = k == nzp - 2;                // rflm(nzp) = 0
   const int64_t x1 = x + 1;
-  const bool top = k == a.nzp - 2;              // rflm(nzp) = 0
-  const double adn = a.adn[x], adn1 = a.adn[x1];
+  const double adn = a.adn[x], adn1 = a.adn[x1], dz = a.dzt[k];
   double sdw = 0.;
   for (int b = 0; b < nb; ++b) {
     const double zpn = N[(int64_t)b * n3 + x], zpn1 = top ? 0. : N[(int64_t)b * n3 + x1];
     const bool on = !(zpn < kNlim), on1 = !top && !(zpn1 < kNlim);
-    const double vc = V[(int64_t)b * n3 + x], vc1 = on1 ? V[(int64_t)b * n3 + x1] : 0.;
+    const double vc = vs[b * nzp + k], vc1 = on1 ? vs[b * nzp + k + 1] : 0.;
     const double rfn = on ? -zpn * vc : 0.;
     const double rfn1 = on1 ? -zpn1 * vc1 : 0.;
     const double dn = (rfn1 * adn1 - rfn * adn) * dz / adn;
     a.n_t[cat][(int64_t)b * n3 + x] = a.n_t[cat][(int64_t)b * n3 + x] - dn;
     for (int s = 0; s < ns; ++s) {
-      const double m = M[((int64_t)s * nb + b) * n3 + x], m1 = on1 ? M[((int64_t)s * nb + b) * n3 + x1] : 0.;
+      const int64_t r = ((int64_t)s * nb + b) * n3;
+      const double m = M[r + x], m1 = on1 ? M[r + x1] : 0.;
       const double rfm = on ? -m * vc : 0.;
       const double rfm1 = on1 ? -m1 * vc1 : 0.;
       const double dm = (rfm1 * adn1 - rfm * adn) * dz / adn;
-      a.m_t[cat][((int64_t)s * nb + b) * n3 + x] = a.m_t[cat][((int64_t)s * nb + b) * n3 + x] - dm;
+      a.m_t[cat][r + x] = a.m_t[cat][r + x] - dm;
       if (s == iw) sdw = sdw + dm;
     }
   }
@@ -184,7 +199,7 @@ __global__ void __launch_bounds__(128) k_sed_slow(SedArgs a, int cat) {
   for (int q = ns * nb - nb - 1; q < ns * nb; ++q) {
     const int s = q / nb, b = q % nb;
     if (N[(int64_t)b * n3 + x] < kNlim) continue;
-    sw = sw + (-M[((int64_t)s * nb + b) * n3 + x] * V[(int64_t)b * n3 + x]);
+    sw = sw + (-M[((int64_t)s * nb + b) * n3 + x] * vs[b * nzp + k]);
   }
   a.rrate[x] = a.rrate[x] - sw * adn * kAlvl;
   a.tlt[x] = a.tlt[x] + sdw * (kAlvl / kCp) * a.th[x] / a.tk[x];
@@ -289,7 +304,7 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
 #pragma unroll
         for (int s = 0; s < MS; ++s)
           if (s == ns - 1) mrim = qv[1 + s];
-        const double dwet = zpn < 1.e-15 ? 0. : ::pow(v / zpn / kPi6, 1. / 3.);
+        const double dwet = zpn < 1.e-15 ? 0. : cbrt_fast(v / zpn / kPi6);
         const double dnsp = zpn < 1.e-15 ? 0. : ice_diameter(a.ice_alpha, a.ice_beta, mpri, mrim, zpn);
         const Shape sh = shape_coeffs_p(a.ice_alpha, a.ice_beta, a.ice_gamma, a.ice_sigma, mpri, mrim, zpn);
 #pragma unroll
@@ -414,12 +429,21 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
 }  // namespace
 
 int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, int sed_ice, hipStream_t s) {
-  const unsigned g256 = (unsigned)((a.n3 + 255) / 256), g128 = (unsigned)((a.n3 + 127) / 128);
+  const unsigned g256 = (unsigned)((a.n3 + 255) / 256);
   hipLaunchKernelGGL(k_sed_zero, dim3(g256 < 65536 ? g256 : 65536), dim3(256), 0, s, a);
   for (int cat = 0; cat < 2; ++cat) {
     if (!(cat == 0 ? sed_aero : sed_cloud)) continue;
-    hipLaunchKernelGGL(k_sed_vslow, dim3(g128, a.nb[cat]), dim3(128), 0, s, a, cat);
-    hipLaunchKernelGGL(k_sed_slow, dim3(g128), dim3(128), 0, s, a, cat);
+    // one workgroup per interior column, one lane per level; the fall speeds
+    // of the column's bins in LDS
+    const unsigned ncol = (unsigned)((a.nxp - 4) * (a.nyp - 4)), nthr = (unsigned)((a.nzp + 63) / 64 * 64);
+    const size_t lds = (size_t)a.nb[cat] * a.nzp * sizeof(double);
+    if (a.nzp > kSedMaxZ || lds > 65536) return (int)hipErrorInvalidValue;
+    if (a.ns <= 3)
+      hipLaunchKernelGGL(k_sed_slow<3>, dim3(ncol), dim3(nthr), lds, s, a, cat);
+    else if (a.ns <= 5)
+      hipLaunchKernelGGL(k_sed_slow<5>, dim3(ncol), dim3(nthr), lds, s, a, cat);
+    else
+      hipLaunchKernelGGL(k_sed_slow<kMaxSpec>, dim3(ncol), dim3(nthr), lds, s, a, cat);
   }
   if (sed_precp) {
     // one workgroup per interior column, one lane per level
