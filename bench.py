@@ -283,6 +283,8 @@ def config_leg(capi, W, cfg, nsteps=10, gradient=False):
            "value": g.ncells * nsteps / wall, "unit": "cell-steps/s", "ms_per_step": wall / nsteps * 1e3,
            "gpu_ms_per_step": kms / max(ksteps, 1), "cells_per_class": [c / max(n, 1) for c in cells],
            "launch_lanes_per_cell": shapes, "field_generation_s": gen}
+    if level == 5:
+        res["sedimentation"] = sedim5_leg(capi, desc, g, fd, nn, mm)
     if gradient:
         djdn = [torch.zeros_like(x) for x in n_p]
         ctx.gradient_device(fd, n_p, m_p, djdn, nsteps=1)
@@ -295,6 +297,50 @@ def config_leg(capi, W, cfg, nsteps=10, gradient=False):
                            "unstable_cells": int(np.count_nonzero(fl & capi.CELL_GRAD_UNSTABLE))}
     ctx.close()
     return res
+
+
+def sedim5_leg(capi, desc, g, fd, n_p, m_p, calls=20):
+    """Level-5 sedimentation on device (salsa_sedim_device: aerosol, cloud,
+    precipitation and ice, DepositionFast flag 4 for the ice) of the config-5
+    state after its steps, on the 128x128x100 grid, timed with HIP events on
+    the context stream.  Algorithmic bytes per call: every element of the
+    four categories read with its tendency read and written (24 B), fall
+    speeds written (8 B per precipitation and ice bin and cell), per cell tk,
+    th, adn read, tlt read and written, rrate and irate written (56 B)."""
+    import numpy as np
+    import torch
+    n3 = g.nzp * g.nxp * g.nyp
+    dev = fd["temp"].device
+    nt = [torch.zeros_like(x) for x in n_p]
+    mt = [torch.zeros_like(x) for x in m_p]
+    rng = np.random.default_rng(0)
+    sf = dict(tk=fd["temp"], th=fd["temp"] * 1.02, adn=fd["dn"],
+              ustar=torch.from_numpy(0.1 + 0.4 * rng.random(g.nxp * g.nyp)).to(dev))
+    z = lambda *sh: torch.zeros(sh, dtype=torch.float64, device=dev)
+    tlt, rr, sr, ir, si = z(n3), z(n3), z(g.nxp * g.nyp), z(n3), z(g.nxp * g.nyp)
+    vt, vi = z(desc.nprc, n3), z(desc.nice, n3)
+    dzt = 1. / np.full(g.nzp, 20.)
+    ctx = capi.Context(desc, capi.params(level=5), g.nzp, g.nxp, g.nyp)
+    call = lambda: ctx.sedim_device(1.0, dzt, sf, n_p, m_p, nt, mt, tlt, rr, sr, vt, sed=(1, 1, 1, 1), irate=ir,
+                                    sfcirate=si, vt_ice=vi)
+    call()
+    ctx.sync()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(calls):
+        call()
+    e1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / calls
+    ctx.close()
+    elems = sum(x.numel() for x in n_p + m_p)
+    nbytes = 24 * elems + 8 * (desc.nprc + desc.nice) * n3 + 56 * n3
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"ms_per_call": ms, "cells": n3, "bytes_per_call": nbytes, "achieved_GBs": gbs,
+            "peak_GBs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS, "bound": "hbm",
+            "max_ice_fall_speed_m_s": float(vi.max().item()), "ice_cells": int((n_p[3].sum(dim=0) > 1e-6).sum().item())}
 
 
 def les_leg(capi, desc, f, fd, n_p, m_p, calls=20):

@@ -152,24 +152,6 @@ __device__ inline Shape shape_coeffs(const DevLayout &D, double mpri, double mri
   return shape_coeffs_p(D.ice_alpha, D.ice_beta, D.ice_gamma, D.ice_sigma, mpri, mrim, numc);
 }
 
-// getDiameter, mo_ice_shape.f90:47-86 (getBulkRho, D_nonsphericalIce,
-// D_nonsphericalRimed, D_spherical :105-143): the maximum dimension of an ice
-// particle of pristine mass mpri and rime mass mrim
-__device__ inline double ice_diameter(double ia, double ib, double mpri, double mrim, double numc) {
-  if (numc < 1.e-6) return 1.e-9;
-  const double mtot = (mpri + mrim) / numc;
-  const double fr = fmax(fmin(mrim / (mtot * numc), 0.9999), 0.0001);
-  const double rho_b = fr * kRhori + (1. - fr) * kRhoic;
-  const double dth = ::pow(kPi6 * kRhoic / ia, 1. / (ib - 3.));
-  const double dgr = ::pow(ia / (kPi6 * rho_b), 1. / (3. - ib));
-  const double dcr = ::pow((1. / (1. - fr)) * ia / (kPi6 * rho_b), 1. / (3. - ib));
-  const double mth = kRhoic * kPi6 * cubed(dth), mgr = rho_b * kPi6 * cubed(dgr), mcr = rho_b * kPi6 * cubed(dcr);
-  if (mtot < mth) return ::pow(mtot / (kPi6 * kRhoic), 1. / 3.);
-  if (mtot >= mth && mtot < mgr) return ::pow(mtot / ia, 1. / ib);
-  if (mtot >= mgr && mtot < mcr) return ::pow(mtot / (kPi6 * rho_b), 1. / 3.);
-  if (mtot >= mcr) return ::pow(mtot / ((1. / (1. - fr)) * ia), 1. / ib);
-  return 1.e-9;
-}
 
 // terminal_vel for ice (Khvorostyanov & Curry 2002), :91-98 with kcVt..kcbre :103-158
 __device__ inline double terminal_vel_ice(double dsph, double rhop, double rhoa, double visc, Shape sh, double dnsp) {

@@ -227,6 +227,34 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_slow(SedArgs a, int cat) {
 // rate, the surface rate and tlt, latent heat alvi; MS: the most mass rows
 // per bin (registers per level).
 constexpr int kSedMaxIce = 16;                // ice bins whose rime changes a workgroup keeps in LDS
+
+// getShapeCoefficients and getDiameter of one ice bin (mo_ice_shape.f90:
+// 47-86, 145-191; numc >= prlim), which form the same mass thresholds: here
+// once.  mth: the pristine threshold rhoic*pi6*Dth**3, which depends on the
+// shape parameters only.
+__device__ inline void ice_shape_diam(const SedArgs &a, double mth, double mpri, double mrim, double numc, Shape &sh,
+                                      double &dnsp) {
+  const double ia = a.ice_alpha, ib = a.ice_beta;
+  sh = Shape{kPi6 * kRhoic, 3., kPi / 4., 2.};
+  dnsp = 1.e-9;
+  if (numc < 1.e-6) return;
+  const double mtot = (mpri + mrim) / numc;
+  const double fr = fmax(fmin(mrim / (mtot * numc), 0.9999), 0.0001);
+  const double rho_b = fr * kRhori + (1. - fr) * kRhoic;
+  const double dgr = ::pow(ia / (kPi6 * rho_b), 1. / (3. - ib));
+  const double h = 1. / (1. - fr);
+  const double dcr = ::pow(h * ia / (kPi6 * rho_b), 1. / (3. - ib));
+  const double mgr = rho_b * kPi6 * cubed(dgr), mcr = rho_b * kPi6 * cubed(dcr);
+  if (mtot >= mth && mtot < mgr) {
+    sh = Shape{ia, ib, a.ice_gamma, a.ice_sigma};
+  } else if (mtot > mcr) {
+    sh = Shape{ia * h, ib, a.ice_gamma * (1. - fr) + kPi / 4. * fr, a.ice_sigma * (1. - fr) + 2. * fr};
+  }
+  if (mtot < mth) dnsp = cbrt_fast(mtot / (kPi6 * kRhoic));
+  else if (mtot >= mth && mtot < mgr) dnsp = ::pow(mtot / ia, 1. / ib);
+  else if (mtot >= mgr && mtot < mcr) dnsp = cbrt_fast(mtot / (kPi6 * rho_b));
+  else if (mtot >= mcr) dnsp = ::pow(mtot / (h * ia), 1. / ib);
+}
 struct SedPrcLds {
   double dz[kSedMaxZ], rdz[kSedMaxZ], adn[kSedMaxZ];
   double vc[2][kSedMaxZ];                 // source fall speed, by bin parity
@@ -272,6 +300,11 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
   load(0, qv, tv);
   Air air = {0., 0.};
   double rr = 0., sw = 0.;
+  double mth = 0.;                              // ice: the pristine mass threshold of mo_ice_shape
+  if (ICE) {
+    const double dth = ::pow(kPi6 * kRhoic / a.ice_alpha, 1. / (a.ice_beta - 3.));
+    mth = kRhoic * kPi6 * cubed(dth);
+  }
   if (lev) {
     S.dz[k] = a.dzt[k];
     S.rdz[k] = 1. / a.dzt[k];
@@ -305,8 +338,9 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
         for (int s = 0; s < MS; ++s)
           if (s == ns - 1) mrim = qv[1 + s];
         const double dwet = zpn < 1.e-15 ? 0. : cbrt_fast(v / zpn / kPi6);
-        const double dnsp = zpn < 1.e-15 ? 0. : ice_diameter(a.ice_alpha, a.ice_beta, mpri, mrim, zpn);
-        const Shape sh = shape_coeffs_p(a.ice_alpha, a.ice_beta, a.ice_gamma, a.ice_sigma, mpri, mrim, zpn);
+        Shape sh;
+        double dnsp;
+        ice_shape_diam(a, mth, mpri, mrim, zpn, sh, dnsp);
 #pragma unroll
         for (int s = 0; s < MS; ++s)
           if (s < ns) ms = ms + qv[1 + s];
