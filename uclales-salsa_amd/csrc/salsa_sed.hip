@@ -255,19 +255,19 @@ __device__ inline void ice_shape_diam(const SedArgs &a, double mth, double mpri,
   else if (mtot >= mgr && mtot < mcr) dnsp = cbrt_fast(mtot / (kPi6 * rho_b));
   else if (mtot >= mcr) dnsp = ::pow(mtot / (h * ia), 1. / ib);
 }
-struct SedPrcLds {
-  double dz[kSedMaxZ], rdz[kSedMaxZ], adn[kSedMaxZ];
-  double vc[2][kSedMaxZ];                 // source fall speed, by bin parity
-  int dec[2][kSedMaxZ];                   // bit 0 source, 1 two levels, 2 deposits; (d + 1) << 8
-  double c1[2][kSedMaxZ], c0[2][kSedMaxZ], q[2][kSedMaxZ];   // by quantity parity
-  int wmax[2][kSedMaxZ / 64];             // largest fall distance per wavefront, by bin parity
-  double rem[kMaxSpec * kMaxBin];         // deposited mass per (species, bin)
-};
+// LDS of one column (Z = blockDim.x levels), carved from the dynamic buffer
+// in this order: dz, rdz, adn [Z]; vc [2][Z] (source fall speed, by bin
+// parity); c1, c0, q [2][Z] (by quantity parity); rem [ns*nb] (deposited
+// mass per species and bin); rout [nb][Z] (ice: rime change per bin); then
+// the ints dec [2][Z] (bit 0 source, 1 two levels, 2 deposits; (d + 1) << 8)
+// and wmax [2][4] (largest fall distance per wavefront, by bin parity).
+__host__ __device__ inline size_t sed_prc_lds_bytes(int Z, int nb, int ns, bool ice) {
+  return (size_t)(11 * Z + ns * nb + (ice ? nb * Z : 0)) * sizeof(double) + (size_t)(2 * Z + 8) * sizeof(int);
+}
 
 template <int MS, bool ICE>
 __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
-  __shared__ SedPrcLds S;
-  __shared__ double rout[ICE ? kSedMaxIce : 1][kSedMaxZ];   // rime change per bin (ICE)
+  extern __shared__ double smem[];
   constexpr int cat = ICE ? 3 : 2;
   constexpr double alv = ICE ? kAls : kAlvl;
   const int nx = a.nxp - 4;
@@ -275,6 +275,12 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
   const int64_t col = (int64_t)i + (int64_t)a.nxp * j, n3 = a.n3;
   const int k = threadIdx.x, nzp = a.nzp, ktop = nzp - 2, nw = blockDim.x / 64;
   const int nb = a.nb[cat], ns = ICE ? a.ns_ice : a.ns, iw = a.iwa, ir = a.irim;
+  const int Z = blockDim.x;
+  double *const Sdz = smem, *const Srdz = smem + Z, *const Sadn = smem + 2 * Z, *const Svc = smem + 3 * Z;
+  double *const Sc1 = smem + 5 * Z, *const Sc0 = smem + 7 * Z, *const Sq = smem + 9 * Z, *const Srem = smem + 11 * Z;
+  double *const Srout = Srem + ns * nb;
+  int *const Sdec = reinterpret_cast<int *>(Srout + (ICE ? nb * Z : 0));
+  int *const Swmax = Sdec + 2 * Z;
   const double *__restrict__ Np = a.n_p[cat];
   const double *__restrict__ Mp = a.m_p[cat];
   double *__restrict__ Nt = a.n_t[cat];
@@ -306,9 +312,9 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
     mth = kRhoic * kPi6 * cubed(dth);
   }
   if (lev) {
-    S.dz[k] = a.dzt[k];
-    S.rdz[k] = 1. / a.dzt[k];
-    S.adn[k] = a.adn[x];
+    Sdz[k] = a.dzt[k];
+    Srdz[k] = 1. / a.dzt[k];
+    Sadn[k] = a.adn[x];
   }
   if (src) {
     air = sd_air(a.tk[x], a.adn[x]);
@@ -345,7 +351,7 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
         for (int s = 0; s < MS; ++s)
           if (s < ns) ms = ms + qv[1 + s];
         const double zdn = ms / zpn / (kPi6 * (dwet * dwet * dwet));
-        vc = terminal_vel_ice(dwet, zdn, S.adn[k], air.avis, sh, dnsp);
+        vc = terminal_vel_ice(dwet, zdn, Sadn[k], air.avis, sh, dnsp);
       } else {
         double v = 0., ms = 0.;
 #pragma unroll
@@ -354,7 +360,7 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
 #pragma unroll
         for (int s = 0; s < MS; ++s)
           if (s < ns) ms = ms + qv[1 + s];
-        vc = sd_vc_sums(zpn, v, ms, S.adn[k], air, nullptr, nullptr);
+        vc = sd_vc_sums(zpn, v, ms, Sadn[k], air, nullptr, nullptr);
       }
       vt[(int64_t)b * n3 + x] = vc;
       // the rate: water (and, for ice, rime) falling out of the level (:1206-1212)
@@ -364,30 +370,30 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
         if (s == iw) wv = qv[1 + s];
         if (s == ir) rv = qv[1 + s];
       }
-      rr = ICE ? rr + (wv + rv) * S.adn[k] * vc * alv : rr + wv * S.adn[k] * vc * alv;
+      rr = ICE ? rr + (wv + rv) * Sadn[k] * vc * alv : rr + wv * Sadn[k] * vc * alv;
       const double fdmax = dtlt * vc;
       double fd = 0.;
       int fi = 0;
       while (fd < fdmax) {                      // bounded: ends at the lowest level
-        fd = fd + S.rdz[k - fi];
+        fd = fd + Srdz[k - fi];
         fi = fi + 1;
         if (k - fi <= 0) { dep = true; break; }
       }
       fi = fi - 1;
       const int kf = k - fi;                    // kf - 1 >= 0; kf <= k + 1 <= nzp - 1
-      fd = fd - S.rdz[kf];
-      const double fdos = fmin(fmax(fdmax - fd, 0.), S.rdz[kf]);
-      pre = fdos * S.dz[kf];
+      fd = fd - Srdz[kf];
+      const double fdos = fmin(fmax(fdmax - fd, 0.), Srdz[kf]);
+      pre = fdos * Sdz[kf];
       two = !(pre > 0.5);
-      dA = S.dz[kf - 1];
-      dB = two ? S.dz[kf] : S.dz[k];
+      dA = Sdz[kf - 1];
+      dB = two ? Sdz[kf] : Sdz[k];
       d = k - kf;                               // >= -1
-      S.vc[pb][k] = vc;
+      Svc[pb * Z + k] = vc;
     }
-    S.dec[pb][k] = on ? (1 | (two ? 2 : 0) | (dep ? 4 : 0) | ((d + 1) << 8)) : 0;
+    Sdec[pb * Z + k] = on ? (1 | (two ? 2 : 0) | (dep ? 4 : 0) | ((d + 1) << 8)) : 0;
     int wm = d;
     for (int o = 32; o > 0; o >>= 1) wm = max(wm, __shfl_xor(wm, o));
-    if ((k & 63) == 0) S.wmax[pb][k >> 6] = wm;
+    if ((k & 63) == 0) Swmax[pb * 4 + (k >> 6)] = wm;
     int dmax = -1;
 #pragma unroll
     for (int qy = 0; qy <= MS; ++qy) {
@@ -403,22 +409,22 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
         } else {
           c1 = q * dA / dB;
         }
-        S.c1[p][k] = c1;
-        S.c0[p][k] = c0;
-        if (dep) S.q[p][k] = q;
+        Sc1[p * Z + k] = c1;
+        Sc0[p * Z + k] = c0;
+        if (dep) Sq[p * Z + k] = q;
       }
       __syncthreads();
       if (qy == 0)
-        for (int w = 0; w < nw; ++w) dmax = max(dmax, S.wmax[pb][w]);
+        for (int w = 0; w < nw; ++w) dmax = max(dmax, Swmax[pb * 4 + (w)]);
       if (lev) {
         double out = 0.;
         if (k <= ktop) {
           for (int s = min(ktop, k + dmax + 1); s > k; --s) {
-            const int ds = S.dec[pb][s];
+            const int ds = Sdec[pb * Z + s];
             if (!(ds & 1)) continue;
             const int kf = s - ((ds >> 8) - 1);
-            if (kf - 1 == k) out = out + S.c1[p][s];
-            if ((ds & 2) && kf == k) out = out + S.c0[p][s];
+            if (kf - 1 == k) out = out + Sc1[p * Z + s];
+            if ((ds & 2) && kf == k) out = out + Sc0[p * Z + s];
           }
           if (on) {
             out = out - q;
@@ -429,13 +435,13 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
         const int64_t r = qy == 0 ? (int64_t)b * n3 : ((int64_t)(qy - 1) * nb + b) * n3;
         (qy == 0 ? Nt : Mt)[r + x] = tv[qy] + out / dtlt;
         if (qy - 1 == iw) sw = sw + out;
-        if (ICE && qy - 1 == ir) rout[ICE ? b : 0][k] = out;
+        if (ICE && qy - 1 == ir) Srout[b * Z + k] = out;
       }
       if (qy > 0 && k == 0) {                   // lane 0 (level 0 is never a source)
         double r = 0.;
         for (int s = min(ktop, dmax + 1); s >= 1; --s)
-          if ((S.dec[pb][s] & 5) == 5) r = r + S.q[p][s] * S.adn[s] * S.vc[pb][s];
-        S.rem[(qy - 1) * nb + b] = r;
+          if ((Sdec[pb * Z + s] & 5) == 5) r = r + Sq[p * Z + s] * Sadn[s] * Svc[pb * Z + s];
+        Srem[(qy - 1) * nb + b] = r;
       }
     }
 #pragma unroll
@@ -448,14 +454,14 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
   if (lev && k <= ktop) {
     // SUM over the water rows in bin order, for ice then the rime rows
     if (ICE)
-      for (int b = 0; b < nb; ++b) sw = sw + rout[ICE ? b : 0][k];
+      for (int b = 0; b < nb; ++b) sw = sw + Srout[b * Z + k];
     a.tlt[x] = a.tlt[x] - (sw / dtlt) * (alv / kCp) * a.th[x] / a.tk[x];
   }
   if (k == 0) {
     // SUM(remprc((ns-1)*nb:ns*nb)) (ice: (ns-2)*nb:ns*nb), 1-based, one entry
     // before the water rows as written (:1256-1260)
     double s2 = 0.;
-    for (int q = (ICE ? ns - 2 : ns - 1) * nb - 1; q < ns * nb; ++q) s2 = s2 + S.rem[q];
+    for (int q = (ICE ? ns - 2 : ns - 1) * nb - 1; q < ns * nb; ++q) s2 = s2 + Srem[q];
     srate[col] = srate[col] + s2 * alv;
   }
 }
@@ -482,24 +488,25 @@ int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, i
   if (sed_precp) {
     // one workgroup per interior column, one lane per level
     const unsigned ncol = (unsigned)((a.nxp - 4) * (a.nyp - 4)), nthr = (unsigned)((a.nzp + 63) / 64 * 64);
-    if (a.nzp > kSedMaxZ || a.nb[2] * a.ns > kMaxSpec * kMaxBin) return (int)hipErrorInvalidValue;
+    const size_t lds = sed_prc_lds_bytes((int)nthr, a.nb[2], a.ns, false);
+    if (a.nzp > kSedMaxZ || lds > 65536) return (int)hipErrorInvalidValue;
     if (a.ns <= 3)
-      hipLaunchKernelGGL((k_sed_prc<3, false>), dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<3, false>), dim3(ncol), dim3(nthr), lds, s, a);
     else if (a.ns <= 5)
-      hipLaunchKernelGGL((k_sed_prc<5, false>), dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<5, false>), dim3(ncol), dim3(nthr), lds, s, a);
     else
-      hipLaunchKernelGGL((k_sed_prc<kMaxSpec, false>), dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<kMaxSpec, false>), dim3(ncol), dim3(nthr), lds, s, a);
   }
   if (sed_ice && a.level == 5) {
     const unsigned ncol = (unsigned)((a.nxp - 4) * (a.nyp - 4)), nthr = (unsigned)((a.nzp + 63) / 64 * 64);
-    if (a.nzp > kSedMaxZ || a.nb[3] > kSedMaxIce || a.nb[3] * a.ns_ice > kMaxSpec * kMaxBin)
-      return (int)hipErrorInvalidValue;
+    const size_t lds = sed_prc_lds_bytes((int)nthr, a.nb[3], a.ns_ice, true);
+    if (a.nzp > kSedMaxZ || a.nb[3] > kSedMaxIce || lds > 65536) return (int)hipErrorInvalidValue;
     if (a.ns_ice <= 3)
-      hipLaunchKernelGGL((k_sed_prc<3, true>), dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<3, true>), dim3(ncol), dim3(nthr), lds, s, a);
     else if (a.ns_ice <= 5)
-      hipLaunchKernelGGL((k_sed_prc<5, true>), dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<5, true>), dim3(ncol), dim3(nthr), lds, s, a);
     else
-      hipLaunchKernelGGL((k_sed_prc<kMaxSpec, true>), dim3(ncol), dim3(nthr), 0, s, a);
+      hipLaunchKernelGGL((k_sed_prc<kMaxSpec, true>), dim3(ncol), dim3(nthr), lds, s, a);
   }
   return (int)hipGetLastError();
 }
