@@ -12,7 +12,9 @@ tests/golden/act_cloudbase.npz):
 * against the oracle's own step with activation on the same inputs, to
   1e-6 of the cell's activated total on at least 95 % of the cells (the
   critical-bin search is a discrete decision on the post-condensation
-  state)."""
+  state);
+* with autoconv2 before it and distr_update after it, the activation sees
+  the state between the two, as the oracle's step does."""
 import numpy as np
 import pytest
 
@@ -114,3 +116,57 @@ def test_gpu_cloud_base_activation(capi, oracle_mod, name, prec):
         e2 = np.maximum(np.abs(gn - on2).max(axis=1) / s2n, np.abs(gv - ov2).max(axis=(1, 2)) / s2v)
         print("  vs the oracle's step: median %.2e, cells above 1e-6: %d" % (np.median(e2), int((e2 > 1e-6).sum())))
         assert (e2 <= 1e-6).mean() >= 0.95
+
+
+@pytest.mark.parametrize("name", ["step_cfg4.npz", "step_cfg5.npz"])
+def test_gpu_cloud_base_activation_with_later_processes(capi, oracle_mod, name):
+    """autoconv2 before and distr_update after the activation (mo_salsa.f90:
+    110-149): the activation sees the state after autoconv2 and before the
+    re-binning, so the result differs from an activation of the step's final
+    state; against the oracle's step with both processes (FP64)."""
+    O = oracle_mod
+    g = golden(name)
+    level = int(g["level"])
+    species = tuple(int(s) for s in g["species"])
+    desc = capi.bin_desc(species=species)
+    nt = desc.nbins + desc.ncld + desc.nprc + (desc.nice if level == 5 else 0)
+    cells = (g["T"], g["p"], g["rs"], g["rsi"], g["rv"], g["numc"][:, :nt], full_volc(g["volc"])[:, :nt])
+    n = len(g["T"])
+    w = np.random.default_rng(8).uniform(0.1, 6., n)
+    print("case %s with autoconv2 and distr_update, %d cells" % (name, n), flush=True)
+    import torch
+    fc = FlatCells(desc, level, *cells)
+    prm = capi.params(level=level, precision=0, hz=0, hxy=0, lsactiv=2, lsauto_mode=2, lsdistupdate=1)
+    ctx = capi.Context(desc, prm, fc.n, 1, 1)
+    n_p = [x.copy() for x in fc.n_p]
+    m_p = [x.copy() for x in fc.m_p]
+    f = dict(fc.f)
+    f["rv"] = f["rv"].copy()
+    nwet = desc.nspec - 1
+    dev = torch.device("cuda")
+    na = torch.zeros((desc.ncld, n), dtype=torch.float64, device=dev)
+    va = torch.zeros((nwet * desc.ncld, n), dtype=torch.float64, device=dev)
+    ctx.set_cloud_base_fields(torch.from_numpy(w).to(dev), na, va)
+    ctx.step_host(f, n_p, m_p, flags=1)
+    ctx.sync()
+    inp = fc.unpack(fc.n_p, fc.m_p, fc.f["rv"])
+    ctx.close()
+    L = oracle_layout_for(g)
+    P = O.make_params(level=level, lsactiv=2, lsauto_mode=2, lsdistupdate=1)
+    nci = np.zeros((n, L.ntotal))
+    vci = np.zeros((n, L.ntotal, 9))
+    nci[:, :nt] = inp[1]
+    vci[:, :nt] = inp[2]
+    _, _, _, _, on, ov = O.step_cells_activ(L, P, g["T"], g["p"], g["rs"], g["rsi"], inp[0], nci, vci, w)
+    dn = fc.f["dn"]
+    rho = np.array([desc.rholiq[s] for s in range(nwet)])
+    on = on / dn[:, None]
+    ov = ov[:, :, :nwet] * rho[None, None, :] / dn[:, None, None]
+    gn = na.cpu().numpy().T
+    gv = va.cpu().numpy().reshape(nwet, desc.ncld, n).transpose(2, 1, 0)
+    e = np.maximum(np.abs(gn - on).max(axis=1) / (np.abs(on).sum(axis=1) + 1e-300),
+                   np.abs(gv - ov).max(axis=(1, 2)) / (np.abs(ov).sum(axis=(1, 2)) + 1e-300))
+    print("  vs the oracle's step: median %.2e, cells above 1e-6: %d, activating %d"
+          % (np.median(e), int((e > 1e-6).sum()), int((on.sum(axis=1) > 0).sum())), flush=True)
+    assert (on.sum(axis=1) > 0).sum() > n // 2
+    assert (e <= 1e-6).mean() >= 0.95
