@@ -5,13 +5,13 @@
 //   k_sed_zero   rrate, sfcrrate, VtPrc (level 5: irate, sfcirate, VtIce) to
 //                zero (sedim_SALSA :792-795, DepositionFast :1127)
 //   k_sed_slow   DepositionSlow (:926-1069) of one category, one workgroup
-//                per interior column and one lane per level: the bins' fall
-//                speeds (with the dry deposition velocity at the lowest
-//                level) into LDS, then the bin fluxes at each level and the
-//                level above, their divergence into the tendencies, the water
-//                divergence into tlt (:810-816, :833-839), the rain rate and
-//                at the lowest level the surface rate.  One launch per
-//                category, so the accumulation into rrate / tlt keeps the
+//                per interior column and one lane per level, one pass over
+//                the bins: each bin's fall speeds (with the dry deposition
+//                velocity at the lowest level) and fluxes, the level above's
+//                fluxes through LDS, their divergence into the tendencies,
+//                the water divergence into tlt (:810-816, :833-839), the rain
+//                rate and at the lowest level the surface rate.  One launch
+//                per category, so the accumulation into rrate / tlt keeps the
 //                reference's order.
 //   k_sed_prc    DepositionFast (:1073-1275) and its part of sedim_SALSA
 //                (:862-878, :884-904) for precipitation (flag 3) or ice
@@ -114,20 +114,26 @@ __global__ void __launch_bounds__(256) k_sed_zero(SedArgs a) {
 }
 
 // DepositionSlow (:926-1069) for category cat (0 aerosol, 1 cloud) on one
-// interior column: one workgroup per column, one lane per level.  First every
-// interior level evaluates the fall speed of each of its bins (0 below the
-// number limit; capped at half a level per step and 2 m/s; at the lowest
-// level plus the dry deposition velocity) into LDS, vs[b][k]; then, bin by
-// bin in order, each level forms the fluxes at its level and the level above
-// (:1036-1049; the neighbour's rows come from the same cache lines) and
-// their divergence into the tendencies, the water divergence into tlt
-// (:810-816, :833-839) and the rain rate over the flux entries
-// (ns-1)*nb .. ns*nb (1-based, one entry early, as written), at the lowest
-// level also the surface rate.  One launch per category keeps the
+// interior column: one workgroup per column, one lane per level, one pass over
+// the bins in order.  Per bin, every interior level reads its number and mass
+// rows once, evaluates the fall speed (0 below the number limit; capped at
+// half a level per step and 2 m/s; at the lowest level plus the dry
+// deposition velocity) and publishes its fluxes times the air density in LDS
+// (two buffers by bin parity, one barrier per bin); each level then takes the
+// level above's from there (:1036-1049; 0 above the top interior level) and
+// forms their divergence into the tendencies and, for the water row, into
+// tlt (:810-816, :833-839).  The rain rate sums the flux entries
+// (ns-1)*nb .. ns*nb (1-based, one entry early, as written) at the lane's
+// level: the water rows' terms wait in LDS until the bin loop has produced
+// the early entry (the last dry row of the last bin), which comes first; at
+// the lowest level also the surface rate.  One launch per category keeps the
 // reference's category order of the rrate / tlt accumulation.
 template <int MS>
-__global__ void __launch_bounds__(kSedMaxZ) k_sed_slow(SedArgs a, int cat) {
-  extern __shared__ double vs[];                // [nb][nzp]
+__global__ void __launch_bounds__(kSedMaxZ, MS <= 3 ? 3 : 1) k_sed_slow(SedArgs a, int cat) {
+  extern __shared__ double sm[];
+  const int Z = (int)blockDim.x;
+  double *flx = sm;                             // [2][1 + MS][Z]: flux * adn of each level, by bin parity
+  double *rt = sm + 2 * (1 + MS) * Z;           // [nb][Z]: the water rows' rain-rate terms of each level
   const int nx = a.nxp - 4;
   const int i = 2 + (int)blockIdx.x % nx, j = 2 + (int)blockIdx.x / nx;
   const int64_t col = (int64_t)i + (int64_t)a.nxp * j, n3 = a.n3;
@@ -137,73 +143,111 @@ __global__ void __launch_bounds__(kSedMaxZ) k_sed_slow(SedArgs a, int cat) {
   const int64_t x = col * nzp + k;
   const double *__restrict__ N = a.n_p[cat];
   const double *__restrict__ M = a.m_p[cat];
+  double *__restrict__ Nt = a.n_t[cat];
+  double *__restrict__ Mt = a.m_t[cat];
+  double ad = 1., dz = 0., cap = 0.;
+  Air ar{0., 0.};
   if (src) {
-    const double ad = a.adn[x];
-    const Air ar = sd_air(a.tk[x], ad);
-    const double cap = fmin(0.5 * (1. / a.dzt[k]) / a.dtlt, 2.0);
-    for (int b = 0; b < nb; ++b) {
-      const double zpn = N[(int64_t)b * n3 + x];
-      double vc = 0.;
-      if (!(zpn < kNlim)) {
-        double v = 0., ms = 0., m[MS];
+    ad = a.adn[x];
+    ar = sd_air(a.tk[x], ad);
+    dz = a.dzt[k];
+    cap = fmin(0.5 * (1. / dz) / a.dtlt, 2.0);
+  }
+  double sdw = 0., t0 = 0.;
+  uint64_t onm = 0;                             // bins at or above the number limit
+  // the rows of bin b (number, masses, their tendencies), loaded one bin
+  // ahead so that a bin's loads are in flight while the previous bin computes
+  double qn = 0., qnt = 0., qm[MS], qmt[MS];
+  auto load = [&](int b, double &n_, double &nt_, double *m_, double *mt_) {
+    n_ = N[(int64_t)b * n3 + x];
+    nt_ = Nt[(int64_t)b * n3 + x];
 #pragma unroll
-        for (int s = 0; s < MS; ++s) m[s] = s < ns ? M[((int64_t)s * nb + b) * n3 + x] : 0.;
+    for (int s = 0; s < MS; ++s) {
+      const int64_t r = ((int64_t)s * nb + b) * n3 + x;
+      m_[s] = s < ns ? M[r] : 0.;
+      mt_[s] = s < ns ? Mt[r] : 0.;
+    }
+  };
 #pragma unroll
-        for (int s = 0; s < MS; ++s)
-          if (s < ns) v = v + m[s] / a.rholiq[s];
+  for (int s = 0; s < MS; ++s) qm[s] = qmt[s] = 0.;
+  if (src) load(0, qn, qnt, qm, qmt);
+  for (int b = 0; b < nb; ++b) {
+    double *F = flx + (b & 1) * (1 + MS) * Z;
+    const double zpn = qn, znt = qnt;
+    double m[MS], mt[MS];
 #pragma unroll
-        for (int s = 0; s < MS; ++s)
-          if (s < ns) ms = ms + m[s];
-        double dwet, GG;
-        vc = sd_vc_sums(zpn, v, ms, ad, ar, &dwet, &GG);
-        vc = fmin(vc, cap);
-        if (k == 1) {                             // the level just above the surface
-          const double ust = a.ustar[col];
-          const double mdiff = (kKb * a.tk[x] * GG) / (3.0 * kPiFull * ar.avis * dwet);
-          const double Sc = (ar.avis / ad) / mdiff;
-          double St = vc * ::pow(ust, 2.0) / kG * (ar.avis / ad);
-          if (St < 0.01) St = 0.01;
-          const double rt = 1.0 / fmax(kEpsR8, (ust * (::pow(Sc, -2.0 / 3.0) + ::pow(10., -3.0 / St))));
-          vc = (1. / rt) + vc;
+    for (int s = 0; s < MS; ++s) {
+      m[s] = qm[s];
+      mt[s] = qmt[s];
+    }
+    if (src && b + 1 < nb) load(b + 1, qn, qnt, qm, qmt);
+    double vc = 0.;
+    const bool on = src && !(zpn < kNlim);
+    if (on) {
+      double v = 0., ms = 0.;
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (s < ns) v = v + m[s] / a.rholiq[s];
+#pragma unroll
+      for (int s = 0; s < MS; ++s)
+        if (s < ns) ms = ms + m[s];
+      double dwet, GG;
+      vc = sd_vc_sums(zpn, v, ms, ad, ar, &dwet, &GG);
+      vc = fmin(vc, cap);
+      if (k == 1) {                               // the level just above the surface
+        const double ust = a.ustar[col];
+        const double mdiff = (kKb * a.tk[x] * GG) / (3.0 * kPiFull * ar.avis * dwet);
+        const double Sc = (ar.avis / ad) / mdiff;
+        double St = vc * ::pow(ust, 2.0) / kG * (ar.avis / ad);
+        if (St < 0.01) St = 0.01;
+        const double rtr = 1.0 / fmax(kEpsR8, (ust * (::pow(Sc, -2.0 / 3.0) + ::pow(10., -3.0 / St))));
+        vc = (1. / rtr) + vc;
+      }
+      onm |= (uint64_t)1 << b;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        if (s == ns - 1) rt[b * Z + k] = -m[s] * vc;
+        if (s == ns - 2 && b == nb - 1) t0 = -m[s] * vc;
+      }
+    }
+    // the fluxes rfn, rfm (0 below the limit and outside the interior) times adn
+    const double Pn = (on ? -zpn * vc : 0.) * ad;
+    double Pm[MS];
+#pragma unroll
+    for (int s = 0; s < MS; ++s) Pm[s] = (on ? -m[s] * vc : 0.) * ad;
+    F[k] = Pn;
+#pragma unroll
+    for (int s = 0; s < MS; ++s)
+      if (s < ns) F[(1 + s) * Z + k] = Pm[s];
+    __syncthreads();
+    if (src) {
+      const double dn = (F[k + 1] - Pn) * dz / ad;
+      Nt[(int64_t)b * n3 + x] = znt - dn;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        if (s < ns) {
+          const double dm = (F[(1 + s) * Z + k + 1] - Pm[s]) * dz / ad;
+          Mt[((int64_t)s * nb + b) * n3 + x] = mt[s] - dm;
+          if (s == iw) sdw = sdw + dm;
         }
       }
-      vs[b * nzp + k] = vc;
     }
   }
-  __syncthreads();
   if (!src) return;                             // no barrier below
-  const bool top = k == nzp - 2;                // rflm(nzp) = 0
-  const int64_t x1 = x + 1;
-  const double adn = a.adn[x], adn1 = a.adn[x1], dz = a.dzt[k];
-  double sdw = 0.;
-  for (int b = 0; b < nb; ++b) {
-    const double zpn = N[(int64_t)b * n3 + x], zpn1 = top ? 0. : N[(int64_t)b * n3 + x1];
-    const bool on = !(zpn < kNlim), on1 = !top && !(zpn1 < kNlim);
-    const double vc = vs[b * nzp + k], vc1 = on1 ? vs[b * nzp + k + 1] : 0.;
-    const double rfn = on ? -zpn * vc : 0.;
-    const double rfn1 = on1 ? -zpn1 * vc1 : 0.;
-    const double dn = (rfn1 * adn1 - rfn * adn) * dz / adn;
-    a.n_t[cat][(int64_t)b * n3 + x] = a.n_t[cat][(int64_t)b * n3 + x] - dn;
-    for (int s = 0; s < ns; ++s) {
-      const int64_t r = ((int64_t)s * nb + b) * n3;
-      const double m = M[r + x], m1 = on1 ? M[r + x1] : 0.;
-      const double rfm = on ? -m * vc : 0.;
-      const double rfm1 = on1 ? -m1 * vc1 : 0.;
-      const double dm = (rfm1 * adn1 - rfm * adn) * dz / adn;
-      a.m_t[cat][r + x] = a.m_t[cat][r + x] - dm;
-      if (s == iw) sdw = sdw + dm;
-    }
-  }
   // rain rate over the flux entries (ns-1)*nb .. ns*nb (1-based), as written
   double sw = 0.;
-  for (int q = ns * nb - nb - 1; q < ns * nb; ++q) {
-    const int s = q / nb, b = q % nb;
-    if (N[(int64_t)b * n3 + x] < kNlim) continue;
-    sw = sw + (-M[((int64_t)s * nb + b) * n3 + x] * vs[b * nzp + k]);
-  }
-  a.rrate[x] = a.rrate[x] - sw * adn * kAlvl;
+  if ((onm >> (nb - 1)) & 1) sw = sw + t0;
+  for (int b = 0; b < nb; ++b)
+    if ((onm >> b) & 1) sw = sw + rt[b * Z + k];
+  a.rrate[x] = a.rrate[x] - sw * ad * kAlvl;
   a.tlt[x] = a.tlt[x] + sdw * (kAlvl / kCp) * a.th[x] / a.tk[x];
-  if (k == 1) a.sfcrrate[col] = a.sfcrrate[col] - sw * adn * kAlvl;
+  if (k == 1) a.sfcrrate[col] = a.sfcrrate[col] - sw * ad * kAlvl;
+}
+
+// LDS of k_sed_slow<MS> for blockDim Z and nb bins: the flux buffers and the
+// rain-rate terms
+size_t sed_slow_lds_bytes(int Z, int nb, int MS) {
+  return ((size_t)2 * (1 + MS) * Z + (size_t)nb * Z) * sizeof(double);
 }
 
 // DepositionFast for one category on one interior column: one workgroup per
@@ -473,14 +517,15 @@ int launch_sedim(const SedArgs &a, int sed_aero, int sed_cloud, int sed_precp, i
   hipLaunchKernelGGL(k_sed_zero, dim3(g256 < 65536 ? g256 : 65536), dim3(256), 0, s, a);
   for (int cat = 0; cat < 2; ++cat) {
     if (!(cat == 0 ? sed_aero : sed_cloud)) continue;
-    // one workgroup per interior column, one lane per level; the fall speeds
-    // of the column's bins in LDS
+    // one workgroup per interior column, one lane per level; the fluxes of
+    // one bin and the column's rain-rate terms in LDS
     const unsigned ncol = (unsigned)((a.nxp - 4) * (a.nyp - 4)), nthr = (unsigned)((a.nzp + 63) / 64 * 64);
-    const size_t lds = (size_t)a.nb[cat] * a.nzp * sizeof(double);
-    if (a.nzp > kSedMaxZ || lds > 65536) return (int)hipErrorInvalidValue;
-    if (a.ns <= 3)
+    const int ms = a.ns <= 3 ? 3 : a.ns <= 5 ? 5 : kMaxSpec;
+    const size_t lds = sed_slow_lds_bytes((int)nthr, a.nb[cat], ms);
+    if (a.nzp > kSedMaxZ || a.nb[cat] > 64 || a.ns < 2 || lds > 65536) return (int)hipErrorInvalidValue;
+    if (ms == 3)
       hipLaunchKernelGGL(k_sed_slow<3>, dim3(ncol), dim3(nthr), lds, s, a, cat);
-    else if (a.ns <= 5)
+    else if (ms == 5)
       hipLaunchKernelGGL(k_sed_slow<5>, dim3(ncol), dim3(nthr), lds, s, a, cat);
     else
       hipLaunchKernelGGL(k_sed_slow<kMaxSpec>, dim3(ncol), dim3(nthr), lds, s, a, cat);
