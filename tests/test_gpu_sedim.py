@@ -3,7 +3,9 @@ droplets, DepositionFast for precipitation and, at level 5, ice; mcrp.f90:
 745-1275) against the oracle's so_sedim / so_sedim5 on the same fields.  The fall speeds go through pow,
 exp, log and cube roots, whose device and host results may differ in the
 last bit (the device forms some powers as products and a Halley-refined
-cube root), and the tendencies are differences of level fluxes: the bar is
+cube root, and the ice fall speed's powers as one exponential of a sum of
+logarithms, a few 1e-14 from the power form), and the tendencies are
+differences of level fluxes: the bar is
 1e-10 of each quantity's scale in its column (the fields themselves for the
 tendencies).
 Fall-distance decisions compare such speeds with level depths; none of the
