@@ -4,11 +4,13 @@
 //
 //   pass A  tend_constrain2 (constrain_SALSA.f90:18-36), update
 //           (step.f90:470-489) on the interior, tend0 (step.f90:379-396):
-//           one thread per array element, rows streamed contiguously.
+//           one thread per array element, rows streamed contiguously, one
+//           launch over the rows of every category.
 //   pass B  sclrset 'mixd' (util.f90:71-107): the bottom row copies k = 2,
 //           the top row extrapolates with dzf2 = dzt(nzp-2)/dzt(nzp-1); the
 //           lateral halos are the periodic images (cyclics/cyclicc of a
-//           process that owns the whole grid).  One thread per (row, column).
+//           process that owns the whole grid).  One launch over every row:
+//           one thread per (halo column, level) or per interior column.
 //   pass D  SALSA_diagnostics (constrain_SALSA.f90:50-402): clamps on every
 //           cell; on interior columns the removal of particles with number
 //           but no mass, cloud / precipitation / ice bins that evaporated
@@ -48,70 +50,99 @@ __device__ __forceinline__ bool interior_col(const LesArgs &a, int i, int j) {
   return i >= 2 && i <= a.nxp - 3 && j >= 2 && j <= a.nyp - 3;
 }
 
-// pass A: one (p, t) array pair; blockIdx.y is the row, x the cell
-__global__ void __launch_bounds__(256) k_les_update(LesArgs a, double *__restrict__ p, double *__restrict__ t) {
+// Row r of the concatenated tracer rows of every category (number rows, then
+// mass rows, category by category): its prognostic and tendency rows.  The
+// same for every thread of a workgroup (r is blockIdx.y).
+__device__ __forceinline__ void les_row(const LesArgs &a, int r, double *&p, double *&t) {
+  const int ncat = a.level == 5 ? 4 : 3;
+  p = t = nullptr;
+  for (int c = 0; c < ncat; ++c) {
+    if (r < a.nb[c]) {
+      p = a.n_p[c] + (int64_t)r * a.n3;
+      t = a.n_t[c] + (int64_t)r * a.n3;
+      return;
+    }
+    r -= a.nb[c];
+    const int nm = a.nb[c] * a.nsw[c];
+    if (r < nm) {
+      p = a.m_p[c] + (int64_t)r * a.n3;
+      t = a.m_t[c] + (int64_t)r * a.n3;
+      return;
+    }
+    r -= nm;
+  }
+}
+
+// pass A, every tracer row in one launch: blockIdx.y the row, x the cell
+__global__ void __launch_bounds__(256) k_les_update(LesArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= (int)a.n3) return;
+  double *p, *t;
+  les_row(a, (int)blockIdx.y, p, t);
+  if (!p) return;
   const int k = x % a.nzp, col = x / a.nzp;
   const int i = col % a.nxp, j = col / a.nxp;
-  const int64_t e = (int64_t)blockIdx.y * a.n3 + x;
-  if (e == 0) *a.count = 0;                   // the removal list of this call starts empty
+  if (x == 0 && blockIdx.y == 0) *a.count = 0; // the removal list of this call starts empty
   const double rdt = a.dtlt;
-  const double pv = p[e], tv = t[e];
+  const double pv = p[x], tv = t[x];
   const double tc = fmax((-1. + 1.e-8) * pv / rdt, tv);
-  if (k >= 1 && k <= a.nzp - 2 && interior_col(a, i, j)) p[e] = pv + tc * rdt;
-  t[e] = 0.;
+  if (k >= 1 && k <= a.nzp - 2 && interior_col(a, i, j)) p[x] = pv + tc * rdt;
+  t[x] = 0.;
 }
 
-// pass B, halo columns: one thread per (halo column, k), blockIdx.y the
-// row; k fastest, so a wavefront reads and writes along one column.  The
-// halo columns are the 2 rows of columns at each y edge (all x) followed by
-// the 2 columns at each x edge of the interior y range.
-__global__ void __launch_bounds__(256) k_les_halo(LesArgs a, double *__restrict__ p) {
+// pass B, every tracer row in one launch, blockIdx.y the row.  Workgroups
+// below gh fill the halo columns: one thread per (halo column, k), k fastest,
+// so a wavefront reads and writes along one column; the halo columns are the
+// 2 rows of columns at each y edge (all x) followed by the 2 columns at each x
+// edge of the interior y range.  The others set the bottom and top rows of
+// the interior columns (sclrset 'mixd'), one thread per column.  The halo
+// reads interior levels 1 .. nzp-2 only, which the bottom / top rows leave
+// alone, so the two parts need no order.
+__global__ void __launch_bounds__(256) k_les_bound(LesArgs a, unsigned gh) {
+  double *p, *t;
+  les_row(a, (int)blockIdx.y, p, t);
+  if (!p) return;
   const int nz = a.nzp, nxp = a.nxp, nyp = a.nyp, nx = nxp - 4, ny = nyp - 4;
-  const int nhc = 4 * nxp + 4 * ny;
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= nhc * nz) return;
-  const int hc = h / nz, k = h % nz;
-  int i, j;
-  if (hc < 4 * nxp) {
-    const int jq = hc / nxp;
-    j = jq < 2 ? jq : nyp - 4 + jq;
-    i = hc % nxp;
+  if (blockIdx.x < gh) {
+    const int nhc = 4 * nxp + 4 * ny;
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nhc * nz) return;
+    const int hc = h / nz, k = h % nz;
+    int i, j;
+    if (hc < 4 * nxp) {
+      const int jq = hc / nxp;
+      j = jq < 2 ? jq : nyp - 4 + jq;
+      i = hc % nxp;
+    } else {
+      const int q = (hc - 4 * nxp) % 4;
+      j = 2 + (hc - 4 * nxp) / 4;
+      i = q < 2 ? q : nxp - 4 + q;
+    }
+    const int ii = i < 2 ? i + nx : (i > nxp - 3 ? i - nx : i);
+    const int jj = j < 2 ? j + ny : (j > nyp - 3 ? j - ny : j);
+    const double *src = p + ((int64_t)ii + (int64_t)nxp * jj) * nz;
+    double v;
+    if (k == 0) {
+      const double s1 = src[1], s2 = src[2];
+      v = s1 - 0. * (s2 - s1);
+    } else if (k == nz - 1) {
+      const double sm = src[nz - 2], sm2 = src[nz - 3];
+      v = sm + a.dzf2 * (sm - sm2);
+    } else {
+      v = src[k];
+    }
+    p[((int64_t)i + (int64_t)nxp * j) * nz + k] = v;
   } else {
-    const int q = (hc - 4 * nxp) % 4;
-    j = 2 + (hc - 4 * nxp) / 4;
-    i = q < 2 ? q : nxp - 4 + q;
+    const int c = (blockIdx.x - gh) * blockDim.x + threadIdx.x;
+    if (c >= nx * ny) return;
+    const int i = 2 + c % nx, j = 2 + c / nx;
+    double *col = p + ((int64_t)i + (int64_t)nxp * j) * nz;
+    const double s1 = col[1], s2 = col[2], sm = col[nz - 2], sm2 = col[nz - 3];
+    col[0] = s1 - 0. * (s2 - s1);
+    col[nz - 1] = sm + a.dzf2 * (sm - sm2);
   }
-  const int ii = i < 2 ? i + nx : (i > nxp - 3 ? i - nx : i);
-  const int jj = j < 2 ? j + ny : (j > nyp - 3 ? j - ny : j);
-  double *row = p + (int64_t)blockIdx.y * a.n3;
-  const double *src = row + ((int64_t)ii + (int64_t)nxp * jj) * nz;
-  double v;
-  if (k == 0) {
-    const double s1 = src[1], s2 = src[2];
-    v = s1 - 0. * (s2 - s1);
-  } else if (k == nz - 1) {
-    const double sm = src[nz - 2], sm2 = src[nz - 3];
-    v = sm + a.dzf2 * (sm - sm2);
-  } else {
-    v = src[k];
-  }
-  row[((int64_t)i + (int64_t)nxp * j) * nz + k] = v;
 }
 
-// pass B, interior columns: bottom and top rows (sclrset 'mixd'), one thread
-// per column, blockIdx.y the row
-__global__ void __launch_bounds__(256) k_les_rows(LesArgs a, double *__restrict__ p) {
-  const int nz = a.nzp, nx = a.nxp - 4, ny = a.nyp - 4;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nx * ny) return;
-  const int i = 2 + c % nx, j = 2 + c / nx;
-  double *col = p + (int64_t)blockIdx.y * a.n3 + ((int64_t)i + (int64_t)a.nxp * j) * nz;
-  const double s1 = col[1], s2 = col[2], sm = col[nz - 2], sm2 = col[nz - 3];
-  col[0] = s1 - 0. * (s2 - s1);
-  col[nz - 1] = sm + a.dzf2 * (sm - sm2);
-}
 // One cell's tracer rows, either in the LES arrays (row stride n3) or staged
 // in LDS (row stride = cells per workgroup): N(c, b) = n[c][b * rs],
 // M(c, s, b) = m[c][(s * nb[c] + b) * rs].
@@ -549,21 +580,14 @@ unsigned grid_for(int64_t n, int block) {
 
 int launch_les_update(const LesArgs &a, hipStream_t s) {
   const int ncat = a.level == 5 ? 4 : 3;
+  int nrow = 0;
+  for (int c = 0; c < ncat; ++c) nrow += a.nb[c] * (1 + a.nsw[c]);
+  if (nrow < 1 || nrow > 65535) return (int)hipErrorInvalidValue;
   const unsigned gx = (unsigned)((a.n3 + 255) / 256);
-  for (int c = 0; c < ncat; ++c) {
-    hipLaunchKernelGGL(k_les_update, dim3(gx, a.nb[c]), dim3(256), 0, s, a, a.n_p[c], a.n_t[c]);
-    hipLaunchKernelGGL(k_les_update, dim3(gx, a.nb[c] * a.nsw[c]), dim3(256), 0, s, a, a.m_p[c], a.m_t[c]);
-  }
+  hipLaunchKernelGGL(k_les_update, dim3(gx, nrow), dim3(256), 0, s, a);
   const unsigned gh = (unsigned)(((4 * a.nxp + 4 * (a.nyp - 4)) * a.nzp + 255) / 256);
   const unsigned gr = (unsigned)(((a.nxp - 4) * (a.nyp - 4) + 255) / 256);
-  for (int c = 0; c < ncat; ++c) {
-    for (int q = 0; q < 2; ++q) {
-      double *arr = q ? a.m_p[c] : a.n_p[c];
-      const unsigned nrow = (unsigned)(q ? a.nb[c] * a.nsw[c] : a.nb[c]);
-      hipLaunchKernelGGL(k_les_halo, dim3(gh, nrow), dim3(256), 0, s, a, arr);
-      hipLaunchKernelGGL(k_les_rows, dim3(gr, nrow), dim3(256), 0, s, a, arr);
-    }
-  }
+  hipLaunchKernelGGL(k_les_bound, dim3(gh + gr, nrow), dim3(256), 0, s, a, gh);
   int msw = 0;
   for (int c = 0; c < ncat; ++c) msw = a.nsw[c] > msw ? a.nsw[c] : msw;
   if (msw <= 3)
@@ -572,8 +596,6 @@ int launch_les_update(const LesArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_les_diag<5>, dim3(grid_for(a.n3, 256)), dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL(k_les_diag<kMaxSpec>, dim3(grid_for(a.n3, 256)), dim3(256), 0, s, a);
-  int nrow = 0;
-  for (int c = 0; c < ncat; ++c) nrow += a.nb[c] * (1 + a.nsw[c]);
   if (nrow <= kLesLdsRows) {
     const int64_t g = (a.n3 + kLesCells - 1) / kLesCells;
     hipLaunchKernelGGL(k_les_remove_lds, dim3((unsigned)(g < 4096 ? g : 4096)), dim3(kLesCells),
