@@ -358,14 +358,25 @@ int salsa_gradient_device_noalloc(salsa_ctx *ctx, const double *press, const dou
  *   charge_time: NULL, or a_chargeTimep [nbins+ncld+nprc][n3]: zeroed where
  *                the reference zeroes it (lcharge)
  *   diag       : [6][n3] rc, srp, snrp, rtot, ri, riri (rows 4-5 at level 5)
- * Needs hz = 1, hxy = 2, nzp >= 4, nxp and nyp >= 6.  Not carried: the gas-phase transfer of tiny
- * aerosols (lscndgas) and the a_indefp clamp (ice_theta_dist).  Device
+ * Needs hz = 1, hxy = 2, nzp >= 4, nxp and nyp >= 6.  The gas-phase transfer
+ * of tiny aerosols (lscndgas) and the a_indefp clamp (ice_theta_dist) run when
+ * their fields are registered with salsa_set_les_extra_fields.  Device
  * pointers; work is enqueued on the context stream like salsa_step_device.
  */
 int salsa_les_update_device(salsa_ctx *ctx, double dtlt, const double *dzt, const double *temp,
                             const double *rh, const double *rhi, const double *rp,
                             double *const n_p[4], double *const n_t[4], double *const m_p[4],
                             double *const m_t[4], double *charge_time, double *diag);
+
+/* The two LES fields that constrain_SALSA touches besides the SALSA tracers,
+ * for the following salsa_les_update_device calls (NULL unregisters):
+ *   gaerop : a_gaerop [5][n3] (device), registered when lscndgas is on: the
+ *            SO4, NO, NH and OC of aerosols removed below 0.1 nm go to gas
+ *            rows 1, 2, 3 and 5 (1-based) as molecules, mass / molar mass *
+ *            avog (SALSA_diagnostics, constrain_SALSA.f90:332-351)
+ *   indefp : a_indefp [nindef][n3] (device), registered with ice_theta_dist:
+ *            clamped to [0, 1] at level 5 (tend_constrain2, :33-34) */
+int salsa_set_les_extra_fields(salsa_ctx *ctx, double *gaerop, double *indefp, int32_t nindef);
 
 /*
  * Sedimentation on device (sedim_SALSA, mcrp.f90:745-921): DepositionSlow

@@ -106,6 +106,8 @@ def lib():
         L.so_les_update.argtypes = [C.POINTER(Layout), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                     dp, dp, dp, dp, P4, P4, P4, P4, C.c_void_p, dp]
         L.so_les_update.restype = C.c_int
+        L.so_les_update_ext.argtypes = L.so_les_update.argtypes + [C.c_void_p, C.c_void_p, C.c_int]
+        L.so_les_update_ext.restype = C.c_int
         L.so_les_corr.argtypes = [dp, dp, C.c_int]
         L.so_les_corr.restype = C.c_double
         P3 = C.c_void_p * 3
@@ -241,11 +243,15 @@ def step_batch(L, P, T, p, rs, rsi, rv, numc, volc, dt=1.0, nsteps=1):
                                dt, nsteps, _dp(numc), _dp(volc))
 
 
-def les_update(L, level, shape, dtlt, dzf2, temp, rh, rhi, rp, n_p, n_t, m_p, m_t, charge_time=None):
+def les_update(L, level, shape, dtlt, dzf2, temp, rh, rhi, rp, n_p, n_t, m_p, m_t, charge_time=None,
+               gaerop=None, indefp=None):
     """tend_constrain2 + update_sclrs + tend0 + SALSA_diagnostics on LES
-    arrays (so_les_update).  n_p/n_t/m_p/m_t: lists of 4 float64 arrays in
+    arrays (so_les_update_ext).  n_p/n_t/m_p/m_t: lists of 4 float64 arrays in
     the LES layout ([rows, n3], n3 in Fortran order), None for the ice
-    category at level 4; updated in place.  Returns diag[6, n3]."""
+    category at level 4; updated in place.  gaerop [5, n3] (the gas phase
+    that removed tiny aerosols feed, lscndgas) and indefp [rows, n3]
+    (clamped to [0, 1] at level 5, ice_theta_dist), when given, are updated
+    in place too.  Returns diag[6, n3]."""
     nzp, nxp, nyp = shape
     n3 = nzp * nxp * nyp
     P4 = C.c_void_p * 4
@@ -265,8 +271,14 @@ def les_update(L, level, shape, dtlt, dzf2, temp, rh, rhi, rp, n_p, n_t, m_p, m_
         assert charge_time.dtype == np.float64 and charge_time.flags["C_CONTIGUOUS"]
         ct = charge_time.ctypes.data
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (temp, rh, rhi, rp)]
-    rc = lib().so_les_update(C.byref(L), level, nzp, nxp, nyp, dtlt, dzf2, *(_dp(a) for a in arrs),
-                             ptrs(n_p), ptrs(n_t), ptrs(m_p), ptrs(m_t), ct, _dp(diag))
+    for a in (gaerop, indefp):
+        assert a is None or (a.dtype == np.float64 and a.flags["C_CONTIGUOUS"] and a.size % n3 == 0)
+    assert gaerop is None or gaerop.size == 5 * n3
+    gp = gaerop.ctypes.data if gaerop is not None else None
+    ip = indefp.ctypes.data if indefp is not None else None
+    nind = indefp.size // n3 if indefp is not None else 0
+    rc = lib().so_les_update_ext(C.byref(L), level, nzp, nxp, nyp, dtlt, dzf2, *(_dp(a) for a in arrs),
+                                 ptrs(n_p), ptrs(n_t), ptrs(m_p), ptrs(m_t), ct, _dp(diag), gp, ip, nind)
     if rc != 0:
         raise ValueError("so_les_update failed: %d" % rc)
     return diag

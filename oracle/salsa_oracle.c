@@ -2184,6 +2184,33 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
                   const double *temp, const double *rh, const double *rhi, const double *rp,
                   double *const n_p[4], double *const n_t[4], double *const m_p[4], double *const m_t[4],
                   double *charge_time, double *diag) {
+  return so_les_update_ext(L, level, nzp, nxp, nyp, dtlt, dzf2, temp, rh, rhi, rp, n_p, n_t, m_p, m_t, charge_time,
+                           diag, 0, 0, 0);
+}
+
+/* gas row (0-based) that a dry species' mass goes to when a tiny aerosol is
+   removed with lscndgas on (constrain_SALSA.f90:332-351): SO4 -> 1, NO -> 2,
+   NH -> 3, OC -> 5 (1-based rows of a_gaerop); -1 for the others */
+static int les_gas_row(int species_id) {
+  switch (species_id) {
+    case 0: return 0;
+    case 1: return 4;
+    case 5: return 1;
+    case 6: return 2;
+    default: return -1;
+  }
+}
+
+/* so_les_update with the two optional fields of constrain_SALSA that the
+   SALSA tracers do not cover: gaerop[5][n3] (a_gaerop; when given, the dry
+   mass of aerosols removed below 0.1 nm goes to the gas phase as molecules,
+   the lscndgas branch of :332-351) and indefp[nindef][n3] (a_indefp; when
+   given and level 5, clamped to [0, 1] in tend_constrain2, :33-34, the
+   ice_theta_dist branch). */
+int so_les_update_ext(const so_layout *L, int level, int nzp, int nxp, int nyp, double dtlt, double dzf2,
+                      const double *temp, const double *rh, const double *rhi, const double *rp,
+                      double *const n_p[4], double *const n_t[4], double *const m_p[4], double *const m_t[4],
+                      double *charge_time, double *diag, double *gaerop, double *indefp, int nindef) {
   const long n3 = (long)nzp * nxp * nyp;
   const int ncat = level == 5 ? 4 : 3;
   const int nb[4] = {L->nbins, L->ncld, L->nprc, L->nice};
@@ -2193,6 +2220,9 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
   const double massTH = 1.e-25;
   if (nzp < 4 || nx < 2 || ny < 2) return -1;     /* the periodic images need two interior columns */
 #define IDX(k, i, j) ((long)(k) + (long)nzp * ((long)(i) + (long)nxp * (long)(j)))
+  /* tend_constrain2's a_indefp clamp (:33-34) */
+  if (indefp && level > 4)
+    for (long e = 0; e < (long)nindef * n3; ++e) indefp[e] = fmin(fmax(indefp[e], 0.), 1.);
   /* tend_constrain2, update (interior), tend0 */
   for (int c = 0; c < ncat; ++c) {
     const int nrow = nb[c] * (1 + nsw[c]);
@@ -2356,14 +2386,19 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
               MP(3, irim, bc) = 0.;
             }
           }
-        /* aerosols below 0.1 nm dry diameter (:317-366); the gas-phase
-           transfer of lscndgas is not carried */
+        /* aerosols below 0.1 nm dry diameter (:317-366); with gaerop their
+           SO4, OC, NO and NH masses go to the gas phase as molecules */
         for (int b = 0; b < nb[0]; ++b) {
           if (!(NP(0, b) > nlim_c)) continue;
           double zvol = 0.;
           for (int s = 0; s < ndry; ++s) zvol = zvol + MP(0, s, b) / L->rholiq[s];
           zvol = zvol / NP(0, b);
           if (zvol < pi6 * 1.e-30) {
+            if (gaerop)
+              for (int s = 0; s < ndry; ++s) {
+                const int g = les_gas_row(L->spec_id[s]);
+                if (g >= 0) gaerop[g * n3 + x] = gaerop[g * n3 + x] + MP(0, s, b) / L->mm[s] * avog;
+              }
             for (int s = 0; s < nwet; ++s) MP(0, s, b) = 0.;
             NP(0, b) = 0.;
           }

@@ -186,6 +186,10 @@ int so_les_update(const so_layout *L, int level, int nzp, int nxp, int nyp, doub
                   const double *temp, const double *rh, const double *rhi, const double *rp,
                   double *const n_p[4], double *const n_t[4], double *const m_p[4], double *const m_t[4],
                   double *charge_time, double *diag);
+int so_les_update_ext(const so_layout *L, int level, int nzp, int nxp, int nyp, double dtlt, double dzf2,
+                      const double *temp, const double *rh, const double *rhi, const double *rp,
+                      double *const n_p[4], double *const n_t[4], double *const m_p[4], double *const m_t[4],
+                      double *charge_time, double *diag, double *gaerop, double *indefp, int nindef);
 double so_les_corr(const double *x, const double *y, int n);
 /* Sedimentation at level 4 (sedim_SALSA, mcrp.f90:745-921, DepositionSlow
  * :926-1069 for aerosols and cloud, DepositionFast :1073-1275 for

@@ -68,6 +68,67 @@ def test_les_update_matches_oracle(capi, oracle_mod, level, seed, grid, dry):
     np.testing.assert_array_equal(ddiag.cpu().numpy()[:rows], odiag[:rows])
 
 
+# The optional fields: a_gaerop (tiny aerosols to the gas phase, lscndgas)
+# on the tiny-aerosol case with OC + SO4 and with NO + NH at level 5 (every
+# gas row the reference names; 139 rows: removal from the LES arrays), on the
+# level-4 SO4 case (78 rows: removal staged in LDS), and with a_indefp
+# (clamped at level 5) on the level-5 SO4+DU case.
+@pytest.mark.parametrize("case", ["tiny_oc_so4", "tiny_no_nh", "les4", "les5"])
+def test_les_update_extra_fields_match_oracle(capi, oracle_mod, case):
+    import torch
+    from test_oracle_les import tiny_aerosol_case
+    if case.startswith("tiny"):
+        level = 5
+        species, ids = (("OC", "SO4"), (1, 0)) if case == "tiny_oc_so4" else (("NO", "NH"), (5, 6))
+        desc = capi.bin_desc(species=species)
+        shape, dzt, f, n_p, n_t, m_p, m_t = tiny_aerosol_case(desc, 4, level)
+        dzt = np.ones(shape[0])                       # dzf2 = 1
+    else:
+        level = 4 if case == "les4" else 5
+        species, ids = (("SO4",), (0,)) if level == 4 else (("SO4", "DU"), (0, 3))
+        desc = capi.bin_desc(species=species)
+        shape, dzt, f, n_p, n_t, m_p, m_t = les_case(desc, level, (12, 10, 9), 3)
+    nzp, nxp, nyp = shape
+    n3 = nzp * nxp * nyp
+    print("case LES update with a_gaerop%s: %s, level %d, grid %s" % (" and a_indefp" if level == 5 else "", case,
+                                                                      level, shape), flush=True)
+    rng = np.random.default_rng(9)
+    gas = rng.uniform(1e10, 1e12, (5, n3))
+    ind = rng.uniform(-1., 2., (3, n3)) if level == 5 else None
+    # oracle
+    L = oracle_mod.make_layout(species=ids)
+    on, ont, om, omt = ([a.copy() for a in x] for x in (n_p, n_t, m_p, m_t))
+    ogas, oind = gas.copy(), None if ind is None else ind.copy()
+    four = lambda x: (list(x) + [None] * 4)[:4]
+    dzf2 = dzt[nzp - 3] / dzt[nzp - 2]
+    odiag = oracle_mod.les_update(L, level, shape, 1.0, dzf2, f["temp"], f["rh"], f["rhi"], f["rp"], four(on),
+                                  four(ont), four(om), four(omt), gaerop=ogas, indefp=oind)
+    # device
+    dev = torch.device("cuda")
+    D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    dn, dnt, dm, dmt = ([D(a) for a in x] for x in (n_p, n_t, m_p, m_t))
+    df = {k: D(v) for k, v in f.items()}
+    dgas, dind = D(gas), None if ind is None else D(ind)
+    ddiag = torch.zeros((6, n3), dtype=torch.float64, device=dev)
+    ctx = capi.Context(desc, capi.params(level=level), nzp, nxp, nyp)
+    ctx.set_les_extra_fields(dgas, dind)
+    ctx.les_update_device(1.0, dzt, df, dn, dnt, dm, dmt, ddiag)
+    ctx.sync()
+    ctx.close()
+    moved = int((ogas != gas).sum())
+    print("  oracle changed %d gas entries" % moved, flush=True)
+    assert moved > 0
+    np.testing.assert_array_equal(dgas.cpu().numpy(), ogas)
+    if ind is not None:
+        np.testing.assert_array_equal(dind.cpu().numpy(), oind)
+        assert (oind == 0).any() and (oind == 1).any()
+    for c in range(len(n_p)):
+        np.testing.assert_array_equal(dn[c].cpu().numpy(), on[c], err_msg="numbers, category %d" % c)
+        np.testing.assert_array_equal(dm[c].cpu().numpy(), om[c], err_msg="masses, category %d" % c)
+    rows = 6 if level == 5 else 4
+    np.testing.assert_array_equal(ddiag.cpu().numpy()[:rows], odiag[:rows])
+
+
 def test_les_update_rejects_other_halos(capi):
     import torch
     desc = capi.bin_desc()

@@ -12,7 +12,10 @@ physics and the reference's own lines fix:
 * conservation of number and of every species' mass over the categories
   when bins move to the aerosol regime;
 * tend_constrain2's floor (no value below 1e-8 of its old self), sclrset
-  'mixd' rows and periodic halos.
+  'mixd' rows and periodic halos;
+* with a_gaerop, the gas gained per species equals the mass of the removed
+  tiny aerosols / molar mass * avog, row by row, the species mass conserved
+  between the phases; with a_indefp, the [0, 1] clamp at level 5 only.
 Runs the FP64 oracle only."""
 import numpy as np
 import pytest
@@ -280,3 +283,111 @@ def test_update_rows_and_halos(oracle_mod, level):
             np.testing.assert_array_equal(hb[..., 0], ib[..., 1])
             top = ib[..., nzp - 2] + dzf2 * (ib[..., nzp - 2] - ib[..., nzp - 3])
             np.testing.assert_array_equal(hb[..., nzp - 1], np.maximum(top, 0.))
+
+
+# tiny aerosols to the gas phase (lscndgas, constrain_SALSA.f90:332-351):
+# the molar masses of mo_submctl.f90:284-287 and the 0-based a_gaerop rows
+# SO4 -> 0, NO -> 1, NH -> 2, OC -> 4 that the reference's lines name
+GAS_MM = {0: 98.08e-3, 1: 150.e-3, 5: 62.01e-3, 6: 18.04e-3}
+GAS_ROW = {0: 0, 5: 1, 6: 2, 1: 4}
+AVOG = 6.0221e+23
+SPECIES5 = ("SO4", "OC", "NO", "NH", "DU")
+IDS5 = (0, 1, 5, 6, 3)
+
+
+def tiny_aerosol_case(desc, seed, level=4):
+    """Aerosol bins in every cell, about a third of them below 0.1 nm dry
+    diameter, saturated air and no other hydrometeors, zero tendencies: the
+    tiny-aerosol removal is the only thing SALSA_diagnostics does."""
+    case = one_cell(desc, level, nz=3)
+    shape, dzt, f, n_p, n_t, m_p, m_t = case
+    f["rh"][:] = 1.0
+    f["rhi"][:] = 1.0
+    rng = np.random.default_rng(seed)
+    nb, ndry, n3 = desc.nbins, desc.iwa, n_p[0].shape[1]
+    n_p[0][:] = 1e6
+    for s in range(ndry):
+        m_p[0][s * nb:(s + 1) * nb] = rng.uniform(0.5, 2., (nb, n3)) * 1e-12
+    m_p[0][ndry * nb:(ndry + 1) * nb] = 1e-11
+    tiny = rng.random((nb, n3)) < 0.35
+    for s in range(ndry):
+        m_p[0][s * nb:(s + 1) * nb][tiny] *= 1e-13
+    # rows and halos as the update leaves them (bottom = level 1, top = level
+    # nzp-2 with dzf2 = 0, halos = periodic images), so that the removal sees
+    # the planted values in every cell
+    nzp, nxp, nyp = shape
+    img = lambda i, n: i + (n - 4) if i < 2 else (i - (n - 4) if i > n - 3 else i)
+    for a in (n_p[0], m_p[0]):
+        v = a.reshape(-1, nyp, nxp, nzp)
+        v[..., 0] = v[..., 1]
+        v[..., nzp - 1] = v[..., nzp - 2]
+        for i in range(nxp):
+            v[:, :, i, :] = v[:, :, img(i, nxp), :]
+        for j in range(nyp):
+            v[:, j, :, :] = v[:, img(j, nyp), :, :]
+    return case
+
+
+def test_tiny_aerosols_feed_the_gas_phase(oracle_mod):
+    """With a_gaerop registered, the SO4, OC, NO and NH of every aerosol bin
+    removed below 0.1 nm dry diameter reach gas rows 1, 5, 2 and 3 (1-based)
+    as molecules: the gas increase of each species equals the removed mass
+    of that species / molar mass * avog, cell by cell (summed over the removed
+    bins in bin order), DU and water are dropped, gas row 4 is untouched, the
+    tracers end as they do without a_gaerop, and the species mass is
+    conserved between the two phases."""
+    O = oracle_mod
+    from salsa_amd import capi
+    desc = capi.bin_desc(species=SPECIES5)
+    L = O.make_layout(species=IDS5)
+    outs = []
+    for with_gas in (False, True):
+        shape, dzt, f, n_p, n_t, m_p, m_t = tiny_aerosol_case(desc, 4)
+        n_in, m_in = n_p[0].copy(), m_p[0].copy()
+        n3 = n_p[0].shape[1]
+        gas0 = np.random.default_rng(5).uniform(1e10, 1e12, (5, n3))
+        gas = gas0.copy() if with_gas else None
+        four = lambda x: (list(x) + [None] * 4)[:4]
+        O.les_update(L, 4, shape, 1.0, 0.0, f["temp"], f["rh"], f["rhi"], f["rp"], four(n_p), four(n_t),
+                     four(m_p), four(m_t), gaerop=gas)
+        outs.append((n_p[0].copy(), m_p[0].copy(), gas))
+    (n0, m0, _), (n1, m1, gas) = outs
+    np.testing.assert_array_equal(n0, n1)
+    np.testing.assert_array_equal(m0, m1)
+    nb, ndry = desc.nbins, desc.iwa
+    removed = (n_in > 0) & (n1 == 0)                          # [nb, n3]
+    assert removed.sum() > 20
+    want = gas0.copy()
+    for x in range(n3):
+        for b in np.nonzero(removed[:, x])[0]:
+            for s, sid in enumerate(IDS5):
+                if sid in GAS_ROW:
+                    g = GAS_ROW[sid]
+                    want[g, x] = want[g, x] + m_in[s * nb + b, x] / GAS_MM[sid] * AVOG
+    np.testing.assert_array_equal(gas, want)
+    np.testing.assert_array_equal(gas[3], gas0[3])
+    for s, sid in enumerate(IDS5):
+        if sid in GAS_ROW:
+            lost = (m_in[s * nb:(s + 1) * nb] * removed).sum()
+            gained = (gas[GAS_ROW[sid]] - gas0[GAS_ROW[sid]]).sum() * GAS_MM[sid] / AVOG
+            assert gained == pytest.approx(lost, rel=1e-9)
+
+
+@pytest.mark.parametrize("level", [4, 5])
+def test_indefp_clamped_at_level_5(oracle_mod, level):
+    """tend_constrain2 clamps a_indefp to [0, 1] on every cell at level 5
+    (constrain_SALSA.f90:33-34) and leaves it alone at level 4."""
+    O = oracle_mod
+    from salsa_amd import capi
+    desc = capi.bin_desc(species=("SO4", "DU") if level == 5 else ("SO4",))
+    case = one_cell(desc, level, nz=2)
+    shape, dzt, f, n_p, n_t, m_p, m_t = case
+    n3 = n_p[0].shape[1]
+    ind0 = np.random.default_rng(2).uniform(-1., 2., (3, n3))
+    ind = ind0.copy()
+    L = O.make_layout(species=(0, 3) if level == 5 else (0,))
+    four = lambda x: (list(x) + [None] * 4)[:4]
+    O.les_update(L, level, shape, 1.0, 1.0, f["temp"], f["rh"], f["rhi"], f["rp"], four(n_p), four(n_t),
+                 four(m_p), four(m_t), indefp=ind)
+    np.testing.assert_array_equal(ind, np.clip(ind0, 0., 1.) if level == 5 else ind0)
+    assert level == 4 or (ind0 < 0).any() and (ind0 > 1).any()

@@ -91,6 +91,9 @@ struct salsa_ctx {
   double *act_state = nullptr;        // [ncells][nbins*(1+nwet)]: the aerosol state it sees
   // salsa_les_update_device: the cells the removals visit ([n3]) and their number
   int32_t *les_list = nullptr;
+  // its optional fields (salsa_set_les_extra_fields): a_gaerop [5][n3], a_indefp [les_nindef][n3]
+  double *les_gas = nullptr, *les_indef = nullptr;
+  int32_t les_nindef = 0;
   // staging for salsa_step_host
   double *stage = nullptr;
   size_t stage_bytes = 0;
@@ -522,6 +525,18 @@ extern "C" int salsa_set_cloud_base_fields(salsa_ctx *c, const double *w, double
   c->act_w = w;
   c->act_n = nactd;
   c->act_v = vactd;
+  return SALSA_OK;
+}
+
+extern "C" int salsa_set_les_extra_fields(salsa_ctx *c, double *gaerop, double *indefp, int32_t nindef) {
+  if (!c) { set_error("null context"); return SALSA_ERR_ARG; }
+  if ((indefp != nullptr) != (nindef > 0) || nindef < 0 || nindef > 4096) {
+    set_error("indefp and nindef (1..4096 rows) are given together");
+    return SALSA_ERR_ARG;
+  }
+  c->les_gas = gaerop;
+  c->les_indef = indefp;
+  c->les_nindef = indefp ? nindef : 0;
   return SALSA_OK;
 }
 
@@ -964,6 +979,15 @@ extern "C" int salsa_les_update_device(salsa_ctx *c, double dtlt, const double *
   }
   a.charge_time = charge_time;
   a.diag = diag;
+  // the gas rows of the dry species (constrain_SALSA.f90:332-351): SO4 -> 1,
+  // NO -> 2, NH -> 3, OC -> 5 of a_gaerop (1-based)
+  a.gas = c->les_gas;
+  for (int s = 0; s < SALSA_MAXSPEC; ++s) {
+    const int id = s < d.nspec_dry ? d.dry_species[s] : -1;
+    a.gas_row[s] = id == 0 ? 0 : id == 5 ? 1 : id == 6 ? 2 : id == 1 ? 4 : -1;
+  }
+  a.indef = c->les_indef;
+  a.nindef = c->les_nindef;
   if (c->n3 >= INT32_MAX) { set_error("grid too large for the LES update"); return SALSA_ERR_ARG; }
   if (!c->les_list && hipMalloc(&c->les_list, ((size_t)c->n3 + 1) * sizeof(int32_t)) != hipSuccess) {
     set_error("LES update work allocation failed");

@@ -241,11 +241,15 @@ struct LesArgs {
   double *charge_time;                // [nbins+ncld+nprc][n3] or null
   double *diag;                       // [6][n3]: rc, srp, snrp, rtot, ri, riri
   int32_t *list, *count;              // work: [n3] cells the removals visit, [1] their number
+  double *gas;                        // [5][n3] a_gaerop (tiny aerosols' SO4, NO, NH, OC go there) or null
+  int32_t gas_row[kMaxSpec];          // per dry species slot: its gas row, -1 for none
+  double *indef;                      // [nindef][n3] a_indefp (clamped to [0, 1] at level 5) or null
+  int32_t nindef;
 };
 
-// Arguments of the level-4 sedimentation passes (salsa_sedim_device):
-// sedim_SALSA with DepositionSlow (aerosol, cloud) and DepositionFast
-// (precipitation) on the LES arrays.
+// Arguments of the sedimentation passes (salsa_sedim_device): sedim_SALSA
+// with DepositionSlow (aerosol, cloud) and DepositionFast (precipitation;
+// ice at level 5) on the LES arrays.
 struct SedArgs {
   int32_t nzp, nxp, nyp, level;
   int64_t n3;

@@ -2,7 +2,8 @@
 // after run_SALSA (step.f90:330-333) for the SALSA tracers, so that the
 // prognostic fields never leave HBM between steps.
 //
-//   pass A  tend_constrain2 (constrain_SALSA.f90:18-36), update
+//   pass A  tend_constrain2 (constrain_SALSA.f90:18-36; a_indefp clamped
+//           when registered), update
 //           (step.f90:470-489) on the interior, tend0 (step.f90:379-396):
 //           one thread per array element, rows streamed contiguously, one
 //           launch over the rows of every category.
@@ -52,7 +53,8 @@ __device__ __forceinline__ bool interior_col(const LesArgs &a, int i, int j) {
 
 // Row r of the concatenated tracer rows of every category (number rows, then
 // mass rows, category by category): its prognostic and tendency rows.  The
-// same for every thread of a workgroup (r is blockIdx.y).
+// same for every thread of a workgroup (r is blockIdx.y).  Rows past the
+// tracers: the a_indefp rows of the update pass (t null).
 __device__ __forceinline__ void les_row(const LesArgs &a, int r, double *&p, double *&t) {
   const int ncat = a.level == 5 ? 4 : 3;
   p = t = nullptr;
@@ -71,6 +73,7 @@ __device__ __forceinline__ void les_row(const LesArgs &a, int r, double *&p, dou
     }
     r -= nm;
   }
+  if (a.indef && r < a.nindef) p = a.indef + (int64_t)r * a.n3;
 }
 
 // pass A, every tracer row in one launch: blockIdx.y the row, x the cell
@@ -83,6 +86,10 @@ __global__ void __launch_bounds__(256) k_les_update(LesArgs a) {
   const int k = x % a.nzp, col = x / a.nzp;
   const int i = col % a.nxp, j = col / a.nxp;
   if (x == 0 && blockIdx.y == 0) *a.count = 0; // the removal list of this call starts empty
+  if (!t) {                                   // tend_constrain2's a_indefp clamp (:33-34)
+    p[x] = fmin(fmax(p[x], 0.), 1.);
+    return;
+  }
   const double rdt = a.dtlt;
   const double pv = p[x], tv = t[x];
   const double tc = fmax((-1. + 1.e-8) * pv / rdt, tv);
@@ -101,7 +108,7 @@ __global__ void __launch_bounds__(256) k_les_update(LesArgs a) {
 __global__ void __launch_bounds__(256) k_les_bound(LesArgs a, unsigned gh) {
   double *p, *t;
   les_row(a, (int)blockIdx.y, p, t);
-  if (!p) return;
+  if (!t) return;
   const int nz = a.nzp, nxp = a.nxp, nyp = a.nyp, nx = nxp - 4, ny = nyp - 4;
   if (blockIdx.x < gh) {
     const int nhc = 4 * nxp + 4 * ny;
@@ -312,13 +319,19 @@ __device__ __forceinline__ bool les_remove(const LesArgs &a, const LesCell &C, i
       }
     }
   }
-  // aerosols below 0.1 nm dry diameter (:317-366); no gas-phase transfer
+  // aerosols below 0.1 nm dry diameter (:317-366); with the gas field
+  // (lscndgas) their SO4, OC, NO and NH go to the gas phase as molecules
   for (int b = 0; b < a.nb[0]; ++b) {
     if (!(N(0, b) > kNlim)) continue;
     double zvol = 0.;
     for (int s = 0; s < ndry; ++s) zvol = zvol + M(0, s, b) / a.rholiq[s];
     zvol = zvol / N(0, b);
     if (zvol < kPi6 * 1.e-30) {
+      if (a.gas)
+        for (int s = 0; s < ndry; ++s) {
+          const int g = a.gas_row[s];
+          if (g >= 0) a.gas[(int64_t)g * n3 + x] = a.gas[(int64_t)g * n3 + x] + M(0, s, b) / a.mm[s] * kAvog;
+        }
       for (int s = 0; s < nwet; ++s) M(0, s, b) = 0.;
       N(0, b) = 0.;
       mod = true;
@@ -582,9 +595,10 @@ int launch_les_update(const LesArgs &a, hipStream_t s) {
   const int ncat = a.level == 5 ? 4 : 3;
   int nrow = 0;
   for (int c = 0; c < ncat; ++c) nrow += a.nb[c] * (1 + a.nsw[c]);
-  if (nrow < 1 || nrow > 65535) return (int)hipErrorInvalidValue;
+  const int nind = a.indef && a.level == 5 ? a.nindef : 0;
+  if (nrow < 1 || nrow + nind > 65535) return (int)hipErrorInvalidValue;
   const unsigned gx = (unsigned)((a.n3 + 255) / 256);
-  hipLaunchKernelGGL(k_les_update, dim3(gx, nrow), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_les_update, dim3(gx, nrow + nind), dim3(256), 0, s, a);
   const unsigned gh = (unsigned)(((4 * a.nxp + 4 * (a.nyp - 4)) * a.nzp + 255) / 256);
   const unsigned gr = (unsigned)(((a.nxp - 4) * (a.nyp - 4) + 255) / 256);
   hipLaunchKernelGGL(k_les_bound, dim3(gh + gr, nrow), dim3(256), 0, s, a, gh);

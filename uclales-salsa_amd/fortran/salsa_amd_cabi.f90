@@ -174,6 +174,13 @@ MODULE mo_salsa_amd_cabi
        TYPE(C_PTR), VALUE :: ctx, w, nactd, vactd         ! device arrays: w, pa_nactd, pa_vactd
      END FUNCTION salsa_set_cloud_base_fields
 
+     INTEGER(C_INT) FUNCTION salsa_set_les_extra_fields(ctx, gaerop, indefp, nindef) &
+                                                        BIND(C, name="salsa_set_les_extra_fields")
+       IMPORT :: C_INT, C_INT32_T, C_PTR
+       TYPE(C_PTR), VALUE        :: ctx, gaerop, indefp     ! device arrays: a_gaerop, a_indefp (or C_NULL_PTR)
+       INTEGER(C_INT32_T), VALUE :: nindef                  ! rows of a_indefp
+     END FUNCTION salsa_set_les_extra_fields
+
      INTEGER(C_INT) FUNCTION salsa_ctx_sync(ctx) BIND(C, name="salsa_ctx_sync")
        IMPORT :: C_INT, C_PTR
        TYPE(C_PTR), VALUE :: ctx

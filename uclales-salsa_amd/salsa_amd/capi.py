@@ -46,6 +46,7 @@ EXPORTS = [
     "salsa_ctx_ntotal", "salsa_ctx_stream", "salsa_last_error", "salsa_ctx_timing",
     "salsa_ctx_kernel_time", "salsa_ctx_phase_time", "salsa_gradient_device_noalloc", "salsa_adjoint_device",
     "salsa_set_cloud_base_fields",
+    "salsa_set_les_extra_fields",
 ]
 
 NOALLOC_MAX_CELLS = 1 << 20
@@ -184,6 +185,8 @@ def lib():
     L.salsa_sedim_device.restype = C.c_int
     L.salsa_set_cloud_base_fields.argtypes = [C.c_void_p] * 4
     L.salsa_set_cloud_base_fields.restype = C.c_int
+    L.salsa_set_les_extra_fields.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+    L.salsa_set_les_extra_fields.restype = C.c_int
     L.salsa_last_error.argtypes = []
     L.salsa_last_error.restype = C.c_char_p
     _lib = L
@@ -380,6 +383,15 @@ class Context:
         None for all three clears them."""
         self._keep_cb = (w, nactd, vactd)
         _check(lib().salsa_set_cloud_base_fields(self._h, _ptr(w), _ptr(nactd), _ptr(vactd)))
+
+    def set_les_extra_fields(self, gaerop=None, indefp=None):
+        """The LES update's optional fields (salsa_set_les_extra_fields):
+        gaerop [5, n3] (a_gaerop, lscndgas) and indefp [rows, n3] (a_indefp,
+        ice_theta_dist), device tensors or None."""
+        self._keep_les = (gaerop, indefp)
+        n3 = self.shape[0] * self.shape[1] * self.shape[2]
+        nind = 0 if indefp is None else int(indefp.numel() // n3)
+        _check(lib().salsa_set_les_extra_fields(self._h, _ptr(gaerop), _ptr(indefp), nind))
 
     def cell_flags(self):
         out = np.zeros(self.ncells, dtype=np.uint8)
