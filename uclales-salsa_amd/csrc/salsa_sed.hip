@@ -42,6 +42,7 @@ namespace {
 
 constexpr int kSedMaxZ = 256;                 // levels per column (SedArgs::dzt)
 constexpr double kPiFull = 3.14159265358979323846, kKb = 1.3807e-23, kG = 9.81, kCp = 1005., kAlvl = 2.5e6;
+constexpr double kLn10 = 2.302585092994045684;
 
 __device__ double sd_terminal_vel(double diam, double rhop, double rhoa, double visc, double beta) {
   if (diam <= 19e-6) return (diam * diam) * (rhop - rhoa) * kGrav * beta / (18. * visc);
@@ -54,13 +55,17 @@ __device__ double sd_terminal_vel(double diam, double rhop, double rhoa, double 
     const double Re2 = beta * ::exp(Y);
     return visc / rhoa / diam * Re2;
   }
-  const double Np = ::pow(kSurfw0, 3.) * ::pow(rhoa, 2.) / (::pow(visc, 4.) * (rhop - rhoa) * kGrav);
-  const double Bo = 4. / 3. * (rhop - rhoa) * kGrav / kSurfw0 * ::pow(diam, 2.);
-  const double X3 = ::log(Bo * ::pow(Np, 1. / 6.));
+  // Np**(1/6) through log(Np) / 6: X3 = log(Bo) + log(Np) / 6 and
+  // Re3 = exp(log(Np) / 6 + Y3), within a few 1e-16 of the powers
+  const double v2 = visc * visc;
+  const double Np = (kSurfw0 * kSurfw0 * kSurfw0) * (rhoa * rhoa) / ((v2 * v2) * (rhop - rhoa) * kGrav);
+  const double L6 = ::log(Np) / 6.;
+  const double Bo = 4. / 3. * (rhop - rhoa) * kGrav / kSurfw0 * (diam * diam);
+  const double X3 = ::log(Bo) + L6;
   const double Z2 = X3 * X3, Z3 = Z2 * X3, Z4 = Z2 * Z2, Z5 = Z4 * X3;
   const double Y3 = -0.500015e1 + 0.523778e1 * X3 - 0.204914e1 * Z2 + 0.475294 * Z3 - 0.542819e-1 * Z4 +
                     0.238449e-2 * Z5;
-  const double Re3 = ::pow(Np, 1. / 6.) * ::exp(Y3);
+  const double Re3 = ::exp(L6 + Y3);
   return fmin(visc / rhoa / diam * Re3, 9.2);
 }
 
@@ -145,14 +150,18 @@ __global__ void __launch_bounds__(kSedMaxZ, MS <= 3 ? 3 : 1) k_sed_slow(SedArgs 
   const double *__restrict__ M = a.m_p[cat];
   double *__restrict__ Nt = a.n_t[cat];
   double *__restrict__ Mt = a.m_t[cat];
-  double ad = 1., dz = 0., cap = 0.;
+  double ad = 1., dz = 0., cap = 0., tkx = 0., ust = 0.;
   Air ar{0., 0.};
   if (src) {
     ad = a.adn[x];
-    ar = sd_air(a.tk[x], ad);
+    tkx = a.tk[x];
+    ar = sd_air(tkx, ad);
     dz = a.dzt[k];
     cap = fmin(0.5 * (1. / dz) / a.dtlt, 2.0);
+    if (k == 1) ust = a.ustar[col];
   }
+  // per-lane factor of the flux divergence (x * dz / adn rounded as x * (dz / adn))
+  const double dza = dz / ad;
   double sdw = 0., t0 = 0.;
   uint64_t onm = 0;                             // bins at or above the number limit
   // the rows of bin b (number, masses, their tendencies), loaded one bin
@@ -195,12 +204,13 @@ __global__ void __launch_bounds__(kSedMaxZ, MS <= 3 ? 3 : 1) k_sed_slow(SedArgs 
       vc = sd_vc_sums(zpn, v, ms, ad, ar, &dwet, &GG);
       vc = fmin(vc, cap);
       if (k == 1) {                               // the level just above the surface
-        const double ust = a.ustar[col];
-        const double mdiff = (kKb * a.tk[x] * GG) / (3.0 * kPiFull * ar.avis * dwet);
+        // Sc**(-2/3) and 10**(-3/St) as exponentials of logarithms (a few
+        // 1e-15 from the powers; Sc is the Schmidt number, 10**(-3/St) >= 1e-300)
+        const double mdiff = (kKb * tkx * GG) / (3.0 * kPiFull * ar.avis * dwet);
         const double Sc = (ar.avis / ad) / mdiff;
-        double St = vc * ::pow(ust, 2.0) / kG * (ar.avis / ad);
+        double St = vc * (ust * ust) / kG * (ar.avis / ad);
         if (St < 0.01) St = 0.01;
-        const double rtr = 1.0 / fmax(kEpsR8, (ust * (::pow(Sc, -2.0 / 3.0) + ::pow(10., -3.0 / St))));
+        const double rtr = 1.0 / fmax(kEpsR8, (ust * (::exp(-2.0 / 3.0 * ::log(Sc)) + ::exp(-3.0 / St * kLn10))));
         vc = (1. / rtr) + vc;
       }
       onm |= (uint64_t)1 << b;
@@ -221,12 +231,12 @@ __global__ void __launch_bounds__(kSedMaxZ, MS <= 3 ? 3 : 1) k_sed_slow(SedArgs 
       if (s < ns) F[(1 + s) * Z + k] = Pm[s];
     __syncthreads();
     if (src) {
-      const double dn = (F[k + 1] - Pn) * dz / ad;
+      const double dn = (F[k + 1] - Pn) * dza;
       Nt[(int64_t)b * n3 + x] = znt - dn;
 #pragma unroll
       for (int s = 0; s < MS; ++s) {
         if (s < ns) {
-          const double dm = (F[(1 + s) * Z + k + 1] - Pm[s]) * dz / ad;
+          const double dm = (F[(1 + s) * Z + k + 1] - Pm[s]) * dza;
           Mt[((int64_t)s * nb + b) * n3 + x] = mt[s] - dm;
           if (s == iw) sdw = sdw + dm;
         }
