@@ -340,7 +340,7 @@ __host__ __device__ inline size_t sed_prc_lds_bytes(int Z, int nb, int ns, bool 
 }
 
 template <int MS, bool ICE>
-__global__ void __launch_bounds__(kSedMaxZ) k_sed_prc(SedArgs a) {
+__global__ void __launch_bounds__(kSedMaxZ, (MS <= 3 && !ICE) ? 4 : 1) k_sed_prc(SedArgs a) {
   extern __shared__ double smem[];
   constexpr int cat = ICE ? 3 : 2;
   constexpr double alv = ICE ? kAls : kAlvl;
