@@ -65,8 +65,11 @@ def _compare(capi, oracle_mod, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, seed, 
     return ont, vt
 
 
+# (the 130- and 250-level grids run columns of three and four wavefronts:
+# the level-above fluxes and the fall-distance gathers cross wavefronts)
 @pytest.mark.parametrize("seed,grid,sed", [(1, (20, 8, 8), (1, 1, 1)), (2, (40, 12, 10), (1, 1, 1)),
-                                           (3, (20, 8, 8), (0, 0, 1)), (4, (20, 8, 8), (1, 0, 0))])
+                                           (3, (20, 8, 8), (0, 0, 1)), (4, (20, 8, 8), (1, 0, 0)),
+                                           (5, (130, 8, 8), (1, 1, 1)), (6, (250, 6, 6), (1, 1, 1))])
 def test_sedim_matches_oracle(capi, oracle_mod, seed, grid, sed):
     desc = capi.bin_desc()
     shape, dzt, fld, n_p, m_p, n_t, m_t = sed_fields(desc, grid, seed)
@@ -146,7 +149,8 @@ def _compare5(capi, oracle_mod, desc, shape, dzt, fld, n_p, m_p, n_t, m_t, seed,
 
 
 @pytest.mark.parametrize("seed,grid,sed", [(21, (20, 8, 8), (1, 1, 1, 1)), (22, (40, 12, 10), (1, 1, 1, 1)),
-                                           (23, (20, 8, 8), (0, 0, 0, 1))])
+                                           (23, (20, 8, 8), (0, 0, 0, 1)), (25, (130, 8, 8), (1, 1, 1, 1)),
+                                           (26, (250, 6, 6), (1, 1, 1, 1))])
 def test_sedim_level5_matches_oracle(capi, oracle_mod, seed, grid, sed):
     """Level 5 (SO4 + dust, ice with rime): the liquid categories and ice
     sedimentation (DepositionFast flag 4) against so_sedim5."""
