@@ -12,12 +12,15 @@
 !
 ! Usage: dropin_drive <input.bin> <output.bin>
 !   input : int32 hdr(12) = magic, nzp, nxp, nyp, level, nbin(1:2), nprc, nice,
-!           nspec_dry, istp, 0; int32 ids(7); real(8) dh(8) = reglim(1:4),
-!           prc_dlo, prc_vratio, ice_dlo, ice_vratio; real(8) tstep, time;
-!           fields press, temp, rsl, rsi, dn, rp, then per category the
-!           number field (nb) and the mass field (nsw*nb), each (nzp,nxp,nyp,.)
+!           nspec_dry, istp, activation mode (0 off, 2 cloud base); int32
+!           ids(7); real(8) dh(8) = reglim(1:4), prc_dlo, prc_vratio, ice_dlo,
+!           ice_vratio; real(8) tstep, time; fields press, temp, rsl, rsi, dn,
+!           rp, then per category the number field (nb) and the mass field
+!           (nsw*nb), each (nzp,nxp,nyp,.); with cloud-base activation the
+!           vertical velocity wp (nzp,nxp,nyp)
 !   output: for each of the two calls, the tendencies of rp, then per
-!           category of the number and mass fields, then the rate fields
+!           category of the number and mass fields, then the rate fields;
+!           with cloud-base activation then pa_nactd and pa_vactd
 PROGRAM dropin_drive
   USE mo_submctl, ONLY : nbin, reglim, bloPrc, bloIce, nspec_dry, listspec, lscoag, lscnd, lsauto, lsactiv, &
                          lsicenucl, lsicemelt, lssecice, lscndgas, nsnucl, lsdistupdate, lscgia, lscgic,    &
@@ -79,6 +82,9 @@ PROGRAM dropin_drive
   lscnd%switch = .TRUE.;  lscnd%state = .TRUE.
   lsauto%switch = .TRUE.; lsauto%state = .TRUE.; lsauto%mode = 1
   lsactiv%state = .FALSE.; lsicenucl%state = .FALSE.; lsicenucl%mode = 1
+  IF (hdr(12) == 2) THEN           ! cloud-base activation (mo_salsa.f90:115-119)
+     lsactiv%switch = .TRUE.; lsactiv%state = .TRUE.; lsactiv%mode = 2
+  END IF
   lsicemelt%state = .FALSE.; lssecice%state = .FALSE.
   lscndgas = .FALSE.; nsnucl = 0
   lsdistupdate = .FALSE.
@@ -108,6 +114,7 @@ PROGRAM dropin_drive
      READ(10) nP(:,:,:,1:nb(c),c)
      READ(10) mP(:,:,:,1:nsw(c)*nb(c),c)
   END DO
+  IF (hdr(12) == 2) READ(10) wp
   CLOSE(10)
 
   Diag = FieldArray()
@@ -151,6 +158,10 @@ PROGRAM dropin_drive
         WRITE(11) mT(:,:,:,1:nsw(c)*nb(c),c)
      END DO
      WRITE(11) d3(:,:,:,6:5+nrf)
+     IF (hdr(12) == 2) THEN
+        WRITE(11) nactd
+        WRITE(11) vactd
+     END IF
   END DO
   CLOSE(11)
   CALL salsa_amd_finalize()

@@ -33,10 +33,10 @@ def capi():
     return c
 
 
-def _write_input(path, desc, f, tstep, time, istp):
+def _write_input(path, desc, f, tstep, time, istp, wp=None):
     g = f.grid
     hdr = np.array([MAGIC, g.nzp, g.nxp, g.nyp, f.level, desc.nbin[0], desc.nbin[1], desc.nprc, desc.nice,
-                    desc.nspec_dry, istp, 0], np.int32)
+                    desc.nspec_dry, istp, 0 if wp is None else 2], np.int32)
     ids = np.zeros(7, np.int32)
     ids[:desc.nspec_dry] = list(desc.dry_species)[:desc.nspec_dry]
     dh = np.array(list(desc.reglim) + [desc.prc_dlo, desc.prc_vratio, desc.ice_dlo, desc.ice_vratio])
@@ -51,9 +51,11 @@ def _write_input(path, desc, f, tstep, time, istp):
         for c in range(ncat):
             np.ascontiguousarray(f.n[c], dtype=np.float64).tofile(fh)
             np.ascontiguousarray(f.m[c], dtype=np.float64).tofile(fh)
+        if wp is not None:
+            np.ascontiguousarray(wp, dtype=np.float64).tofile(fh)
 
 
-def _read_output(path, desc, f):
+def _read_output(path, desc, f, act=False):
     g = f.grid
     n3 = g.nzp * g.nxp * g.nyp
     ncat = 4 if f.level == 5 else 3
@@ -74,7 +76,15 @@ def _read_output(path, desc, f):
             off += a
         rates = raw[off:off + nrf * n3].reshape(nrf, n3)
         off += nrf * n3
-        calls.append((rv_t, n_t, m_t, rates))
+        if act:
+            nwet = desc.nspec - 1
+            na = raw[off:off + desc.ncld * n3].reshape(desc.ncld, n3)
+            off += desc.ncld * n3
+            va = raw[off:off + nwet * desc.ncld * n3].reshape(nwet * desc.ncld, n3)
+            off += nwet * desc.ncld * n3
+            calls.append((rv_t, n_t, m_t, rates, na, va))
+        else:
+            calls.append((rv_t, n_t, m_t, rates))
     assert off == raw.size
     return calls
 
@@ -141,3 +151,60 @@ def test_fortran_dropin_run_salsa(capi, oracle_mod, tmp_path):
     iwa = desc.iwa
     d = np.abs(outs[0][:, :desc.nbins, iwa] - outs[1][:, :desc.nbins, iwa])
     assert np.count_nonzero(d > 1e-6 * np.abs(outs[1][:, :desc.nbins, iwa])) > 0.05 * d.size
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="oracle/_ref/dropin_drive is built by `make -C oracle ref`")
+def test_fortran_dropin_cloud_base_activation(capi, tmp_path):
+    """lsactiv mode 2 through the drop-in: run_SALSA copies wp to the
+    device, the step activates at cloud base, and pa_nactd / pa_vactd come
+    back in the reference's layout (mo_salsa_driver.f90:360-366).  Both
+    calls (initialization and time step) bitwise those of salsa_step_host
+    with the same fields registered from Python, halos zero, and droplets
+    activated where the updraft is positive."""
+    import torch
+    from salsa_amd import workload as W
+    desc = capi.bin_desc()
+    f = W.config_fields(desc, 1)
+    g = f.grid
+    n3 = g.nzp * g.nxp * g.nyp
+    wp = np.random.default_rng(4).uniform(-0.5, 4., n3)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_input(fin, desc, f, 1.0, 10.0, 1, wp=wp)
+    print("dropin_drive with cloud-base activation on the config-1 grid %dx%dx%d" % (g.nzp, g.nxp, g.nyp),
+          flush=True)
+    env = dict(os.environ, SALSA_AMD_PRECISION="fp64")
+    r = subprocess.run([EXE, fin, fout], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0 and "dropin_drive done" in r.stdout, (r.stdout[-2000:], r.stderr[-2000:])
+    calls = _read_output(fout, desc, f, act=True)
+    nwet = desc.nspec - 1
+    dev = torch.device("cuda")
+    for k, flags in enumerate((capi.STEP_INIT, 0)):
+        ctx = capi.Context(desc, capi.params(level=4, precision=capi.SALSA_FP64, lsactiv=2), g.nzp, g.nxp, g.nyp)
+        dw = torch.from_numpy(wp).to(dev)
+        na = torch.zeros((desc.ncld, n3), dtype=torch.float64, device=dev)
+        va = torch.zeros((nwet * desc.ncld, n3), dtype=torch.float64, device=dev)
+        ctx.set_cloud_base_fields(dw, na, va)
+        fd = dict(press=f.press, temp=f.temp, rsl=f.rsl, rsi=f.rsi, dn=f.dn, rv=f.rv)
+        n_t = [np.zeros_like(x) for x in f.n[:3]]
+        m_t = [np.zeros_like(x) for x in f.m[:3]]
+        rv_t = np.zeros_like(f.rv)
+        rates = np.zeros((12, n3))
+        ctx.step_host(fd, f.n[:3], f.m[:3], n_t, m_t, rv_t, flags=flags, rates=rates, time=10.0, istp=1)
+        ctx.sync()
+        pna, pva = na.cpu().numpy(), va.cpu().numpy()
+        ctx.close()
+        frv, fn, fm, frates, fna, fva = calls[k]
+        assert np.array_equal(frv, rv_t), k
+        for c in range(3):
+            assert np.array_equal(fn[c], n_t[c]), (k, c)
+            assert np.array_equal(fm[c], m_t[c]), (k, c)
+        assert np.array_equal(frates, rates), k
+        assert np.array_equal(fna, pna), k
+        assert np.array_equal(fva, pva), k
+        # interior cells with an updraft activate; halos and w <= 0 stay zero
+        interior = np.zeros((g.nyp, g.nxp, g.nzp), bool)
+        interior[2:-2, 2:-2, 1:-1] = True
+        interior = interior.reshape(-1)
+        act = fna.sum(axis=0) > 0
+        print("  call %d: %d cells activate, largest pa_nactd %.3e /kg" % (k, int(act.sum()), fna.max()), flush=True)
+        assert act.sum() > 0 and not (act & ~interior).any() and not (act & (wp <= 0)).any()

@@ -10,9 +10,10 @@
 !> process-rate Diag fields of the interior cells.
 !>
 !> Only the processes of this step are provided: coagulation, condensation,
-!> autoconv2 (lsauto mode 2), interstitial activation (lsactiv mode 1) and
-!> distr_update.  A configuration that also switches on secondary ice,
-!> cloud-base activation, ice nucleation, melting, gas-phase condensation or
+!> autoconv2 (lsauto mode 2), cloud activation (lsactiv mode 1, interstitial;
+!> mode 2, cloud base, from wp into pa_nactd / pa_vactd through device copies
+!> of the three fields) and distr_update.  A configuration that also switches
+!> on secondary ice, ice nucleation, melting, gas-phase condensation or
 !> nucleation stops with a message naming the process (mo_salsa.f90:92-153
 !> calls them in the same step).
 !>
@@ -35,6 +36,35 @@ MODULE mo_salsa_amd_driver
 
   TYPE(C_PTR), SAVE :: ctx = C_NULL_PTR
   REAL(C_DOUBLE), ALLOCATABLE, TARGET, SAVE :: rates(:,:,:,:)
+  ! cloud-base activation: device copies of wp, pa_nactd and pa_vactd
+  TYPE(C_PTR), SAVE :: d_w = C_NULL_PTR, d_nactd = C_NULL_PTR, d_vactd = C_NULL_PTR
+  INTEGER(C_SIZE_T), SAVE :: act_n3 = 0
+
+  ! the HIP runtime calls the activation fields need (hip_runtime_api.h)
+  INTEGER(C_INT), PARAMETER :: hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2
+  INTERFACE
+     INTEGER(C_INT) FUNCTION hipMalloc(ptr, bytes) BIND(C, name="hipMalloc")
+       IMPORT :: C_INT, C_PTR, C_SIZE_T
+       TYPE(C_PTR)              :: ptr
+       INTEGER(C_SIZE_T), VALUE :: bytes
+     END FUNCTION hipMalloc
+     INTEGER(C_INT) FUNCTION hipFree(ptr) BIND(C, name="hipFree")
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: ptr
+     END FUNCTION hipFree
+     INTEGER(C_INT) FUNCTION hipMemset(ptr, val, bytes) BIND(C, name="hipMemset")
+       IMPORT :: C_INT, C_PTR, C_SIZE_T
+       TYPE(C_PTR), VALUE       :: ptr
+       INTEGER(C_INT), VALUE    :: val
+       INTEGER(C_SIZE_T), VALUE :: bytes
+     END FUNCTION hipMemset
+     INTEGER(C_INT) FUNCTION hipMemcpy(dst, src, bytes, kind) BIND(C, name="hipMemcpy")
+       IMPORT :: C_INT, C_PTR, C_SIZE_T
+       TYPE(C_PTR), VALUE       :: dst, src
+       INTEGER(C_SIZE_T), VALUE :: bytes
+       INTEGER(C_INT), VALUE    :: kind
+     END FUNCTION hipMemcpy
+  END INTERFACE
 
   CHARACTER(len=4), PARAMETER :: dry_names(7) = ['SO4 ', 'OC  ', 'BC  ', 'DU  ', 'SS  ', 'NO  ', 'NH  ']
   CHARACTER(len=16), PARAMETER :: rate_names(SALSA_NRATE_L5) = [CHARACTER(len=16) ::                      &
@@ -53,11 +83,11 @@ CONTAINS
     REAL, INTENT(in)    :: time
     INTEGER, INTENT(in) :: istp
     LOGICAL, INTENT(in) :: initialize
-    REAL, INTENT(in)    :: wp(nzp,nxp,nyp)
+    REAL, INTENT(in), TARGET  :: wp(nzp,nxp,nyp)
     INTEGER, INTENT(in) :: level
     LOGICAL, INTENT(in) :: lcharge
-    REAL, INTENT(out)   :: pa_vactd(nzp,nxp,nyp,ns*ncld)
-    REAL, INTENT(out)   :: pa_nactd(nzp,nxp,nyp,ncld)
+    REAL, INTENT(out), TARGET :: pa_vactd(nzp,nxp,nyp,ns*ncld)
+    REAL, INTENT(out), TARGET :: pa_nactd(nzp,nxp,nyp,ncld)
 
     TYPE(FloatArray3d), POINTER :: ppress => NULL(), ptk => NULL(), prs => NULL(), prsi => NULL(), &
                                    pdn => NULL(), prv => NULL(), prt => NULL(), fld => NULL()
@@ -66,12 +96,28 @@ CONTAINS
                                    pnpp => NULL(), pnpt => NULL(), pmpp => NULL(), pmpt => NULL(), &
                                    pnip => NULL(), pnit => NULL(), pmip => NULL(), pmit => NULL()
     TYPE(C_PTR) :: n_p(4), n_t(4), m_p(4), m_t(4), crsi
-    INTEGER :: nrf, f
+    INTEGER :: nrf, f, nwet
     INTEGER(C_INT) :: rc
     INTEGER(C_INT32_T) :: sflags
+    INTEGER(C_SIZE_T) :: n3
+    LOGICAL :: cloud_base
 
     CALL check_switches()
     IF (.NOT. C_ASSOCIATED(ctx)) CALL create_context(level, lcharge, nzp, nxp, nyp)
+
+    ! cloud-base activation (lsactiv mode 2): wp to the device before the
+    ! step, the activated droplets back after it (mo_salsa_driver.f90:360-366)
+    pa_nactd = 0.
+    pa_vactd = 0.
+    cloud_base = lsactiv%state .AND. lsactiv%mode == 2
+    nwet = spec%getNSpec(type="wet")
+    n3 = INT(nzp, C_SIZE_T) * INT(nxp, C_SIZE_T) * INT(nyp, C_SIZE_T)
+    IF (cloud_base) THEN
+       IF (ns < nwet) CALL fail('pa_vactd holds fewer species than the wet species')
+       IF (.NOT. C_ASSOCIATED(d_w)) CALL alloc_activation_fields(n3, nwet)
+       rc = hipMemcpy(d_w, C_LOC(wp(1,1,1)), 8_C_SIZE_T * n3, hipMemcpyHostToDevice)
+       IF (rc /= 0) CALL fail('copying wp to the device')
+    END IF
 
     CALL Diag%getData(1, ppress, name="press")
     CALL Diag%getData(1, ptk, name="temp")
@@ -129,15 +175,47 @@ CONTAINS
        fld%d(:,:,:) = rates(:,:,:,f)
     END DO
 
-    ! newly activated droplets: activation is not part of this step
-    pa_nactd = 0.
-    pa_vactd = 0.
-    IF (.FALSE.) WRITE(*,*) wp(1,1,1)
+    ! the droplets activated at cloud base (interior cells; the rest stay 0);
+    ! pa_vactd's first nwet*ncld slabs, species-major as getMassIndex
+    IF (cloud_base) THEN
+       rc = hipMemcpy(C_LOC(pa_nactd(1,1,1,1)), d_nactd, 8_C_SIZE_T * n3 * INT(ncld, C_SIZE_T),            &
+                      hipMemcpyDeviceToHost)
+       IF (rc == 0) rc = hipMemcpy(C_LOC(pa_vactd(1,1,1,1)), d_vactd,                                       &
+                                   8_C_SIZE_T * n3 * INT(nwet * ncld, C_SIZE_T), hipMemcpyDeviceToHost)
+       IF (rc /= 0) CALL fail('copying pa_nactd / pa_vactd from the device')
+    END IF
   END SUBROUTINE run_SALSA
 
+  ! device copies of wp, pa_nactd and pa_vactd, registered with the context
+  ! (salsa_set_cloud_base_fields); the kernel writes the interior cells only,
+  ! so the rest is zeroed once here
+  SUBROUTINE alloc_activation_fields(n3, nwet)
+    INTEGER(C_SIZE_T), INTENT(in) :: n3
+    INTEGER, INTENT(in) :: nwet
+    INTEGER(C_INT) :: rc
+    INTEGER(C_SIZE_T) :: bn, bv
+    bn = 8_C_SIZE_T * n3 * INT(ncld, C_SIZE_T)
+    bv = 8_C_SIZE_T * n3 * INT(nwet * ncld, C_SIZE_T)
+    rc = hipMalloc(d_w, 8_C_SIZE_T * n3)
+    IF (rc == 0) rc = hipMalloc(d_nactd, bn)
+    IF (rc == 0) rc = hipMalloc(d_vactd, bv)
+    IF (rc == 0) rc = hipMemset(d_nactd, 0_C_INT, bn)
+    IF (rc == 0) rc = hipMemset(d_vactd, 0_C_INT, bv)
+    IF (rc /= 0) CALL fail('device memory for the activation fields')
+    act_n3 = n3
+    rc = salsa_set_cloud_base_fields(ctx, d_w, d_nactd, d_vactd)
+    IF (rc /= SALSA_OK) CALL fail('cloud-base activation fields')
+  END SUBROUTINE alloc_activation_fields
+
   SUBROUTINE salsa_amd_finalize()
+    INTEGER(C_INT) :: rc
     IF (C_ASSOCIATED(ctx)) CALL salsa_ctx_destroy(ctx)
     ctx = C_NULL_PTR
+    IF (C_ASSOCIATED(d_w)) rc = hipFree(d_w)
+    IF (C_ASSOCIATED(d_nactd)) rc = hipFree(d_nactd)
+    IF (C_ASSOCIATED(d_vactd)) rc = hipFree(d_vactd)
+    d_w = C_NULL_PTR; d_nactd = C_NULL_PTR; d_vactd = C_NULL_PTR
+    act_n3 = 0
     IF (ALLOCATED(rates)) DEALLOCATE(rates)
   END SUBROUTINE salsa_amd_finalize
 
@@ -188,7 +266,9 @@ CONTAINS
     p%lscoag_mode = INT(lscoag%mode, C_INT32_T)
     p%cgintvl = REAL(cgintvl, C_DOUBLE)
     p%lsdistupdate = MERGE(1_C_INT32_T, 0_C_INT32_T, lsdistupdate)   ! distr_update after condensation
-    p%lsactiv = MERGE(1_C_INT32_T, 0_C_INT32_T, lsactiv%state .AND. lsactiv%mode == 1)   ! interstitial activation
+    p%lsactiv = 0_C_INT32_T
+    IF (lsactiv%state .AND. lsactiv%mode == 1) p%lsactiv = 1_C_INT32_T   ! interstitial activation
+    IF (lsactiv%state .AND. lsactiv%mode == 2) p%lsactiv = 2_C_INT32_T   ! cloud base
     rc = salsa_ctx_create(desc, p, INT(nzp, C_INT32_T), INT(nxp, C_INT32_T), INT(nyp, C_INT32_T), ctx)
     IF (rc /= SALSA_OK) CALL fail('context')
   END SUBROUTINE create_context
@@ -196,7 +276,7 @@ CONTAINS
   ! Processes that salsa() would run in the same step but this library does not
   SUBROUTINE check_switches()
     IF (lssecice%state) CALL outside('secondary ice production (lssecice)')
-    IF (lsactiv%state .AND. lsactiv%mode /= 1) CALL outside('cloud-base activation (lsactiv mode 2)')
+    IF (lsactiv%state .AND. lsactiv%mode /= 1 .AND. lsactiv%mode /= 2) CALL outside('cloud activation mode')
     IF (lsicenucl%state .OR. lsicenucl%mode == 2) CALL outside('ice nucleation (lsicenucl)')
     IF (lsicemelt%state) CALL outside('ice melting (lsicemelt)')
     IF (lscndgas) CALL outside('gas-phase condensation (lscndgas)')
